@@ -1,0 +1,176 @@
+"""ctypes binding of libbsim.so (the C ABI declared in include/bsim.h).
+
+The HIP library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("BSIM_LIBRARY", _PKG / "libbsim.so"))
+
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+class SolverConfig(C.Structure):
+    """LinearSolverConfig (solver_control.h:165-187) + AMG/precision controls."""
+
+    _fields_ = [("solver", C.c_int), ("pc", C.c_int), ("atol", C.c_double), ("rtol", C.c_double),
+                ("max_iter", C.c_int), ("warm_start", C.c_int), ("rebuild_interval", C.c_int),
+                ("reuse_growth", C.c_double), ("amg_max_levels", C.c_int), ("amg_coarse_size", C.c_int),
+                ("amg_pre_sweeps", C.c_int), ("amg_post_sweeps", C.c_int), ("amg_omega", C.c_double),
+                ("amg_strength", C.c_double), ("amg_correction", C.c_double), ("precision", C.c_int),
+                ("inner_rtol", C.c_double)]
+
+
+def solver_config(**kw) -> SolverConfig:
+    base = dict(solver=1, pc=2, atol=1e-12, rtol=1e-8, max_iter=1000, warm_start=0, rebuild_interval=8,
+                reuse_growth=2.0, amg_max_levels=12, amg_coarse_size=48, amg_pre_sweeps=2, amg_post_sweeps=2,
+                amg_omega=0.8, amg_strength=0.25, amg_correction=1.0, precision=0, inner_rtol=1e-3)
+    base.update(kw)
+    return SolverConfig(**base)
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("initial", C.c_double),
+                ("final_residual", C.c_double), ("relative", C.c_double)]
+
+
+class SimpleConfig(C.Structure):
+    """Steady SIMPLE controls (src/physics/simple/main.cpp:21-31) and per-equation methods."""
+
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double),
+                ("m_interp", C.c_int), ("m_grad", C.c_int), ("m_conv", C.c_int), ("m_diff", C.c_int),
+                ("m_coef_interp", C.c_int), ("m_coef_grad", C.c_int),
+                ("p_interp", C.c_int), ("p_grad", C.c_int), ("p_diff", C.c_int),
+                ("p_coef_interp", C.c_int), ("p_coef_grad", C.c_int),
+                ("alpha_u", C.c_double), ("alpha_p", C.c_double), ("non_orth_corr", C.c_int),
+                ("tol_mass", C.c_double), ("tol_u", C.c_double), ("tol_mom", C.c_double), ("tol_p", C.c_double),
+                ("mom", SolverConfig), ("pre", SolverConfig)]
+
+
+class IterReport(C.Structure):
+    _fields_ = [("rU", C.c_double), ("dU", C.c_double), ("dP", C.c_double), ("mass", C.c_double),
+                ("linP", C.c_double), ("mom_iters", C.c_int * 3), ("p_iters", C.c_int),
+                ("mom_status", C.c_int), ("p_status", C.c_int), ("converged", C.c_int), ("referenced", C.c_int)]
+
+
+def simple_config(**kw) -> SimpleConfig:
+    base = dict(rho=1.0, mu=0.01, m_interp=0, m_grad=0, m_conv=1, m_diff=0, m_coef_interp=0, m_coef_grad=0,
+                p_interp=0, p_grad=0, p_diff=0, p_coef_interp=0, p_coef_grad=0, alpha_u=0.7, alpha_p=0.3,
+                non_orth_corr=1, tol_mass=1e-6, tol_u=1e-6, tol_mom=1e-6, tol_p=1e-6)
+    mom = kw.pop("mom", None) or solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
+    pre = kw.pop("pre", None) or solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14)
+    base.update(kw)
+    return SimpleConfig(mom=mom, pre=pre, **base)
+
+
+def ptr(a: np.ndarray | None, kind=dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(kind)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libbsim.so is not built ({LIB_PATH}); run babelsim_amd.build")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != 0:
+        msg = lib().bsim_last_error().decode()
+        raise (ValueError if code == 2 else RuntimeError)(msg)
+
+
+def _declare(L: C.CDLL) -> None:
+    L.bsim_last_error.restype = C.c_char_p
+    L.bsim_mesh_generate_hex.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
+                                         C.c_uint64, C.POINTER(C.c_void_p)]
+    L.bsim_mesh_from_arrays.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, ip, ip, C.c_int, C.POINTER(C.c_void_p)]
+    L.bsim_mesh_prepare.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), ip, ip]
+    L.bsim_mesh_sizes.argtypes = [C.c_void_p, ip, ip, ip, ip, ip]
+    L.bsim_mesh_sizes.restype = None
+    L.bsim_mesh_arrays.argtypes = [C.c_void_p, dp, ip, ip, ip, ip, ip]
+    L.bsim_mesh_arrays.restype = None
+    L.bsim_mesh_free.argtypes = [C.c_void_p]
+    L.bsim_mesh_free.restype = None
+    for name, args in _DEVICE_SIGNATURES.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+
+
+vp = C.c_void_p
+_DEVICE_SIGNATURES: dict = {}
+
+
+class MeshData:
+    """A polyhedral face list (Mesh::polyhedral inputs) held by the library."""
+
+    def __init__(self, handle: int):
+        self.handle = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.bsim_mesh_free(self.handle)
+            self.handle = None
+
+    @classmethod
+    def hex(cls, n, lo=(0, 0, 0), hi=(1, 1, 1), vertices=None, jitter=0.0, split=0.0, seed=1) -> "MeshData":
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        v = None if vertices is None else np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1)
+        h = C.c_void_p()
+        check(lib().bsim_mesh_generate_hex(int(n[0]), int(n[1]), int(n[2]), ptr(lo), ptr(hi), ptr(v),
+                                           float(jitter), float(split), int(seed), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, a: dict) -> "MeshData":
+        v = np.ascontiguousarray(a["vertices"], dtype=np.float64).reshape(-1)
+        fpo = np.ascontiguousarray(a["fpo"], dtype=np.int32)
+        fpi = np.ascontiguousarray(a["fpi"], dtype=np.int32)
+        own = np.ascontiguousarray(a["owner"], dtype=np.int32)
+        nei = np.ascontiguousarray(a["neighbour"], dtype=np.int32)
+        pat = np.ascontiguousarray(a["patch"], dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().bsim_mesh_from_arrays(len(v) // 3, ptr(v), len(own), ptr(fpo, ip), ptr(fpi, ip), ptr(own, ip),
+                                          ptr(nei, ip), ptr(pat, ip), int(a["npatch"]), C.byref(h)))
+        return cls(h.value)
+
+    def sizes(self):
+        vals = [C.c_int() for _ in range(5)]
+        lib().bsim_mesh_sizes(self.handle, *[C.byref(x) for x in vals])
+        return tuple(x.value for x in vals)  # nv, nf, nfp, npatch, ncells
+
+    def arrays(self) -> dict:
+        nv, nf, nfp, npatch, nc = self.sizes()
+        out = dict(vertices=np.zeros(3 * nv), fpo=np.zeros(nf + 1, np.int32), fpi=np.zeros(nfp, np.int32),
+                   owner=np.zeros(nf, np.int32), neighbour=np.zeros(nf, np.int32), patch=np.zeros(nf, np.int32))
+        lib().bsim_mesh_arrays(self.handle, ptr(out["vertices"]), ptr(out["fpo"], ip), ptr(out["fpi"], ip),
+                               ptr(out["owner"], ip), ptr(out["neighbour"], ip), ptr(out["patch"], ip))
+        out["npatch"] = npatch
+        out["ncells"] = nc
+        return out
+
+    def prepare(self, renumber: bool = True):
+        nv, nf, nfp, npatch, nc = self.sizes()
+        cperm = np.zeros(nc, np.int32)
+        fperm = np.zeros(nf, np.int32)
+        h = C.c_void_p()
+        check(lib().bsim_mesh_prepare(self.handle, int(renumber), C.byref(h), ptr(cperm, ip), ptr(fperm, ip)))
+        return MeshData(h.value), cperm, fperm

@@ -1,0 +1,81 @@
+// C ABI: host mesh lists (generation, wrapping, renumbering).
+#include "../../include/bsim.h"
+#include "errors.hpp"
+#include "mesh_host.hpp"
+
+#include <cstring>
+
+struct bsim_mesh {
+    bsim::MeshDesc d;
+};
+
+namespace bsim {
+thread_local std::string g_lastError;
+void setError(const std::string& message) { g_lastError = message; }
+}  // namespace bsim
+
+extern "C" {
+
+const char* bsim_last_error(void) { return bsim::g_lastError.c_str(); }
+int bsim_version(void) { return 1; }
+
+int bsim_mesh_generate_hex(int nx, int ny, int nz, const double* lo, const double* hi, const double* vertices,
+                           double jitter, double split_fraction, uint64_t seed, bsim_mesh** out) {
+    return bsim::guarded([&] {
+        auto* m = new bsim_mesh;
+        try { m->d = bsim::generateHex(nx, ny, nz, lo, hi, vertices, jitter, split_fraction, seed); }
+        catch (...) { delete m; throw; }
+        *out = m;
+    });
+}
+
+int bsim_mesh_from_arrays(int nv, const double* vertices, int nf, const int* fpo, const int* fpi, const int* owner,
+                          const int* neighbour, const int* patch, int npatch, bsim_mesh** out) {
+    return bsim::guarded([&] {
+        if (nv <= 0 || nf <= 0 || npatch <= 0) throw std::invalid_argument("mesh arrays are empty");
+        auto* m = new bsim_mesh;
+        m->d.vert.assign(vertices, vertices + 3 * static_cast<long long>(nv));
+        m->d.fpo.assign(fpo, fpo + nf + 1);
+        m->d.fpi.assign(fpi, fpi + fpo[nf]);
+        m->d.own.assign(owner, owner + nf);
+        m->d.nei.assign(neighbour, neighbour + nf);
+        m->d.patch.assign(patch, patch + nf);
+        m->d.npatch = npatch;
+        *out = m;
+    });
+}
+
+int bsim_mesh_prepare(const bsim_mesh* in, int renumber, bsim_mesh** out, int* cell_new_to_old,
+                      int* face_new_to_old) {
+    return bsim::guarded([&] {
+        std::vector<int> c, f;
+        auto* m = new bsim_mesh;
+        m->d = bsim::prepare(in->d, renumber != 0, c, f);
+        if (cell_new_to_old) std::memcpy(cell_new_to_old, c.data(), sizeof(int) * c.size());
+        if (face_new_to_old) std::memcpy(face_new_to_old, f.data(), sizeof(int) * f.size());
+        *out = m;
+    });
+}
+
+void bsim_mesh_sizes(const bsim_mesh* m, int* nv, int* nf, int* nfp, int* npatch, int* ncells) {
+    if (nv) *nv = static_cast<int>(m->d.vert.size() / 3);
+    if (nf) *nf = m->d.nfaces();
+    if (nfp) *nfp = static_cast<int>(m->d.fpi.size());
+    if (npatch) *npatch = m->d.npatch;
+    if (ncells) *ncells = m->d.ncells();
+}
+
+void bsim_mesh_arrays(const bsim_mesh* m, double* vertices, int* fpo, int* fpi, int* owner, int* neighbour,
+                      int* patch) {
+    const auto& d = m->d;
+    if (vertices) std::memcpy(vertices, d.vert.data(), sizeof(double) * d.vert.size());
+    if (fpo) std::memcpy(fpo, d.fpo.data(), sizeof(int) * d.fpo.size());
+    if (fpi) std::memcpy(fpi, d.fpi.data(), sizeof(int) * d.fpi.size());
+    if (owner) std::memcpy(owner, d.own.data(), sizeof(int) * d.own.size());
+    if (neighbour) std::memcpy(neighbour, d.nei.data(), sizeof(int) * d.nei.size());
+    if (patch) std::memcpy(patch, d.patch.data(), sizeof(int) * d.patch.size());
+}
+
+void bsim_mesh_free(bsim_mesh* m) { delete m; }
+
+}  // extern "C"

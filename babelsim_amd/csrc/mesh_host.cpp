@@ -1,0 +1,245 @@
+// Host-side mesh generation and renumbering (setup only, never on the timed
+// path).  The generator produces the synthetic polyhedral cavities used by the
+// benchmark and tests; prepare() fixes the cell and face order the device
+// kernels rely on for coalesced face and row access.
+#include "mesh_host.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+
+namespace bsim {
+
+int MeshDesc::ncells() const {
+    int n = 0;
+    for (std::size_t f = 0; f < own.size(); ++f) {
+        n = std::max(n, own[f] + 1);
+        if (nei[f] >= 0) n = std::max(n, nei[f] + 1);
+    }
+    return n;
+}
+
+namespace {
+
+std::uint64_t mix(std::uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+double unit(std::uint64_t seed, std::uint64_t a, std::uint64_t b) {
+    return static_cast<double>(mix(seed ^ mix(a * 8 + b)) >> 11) * 0x1.0p-53;
+}
+
+// Local hex vertices: v0=(0,0,0) v1=(1,0,0) v2=(1,1,0) v3=(0,1,0) and +4 at dz=1.
+constexpr int kLocal[2][2][2] = {{{0, 1}, {3, 2}}, {{4, 5}, {7, 6}}};  // [dz][dy][dx]
+constexpr int kOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+constexpr int kFace[6][4] = {{0, 4, 7, 3}, {1, 2, 6, 5}, {0, 1, 5, 4}, {3, 7, 6, 2}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+// Diagonal plane per family: prism A holds vertices with d[first] >= d[second].
+constexpr int kPlane[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+constexpr int kInterior[3][4] = {{0, 1, 6, 7}, {0, 3, 6, 5}, {0, 2, 6, 4}};
+
+}  // namespace
+
+MeshDesc generateHex(int nx, int ny, int nz, const double lo[3], const double hi[3], const double* vertices,
+                     double jitter, double splitFraction, std::uint64_t seed) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("hex counts must be positive");
+    if (!(jitter >= 0.0 && jitter < 0.5)) throw std::invalid_argument("jitter must lie in [0, 0.5)");
+    if (!(splitFraction >= 0.0 && splitFraction <= 1.0)) throw std::invalid_argument("split fraction must lie in [0, 1]");
+    const long long nvx = nx + 1, nvy = ny + 1, nvz = nz + 1;
+    const long long nv = nvx * nvy * nvz;
+    const long long nh = static_cast<long long>(nx) * ny * nz;
+    if (nh * 2 + 1 > 0x7fffffffLL || nv > 0x7fffffffLL) throw std::invalid_argument("mesh exceeds 32-bit indexing");
+    MeshDesc m;
+    m.npatch = 6;
+    m.vert.resize(3 * nv);
+    const double h[3] = {(hi[0] - lo[0]) / nx, (hi[1] - lo[1]) / ny, (hi[2] - lo[2]) / nz};
+    for (long long k = 0; k < nvz; ++k)
+        for (long long j = 0; j < nvy; ++j)
+            for (long long i = 0; i < nvx; ++i) {
+                const long long v = i + nvx * (j + nvy * k);
+                double p[3];
+                if (vertices) { p[0] = vertices[3 * v]; p[1] = vertices[3 * v + 1]; p[2] = vertices[3 * v + 2]; }
+                else {
+                    p[0] = lo[0] + (hi[0] - lo[0]) * static_cast<double>(i) / nx;
+                    p[1] = lo[1] + (hi[1] - lo[1]) * static_cast<double>(j) / ny;
+                    p[2] = lo[2] + (hi[2] - lo[2]) * static_cast<double>(k) / nz;
+                }
+                const bool interior = i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
+                if (interior && jitter > 0.0)
+                    for (int d = 0; d < 3; ++d) p[d] += (2.0 * unit(seed, static_cast<std::uint64_t>(v), d) - 1.0) * jitter * h[d];
+                m.vert[3 * v] = p[0]; m.vert[3 * v + 1] = p[1]; m.vert[3 * v + 2] = p[2];
+            }
+    // Split decisions and cell numbering: a split hex becomes two consecutive cells.
+    std::vector<signed char> family(nh, -1);
+    std::vector<int> base(nh);
+    int next = 0;
+    const std::uint64_t splitSeed = mix(seed ^ 0x5b1175eedULL);
+    for (long long hx = 0; hx < nh; ++hx) {
+        if (splitFraction > 0.0 && unit(splitSeed, static_cast<std::uint64_t>(hx), 0) < splitFraction)
+            family[hx] = static_cast<signed char>(mix(splitSeed + static_cast<std::uint64_t>(hx)) % 3);
+        base[hx] = next;
+        next += family[hx] >= 0 ? 2 : 1;
+    }
+    auto hexId = [&](int a, int b, int c) { return static_cast<long long>(a) + nx * (static_cast<long long>(b) + static_cast<long long>(ny) * c); };
+    auto vid = [&](int a, int b, int c, int lv) {
+        return static_cast<int>((a + kOff[lv][0]) + nvx * ((b + kOff[lv][1]) + nvy * (c + kOff[lv][2])));
+    };
+    // Cell of hex (a,b,c) containing the lattice vertices `g` (global ids).
+    auto cellOf = [&](int a, int b, int c, const int* g, int n) {
+        const long long hx = hexId(a, b, c);
+        if (family[hx] < 0) return base[hx];
+        const int first = kPlane[family[hx]][0], second = kPlane[family[hx]][1];
+        bool inA = true;
+        for (int t = 0; t < n; ++t) {
+            const long long v = g[t];
+            const int d[3] = {static_cast<int>(v % nvx) - a, static_cast<int>((v / nvx) % nvy) - b, static_cast<int>(v / (nvx * nvy)) - c};
+            if (d[first] < d[second]) { inA = false; break; }
+        }
+        return base[hx] + (inA ? 0 : 1);
+    };
+    m.fpo.push_back(0);
+    auto emit = [&](const int* ring, int n, int owner, int neighbour, int patch) {
+        for (int t = 0; t < n; ++t) m.fpi.push_back(ring[t]);
+        m.fpo.push_back(static_cast<int>(m.fpi.size()));
+        m.own.push_back(owner); m.nei.push_back(neighbour); m.patch.push_back(patch);
+    };
+    for (int c = 0; c < nz; ++c)
+        for (int b = 0; b < ny; ++b)
+            for (int a = 0; a < nx; ++a) {
+                const long long hx = hexId(a, b, c);
+                if (family[hx] >= 0) {
+                    int ring[4];
+                    for (int t = 0; t < 4; ++t) ring[t] = vid(a, b, c, kInterior[family[hx]][t]);
+                    emit(ring, 4, base[hx], base[hx] + 1, -1);
+                }
+                for (int lf = 0; lf < 6; ++lf) {
+                    const int dir = lf / 2;
+                    const bool plus = lf % 2 == 1;
+                    int nb[3] = {a, b, c};
+                    nb[dir] += plus ? 1 : -1;
+                    const int cnt[3] = {nx, ny, nz};
+                    const bool boundary = nb[dir] < 0 || nb[dir] >= cnt[dir];
+                    if (!boundary && !plus) continue;  // the lower hex emits shared faces
+                    bool cut = family[hx] == dir;
+                    if (!boundary && family[hexId(nb[0], nb[1], nb[2])] == dir) cut = true;
+                    int ring[4];
+                    for (int t = 0; t < 4; ++t) ring[t] = vid(a, b, c, kFace[lf][t]);
+                    auto place = [&](const int* r, int n) {
+                        const int owner = cellOf(a, b, c, r, n);
+                        if (boundary) emit(r, n, owner, -1, lf);
+                        else emit(r, n, owner, cellOf(nb[0], nb[1], nb[2], r, n), -1);
+                    };
+                    if (!cut) { place(ring, 4); continue; }
+                    // Main diagonal: the ring vertices whose two in-plane offsets agree.
+                    int p0 = -1;
+                    for (int t = 0; t < 4; ++t) {
+                        const int lv = kFace[lf][t];
+                        const int u = kOff[lv][(dir + 1) % 3], w = kOff[lv][(dir + 2) % 3];
+                        if (u == 0 && w == 0) { p0 = t; break; }
+                    }
+                    const int t0[3] = {ring[p0], ring[(p0 + 1) % 4], ring[(p0 + 2) % 4]};
+                    const int t1[3] = {ring[(p0 + 2) % 4], ring[(p0 + 3) % 4], ring[p0]};
+                    place(t0, 3);
+                    place(t1, 3);
+                }
+            }
+    (void)kLocal;
+    return m;
+}
+
+MeshDesc prepare(const MeshDesc& in, bool renumber, std::vector<int>& cellNewToOld, std::vector<int>& faceNewToOld) {
+    const int nc = in.ncells();
+    const int nf = in.nfaces();
+    std::vector<int> oldToNew(nc);
+    cellNewToOld.resize(nc);
+    if (renumber) {
+        // Cell graph (unique neighbours).
+        std::vector<int> deg(nc, 0);
+        for (int f = 0; f < nf; ++f) if (in.nei[f] >= 0) { ++deg[in.own[f]]; ++deg[in.nei[f]]; }
+        std::vector<int> ptr(nc + 1, 0);
+        for (int i = 0; i < nc; ++i) ptr[i + 1] = ptr[i] + deg[i];
+        std::vector<int> adj(ptr[nc]), fill(ptr.begin(), ptr.end() - 1);
+        for (int f = 0; f < nf; ++f)
+            if (in.nei[f] >= 0) { adj[fill[in.own[f]]++] = in.nei[f]; adj[fill[in.nei[f]]++] = in.own[f]; }
+        for (int i = 0; i < nc; ++i) {
+            std::sort(adj.begin() + ptr[i], adj.begin() + ptr[i + 1]);
+            deg[i] = static_cast<int>(std::unique(adj.begin() + ptr[i], adj.begin() + ptr[i + 1]) - (adj.begin() + ptr[i]));
+        }
+        std::vector<int> level(nc, -1), order;
+        order.reserve(nc);
+        auto bfs = [&](int root, std::vector<int>& lv) {  // returns last-level node of minimum degree
+            std::vector<int> q{root};
+            lv.assign(nc, -1);
+            lv[root] = 0;
+            for (std::size_t h = 0; h < q.size(); ++h) {
+                const int u = q[h];
+                for (int k = ptr[u]; k < ptr[u] + deg[u]; ++k)
+                    if (lv[adj[k]] < 0) { lv[adj[k]] = lv[u] + 1; q.push_back(adj[k]); }
+            }
+            int far = q.back();
+            for (int u : q) if (lv[u] == lv[far] && (deg[u] < deg[far] || (deg[u] == deg[far] && u < far))) far = u;
+            return std::make_pair(far, lv[q.back()]);
+        };
+        std::vector<char> seen(nc, 0);
+        std::vector<int> lv;
+        for (int start = 0; start < nc; ++start) {
+            if (seen[start]) continue;
+            // Pseudo-peripheral root of this component (George-Liu).
+            int root = start;
+            auto r = bfs(root, lv);
+            for (int it = 0; it < 4; ++it) {
+                auto r2 = bfs(r.first, lv);
+                if (r2.second <= r.second) break;
+                root = r.first;
+                r = r2;
+            }
+            root = r.first;
+            std::size_t head = order.size();
+            order.push_back(root);
+            seen[root] = 1;
+            std::vector<int> nb;
+            while (head < order.size()) {
+                const int u = order[head++];
+                nb.clear();
+                for (int k = ptr[u]; k < ptr[u] + deg[u]; ++k) if (!seen[adj[k]]) { nb.push_back(adj[k]); seen[adj[k]] = 1; }
+                std::sort(nb.begin(), nb.end(), [&](int x, int y) { return deg[x] != deg[y] ? deg[x] < deg[y] : x < y; });
+                order.insert(order.end(), nb.begin(), nb.end());
+            }
+        }
+        std::reverse(order.begin(), order.end());
+        for (int i = 0; i < nc; ++i) { cellNewToOld[i] = order[i]; oldToNew[order[i]] = i; }
+    } else {
+        std::iota(cellNewToOld.begin(), cellNewToOld.end(), 0);
+        std::iota(oldToNew.begin(), oldToNew.end(), 0);
+    }
+    std::vector<long long> key(nf);
+    std::vector<int> idx(nf);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int f = 0; f < nf; ++f) {
+        const int o = oldToNew[in.own[f]];
+        if (in.nei[f] >= 0) {
+            const int n = oldToNew[in.nei[f]];
+            key[f] = static_cast<long long>(std::min(o, n)) * (static_cast<long long>(nc) + 1) + std::max(o, n);
+        } else {
+            key[f] = static_cast<long long>(nc + 1) * (nc + 1) + static_cast<long long>(in.patch[f]) * (nc + 1) + o;
+        }
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return key[x] < key[y]; });
+    MeshDesc out;
+    out.vert = in.vert;
+    out.npatch = in.npatch;
+    out.fpo.push_back(0);
+    faceNewToOld = idx;
+    for (int nfc = 0; nfc < nf; ++nfc) {
+        const int f = idx[nfc];
+        for (int k = in.fpo[f]; k < in.fpo[f + 1]; ++k) out.fpi.push_back(in.fpi[k]);
+        out.fpo.push_back(static_cast<int>(out.fpi.size()));
+        out.own.push_back(oldToNew[in.own[f]]);
+        out.nei.push_back(in.nei[f] >= 0 ? oldToNew[in.nei[f]] : -1);
+        out.patch.push_back(in.patch[f]);
+    }
+    return out;
+}
+
+}  // namespace bsim

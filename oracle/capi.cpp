@@ -1,0 +1,319 @@
+// Oracle C ABI wrappers (test infrastructure only).
+#include "oracle.h"
+
+#include "simple.hpp"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+using namespace orc;
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F> int guard(F&& f) {
+    try { f(); return 0; }
+    catch (const std::exception& e) { g_error = e.what(); return 1; }
+    catch (...) { g_error = "unknown error"; return 1; }
+}
+
+SolverConfig toConfig(const orc_solver_config* c) {
+    SolverConfig s;
+    s.solver = c->solver; s.pc = c->pc; s.atol = c->atol; s.rtol = c->rtol; s.maxIter = c->max_iter;
+    s.warmStart = c->warm_start; s.rebuildInterval = c->rebuild_interval; s.reuseGrowth = c->reuse_growth;
+    s.amgMaxLevels = c->amg_max_levels; s.amgCoarseSize = c->amg_coarse_size;
+    s.amgPreSweeps = c->amg_pre_sweeps; s.amgPostSweeps = c->amg_post_sweeps;
+    s.amgOmega = c->amg_omega; s.amgStrength = c->amg_strength; s.amgCorrection = c->amg_correction;
+    return s;
+}
+
+template <class T> Field<T> makeField(const Mesh& m, const double* x, const int* bct, const double* bcv,
+                                      const double* ctx) {
+    Field<T> f(m);
+    constexpr int k = ncomp<T>();
+    for (Index i = 0; i < m.nc; ++i) for (int c = 0; c < k; ++c) setComp(f.v[i], c, x[i * k + c]);
+    for (Index p = 0; p < m.np; ++p) {
+        f.bc[p].type = bct[p];
+        for (int c = 0; c < k; ++c) setComp(f.bc[p].value, c, bcv[p * k + c]);
+    }
+    if (ctx) f.fluxCtx.assign(ctx, ctx + m.nf);
+    return f;
+}
+
+template <class T> void store(const std::vector<T>& v, double* out) {
+    constexpr int k = ncomp<T>();
+    for (std::size_t i = 0; i < v.size(); ++i) for (int c = 0; c < k; ++c) out[i * k + c] = comp(v[i], c);
+}
+void storeG(const std::vector<V3>& v, double* out) { store(v, out); }
+void storeG(const std::vector<T3>& v, double* out) {
+    for (std::size_t i = 0; i < v.size(); ++i)
+        for (int c = 0; c < 3; ++c) for (int d = 0; d < 3; ++d) out[i * 9 + c * 3 + d] = v[i].r[c][d];
+}
+
+template <class T> void storeLdu(const Mesh& m, const Ldu<T>& q, double* diag, double* upper, double* lower,
+                                 double* source) {
+    constexpr int k = ncomp<T>();
+    for (Index i = 0; i < m.nc; ++i) {
+        diag[i] += q.diag[i];
+        for (int c = 0; c < k; ++c) source[i * k + c] += comp(q.source[i], c);
+    }
+    for (Index f = 0; f < m.nf; ++f) { upper[f] += q.upper[f]; lower[f] += q.lower[f]; }
+}
+
+Csr makeCsr(int n, const int* ptr, const int* col, const double* diag, const double* val) {
+    Csr A;
+    A.n = n;
+    A.ptr.assign(ptr, ptr + n + 1);
+    A.col.assign(col, col + ptr[n]);
+    A.diag.assign(diag, diag + n);
+    A.val.assign(val, val + ptr[n]);
+    return A;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_error.c_str(); }
+
+void* orc_mesh_new(int nv, const double* vertices, int nf, const int* fpo, const int* fpi, const int* owner,
+                   const int* neighbour, const int* patch, int npatch) {
+    Mesh* out = nullptr;
+    guard([&] {
+        std::vector<V3> vx(nv);
+        for (int i = 0; i < nv; ++i) vx[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        std::vector<Index> o(fpo, fpo + nf + 1), r(fpi, fpi + fpo[nf]);
+        out = new Mesh(buildMesh(vx, o, r, std::vector<Index>(owner, owner + nf),
+                                 std::vector<Index>(neighbour, neighbour + nf), std::vector<Index>(patch, patch + nf),
+                                 npatch));
+    });
+    return out;
+}
+void orc_mesh_free(void* mesh) { delete static_cast<Mesh*>(mesh); }
+void orc_mesh_sizes(void* mesh, int* nc, int* nf, int* orthogonal, int* nfp) {
+    const Mesh& m = *static_cast<Mesh*>(mesh);
+    *nc = m.nc; *nf = m.nf; *orthogonal = m.orthogonal ? 1 : 0; *nfp = static_cast<int>(m.fpi.size());
+}
+void orc_mesh_cells(void* mesh, double* centres, double* volumes) {
+    const Mesh& m = *static_cast<Mesh*>(mesh);
+    store(m.C, centres);
+    std::memcpy(volumes, m.V.data(), sizeof(double) * m.nc);
+}
+void orc_mesh_faces(void* mesh, double* cf, double* sf, double* af, double* nrm, double* w, double* orth,
+                    double* kf, double* skew) {
+    const Mesh& m = *static_cast<Mesh*>(mesh);
+    store(m.Cf, cf); store(m.Sf, sf); store(m.nrm, nrm); store(m.kf, kf); store(m.skew, skew);
+    std::memcpy(af, m.Af.data(), sizeof(double) * m.nf);
+    std::memcpy(w, m.w.data(), sizeof(double) * m.nf);
+    std::memcpy(orth, m.orth.data(), sizeof(double) * m.nf);
+}
+void orc_mesh_rings(void* mesh, int* fpo, int* fpi) {
+    const Mesh& m = *static_cast<Mesh*>(mesh);
+    std::memcpy(fpo, m.fpo.data(), sizeof(int) * m.fpo.size());
+    std::memcpy(fpi, m.fpi.data(), sizeof(int) * m.fpi.size());
+}
+
+int orc_gradient(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv, const double* ctx,
+                 int method, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        if (ncomp == 1) { auto f = makeField<double>(m, x, bct, bcv, ctx); std::vector<V3> g; gradient(f, g, method); storeG(g, out); }
+        else { auto f = makeField<V3>(m, x, bct, bcv, ctx); std::vector<T3> g; gradient(f, g, method); storeG(g, out); }
+    });
+}
+int orc_gradient_traces(void* mesh, const double* x, const int* bct, const double* bcv, const double* grad,
+                        double* traces) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        auto f = makeField<double>(m, x, bct, bcv, nullptr);
+        std::vector<V3> g(m.nc), t;
+        for (Index i = 0; i < m.nc; ++i) g[i] = {grad[3 * i], grad[3 * i + 1], grad[3 * i + 2]};
+        gradientTraces(f, g, t);
+        store(t, traces);
+    });
+}
+int orc_interpolate(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv, const double* ctx,
+                    int method, int gmethod, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        if (ncomp == 1) { auto f = makeField<double>(m, x, bct, bcv, ctx); std::vector<double> o; interpolate(f, o, method, gmethod); store(o, out); }
+        else { auto f = makeField<V3>(m, x, bct, bcv, ctx); std::vector<V3> o; interpolate(f, o, method, gmethod); store(o, out); }
+    });
+}
+int orc_flux(void* mesh, const double* U, const int* bct, const double* bcv, const double* ctx, int method,
+             int gmethod, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        auto f = makeField<V3>(m, U, bct, bcv, ctx);
+        std::vector<double> o;
+        flux(f, o, method, gmethod);
+        store(o, out);
+    });
+}
+int orc_normal_gradient(void* mesh, const double* x, const int* bct, const double* bcv, int gmethod, int dmethod,
+                        double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        auto f = makeField<double>(m, x, bct, bcv, nullptr);
+        std::vector<double> o;
+        normalGradient(f, nullptr, gmethod, dmethod, o);
+        store(o, out);
+    });
+}
+int orc_divergence_face(void* mesh, const double* phi, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        std::vector<double> o;
+        divergenceFace(m, std::vector<double>(phi, phi + m.nf), o);
+        store(o, out);
+    });
+}
+int orc_explicit_laplacian(void* mesh, const double* x, const int* bct, const double* bcv, const double* kf,
+                           double k, int gmethod, int dmethod, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        auto f = makeField<double>(m, x, bct, bcv, nullptr);
+        std::vector<double> kv, o;
+        if (kf) kv.assign(kf, kf + m.nf);
+        explicitLaplacian(f, kf ? &kv : nullptr, k, gmethod, dmethod, o);
+        store(o, out);
+    });
+}
+int orc_explicit_convection(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                            const double* phi, int conv, int interp, int gmethod, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        std::vector<double> ph(phi, phi + m.nf);
+        if (ncomp == 1) { auto f = makeField<double>(m, x, bct, bcv, phi); std::vector<double> o; explicitConvection(ph, f, o, conv, interp, gmethod); store(o, out); }
+        else { auto f = makeField<V3>(m, x, bct, bcv, phi); std::vector<V3> o; explicitConvection(ph, f, o, conv, interp, gmethod); store(o, out); }
+    });
+}
+int orc_assemble_convection(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                            const double* phi, int conv, int interp, int gmethod, double scale, double* diag,
+                            double* upper, double* lower, double* source) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        std::vector<double> ph(phi, phi + m.nf);
+        if (ncomp == 1) {
+            auto f = makeField<double>(m, x, bct, bcv, phi); Ldu<double> q; q.init(m);
+            addConvection(q, ph, f, conv, interp, gmethod, scale); storeLdu(m, q, diag, upper, lower, source);
+        } else {
+            auto f = makeField<V3>(m, x, bct, bcv, phi); Ldu<V3> q; q.init(m);
+            addConvection(q, ph, f, conv, interp, gmethod, scale); storeLdu(m, q, diag, upper, lower, source);
+        }
+    });
+}
+int orc_assemble_diffusion(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                           const double* ctx, const double* kf, int gmethod, int dmethod, double* diag,
+                           double* upper, double* lower, double* source) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        std::vector<double> k(kf, kf + m.nf);
+        if (ncomp == 1) {
+            auto f = makeField<double>(m, x, bct, bcv, ctx); Ldu<double> q; q.init(m);
+            addDiffusion<double>(q, k, f, gmethod, dmethod, nullptr); storeLdu(m, q, diag, upper, lower, source);
+        } else {
+            auto f = makeField<V3>(m, x, bct, bcv, ctx); Ldu<V3> q; q.init(m);
+            addDiffusion<V3>(q, k, f, gmethod, dmethod, nullptr); storeLdu(m, q, diag, upper, lower, source);
+        }
+    });
+}
+
+int orc_csr_solve(int n, const int* ptr, const int* col, const double* diag, const double* val, const double* b,
+                  double* x, const orc_solver_config* cfg, orc_solve_result* res, double* hist, int cap,
+                  int* hist_len) {
+    return guard([&] {
+        LinearSystem sys;
+        sys.A = makeCsr(n, ptr, col, diag, val);
+        sys.update(sys.A.diag, sys.A.val);
+        sys.prepare(toConfig(cfg));
+        std::vector<double> bv(b, b + n), xv(x, x + n);
+        const SolveResult r = sys.solve(bv, xv, 0);
+        std::memcpy(x, xv.data(), sizeof(double) * n);
+        res->status = r.status; res->iterations = r.iterations; res->initial = r.initial;
+        res->final_residual = r.final_; res->relative = r.relative;
+        const int len = static_cast<int>(sys.hist.size());
+        if (hist_len) *hist_len = len;
+        if (hist) std::memcpy(hist, sys.hist.data(), sizeof(double) * std::min(len, cap));
+    });
+}
+
+int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,
+                      const orc_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1) {
+    return guard([&] {
+        Amg amg;
+        amg.setup(makeCsr(n, ptr, col, diag, val), toConfig(cfg));
+        *nlevels = static_cast<int>(amg.levels.size());
+        for (std::size_t l = 0; l < amg.levels.size(); ++l) sizes[l] = amg.levels[l].A.n;
+        if (agg0 && !amg.levels[0].agg.empty()) std::memcpy(agg0, amg.levels[0].agg.data(), sizeof(int) * n);
+        if (coarse_diag1 && amg.levels.size() > 1)
+            std::memcpy(coarse_diag1, amg.levels[1].A.diag.data(), sizeof(double) * amg.levels[1].A.n);
+    });
+}
+
+void* orc_simple_new(void* mesh, const orc_simple_config* cfg, const int* ubct, const double* ubcv,
+                     const int* pbct, const double* pbcv) {
+    Simple* s = nullptr;
+    guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        SimpleConfig c;
+        c.rho = cfg->rho; c.mu = cfg->mu;
+        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
+        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
+        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
+        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
+        c.alphaU = cfg->alpha_u; c.alphaP = cfg->alpha_p; c.nonOrthCorr = cfg->non_orth_corr;
+        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
+        c.mom = toConfig(&cfg->mom); c.pre = toConfig(&cfg->pre);
+        s = new Simple(m, c);
+        std::vector<BC<V3>> ub(m.np);
+        std::vector<BC<double>> pb(m.np);
+        for (Index p = 0; p < m.np; ++p) {
+            ub[p].type = ubct[p]; ub[p].value = {ubcv[3 * p], ubcv[3 * p + 1], ubcv[3 * p + 2]};
+            pb[p].type = pbct[p]; pb[p].value = pbcv[p];
+        }
+        s->setBoundaries(ub, pb);
+    });
+    return s;
+}
+void orc_simple_free(void* s) { delete static_cast<Simple*>(s); }
+int orc_simple_set_fields(void* sp, const double* U, const double* p) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        for (Index i = 0; i < s.m.nc; ++i) { s.U.v[i] = {U[3 * i], U[3 * i + 1], U[3 * i + 2]}; s.p.v[i] = p[i]; }
+    });
+}
+int orc_simple_start(void* sp) { return guard([&] { static_cast<Simple*>(sp)->start(); }); }
+int orc_simple_iterate(void* sp, int n, orc_iter_report* reports) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        for (int k = 0; k < n; ++k) {
+            const IterReport r = s.iterate();
+            orc_iter_report& o = reports[k];
+            o.rU = r.rU; o.dU = r.dU; o.dP = r.dP; o.mass = r.mass; o.linP = r.linP;
+            for (int c = 0; c < 3; ++c) o.mom_iters[c] = r.momIters[c];
+            o.p_iters = r.pIters; o.mom_status = r.momStatus; o.p_status = r.pStatus;
+            o.converged = r.converged; o.referenced = r.referenced;
+        }
+    });
+}
+int orc_simple_get_fields(void* sp, double* U, double* p, double* phi) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        if (U) store(s.U.v, U);
+        if (p) store(s.p.v, p);
+        if (phi) store(s.phi, phi);
+    });
+}
+int orc_simple_history(void* sp, int which, double* out, int cap, int* len) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        const std::vector<double>& h = which < 3 ? s.momHist[which] : s.pHist;
+        *len = static_cast<int>(h.size());
+        std::memcpy(out, h.data(), sizeof(double) * std::min<int>(cap, static_cast<int>(h.size())));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,493 @@
+// Oracle linear algebra: CSR assembly from LDU, the PETSc CG/BiCGSTAB
+// recurrences (dependencies/petsc/src/ksp/ksp/impls/cg/cg.c KSPSolve_CG,
+// bcgs/bcgs.c:14-140), the residual/reuse contract of
+// src/backend/petsc_linear_system.cpp:257-481 and the build's own aggregation
+// AMG.  Test infrastructure only.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace orc {
+
+// Off-diagonal entries of row i: interior faces of cell i sorted by
+// (neighbour, face).  Split sub-faces that couple the same pair stay separate
+// entries with the same column; their products sum like PETSc's COO
+// accumulation (petsc_assembly_plan.cpp:33-73).
+Csr csrPattern(const Mesh& m) {
+    Csr A;
+    A.n = m.nc;
+    A.ptr.assign(m.nc + 1, 0);
+    std::vector<std::vector<std::pair<Index, Index>>> rows(m.nc);
+    for (Index f = 0; f < m.nf; ++f) {
+        if (m.nei[f] == kNone) continue;
+        rows[m.own[f]].push_back({m.nei[f], f});
+        rows[m.nei[f]].push_back({m.own[f], f});
+    }
+    for (Index i = 0; i < m.nc; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        A.ptr[i + 1] = A.ptr[i] + static_cast<Index>(rows[i].size());
+        for (auto& e : rows[i]) A.col.push_back(e.first);
+    }
+    A.diag.assign(m.nc, 0.0);
+    A.val.assign(A.col.size(), 0.0);
+    return A;
+}
+
+void lduToCsr(const Mesh& m, const std::vector<double>& diag, const std::vector<double>& upper,
+              const std::vector<double>& lower, Csr& A) {
+    // Row owner holds upper[f] (column neighbour); row neighbour holds lower[f].
+    A = csrPattern(m);
+    std::vector<std::vector<std::pair<Index, Index>>> rows(m.nc);
+    for (Index f = 0; f < m.nf; ++f) {
+        if (m.nei[f] == kNone) continue;
+        rows[m.own[f]].push_back({m.nei[f], f});
+        rows[m.nei[f]].push_back({m.own[f], f});
+    }
+    for (Index i = 0; i < m.nc; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        for (std::size_t k = 0; k < rows[i].size(); ++k) {
+            const Index f = rows[i][k].second;
+            A.val[A.ptr[i] + k] = m.own[f] == i ? upper[f] : lower[f];
+        }
+        A.diag[i] = diag[i];
+    }
+}
+
+void spmv(const Csr& A, const double* x, double* y) {
+    for (Index i = 0; i < A.n; ++i) {
+        double s = A.diag[i] * x[i];
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k) s += A.val[k] * x[A.col[k]];
+        y[i] = s;
+    }
+}
+
+namespace {
+
+double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (std::size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+double nrm(const std::vector<double>& a) { return std::sqrt(dotv(a, a)); }
+
+// Quantised relative strength: robust to last-bit differences between two
+// assemblies of the same matrix, so both sides build the same aggregates.
+constexpr double kStrengthGrid = 4096.0;
+
+}  // namespace
+
+// Greedy three-pass aggregation on the quantised strength graph.
+void aggregate(const Csr& A, double theta, std::vector<Index>& agg, Index& nc) {
+    const Index n = A.n;
+    const long long thr = static_cast<long long>(std::ceil(theta * kStrengthGrid));
+    std::vector<long long> q(A.val.size(), 0);
+    for (Index i = 0; i < n; ++i) {
+        double mx = 0.0;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k) mx = std::max(mx, std::abs(A.val[k]));
+        if (!(mx > 0.0)) continue;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+            q[k] = static_cast<long long>(std::floor(std::abs(A.val[k]) / mx * kStrengthGrid + 0.5));
+    }
+    auto strong = [&](Index k) { return q[k] >= thr && A.col[k] != -1; };
+    agg.assign(n, -1);
+    nc = 0;
+    // Pass 1: roots whose strong neighbourhood is untouched.
+    for (Index i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free = true;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1] && free; ++k)
+            if (strong(k) && agg[A.col[k]] >= 0) free = false;
+        if (!free) continue;
+        agg[i] = nc;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+            if (strong(k)) agg[A.col[k]] = nc;
+        ++nc;
+    }
+    // Pass 2: attach to the strongest pass-1 aggregate (ties: lowest column).
+    const std::vector<Index> first = agg;
+    for (Index i = 0; i < n; ++i) {
+        if (first[i] >= 0) continue;
+        long long best = -1;
+        Index bestCol = -1;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+            if (!strong(k) || first[A.col[k]] < 0) continue;
+            if (q[k] > best || (q[k] == best && A.col[k] < bestCol)) { best = q[k]; bestCol = A.col[k]; }
+        }
+        if (bestCol >= 0) agg[i] = first[bestCol];
+    }
+    // Pass 3: leftovers aggregate with their unassigned strong neighbours.
+    for (Index i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        agg[i] = nc;
+        for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+            if (strong(k) && agg[A.col[k]] < 0) agg[A.col[k]] = nc;
+        ++nc;
+    }
+}
+
+// Galerkin product with piecewise-constant prolongation.  Each coarse value is
+// the sum of its fine contributions in (fine row, diagonal first, entry) order.
+void galerkin(const Csr& A, const std::vector<Index>& agg, Index nc, Csr& Ac) {
+    std::vector<std::vector<Index>> members(nc);
+    for (Index i = 0; i < A.n; ++i) members[agg[i]].push_back(i);
+    Ac.n = nc;
+    Ac.ptr.assign(nc + 1, 0);
+    Ac.col.clear(); Ac.val.clear();
+    Ac.diag.assign(nc, 0.0);
+    for (Index I = 0; I < nc; ++I) {
+        std::vector<Index> cols;
+        for (Index i : members[I])
+            for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+                if (agg[A.col[k]] != I) cols.push_back(agg[A.col[k]]);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        std::vector<double> vals(cols.size(), 0.0);
+        double d = 0.0;
+        for (Index i : members[I]) {
+            d += A.diag[i];
+            for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+                const Index J = agg[A.col[k]];
+                if (J == I) d += A.val[k];
+                else vals[std::lower_bound(cols.begin(), cols.end(), J) - cols.begin()] += A.val[k];
+            }
+        }
+        Ac.diag[I] = d;
+        for (std::size_t e = 0; e < cols.size(); ++e) { Ac.col.push_back(cols[e]); Ac.val.push_back(vals[e]); }
+        Ac.ptr[I + 1] = static_cast<Index>(Ac.col.size());
+    }
+}
+
+void Amg::setup(const Csr& A, const SolverConfig& c) {
+    cfg = c;
+    levels.clear();
+    levels.push_back(AmgLevel{A, {}, 0});
+    while (static_cast<int>(levels.size()) < cfg.amgMaxLevels && levels.back().A.n > cfg.amgCoarseSize) {
+        AmgLevel& L = levels.back();
+        Index nc = 0;
+        aggregate(L.A, cfg.amgStrength, L.agg, nc);
+        if (nc == 0 || nc >= L.A.n) { L.agg.clear(); break; }
+        L.nc = nc;
+        Csr Ac;
+        galerkin(L.A, L.agg, nc, Ac);
+        levels.push_back(AmgLevel{std::move(Ac), {}, 0});
+    }
+    const Csr& Cc = levels.back().A;
+    const Index n = Cc.n;
+    if (n > 4096) throw std::runtime_error("AMG coarsest level is too large for the dense solve");
+    coarseLU.assign(static_cast<std::size_t>(n) * n, 0.0);
+    for (Index i = 0; i < n; ++i) {
+        coarseLU[static_cast<std::size_t>(i) * n + i] = Cc.diag[i];
+        for (Index k = Cc.ptr[i]; k < Cc.ptr[i + 1]; ++k) coarseLU[static_cast<std::size_t>(i) * n + Cc.col[k]] += Cc.val[k];
+    }
+    // LU with partial pivoting, row-major, in place.
+    coarsePiv.resize(n);
+    for (Index k = 0; k < n; ++k) {
+        Index p = k;
+        for (Index i = k + 1; i < n; ++i)
+            if (std::abs(coarseLU[static_cast<std::size_t>(i) * n + k]) > std::abs(coarseLU[static_cast<std::size_t>(p) * n + k])) p = i;
+        coarsePiv[k] = p;
+        if (p != k)
+            for (Index j = 0; j < n; ++j) std::swap(coarseLU[static_cast<std::size_t>(k) * n + j], coarseLU[static_cast<std::size_t>(p) * n + j]);
+        const double piv = coarseLU[static_cast<std::size_t>(k) * n + k];
+        if (!(std::abs(piv) > 0.0)) throw std::runtime_error("AMG coarsest matrix is singular");
+        for (Index i = k + 1; i < n; ++i) {
+            const double l = coarseLU[static_cast<std::size_t>(i) * n + k] / piv;
+            coarseLU[static_cast<std::size_t>(i) * n + k] = l;
+            for (Index j = k + 1; j < n; ++j) coarseLU[static_cast<std::size_t>(i) * n + j] -= l * coarseLU[static_cast<std::size_t>(k) * n + j];
+        }
+    }
+    wx.assign(levels.size(), {}); wb.assign(levels.size(), {}); wr.assign(levels.size(), {});
+    for (std::size_t l = 0; l < levels.size(); ++l) {
+        wx[l].assign(levels[l].A.n, 0.0); wb[l].assign(levels[l].A.n, 0.0); wr[l].assign(levels[l].A.n, 0.0);
+    }
+}
+
+void Amg::vcycle(int l) {
+    const Csr& A = levels[l].A;
+    std::vector<double>& x = wx[l];
+    std::vector<double>& b = wb[l];
+    std::vector<double>& r = wr[l];
+    const Index n = A.n;
+    if (l == static_cast<int>(levels.size()) - 1) {  // dense LU solve
+        for (Index i = 0; i < n; ++i) x[i] = b[i];
+        for (Index k = 0; k < n; ++k) std::swap(x[k], x[coarsePiv[k]]);
+        for (Index i = 0; i < n; ++i)
+            for (Index j = 0; j < i; ++j) x[i] -= coarseLU[static_cast<std::size_t>(i) * n + j] * x[j];
+        for (Index i = n - 1; i >= 0; --i) {
+            for (Index j = i + 1; j < n; ++j) x[i] -= coarseLU[static_cast<std::size_t>(i) * n + j] * x[j];
+            x[i] /= coarseLU[static_cast<std::size_t>(i) * n + i];
+        }
+        return;
+    }
+    const double w = cfg.amgOmega;
+    auto sweep = [&]() {  // x <- x + w (b - A x) / d
+        spmv(A, x.data(), r.data());
+        for (Index i = 0; i < n; ++i) r[i] = x[i] + w * (b[i] - r[i]) / A.diag[i];
+        x.swap(r);
+    };
+    for (Index i = 0; i < n; ++i) x[i] = w * b[i] / A.diag[i];
+    for (int s = 1; s < cfg.amgPreSweeps; ++s) sweep();
+    spmv(A, x.data(), r.data());
+    for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    std::vector<double>& bc = wb[l + 1];
+    std::fill(bc.begin(), bc.end(), 0.0);
+    for (Index i = 0; i < n; ++i) bc[levels[l].agg[i]] += r[i];
+    vcycle(l + 1);
+    const std::vector<double>& xc = wx[l + 1];
+    for (Index i = 0; i < n; ++i) x[i] += cfg.amgCorrection * xc[levels[l].agg[i]];
+    for (int s = 0; s < cfg.amgPostSweeps; ++s) sweep();
+}
+
+void Amg::apply(const double* r, double* z) {
+    std::copy(r, r + levels[0].A.n, wb[0].begin());
+    vcycle(0);
+    std::copy(wx[0].begin(), wx[0].end(), z);
+}
+
+LinearSystem::~LinearSystem() { delete amg; }
+
+void LinearSystem::update(const std::vector<double>& diag, const std::vector<double>& val) {
+    // petsc_linear_system.cpp:257-282: exact comparison decides a value update.
+    const bool same = ready && diag == prevDiag && val == prevVal;
+    if (!same) {
+        prevDiag = diag; prevVal = val;
+        A.diag = diag; A.val = val;
+        ready = true; setupPending = true; ++matrixUpdates;
+    } else {
+        ++rhsOnly;
+    }
+}
+
+static bool sameConfig(const SolverConfig& a, const SolverConfig& b) {
+    return a.solver == b.solver && a.pc == b.pc && a.atol == b.atol && a.rtol == b.rtol && a.maxIter == b.maxIter &&
+           a.warmStart == b.warmStart && a.rebuildInterval == b.rebuildInterval && a.reuseGrowth == b.reuseGrowth &&
+           a.amgMaxLevels == b.amgMaxLevels && a.amgCoarseSize == b.amgCoarseSize &&
+           a.amgPreSweeps == b.amgPreSweeps && a.amgPostSweeps == b.amgPostSweeps && a.amgOmega == b.amgOmega &&
+           a.amgStrength == b.amgStrength && a.amgCorrection == b.amgCorrection;
+}
+
+// petsc_linear_system.cpp:284-301
+void LinearSystem::prepare(const SolverConfig& c) {
+    if (!configured || !sameConfig(c, cfg)) {
+        cfg = c; configured = true; setupPending = true; forceRebuild = true;
+    }
+    if (!setupPending && !forceRebuild) return;
+    const bool reuse = cfg.pc != 0 && pcSetups != 0 && !forceRebuild &&
+                       matrixUpdates - pcMatrixUpdate < static_cast<std::uint64_t>(cfg.rebuildInterval) &&
+                       pcRhsSolves < static_cast<std::uint64_t>(cfg.rebuildInterval);
+    setup(reuse);
+}
+
+// petsc_linear_system.cpp:303-324; the PC keeps the matrix it was built from.
+static std::vector<double> jacobiInv;
+void LinearSystem::setup(bool reuse) {
+    if (!reuse) {
+        if (cfg.pc == 2) {
+            if (!amg) amg = new Amg();
+            amg->setup(A, cfg);
+        } else if (cfg.pc == 1) {
+            if (!amg) amg = new Amg();
+            amg->levels.assign(1, AmgLevel{A, {}, 0});
+        }
+        if (cfg.pc != 0) ++pcSetups;
+        pcMatrixUpdate = matrixUpdates;
+        pcRhsSolves = 0;
+        freshIterations[0] = freshIterations[1] = freshIterations[2] = 0;
+    }
+    setupPending = false;
+    forceRebuild = false;
+}
+
+namespace {
+
+struct Ksp {
+    const Csr& A;
+    const SolverConfig& cfg;
+    Amg* pc;
+    double target;
+    std::vector<double>* hist;
+    int its = 0;
+    int reason = 0;  // >0 converged, <0 diverged (-3 = max iterations)
+    void applyPC(const std::vector<double>& r, std::vector<double>& z) {
+        const Index n = A.n;
+        if (cfg.pc == 0) { z = r; return; }
+        if (cfg.pc == 1) {
+            const Csr& P = pc->levels[0].A;
+            for (Index i = 0; i < n; ++i) z[i] = r[i] * (1.0 / P.diag[i]);
+            return;
+        }
+        pc->apply(r.data(), z.data());
+    }
+    bool test(double dp) {  // petsc_linear_system.cpp:164-175
+        if (!std::isfinite(dp)) { reason = -9; return true; }
+        if (dp <= target) { reason = 2; return true; }
+        return false;
+    }
+    void cg(const std::vector<double>& b, std::vector<double>& x, bool guessZero, int maxIt) {
+        const Index n = A.n;
+        std::vector<double> r(n), z(n), p(n), w(n);
+        if (!guessZero) { spmv(A, x.data(), r.data()); for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i]; }
+        else r = b;
+        double dp = nrm(r);
+        its = 0;
+        if (hist) hist->push_back(dp);
+        if (test(dp)) return;
+        applyPC(r, z);
+        double beta = dotv(z, r), betaold = 1.0, dpi = 0.0, dpiold;
+        if (!std::isfinite(beta)) { reason = -9; return; }
+        int i = 0;
+        do {
+            its = i + 1;
+            if (beta == 0.0) { reason = 3; break; }
+            if (i > 0 && beta * betaold < 0.0) { reason = -8; break; }
+            if (i == 0) p = z;
+            else { const double bb = beta / betaold; for (Index k = 0; k < n; ++k) p[k] = z[k] + bb * p[k]; }
+            dpiold = dpi;
+            spmv(A, p.data(), w.data());
+            dpi = dotv(p, w);
+            if (!std::isfinite(dpi)) { reason = -9; break; }
+            betaold = beta;
+            if (dpi == 0.0 || (i > 0 && ((dpi > 0) - (dpi < 0)) * ((dpiold > 0) - (dpiold < 0)) < 0)) { reason = -10; break; }
+            const double a = beta / dpi;
+            for (Index k = 0; k < n; ++k) x[k] += a * p[k];
+            for (Index k = 0; k < n; ++k) r[k] += -a * w[k];
+            dp = nrm(r);
+            if (hist) hist->push_back(dp);
+            if (test(dp)) break;
+            applyPC(r, z);
+            beta = dotv(z, r);
+            if (!std::isfinite(beta)) { reason = -9; break; }
+            ++i;
+        } while (i < maxIt);
+        if (i >= maxIt) reason = -3;
+    }
+    void bcgs(const std::vector<double>& b, std::vector<double>& x, bool guessZero, int maxIt) {
+        const Index n = A.n;
+        std::vector<double> r(n), rp(n), v(n, 0.0), t(n), s(n), p(n, 0.0), tmp(n), guess;
+        if (!guessZero) {
+            spmv(A, x.data(), r.data());
+            for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+            guess = x;
+            std::fill(x.begin(), x.end(), 0.0);
+        } else {
+            r = b;
+        }
+        double dp = nrm(r);
+        its = 0;
+        if (hist) hist->push_back(dp);
+        auto finish = [&]() {
+            applyPC(x, tmp);  // KSPUnwindPreconditioner for right preconditioning
+            x = tmp;
+            if (!guess.empty()) for (Index i = 0; i < n; ++i) x[i] += guess[i];
+        };
+        if (test(dp)) {
+            if (!guess.empty()) for (Index i = 0; i < n; ++i) x[i] += guess[i];
+            return;
+        }
+        rp = r;
+        double rhoold = 1.0, alpha = 1.0, omegaold = 1.0, rho, omega, d1, d2;
+        int i = 0;
+        do {
+            rho = dotv(r, rp);
+            const double beta = (rho / rhoold) * (alpha / omegaold);
+            for (Index k = 0; k < n; ++k) p[k] = r[k] + (-omegaold * beta) * v[k] + beta * p[k];
+            applyPC(p, tmp); spmv(A, tmp.data(), v.data());
+            d1 = dotv(v, rp);
+            if (!std::isfinite(d1)) { reason = -9; break; }
+            if (d1 == 0.0) { reason = -5; break; }
+            alpha = rho / d1;
+            for (Index k = 0; k < n; ++k) s[k] = -alpha * v[k] + r[k];
+            applyPC(s, r); spmv(A, r.data(), t.data());
+            d1 = dotv(s, t); d2 = dotv(t, t);
+            if (d2 == 0.0) {
+                d1 = dotv(s, s);
+                if (d1 != 0.0) { reason = -5; break; }
+                for (Index k = 0; k < n; ++k) x[k] += alpha * p[k];
+                ++its;
+                reason = 2;
+                if (hist) hist->push_back(dp);
+                break;
+            }
+            omega = d1 / d2;
+            for (Index k = 0; k < n; ++k) x[k] = alpha * p[k] + omega * s[k] + x[k];
+            for (Index k = 0; k < n; ++k) r[k] = -omega * t[k] + s[k];
+            dp = nrm(r);
+            rhoold = rho; omegaold = omega;
+            ++its;
+            if (hist) hist->push_back(dp);
+            if (test(dp)) break;
+            if (rho == 0.0) { reason = -5; break; }
+            ++i;
+        } while (i < maxIt);
+        if (i >= maxIt) reason = -3;
+        finish();
+    }
+};
+
+}  // namespace
+
+// petsc_linear_system.cpp:355-481 with the contract of solver_control.h:196-210.
+SolveResult LinearSystem::solve(const std::vector<double>& b, std::vector<double>& x, int rhsIndex) {
+    const Index n = A.n;
+    const std::vector<double> guess = x;
+    auto resetGuess = [&]() { if (cfg.warmStart) x = guess; else std::fill(x.begin(), x.end(), 0.0); };
+    resetGuess();
+    hist.clear();
+    const double rhsNorm = nrm(b);
+    auto trueResidual = [&]() {
+        std::vector<double> r(n);
+        spmv(A, x.data(), r.data());
+        for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        return nrm(r);
+    };
+    double initial = rhsNorm;
+    if (cfg.warmStart) initial = trueResidual();
+    const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;
+    const double target = std::max(cfg.atol, cfg.rtol * scale);
+    const bool reused = cfg.pc != 0 && pcMatrixUpdate < matrixUpdates;
+    SolveResult res;
+    if (std::isfinite(initial) && initial <= target) {
+        res.status = Converged; res.iterations = 0; res.initial = initial; res.final_ = initial;
+        res.relative = initial / scale;
+        return res;
+    }
+    int total = 0;
+    bool pcReused = reused;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        int attemptIts = 0, reason = 0;
+        double fin = 0.0, internalTarget = target;
+        bool guessZero = !cfg.warmStart;
+        for (int restart = 0; restart < 3; ++restart) {
+            Ksp k{A, cfg, amg, internalTarget, &hist};
+            if (cfg.solver == 0) k.cg(b, x, guessZero, cfg.maxIter - attemptIts);
+            else k.bcgs(b, x, guessZero, cfg.maxIter - attemptIts);
+            attemptIts += k.its; total += k.its; reason = k.reason;
+            fin = trueResidual();
+            if ((std::isfinite(fin) && fin <= target) || reason <= 0 || !std::isfinite(fin) || attemptIts >= cfg.maxIter) break;
+            internalTarget *= 0.5;
+            guessZero = false;
+        }
+        int status = NumericalFailure;
+        if (std::isfinite(fin) && fin <= target) status = Converged;
+        else if (reason == -3 || attemptIts >= cfg.maxIter) status = MaxIterations;
+        res.status = status; res.iterations = total; res.initial = initial; res.final_ = fin; res.relative = fin / scale;
+        if (status != Converged && attempt == 0 && pcReused) {
+            setup(false);
+            ++retries;
+            resetGuess();
+            pcReused = false;
+            continue;
+        }
+        if (status == Converged) {
+            int& fresh = freshIterations[rhsIndex];
+            if (pcMatrixUpdate == matrixUpdates) fresh = std::max(fresh, attemptIts);
+            else if (attemptIts > cfg.reuseGrowth * static_cast<double>(std::max(fresh, 1))) forceRebuild = true;
+        }
+        break;
+    }
+    if (total > 0) ++pcRhsSolves;
+    return res;
+}
+
+}  // namespace orc

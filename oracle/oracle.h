@@ -1,0 +1,92 @@
+/* Oracle C ABI (test infrastructure only; loaded by tests/ through ctypes).
+ * The struct layouts match include/bsim.h so one set of Python structures
+ * drives both the oracle and the HIP library. */
+#ifndef BSIM_ORACLE_H
+#define BSIM_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int solver, pc;
+    double atol, rtol;
+    int max_iter, warm_start, rebuild_interval;
+    double reuse_growth;
+    int amg_max_levels, amg_coarse_size, amg_pre_sweeps, amg_post_sweeps;
+    double amg_omega, amg_strength, amg_correction;
+    int precision;           /* ignored by the oracle (always FP64) */
+    double inner_rtol;       /* ignored by the oracle */
+} orc_solver_config;
+
+typedef struct {
+    int status, iterations;
+    double initial, final_residual, relative;
+} orc_solve_result;
+
+typedef struct {
+    double rho, mu;
+    int m_interp, m_grad, m_conv, m_diff, m_coef_interp, m_coef_grad;
+    int p_interp, p_grad, p_diff, p_coef_interp, p_coef_grad;
+    double alpha_u, alpha_p;
+    int non_orth_corr;
+    double tol_mass, tol_u, tol_mom, tol_p;
+    orc_solver_config mom, pre;
+} orc_simple_config;
+
+typedef struct {
+    double rU, dU, dP, mass, linP;
+    int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
+} orc_iter_report;
+
+const char* orc_last_error(void);
+void* orc_mesh_new(int nv, const double* vertices, int nf, const int* fpo, const int* fpi, const int* owner,
+                   const int* neighbour, const int* patch, int npatch);
+void orc_mesh_free(void* mesh);
+void orc_mesh_sizes(void* mesh, int* nc, int* nf, int* orthogonal, int* nfp);
+void orc_mesh_cells(void* mesh, double* centres, double* volumes);
+void orc_mesh_faces(void* mesh, double* cf, double* sf, double* af, double* nrm, double* w, double* orth,
+                    double* kf, double* skew);
+void orc_mesh_rings(void* mesh, int* fpo, int* fpi);
+
+int orc_gradient(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv, const double* flux_ctx,
+                 int method, double* out);
+int orc_gradient_traces(void* mesh, const double* x, const int* bct, const double* bcv, const double* grad,
+                        double* traces);
+int orc_interpolate(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                    const double* flux_ctx, int method, int gmethod, double* out);
+int orc_flux(void* mesh, const double* U, const int* bct, const double* bcv, const double* flux_ctx, int method,
+             int gmethod, double* out);
+int orc_normal_gradient(void* mesh, const double* x, const int* bct, const double* bcv, int gmethod, int dmethod,
+                        double* out);
+int orc_divergence_face(void* mesh, const double* phi, double* out);
+int orc_explicit_laplacian(void* mesh, const double* x, const int* bct, const double* bcv, const double* kf,
+                           double k, int gmethod, int dmethod, double* out);
+int orc_explicit_convection(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                            const double* phi, int conv, int interp, int gmethod, double* out);
+int orc_assemble_convection(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                            const double* phi, int conv, int interp, int gmethod, double scale, double* diag,
+                            double* upper, double* lower, double* source);
+int orc_assemble_diffusion(void* mesh, int ncomp, const double* x, const int* bct, const double* bcv,
+                           const double* flux_ctx, const double* kf, int gmethod, int dmethod, double* diag,
+                           double* upper, double* lower, double* source);
+
+int orc_csr_solve(int n, const int* ptr, const int* col, const double* diag, const double* val, const double* b,
+                  double* x, const orc_solver_config* cfg, orc_solve_result* res, double* hist, int hist_cap,
+                  int* hist_len);
+int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,
+                      const orc_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
+
+void* orc_simple_new(void* mesh, const orc_simple_config* cfg, const int* ubct, const double* ubcv,
+                     const int* pbct, const double* pbcv);
+void orc_simple_free(void* s);
+int orc_simple_set_fields(void* s, const double* U, const double* p);
+int orc_simple_start(void* s);
+int orc_simple_iterate(void* s, int n, orc_iter_report* reports);
+int orc_simple_get_fields(void* s, double* U, double* p, double* phi);
+int orc_simple_history(void* s, int which, double* out, int cap, int* len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

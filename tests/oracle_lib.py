@@ -1,0 +1,209 @@
+"""ctypes access to the oracle (oracle/liboracle.so) and to the reference's own
+compiled discretisation layer (oracle/_ref/libbsimref.so).  Test-only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from babelsim_amd._capi import IterReport, SimpleConfig, SolveResult, SolverConfig, ip, ptr
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE = ROOT / "oracle" / "liboracle.so"
+REF = ROOT / "oracle" / "_ref" / "libbsimref.so"
+
+_orc = None
+_ref = None
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+
+
+def _sig(L, prefix):
+    f = lambda n: getattr(L, prefix + n)  # noqa: E731
+    f("mesh_new").restype = vp
+    f("mesh_new").argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, ip, ip, C.c_int]
+    f("mesh_free").argtypes = [vp]
+    f("mesh_free").restype = None
+    f("last_error").restype = C.c_char_p
+    f("mesh_cells").argtypes = [vp, dp, dp]
+    f("mesh_cells").restype = None
+    f("mesh_faces").argtypes = [vp] + [dp] * 8
+    f("mesh_faces").restype = None
+    f("gradient").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, dp]
+    f("interpolate").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, dp]
+    f("flux").argtypes = [vp, dp, ip, dp, dp, C.c_int, C.c_int, dp]
+    f("normal_gradient").argtypes = [vp, dp, ip, dp, C.c_int, C.c_int, dp]
+    f("divergence_face").argtypes = [vp, dp, dp]
+    f("explicit_laplacian").argtypes = [vp, dp, ip, dp, dp, C.c_double, C.c_int, C.c_int, dp]
+    f("explicit_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, dp]
+    f("assemble_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         dp, dp, dp, dp]
+    f("assemble_diffusion").argtypes = [vp, C.c_int, dp, ip, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp]
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        if not ORACLE.exists():
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(str(ORACLE))
+        _sig(L, "orc_")
+        L.orc_mesh_sizes.argtypes = [vp, ip, ip, ip, ip]
+        L.orc_mesh_sizes.restype = None
+        L.orc_mesh_rings.argtypes = [vp, ip, ip]
+        L.orc_mesh_rings.restype = None
+        L.orc_gradient_traces.argtypes = [vp, dp, ip, dp, dp, dp]
+        L.orc_csr_solve.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig),
+                                    C.POINTER(SolveResult), dp, C.c_int, ip]
+        L.orc_amg_hierarchy.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp]
+        L.orc_simple_new.restype = vp
+        L.orc_simple_new.argtypes = [vp, C.POINTER(SimpleConfig), ip, dp, ip, dp]
+        L.orc_simple_free.argtypes = [vp]
+        L.orc_simple_free.restype = None
+        L.orc_simple_set_fields.argtypes = [vp, dp, dp]
+        L.orc_simple_start.argtypes = [vp]
+        L.orc_simple_iterate.argtypes = [vp, C.c_int, C.POINTER(IterReport)]
+        L.orc_simple_get_fields.argtypes = [vp, dp, dp, dp]
+        L.orc_simple_history.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+        _orc = L
+    return _orc
+
+
+def reference():
+    """The reference's operators.cpp + mesh.cpp (None when not built here)."""
+    global _ref
+    if _ref is None:
+        if not REF.exists():
+            return None
+        L = C.CDLL(str(REF))
+        _sig(L, "ref_")
+        L.ref_mesh_sizes.argtypes = [vp, ip, ip, ip]
+        L.ref_mesh_sizes.restype = None
+        _ref = L
+    return _ref
+
+
+class CpuMesh:
+    """A mesh built either by the oracle or by the compiled reference."""
+
+    def __init__(self, L, prefix: str, arrays: dict):
+        self.L, self.prefix = L, prefix
+        self.a = arrays
+        v = np.ascontiguousarray(arrays["vertices"], np.float64)
+        fpo = np.ascontiguousarray(arrays["fpo"], np.int32)
+        fpi = np.ascontiguousarray(arrays["fpi"], np.int32)
+        own = np.ascontiguousarray(arrays["owner"], np.int32)
+        nei = np.ascontiguousarray(arrays["neighbour"], np.int32)
+        pat = np.ascontiguousarray(arrays["patch"], np.int32)
+        self.nf = len(own)
+        self.npatch = int(arrays["npatch"])
+        h = getattr(L, prefix + "mesh_new")(len(v) // 3, ptr(v), self.nf, ptr(fpo, ip), ptr(fpi, ip), ptr(own, ip),
+                                            ptr(nei, ip), ptr(pat, ip), self.npatch)
+        if not h:
+            raise ValueError(getattr(L, prefix + "last_error")().decode())
+        self.h = vp(h)
+        self.nc = int(max(own.max(), nei.max()) + 1)
+        self.owner, self.neighbour, self.patch = own, nei, pat
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.L, self.prefix + "mesh_free")(self.h)
+            self.h = None
+
+    def call(self, name, *args):
+        rc = getattr(self.L, self.prefix + name)(self.h, *args)
+        if rc:
+            raise ValueError(getattr(self.L, self.prefix + "last_error")().decode())
+
+    def geometry(self) -> dict:
+        nc, nf = self.nc, self.nf
+        g = dict(C=np.zeros(3 * nc), V=np.zeros(nc), Cf=np.zeros(3 * nf), Sf=np.zeros(3 * nf), Af=np.zeros(nf),
+                 n=np.zeros(3 * nf), w=np.zeros(nf), orth=np.zeros(nf), k=np.zeros(3 * nf), skew=np.zeros(3 * nf))
+        getattr(self.L, self.prefix + "mesh_cells")(self.h, ptr(g["C"]), ptr(g["V"]))
+        getattr(self.L, self.prefix + "mesh_faces")(self.h, ptr(g["Cf"]), ptr(g["Sf"]), ptr(g["Af"]), ptr(g["n"]),
+                                                    ptr(g["w"]), ptr(g["orth"]), ptr(g["k"]), ptr(g["skew"]))
+        for key in ("C", "Cf", "Sf", "n", "k", "skew"):
+            g[key] = g[key].reshape(-1, 3)
+        return g
+
+    @staticmethod
+    def _bc(bct, bcv, ncomp):
+        return (np.ascontiguousarray(bct, np.int32),
+                np.ascontiguousarray(np.asarray(bcv, np.float64).reshape(-1)))
+
+    def gradient(self, x, bct, bcv, method, ctx=None):
+        ncomp = 1 if x.ndim == 1 else 3
+        t, v = self._bc(bct, bcv, ncomp)
+        out = np.zeros(self.nc * 3 * ncomp)
+        xc = np.ascontiguousarray(x, np.float64)
+        self.call("gradient", ncomp, ptr(xc), ptr(t, ip), ptr(v), ptr(ctx), method, ptr(out))
+        return out.reshape(self.nc, 3) if ncomp == 1 else out.reshape(self.nc, 3, 3)
+
+    def interpolate(self, x, bct, bcv, method, gmethod, ctx=None):
+        ncomp = 1 if x.ndim == 1 else 3
+        t, v = self._bc(bct, bcv, ncomp)
+        out = np.zeros(self.nf * ncomp)
+        xc = np.ascontiguousarray(x, np.float64)
+        self.call("interpolate", ncomp, ptr(xc), ptr(t, ip), ptr(v), ptr(ctx), method, gmethod, ptr(out))
+        return out if ncomp == 1 else out.reshape(self.nf, 3)
+
+    def flux(self, U, bct, bcv, method, gmethod, ctx=None):
+        t, v = self._bc(bct, bcv, 3)
+        out = np.zeros(self.nf)
+        self.call("flux", ptr(np.ascontiguousarray(U, np.float64)), ptr(t, ip), ptr(v), ptr(ctx), method, gmethod,
+                  ptr(out))
+        return out
+
+    def normal_gradient(self, x, bct, bcv, gmethod, dmethod):
+        t, v = self._bc(bct, bcv, 1)
+        out = np.zeros(self.nf)
+        self.call("normal_gradient", ptr(np.ascontiguousarray(x, np.float64)), ptr(t, ip), ptr(v), gmethod,
+                  dmethod, ptr(out))
+        return out
+
+    def divergence_face(self, phi):
+        out = np.zeros(self.nc)
+        self.call("divergence_face", ptr(np.ascontiguousarray(phi, np.float64)), ptr(out))
+        return out
+
+    def explicit_laplacian(self, x, bct, bcv, kf, k, gmethod, dmethod):
+        t, v = self._bc(bct, bcv, 1)
+        out = np.zeros(self.nc)
+        kfa = None if kf is None else np.ascontiguousarray(kf, np.float64)
+        self.call("explicit_laplacian", ptr(np.ascontiguousarray(x, np.float64)), ptr(t, ip), ptr(v), ptr(kfa),
+                  float(k), gmethod, dmethod, ptr(out))
+        return out
+
+    def explicit_convection(self, x, bct, bcv, phi, conv, interp, gmethod):
+        ncomp = 1 if x.ndim == 1 else 3
+        t, v = self._bc(bct, bcv, ncomp)
+        out = np.zeros(self.nc * ncomp)
+        self.call("explicit_convection", ncomp, ptr(np.ascontiguousarray(x, np.float64)), ptr(t, ip), ptr(v),
+                  ptr(np.ascontiguousarray(phi, np.float64)), conv, interp, gmethod, ptr(out))
+        return out if ncomp == 1 else out.reshape(self.nc, 3)
+
+    def assemble(self, kind, x, bct, bcv, phi=None, kf=None, conv=1, interp=0, gmethod=0, dmethod=0, scale=1.0,
+                 ctx=None):
+        ncomp = 1 if x.ndim == 1 else 3
+        t, v = self._bc(bct, bcv, ncomp)
+        diag, up, lo, src = np.zeros(self.nc), np.zeros(self.nf), np.zeros(self.nf), np.zeros(self.nc * ncomp)
+        xc = np.ascontiguousarray(x, np.float64)
+        if kind == "convection":
+            self.call("assemble_convection", ncomp, ptr(xc), ptr(t, ip), ptr(v),
+                      ptr(np.ascontiguousarray(phi, np.float64)), conv, interp, gmethod, float(scale),
+                      ptr(diag), ptr(up), ptr(lo), ptr(src))
+        else:
+            self.call("assemble_diffusion", ncomp, ptr(xc), ptr(t, ip), ptr(v), ptr(ctx),
+                      ptr(np.ascontiguousarray(kf, np.float64)), gmethod, dmethod, ptr(diag), ptr(up), ptr(lo),
+                      ptr(src))
+        return dict(diag=diag, upper=up, lower=lo, source=src if ncomp == 1 else src.reshape(self.nc, 3))
+
+
+def oracle_mesh(arrays: dict) -> CpuMesh:
+    return CpuMesh(oracle(), "orc_", arrays)
+
+
+def reference_mesh(arrays: dict) -> CpuMesh | None:
+    L = reference()
+    return None if L is None else CpuMesh(L, "ref_", arrays)
