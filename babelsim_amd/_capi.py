@@ -34,7 +34,7 @@ class SolverConfig(C.Structure):
 def solver_config(**kw) -> SolverConfig:
     base = dict(solver=1, pc=2, atol=1e-12, rtol=1e-8, max_iter=1000, warm_start=0, rebuild_interval=8,
                 reuse_growth=2.0, amg_max_levels=12, amg_coarse_size=48, amg_pre_sweeps=2, amg_post_sweeps=2,
-                amg_omega=0.8, amg_strength=0.25, amg_correction=1.0, precision=0, inner_rtol=1e-3)
+                amg_omega=0.8, amg_strength=0.5, amg_correction=1.9, precision=0, inner_rtol=1e-3)
     base.update(kw)
     return SolverConfig(**base)
 
@@ -108,14 +108,53 @@ def _declare(L: C.CDLL) -> None:
     L.bsim_mesh_free.argtypes = [C.c_void_p]
     L.bsim_mesh_free.restype = None
     for name, args in _DEVICE_SIGNATURES.items():
-        if hasattr(L, name):
-            fn = getattr(L, name)
-            fn.argtypes = args
-            fn.restype = C.c_int
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _declare_void(L)
 
 
 vp = C.c_void_p
-_DEVICE_SIGNATURES: dict = {}
+u64p = C.POINTER(C.c_uint64)
+_DEVICE_SIGNATURES: dict = {
+    "bsim_ctx_create": [vp, C.c_int, C.c_int, C.POINTER(vp)],
+    "bsim_ctx_sizes": [vp, ip, ip, ip, ip, ip],
+    "bsim_ctx_geometry": [vp] + [dp] * 10,
+    "bsim_op_gradient": [vp, C.c_int, dp, ip, dp, dp, C.c_int, dp],
+    "bsim_op_interpolate": [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, dp],
+    "bsim_op_flux": [vp, dp, ip, dp, dp, C.c_int, C.c_int, dp],
+    "bsim_op_normal_gradient": [vp, dp, ip, dp, C.c_int, C.c_int, dp],
+    "bsim_op_divergence_face": [vp, dp, dp],
+    "bsim_op_explicit_laplacian": [vp, dp, ip, dp, dp, C.c_double, C.c_int, C.c_int, dp],
+    "bsim_op_explicit_convection": [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, dp],
+    "bsim_op_assemble_convection": [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    dp, dp, dp, dp],
+    "bsim_op_assemble_diffusion": [vp, C.c_int, dp, ip, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp],
+    "bsim_system_create": [vp, C.POINTER(vp)],
+    "bsim_system_solve_ldu": [vp, dp, dp, dp, C.c_int, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult)],
+    "bsim_system_history": [vp, dp, C.c_int, ip],
+    "bsim_system_counters": [vp, u64p],
+    "bsim_csr_solve": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult),
+                       dp, C.c_int, ip],
+    "bsim_csr_spmv": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp],
+    "bsim_amg_hierarchy": [C.c_int, C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp],
+    "bsim_simple_create": [vp, C.POINTER(SimpleConfig), ip, dp, ip, dp, C.POINTER(vp)],
+    "bsim_simple_set_fields": [vp, dp, dp],
+    "bsim_simple_start": [vp],
+    "bsim_simple_iterate": [vp, C.c_int, C.POINTER(IterReport)],
+    "bsim_simple_get_fields": [vp, dp, dp, dp],
+    "bsim_simple_history": [vp, C.c_int, dp, C.c_int, ip],
+    "bsim_simple_timing": [vp, dp],
+    "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, dp, ip, ip],
+}
+_VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp]}
+
+
+def _declare_void(L):
+    for name, args in _VOID.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = None
 
 
 class MeshData:

@@ -34,8 +34,9 @@ def _hipcc() -> str:
 def _compile(src: Path) -> Path:
     out = OBJ / (src.name + ".o")
     cmd = [_hipcc(), *COMMON]
-    if src.suffix == ".hip":
-        cmd += ["-x", "hip", *DEVICE]
+    # Host sources include the HIP runtime headers too, so everything is
+    # compiled in HIP mode for gfx950.
+    cmd += ["-x", "hip", *DEVICE]
     cmd += ["-c", str(src), "-o", str(out)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

@@ -1,0 +1,476 @@
+// C ABI of the device library (include/bsim.h): contexts, operators, linear
+// systems and the SIMPLE driver.  Host arrays are in the caller's numbering.
+#include "../../include/bsim.h"
+#include "errors.hpp"
+#include "simple.hpp"
+
+#include <cstring>
+
+struct bsim_mesh {
+    bsim::MeshDesc d;
+};
+struct bsim_ctx {
+    std::unique_ptr<bsim::Context> c;
+};
+struct bsim_system {
+    bsim_ctx* ctx;
+    std::unique_ptr<bsim::DeviceSystem> sys;
+};
+struct bsim_simple {
+    bsim_ctx* ctx;
+    std::unique_ptr<bsim::SimpleSolver> s;
+};
+
+using namespace bsim;
+
+namespace {
+
+struct OpField {
+    DBuf<double> v, bcv, ctx;
+    DBuf<int> bct;
+    FieldArg arg;
+};
+
+OpField makeField(Context& C, int ncomp, const double* x, const int* bct, const double* bcv, const double* ctx) {
+    if (ncomp != 1 && ncomp != 3) throw std::invalid_argument("fields have 1 or 3 components");
+    OpField f;
+    hipStream_t s = C.stream;
+    const int np = C.mesh->np;
+    for (int p = 0; p < np; ++p)
+        if (bct[p] < 0 || bct[p] > 4) throw std::invalid_argument("invalid boundary type");
+    f.v.upload(C.cellsIn(x, ncomp), s);
+    f.bct.upload(bct, np, s);
+    f.bcv.upload(bcv, static_cast<std::size_t>(np) * ncomp, s);
+    if (ctx) f.ctx.upload(C.facesIn(ctx, 1), s);
+    f.arg = FieldArg{f.v.p, f.bct.p, f.bcv.p, ctx ? f.ctx.p : nullptr, nullptr};
+    return f;
+}
+
+void syncAndCheck(Context& C) {
+    BSIM_HIP(hipStreamSynchronize(C.stream));
+    C.checkErrors();
+}
+
+// CSR off-diagonal arrays -> LDU upper/lower in caller face numbering (accumulating).
+void lduOut(Context& C, const std::vector<double>& dg, const std::vector<double>& off, double* diag, double* upper,
+            double* lower) {
+    const DeviceMesh& M = *C.mesh;
+    for (int i = 0; i < M.nc; ++i) diag[M.cellNewToOld[i]] += dg[i];
+    for (int i = 0; i < M.nc; ++i)
+        for (int k = M.hmptr[i]; k < M.hmptr[i + 1]; ++k) {
+            const int f = M.hmface[k];
+            if (M.hown[f] == i) upper[M.faceNewToOld[f]] += off[k];
+            else lower[M.faceNewToOld[f]] += off[k];
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsim_ctx_create(const bsim_mesh* mesh, int renumber, int device, bsim_ctx** out) {
+    return guarded([&] {
+        if (!mesh) throw std::invalid_argument("mesh is null");
+        auto* c = new bsim_ctx;
+        try { c->c = std::make_unique<Context>(mesh->d, renumber != 0, device); }
+        catch (...) { delete c; throw; }
+        *out = c;
+    });
+}
+
+void bsim_ctx_destroy(bsim_ctx* ctx) { delete ctx; }
+
+int bsim_ctx_sizes(bsim_ctx* ctx, int* nc, int* nf, int* nfi, int* nnz, int* orthogonal) {
+    return guarded([&] {
+        const DeviceMesh& M = *ctx->c->mesh;
+        if (nc) *nc = M.nc;
+        if (nf) *nf = M.nf;
+        if (nfi) *nfi = M.nfi;
+        if (nnz) *nnz = M.nnzOff;
+        if (orthogonal) *orthogonal = M.orthogonal ? 1 : 0;
+    });
+}
+
+int bsim_ctx_geometry(bsim_ctx* ctx, double* C_, double* V, double* Cf, double* Sf, double* Af, double* nrm, double* w,
+                      double* orth, double* kf, double* skew) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        hipStream_t s = C.stream;
+        if (C_) C.cellsOut(M.C.toHost(s), C_, 3);
+        if (V) C.cellsOut(M.V.toHost(s), V, 1);
+        if (Cf) C.facesOut(M.Cf.toHost(s), Cf, 3);
+        if (Sf) C.facesOut(M.Sf.toHost(s), Sf, 3);
+        if (Af) C.facesOut(M.Af.toHost(s), Af, 1);
+        if (nrm) C.facesOut(M.nrm.toHost(s), nrm, 3);
+        if (w) C.facesOut(M.w.toHost(s), w, 1);
+        if (orth) C.facesOut(M.orth.toHost(s), orth, 1);
+        if (kf) C.facesOut(M.kf.toHost(s), kf, 3);
+        if (skew) C.facesOut(M.skew.toHost(s), skew, 3);
+    });
+}
+
+int bsim_op_gradient(bsim_ctx* ctx, int ncomp, const double* x, const int* bct, const double* bcv,
+                     const double* flux_ctx, int method, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, ncomp, x, bct, bcv, flux_ctx);
+        DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
+        ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, method, g.p, work.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.cellsOut(g.toHost(C.stream), out, 3 * ncomp);
+    });
+}
+
+int bsim_op_interpolate(bsim_ctx* ctx, int ncomp, const double* x, const int* bct, const double* bcv,
+                        const double* flux_ctx, int method, int gmethod, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, ncomp, x, bct, bcv, flux_ctx);
+        DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
+        DBuf<double> o(static_cast<std::size_t>(ncomp) * M.nf);
+        if (method == kCorrected) ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::interpolate(M.view(), ncomp, f.arg, method, g.p, o.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.facesOut(o.toHost(C.stream), out, ncomp);
+    });
+}
+
+int bsim_op_flux(bsim_ctx* ctx, const double* U, const int* bct, const double* bcv, const double* flux_ctx, int method,
+                 int gmethod, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        if (method != kLinear && method != kCorrected) throw std::invalid_argument("unsupported flux interpolation method");
+        OpField f = makeField(C, 3, U, bct, bcv, flux_ctx);
+        DBuf<double> seed;
+        if (!flux_ctx) {  // operators.cpp:977-987: seed directions from owner velocity
+            seed.alloc(M.nf);
+            ops::seedFluxContext(M.view(), f.v.p, seed.p, C.stream);
+            f.arg.ctx = seed.p;
+        }
+        DBuf<double> g(9 * static_cast<std::size_t>(M.nc)), work(9 * static_cast<std::size_t>(M.nc)), o(M.nf);
+        const double* gp = nullptr;
+        if (method == kCorrected) { ops::gradient(M.view(), M.orthogonal, 3, f.arg, gmethod, g.p, work.p, C.err.p, C.stream); gp = g.p; }
+        ops::flux(M.view(), f.arg, gp, o.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.facesOut(o.toHost(C.stream), out, 1);
+    });
+}
+
+int bsim_op_normal_gradient(bsim_ctx* ctx, const double* x, const int* bct, const double* bcv, int gmethod,
+                            int dmethod, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, 1, x, bct, bcv, nullptr);
+        DBuf<double> g(3 * static_cast<std::size_t>(M.nc)), work(3 * static_cast<std::size_t>(M.nc)), o(M.nf);
+        ops::gradient(M.view(), M.orthogonal, 1, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::normalGradient(M.view(), f.arg, g.p, dmethod, o.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.facesOut(o.toHost(C.stream), out, 1);
+    });
+}
+
+int bsim_op_divergence_face(bsim_ctx* ctx, const double* phi, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        DBuf<double> ph, o(M.nc);
+        ph.upload(C.facesIn(phi, 1), C.stream);
+        ops::divergenceFace(M.view(), ph.p, o.p, C.stream);
+        syncAndCheck(C);
+        C.cellsOut(o.toHost(C.stream), out, 1);
+    });
+}
+
+int bsim_op_explicit_laplacian(bsim_ctx* ctx, const double* x, const int* bct, const double* bcv, const double* kf,
+                               double k, int gmethod, int dmethod, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, 1, x, bct, bcv, nullptr);
+        DBuf<double> g(3 * static_cast<std::size_t>(M.nc)), work(3 * static_cast<std::size_t>(M.nc)), o(M.nc), kd;
+        if (kf) kd.upload(C.facesIn(kf, 1), C.stream);
+        if (dmethod != kOrthogonal) ops::gradient(M.view(), M.orthogonal, 1, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::explicitLaplacian(M.view(), f.arg, kf ? kd.p : nullptr, k, g.p, dmethod, o.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.cellsOut(o.toHost(C.stream), out, 1);
+    });
+}
+
+int bsim_op_explicit_convection(bsim_ctx* ctx, int ncomp, const double* x, const int* bct, const double* bcv,
+                                const double* phi, int conv, int interp, int gmethod, double* out) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, ncomp, x, bct, bcv, phi);
+        DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
+        DBuf<double> o(static_cast<std::size_t>(ncomp) * M.nc);
+        if (conv == kLinearUpwind || (conv == kCentral && interp == kCorrected))
+            ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::explicitConvection(M.view(), ncomp, f.arg, f.ctx.p, conv, interp, g.p, o.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        C.cellsOut(o.toHost(C.stream), out, ncomp);
+    });
+}
+
+int bsim_op_assemble_convection(bsim_ctx* ctx, int ncomp, const double* x, const int* bct, const double* bcv,
+                                const double* phi, int conv, int interp, int gmethod, double scale, double* diag,
+                                double* upper, double* lower, double* source) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, ncomp, x, bct, bcv, phi);
+        DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
+        DBuf<double> dg(M.nc), off(std::max(M.nnzOff, 1)), src(static_cast<std::size_t>(ncomp) * M.nc);
+        dg.zero(C.stream); off.zero(C.stream); src.zero(C.stream);
+        if (conv == kLinearUpwind || (conv == kCentral && interp == kCorrected))
+            ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::assembleConvection(M.view(), ncomp, f.arg, f.ctx.p, conv, interp, g.p, scale, dg.p, off.p, src.p, C.err.p,
+                                C.stream);
+        syncAndCheck(C);
+        lduOut(C, dg.toHost(C.stream), off.toHost(C.stream), diag, upper, lower);
+        std::vector<double> s(static_cast<std::size_t>(ncomp) * M.nc, 0.0);
+        C.cellsOut(src.toHost(C.stream), s.data(), ncomp);
+        for (std::size_t i = 0; i < s.size(); ++i) source[i] += s[i];
+    });
+}
+
+int bsim_op_assemble_diffusion(bsim_ctx* ctx, int ncomp, const double* x, const int* bct, const double* bcv,
+                               const double* flux_ctx, const double* kf, int gmethod, int dmethod, double* diag,
+                               double* upper, double* lower, double* source) {
+    return guarded([&] {
+        Context& C = *ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        OpField f = makeField(C, ncomp, x, bct, bcv, flux_ctx);
+        DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
+        DBuf<double> dg(M.nc), off(std::max(M.nnzOff, 1)), src(static_cast<std::size_t>(ncomp) * M.nc), kd;
+        dg.zero(C.stream); off.zero(C.stream); src.zero(C.stream);
+        kd.upload(C.facesIn(kf, 1), C.stream);
+        if (dmethod != kOrthogonal) ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
+        ops::assembleDiffusion(M.view(), ncomp, f.arg, kd.p, g.p, dmethod, 1.0, dg.p, off.p, src.p, C.err.p, C.stream);
+        syncAndCheck(C);
+        lduOut(C, dg.toHost(C.stream), off.toHost(C.stream), diag, upper, lower);
+        std::vector<double> s(static_cast<std::size_t>(ncomp) * M.nc, 0.0);
+        C.cellsOut(src.toHost(C.stream), s.data(), ncomp);
+        for (std::size_t i = 0; i < s.size(); ++i) source[i] += s[i];
+    });
+}
+
+int bsim_system_create(bsim_ctx* ctx, bsim_system** out) {
+    return guarded([&] {
+        const DeviceMesh& M = *ctx->c->mesh;
+        auto* s = new bsim_system;
+        s->ctx = ctx;
+        s->sys = std::make_unique<DeviceSystem>(ctx->c->stream, M.nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
+        *out = s;
+    });
+}
+
+int bsim_system_solve_ldu(bsim_system* sys, const double* diag, const double* upper, const double* lower, int nrhs,
+                          const double* source, double* x, const bsim_solver_config* cfg, bsim_solve_result* results) {
+    return guarded([&] {
+        if (nrhs != 1 && nrhs != 3) throw std::invalid_argument("linear systems have one or three right-hand sides");
+        Context& C = *sys->ctx->c;
+        const DeviceMesh& M = *C.mesh;
+        hipStream_t s = C.stream;
+        const SolverCfg sc = toSolverCfg(cfg);
+        DeviceSystem& D = *sys->sys;
+        // LDU -> CSR values (petsc_assembly_plan.cpp:62-87, row owner holds upper).
+        std::vector<double> dg = C.cellsIn(diag, 1), off(M.nnzOff);
+        for (int i = 0; i < M.nc; ++i)
+            for (int k = M.hmptr[i]; k < M.hmptr[i + 1]; ++k) {
+                const int f = M.hmface[k];
+                off[k] = M.hown[f] == i ? upper[M.faceNewToOld[f]] : lower[M.faceNewToOld[f]];
+            }
+        BSIM_HIP(hipMemcpyAsync(D.candDiag(), dg.data(), sizeof(double) * M.nc, hipMemcpyHostToDevice, s));
+        if (M.nnzOff) BSIM_HIP(hipMemcpyAsync(D.candVal(), off.data(), sizeof(double) * M.nnzOff, hipMemcpyHostToDevice, s));
+        D.update();
+        D.prepare(sc);
+        const std::vector<double> b = C.cellsIn(source, nrhs), x0 = C.cellsIn(x, nrhs);
+        std::vector<double> bc(M.nc), xc(M.nc), xout(x0.size());
+        DBuf<double> db(M.nc), dx(M.nc);
+        for (int comp = 0; comp < nrhs; ++comp) {
+            for (int i = 0; i < M.nc; ++i) { bc[i] = b[static_cast<std::size_t>(i) * nrhs + comp]; xc[i] = x0[static_cast<std::size_t>(i) * nrhs + comp]; }
+            db.upload(bc, s);
+            dx.upload(xc, s);
+            const SolveRes r = D.solve(db.p, dx.p, comp);
+            const std::vector<double> h = dx.toHost(s);
+            for (int i = 0; i < M.nc; ++i) xout[static_cast<std::size_t>(i) * nrhs + comp] = h[i];
+            results[comp] = bsim_solve_result{r.status, r.iterations, r.initial, r.final_, r.relative};
+        }
+        C.cellsOut(xout, x, nrhs);
+    });
+}
+
+int bsim_system_history(bsim_system* sys, double* out, int cap, int* len) {
+    return guarded([&] {
+        const auto& h = sys->sys->hist;
+        *len = static_cast<int>(h.size());
+        if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<std::size_t>(cap, h.size()));
+    });
+}
+
+int bsim_system_counters(bsim_system* sys, uint64_t* o) {
+    return guarded([&] {
+        const auto& c = sys->sys->counters;
+        const uint64_t v[10] = {c.matrixUpdates, c.rhsOnly, c.pcSetups, c.pcReuses, c.retries, c.solves,
+                                c.krylovIterations, c.trueResidualChecks, c.spmvs, c.pcApplications};
+        std::memcpy(o, v, sizeof(v));
+    });
+}
+
+void bsim_system_destroy(bsim_system* sys) { delete sys; }
+
+namespace {
+struct StandaloneCsr {
+    hipStream_t stream = nullptr;
+    DBuf<int> ptr, col;
+    std::unique_ptr<DeviceSystem> sys;
+    StandaloneCsr(int device, int n, const int* p, const int* c, const double* d, const double* v) {
+        int count = 0;
+        BSIM_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw std::invalid_argument("device index is out of range");
+        if (n <= 0) throw std::invalid_argument("matrix is empty");
+        BSIM_HIP(hipSetDevice(device));
+        BSIM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        std::vector<int> hp(p, p + n + 1), hc(c, c + p[n]);
+        for (int k = 0; k < p[n]; ++k)
+            if (hc[k] < 0 || hc[k] >= n) throw std::invalid_argument("column index out of range");
+        ptr.upload(hp, stream);
+        col.upload(hc, stream);
+        sys = std::make_unique<DeviceSystem>(stream, n, ptr.p, col.p, hp, hc);
+        BSIM_HIP(hipMemcpyAsync(sys->candDiag(), d, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+        if (p[n]) BSIM_HIP(hipMemcpyAsync(sys->candVal(), v, sizeof(double) * p[n], hipMemcpyHostToDevice, stream));
+        sys->update();
+    }
+    ~StandaloneCsr() {
+        sys.reset();
+        ptr.release();
+        col.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+}  // namespace
+
+int bsim_csr_solve(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                   const double* b, double* x, const bsim_solver_config* cfg, bsim_solve_result* res, double* hist,
+                   int cap, int* len) {
+    return guarded([&] {
+        StandaloneCsr S(device, n, ptr, col, diag, val);
+        S.sys->prepare(toSolverCfg(cfg));
+        DBuf<double> db, dx;
+        db.upload(b, n, S.stream);
+        dx.upload(x, n, S.stream);
+        const SolveRes r = S.sys->solve(db.p, dx.p, 0);
+        dx.download(x, n, S.stream);
+        BSIM_HIP(hipStreamSynchronize(S.stream));
+        *res = bsim_solve_result{r.status, r.iterations, r.initial, r.final_, r.relative};
+        if (len) *len = static_cast<int>(S.sys->hist.size());
+        if (hist) std::memcpy(hist, S.sys->hist.data(), sizeof(double) * std::min<std::size_t>(cap, S.sys->hist.size()));
+    });
+}
+
+int bsim_csr_spmv(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                  const double* x, double* y) {
+    return guarded([&] {
+        StandaloneCsr S(device, n, ptr, col, diag, val);
+        DBuf<double> dx, dy(n);
+        dx.upload(x, n, S.stream);
+        S.sys->spmv(dx.p, dy.p);
+        dy.download(y, n, S.stream);
+        BSIM_HIP(hipStreamSynchronize(S.stream));
+    });
+}
+
+int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                       const bsim_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1) {
+    return guarded([&] {
+        StandaloneCsr S(device, n, ptr, col, diag, val);
+        SolverCfg c = toSolverCfg(cfg);
+        c.pc = 2;
+        AmgDevice amg(S.stream);
+        amg.setup(S.sys->view(), S.sys->hptr, S.sys->hcol, c);
+        const std::vector<int> sz = amg.sizes();
+        *nlevels = static_cast<int>(sz.size());
+        std::memcpy(sizes, sz.data(), sizeof(int) * sz.size());
+        const std::vector<int> a0 = amg.aggregates0();
+        if (agg0 && !a0.empty()) std::memcpy(agg0, a0.data(), sizeof(int) * a0.size());
+        if (coarse_diag1 && sz.size() > 1) {
+            const std::vector<double> d1 = amg.coarseDiag(1);
+            std::memcpy(coarse_diag1, d1.data(), sizeof(double) * d1.size());
+        }
+    });
+}
+
+int bsim_simple_create(bsim_ctx* ctx, const bsim_simple_config* cfg, const int* ubct, const double* ubcv,
+                       const int* pbct, const double* pbcv, bsim_simple** out) {
+    return guarded([&] {
+        SimpleCfg c;
+        c.rho = cfg->rho; c.mu = cfg->mu;
+        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
+        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
+        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
+        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
+        c.alphaU = cfg->alpha_u; c.alphaP = cfg->alpha_p; c.nonOrthCorr = cfg->non_orth_corr;
+        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
+        c.mom = toSolverCfg(&cfg->mom);
+        c.pre = toSolverCfg(&cfg->pre);
+        if (c.nonOrthCorr < 0 || c.nonOrthCorr > 20) throw std::invalid_argument("nonOrthogonalCorrections must lie in [0, 20]");
+        auto* s = new bsim_simple;
+        s->ctx = ctx;
+        try { s->s = std::make_unique<SimpleSolver>(*ctx->c, c, ubct, ubcv, pbct, pbcv); }
+        catch (...) { delete s; throw; }
+        *out = s;
+    });
+}
+
+int bsim_simple_set_fields(bsim_simple* s, const double* U, const double* p) {
+    return guarded([&] { s->s->setFields(U, p); });
+}
+int bsim_simple_start(bsim_simple* s) { return guarded([&] { s->s->start(); }); }
+int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports) {
+    return guarded([&] {
+        for (int k = 0; k < n; ++k) {
+            const IterRep r = s->s->iterate();
+            bsim_iter_report& o = reports[k];
+            o.rU = r.rU; o.dU = r.dU; o.dP = r.dP; o.mass = r.mass; o.linP = r.linP;
+            for (int c = 0; c < 3; ++c) o.mom_iters[c] = r.momIters[c];
+            o.p_iters = r.pIters; o.mom_status = r.momStatus; o.p_status = r.pStatus;
+            o.converged = r.converged; o.referenced = r.referenced;
+        }
+    });
+}
+int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi) {
+    return guarded([&] { s->s->getFields(U, p, phi); });
+}
+int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* len) {
+    return guarded([&] {
+        if (which < 0 || which > 3) throw std::invalid_argument("history index is 0..3");
+        const std::vector<double>& h = which < 3 ? s->s->momHist[which] : s->s->pHist;
+        *len = static_cast<int>(h.size());
+        if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<std::size_t>(cap, h.size()));
+    });
+}
+int bsim_simple_timing(bsim_simple* s, double* o) {
+    return guarded([&] {
+        const SimpleTiming& t = s->s->timing;
+        const double v[8] = {t.assembly, t.momentumSolve, t.rhieChow, t.pressureAssembly, t.pressureSolve,
+                             t.correction, t.convergence, static_cast<double>(t.iterations)};
+        std::memcpy(o, v, sizeof(v));
+    });
+}
+int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, int* n, int* nnz) {
+    return guarded([&] {
+        DeviceSystem& D = which == 0 ? s->s->momentumSystem() : s->s->pressureSystem();
+        *seconds = D.timeSpmv(reps);
+        *n = D.n;
+        *nnz = D.nnz;
+    });
+}
+void bsim_simple_destroy(bsim_simple* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Shared device helpers: HIP error checks, RAII device buffers, small vector
+// algebra and deterministic two-stage reductions for 64-wide wavefronts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bsim {
+
+#define BSIM_HIP(expr)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +  \
+                                     __FILE__ + ":" + std::to_string(__LINE__) + ": " #expr);       \
+    } while (0)
+
+#define BSIM_LAUNCH_CHECK() BSIM_HIP(hipGetLastError())
+
+constexpr int kBlock = 256;          // 4 wavefronts of 64
+constexpr int kMaxReduceBlocks = 1024;
+
+inline int blocksFor(long long n, int block = kBlock) {
+    long long b = (n + block - 1) / block;
+    return static_cast<int>(b < 1 ? 1 : b);
+}
+
+// Owning device allocation of T[n].
+template <class T> struct DBuf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(std::size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(std::size_t count) {
+        release();
+        n = count;
+        if (count) BSIM_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void ensure(std::size_t count) { if (count > n) alloc(count); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* src, std::size_t count, hipStream_t s) {
+        ensure(count);
+        if (count) BSIM_HIP(hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void download(T* dst, std::size_t count, hipStream_t s) const {
+        if (count) BSIM_HIP(hipMemcpyAsync(dst, p, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    }
+    std::vector<T> toHost(hipStream_t s, std::size_t count = SIZE_MAX) const {
+        std::vector<T> v(count == SIZE_MAX ? n : count);
+        download(v.data(), v.size(), s);
+        BSIM_HIP(hipStreamSynchronize(s));
+        return v;
+    }
+    void zero(hipStream_t s) {
+        if (n) BSIM_HIP(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+// ---- device vector helpers (component-wise, AoS double[3]) ----
+struct D3 {
+    double x, y, z;
+};
+__host__ __device__ inline D3 d3(double a, double b, double c) { return {a, b, c}; }
+__host__ __device__ inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline D3 cross(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline double comp(D3 a, int c) { return c == 0 ? a.x : (c == 1 ? a.y : a.z); }
+__device__ inline D3 ld3(const double* p, long long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ inline void st3(double* p, long long i, D3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+
+// ---- deterministic reductions ----
+// Stage 1: each block writes its partial sums (fixed grid, fixed in-block tree).
+// Stage 2: the last block to finish sums the partials in index order with a
+// fixed tree, so the result does not depend on scheduling.
+template <int K> __device__ inline void blockSum(double (&v)[K], double* smem /* K*kBlock */) {
+    const int t = threadIdx.x;
+    for (int k = 0; k < K; ++k) smem[k * kBlock + t] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < K; ++k) smem[k * kBlock + t] += smem[k * kBlock + t + s];
+        __syncthreads();
+    }
+    for (int k = 0; k < K; ++k) v[k] = smem[k * kBlock];
+    __syncthreads();
+}
+
+// Workspace for one reduction family: partials[K*maxBlocks], a completion counter.
+struct ReduceSlot {
+    double* partials;
+    unsigned int* counter;
+};
+
+// Returns true in exactly one block (the last); that block then holds the
+// final K sums in `out` (valid for all its threads).
+template <int K>
+__device__ inline bool reduceFinish(double (&v)[K], ReduceSlot slot, double* smem, double (&out)[K]) {
+    blockSum<K>(v, smem);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; ++k) slot.partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
+        __threadfence();
+        const unsigned int done = atomicAdd(slot.counter, 1u);
+        last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[K];
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock)
+        for (int k = 0; k < K; ++k) acc[k] += __hip_atomic_load(&slot.partials[k * kMaxReduceBlocks + b], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    blockSum<K>(acc, smem);
+    for (int k = 0; k < K; ++k) out[k] = acc[k];
+    if (threadIdx.x == 0) *slot.counter = 0u;  // re-arm for the next launch on this slot
+    return true;
+}
+
+inline int reduceBlocks(long long n) {
+    const int b = blocksFor(n);
+    return b < kMaxReduceBlocks ? b : kMaxReduceBlocks;
+}
+
+}  // namespace bsim

@@ -1,0 +1,470 @@
+// Row-gather HIP kernels for the face operators of
+// src/discretization/operators.cpp.  One thread owns one cell (or one face)
+// and reads the data of its faces; values that BabelSim scatters from a face
+// to its owner and neighbour are recomputed on both sides instead.
+#include "fvm_device.hpp"
+
+namespace bsim {
+namespace {
+
+using namespace dev;
+
+// ---------------------------------------------------------------- gradients
+template <int NC, bool PASS2>
+__global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    D3 acc[NC];
+    for (int k = 0; k < NC; ++k) acc[k] = d3(0, 0, 0);
+    const double iv = m.invV[c];
+    double val[NC];
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        if (PASS2) correctedInternal<NC>(x, g0, m, f, val);
+        else internalValue<NC>(x, m, f, val);
+        const D3 sf = ld3(m.Sf, f);
+        const bool owner = m.own[f] == c;
+        for (int k = 0; k < NC; ++k) {
+            const D3 contrib = val[k] * sf;
+            acc[k] = owner ? acc[k] + contrib * iv : acc[k] - contrib * iv;
+        }
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        if (PASS2) reconstructedBoundary<NC>(x, g0, m, f, false, 0.0, val, err);
+        else traceValue<NC>(x, m, f, false, 0.0, val, err);
+        const D3 sf = ld3(m.Sf, f);
+        for (int k = 0; k < NC; ++k) acc[k] = acc[k] + (val[k] * sf) * iv;
+    }
+    for (int k = 0; k < NC; ++k) st3(out, static_cast<long long>(NC) * c + k, acc[k]);
+}
+
+
+template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double rhs[3 * NC];
+    for (int i = 0; i < 3 * NC; ++i) rhs[i] = 0.0;
+    const D3 cc = ld3(m.C, c);
+    double xc[NC], val[NC];
+    cellValue<NC>(x, c, xc);
+    auto addRow = [&](D3 d, const double* diff) {
+        const double d2 = dot(d, d);
+        if (!(d2 > 0.0)) { atomicOr(err, kErrSingularLs); return; }
+        const double wgt = 1.0 / d2;
+        const D3 wd = wgt * d;
+        const double wv[3] = {wd.x, wd.y, wd.z}, dv[3] = {d.x, d.y, d.z};
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) A[i][j] += wv[i] * dv[j];
+        for (int i = 0; i < 3; ++i) for (int k = 0; k < NC; ++k) rhs[i * NC + k] += wv[i] * diff[k];
+    };
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int other = m.mcol[e];
+        double diff[NC];
+        cellValue<NC>(x, other, val);
+        for (int k = 0; k < NC; ++k) diff[k] = val[k] - xc[k];
+        addRow(ld3(m.C, other) - cc, diff);
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        const int type = condition<NC>(x, m, f, false, 0.0, err).type;
+        const D3 off = ld3(m.Cf, f) - cc;
+        const D3 n = ld3(m.nrm, f);
+        const D3 d = type == kFixedValue ? 2.0 * off : (2.0 * dot(off, n)) * n;
+        traceValue<NC>(x, m, f, false, 0.0, val, err);
+        double diff[NC];
+        for (int k = 0; k < NC; ++k) diff[k] = 2.0 * (val[k] - xc[k]);
+        addRow(d, diff);
+    }
+    double sol[3 * NC];
+    if (!solveSym3<NC>(A, rhs, sol)) { atomicOr(err, kErrSingularLs); return; }
+    for (int k = 0; k < NC; ++k) st3(out, static_cast<long long>(NC) * c + k, d3(sol[0 * NC + k], sol[1 * NC + k], sol[2 * NC + k]));
+}
+
+// -------------------------------------------------------- face quantities
+template <int NC> __global__ void kInterp(MeshView m, FieldArg x, const double* g, double* out, int* err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    double v[NC];
+    if (f < m.nfi) {
+        if (g) correctedInternal<NC>(x, g, m, f, v);
+        else internalValue<NC>(x, m, f, v);
+    } else {
+        if (g) reconstructedBoundary<NC>(x, g, m, f, false, 0.0, v, err);
+        else traceValue<NC>(x, m, f, false, 0.0, v, err);
+    }
+    for (int k = 0; k < NC; ++k) out[static_cast<long long>(NC) * f + k] = v[k];
+}
+
+__global__ void kSeedCtx(MeshView m, const double* U, double* ctx) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    ctx[f] = f < m.nfi ? 0.0 : dot(ld3(U, m.own[f]), ld3(m.Sf, f));
+}
+
+__global__ void kFlux(MeshView m, FieldArg U, const double* g, double* out, int* err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    double v[3];
+    if (f < m.nfi) {
+        if (g) correctedInternal<3>(U, g, m, f, v);
+        else internalValue<3>(U, m, f, v);
+    } else {
+        if (g) reconstructedBoundary<3>(U, g, m, f, false, 0.0, v, err);
+        else traceValue<3>(U, m, f, false, 0.0, v, err);
+    }
+    out[f] = dot(d3(v[0], v[1], v[2]), ld3(m.Sf, f));
+}
+
+__global__ void kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double* out, int* err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    double v[1];
+    integratedNormalGradient<1>(x, g, m, f, dm, v, err);
+    out[f] = v[0] / m.Af[f];
+}
+
+__global__ void kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double* out,
+                               int* err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const double k = kf[f];
+    if (!(k >= 0.0) || !isfinite(k)) atomicOr(err, kErrNegativeDiffusivity);
+    double v[1];
+    integratedNormalGradient<1>(x, g, m, f, dm, v, err);
+    out[f] = k * v[0];
+}
+
+__global__ void kDivFace(MeshView m, const double* phi, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const double iv = m.invV[c];
+    double acc = 0.0;
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        acc = m.own[f] == c ? acc + phi[f] * iv : acc - phi[f] * iv;
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) acc += phi[m.bface[e]] * iv;
+    out[c] = acc;
+}
+
+__global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, double kc, const double* g, int dm,
+                                   double* out, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const double iv = m.invV[c];
+    double acc = 0.0;
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        double q[1];
+        if (correctedDiffusion(dm)) integratedNormalGradient<1>(x, g, m, f, dm, q, err);
+        else q[0] = m.orth[f] * (x.v[m.nei[f]] - x.v[m.own[f]]);
+        const double integ = (kf ? kf[f] : kc) * q[0];
+        acc = m.own[f] == c ? acc + integ * iv : acc - integ * iv;
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        const Cnd<1> cd = condition<1>(x, m, f, false, 0.0, err);
+        double q = 0.0;
+        if (cd.type == kFixedValue) {
+            double tv[1];
+            traceValue<1>(x, m, f, true, -1.0, tv, err);
+            q = m.orth[f] * (tv[0] - x.v[c]);
+            if (correctedDiffusion(dm)) { double r[1]; integratedNormalGradient<1>(x, g, m, f, dm, r, err); q = r[0]; }
+        } else if (cd.type == kFixedGradient) {
+            q = cd.v[0] * m.Af[f];
+        }
+        acc += ((kf ? kf[f] : kc) * q) * iv;
+    }
+    out[c] = acc;
+}
+
+template <int NC>
+__global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
+                                    double* out, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
+    const double iv = m.invV[c];
+    double acc[NC], fv[NC], corr[NC];
+    for (int k = 0; k < NC; ++k) acc[k] = 0.0;
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        const double q = phi[f];
+        if (conv == kUpwind || conv == kLinearUpwind) {
+            const int up = q >= 0.0 ? m.own[f] : m.nei[f];
+            cellValue<NC>(x, up, fv);
+            if (conv == kLinearUpwind) { luCorrection<NC>(g, m, up, f, corr); for (int k = 0; k < NC; ++k) fv[k] += corr[k]; }
+        } else if (haveG) {
+            correctedInternal<NC>(x, g, m, f, fv);
+        } else {
+            internalValue<NC>(x, m, f, fv);
+        }
+        const bool owner = m.own[f] == c;
+        for (int k = 0; k < NC; ++k) {
+            const double integ = q * fv[k];
+            acc[k] = owner ? acc[k] + integ * iv : acc[k] - integ * iv;
+        }
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        const double q = phi[f];
+        if (conv == kUpwind || conv == kLinearUpwind) {
+            if (q >= 0.0) {
+                cellValue<NC>(x, c, fv);
+                if (conv == kLinearUpwind) { luCorrection<NC>(g, m, c, f, corr); for (int k = 0; k < NC; ++k) fv[k] += corr[k]; }
+            } else {
+                traceValue<NC>(x, m, f, true, q, fv, err);
+            }
+        } else if (haveG) {
+            reconstructedBoundary<NC>(x, g, m, f, true, q, fv, err);
+        } else {
+            traceValue<NC>(x, m, f, true, q, fv, err);
+        }
+        for (int k = 0; k < NC; ++k) acc[k] += (q * fv[k]) * iv;
+    }
+    for (int k = 0; k < NC; ++k) out[static_cast<long long>(NC) * c + k] = acc[k];
+}
+
+// ---------------------------------------------------------------- assembly
+template <int NC>
+__global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
+                                    double scale, double* diag, double* off, double* src, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
+    double dAcc = 0.0, sAcc[NC], corr[NC], a[NC], b[NC];
+    for (int k = 0; k < NC; ++k) sAcc[k] = 0.0;
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        const int o = m.own[f], n = m.nei[f];
+        const bool owner = o == c;
+        const double F = scale * phi[f];
+        if (conv == kUpwind || conv == kLinearUpwind) {
+            if (owner) { dAcc += fmax(F, 0.0); off[e] += fmin(F, 0.0); }
+            else { dAcc += fmax(-F, 0.0); off[e] += fmin(-F, 0.0); }
+            if (conv == kLinearUpwind) {
+                luCorrection<NC>(g, m, F >= 0.0 ? o : n, f, corr);
+                for (int k = 0; k < NC; ++k) sAcc[k] = owner ? sAcc[k] - F * corr[k] : sAcc[k] + F * corr[k];
+            }
+        } else {
+            const double w = m.w[f];
+            if (owner) { dAcc += F * w; off[e] += F * (1.0 - w); }
+            else { off[e] -= F * w; dAcc -= F * (1.0 - w); }
+            if (haveG) {
+                correctedInternal<NC>(x, g, m, f, a);
+                internalValue<NC>(x, m, f, b);
+                for (int k = 0; k < NC; ++k) {
+                    const double cr = a[k] - b[k];
+                    sAcc[k] = owner ? sAcc[k] - F * cr : sAcc[k] + F * cr;
+                }
+            }
+        }
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        const double F = scale * phi[f];
+        const Cnd<NC> cd = condition<NC>(x, m, f, true, F, err);
+        if (conv == kUpwind || conv == kLinearUpwind) {
+            if (F >= 0.0) {
+                dAcc += F;
+                if (conv == kLinearUpwind) {
+                    luCorrection<NC>(g, m, c, f, corr);
+                    for (int k = 0; k < NC; ++k) sAcc[k] -= F * corr[k];
+                }
+            } else {
+                traceValue<NC>(x, m, f, true, F, a, err);
+                for (int k = 0; k < NC; ++k) sAcc[k] -= F * a[k];
+            }
+            continue;
+        }
+        if (cd.type == kFixedValue) {
+            for (int k = 0; k < NC; ++k) sAcc[k] -= F * cd.v[k];
+        } else if (cd.type == kFixedGradient) {
+            const double dist = dot(ld3(m.Cf, f) - ld3(m.C, c), ld3(m.nrm, f));
+            dAcc += F;
+            for (int k = 0; k < NC; ++k) sAcc[k] -= (F * dist) * cd.v[k];
+        } else {
+            dAcc += F;
+        }
+        if (haveG) {
+            reconstructedBoundary<NC>(x, g, m, f, true, F, a, err);
+            traceValue<NC>(x, m, f, true, F, b, err);
+            for (int k = 0; k < NC; ++k) sAcc[k] -= F * (a[k] - b[k]);
+        }
+    }
+    diag[c] += dAcc;
+    for (int k = 0; k < NC; ++k) src[static_cast<long long>(NC) * c + k] += sAcc[k];
+}
+
+template <int NC>
+__global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double factor,
+                                   double* diag, double* off, double* src, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const bool corrected = correctedDiffusion(dm);
+    double dAcc = 0.0, sAcc[NC], ing[NC], bv[NC];
+    for (int k = 0; k < NC; ++k) sAcc[k] = 0.0;
+    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        const int f = m.mface[e];
+        const double gamma = kf[f];
+        if (gamma < 0.0 || !isfinite(gamma)) { atomicOr(err, kErrNegativeDiffusivity); continue; }
+        if (gamma == 0.0) continue;
+        const double coef = gamma * m.orth[f];
+        dAcc += coef;
+        off[e] += factor * (0.0 - coef);
+        if (corrected) {
+            const long long o = m.own[f], n = m.nei[f];
+            integratedNormalGradient<NC>(x, g, m, f, dm, ing, err);
+            const bool owner = o == c;
+            for (int k = 0; k < NC; ++k) {
+                const double orthogonal = m.orth[f] * (x.v[NC * n + k] - x.v[NC * o + k]);
+                const double corr = gamma * (ing[k] - orthogonal);
+                sAcc[k] = owner ? sAcc[k] + corr : sAcc[k] - corr;
+            }
+        }
+    }
+    for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+        const int f = m.bface[e];
+        const double gamma = kf[f];
+        if (gamma < 0.0 || !isfinite(gamma)) { atomicOr(err, kErrNegativeDiffusivity); continue; }
+        if (gamma == 0.0) continue;
+        const double coef = gamma * m.orth[f];
+        const Cnd<NC> cd = condition<NC>(x, m, f, false, 0.0, err);
+        if (cd.type == kFixedValue || (NC == 3 && cd.type == kSymmetry)) {
+            if (NC == 3) traceValue<NC>(x, m, f, true, -1.0, bv, err);
+            else for (int k = 0; k < NC; ++k) bv[k] = cd.v[k];
+            dAcc += coef;
+            for (int k = 0; k < NC; ++k) sAcc[k] += coef * bv[k];
+            if (corrected) {
+                integratedNormalGradient<NC>(x, g, m, f, dm, ing, err);
+                for (int k = 0; k < NC; ++k) {
+                    const double orthogonal = m.orth[f] * (bv[k] - x.v[static_cast<long long>(NC) * c + k]);
+                    sAcc[k] += gamma * (ing[k] - orthogonal);
+                }
+            }
+        } else if (cd.type == kFixedGradient) {
+            for (int k = 0; k < NC; ++k) sAcc[k] += NC == 1 ? (gamma * cd.v[k]) * m.Af[f] : (gamma * m.Af[f]) * cd.v[k];
+        }
+    }
+    diag[c] += factor * dAcc;
+    for (int k = 0; k < NC; ++k) src[static_cast<long long>(NC) * c + k] += factor * sAcc[k];
+}
+
+__global__ void kGradTraces(MeshView m, FieldArg x, const double* g, double* traces) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = m.nfi + b;
+    if (f >= m.nf) return;
+    int dummy = 0;
+    const int o = m.own[f];
+    const D3 n = ld3(m.nrm, f);
+    const D3 d = ld3(m.Cf, f) - ld3(m.C, o);
+    D3 gv = ld3(g, o);
+    const Cnd<1> bc = condition<1>(x, m, f, false, 0.0, &dummy);
+    const double sn = bc.type == kFixedValue ? (bc.v[0] - x.v[o] - dot(gv, d - dot(d, n) * n)) / dot(d, n)
+                    : bc.type == kFixedGradient ? bc.v[0] : 0.0;
+    gv = gv + (sn - dot(gv, n)) * n;
+    st3(traces, f, gv);
+}
+
+}  // namespace
+
+namespace ops {
+
+void gradient(const MeshView& m, bool orthogonal, int ncomp, FieldArg x, int method, double* out, double* work,
+              int* err, hipStream_t s) {
+    const int nb = blocksFor(m.nc);
+    if (method == kLeastSquares) {
+        if (ncomp == 1) kGradLS<1><<<nb, kBlock, 0, s>>>(m, x, out, err);
+        else kGradLS<3><<<nb, kBlock, 0, s>>>(m, x, out, err);
+    } else if (method == kGreenGauss) {
+        double* first = orthogonal ? out : work;
+        if (ncomp == 1) {
+            kGradGG<1, false><<<nb, kBlock, 0, s>>>(m, x, nullptr, first, err);
+            if (!orthogonal) kGradGG<1, true><<<nb, kBlock, 0, s>>>(m, x, first, out, err);
+        } else {
+            kGradGG<3, false><<<nb, kBlock, 0, s>>>(m, x, nullptr, first, err);
+            if (!orthogonal) kGradGG<3, true><<<nb, kBlock, 0, s>>>(m, x, first, out, err);
+        }
+    } else {
+        throw std::invalid_argument("unsupported gradient method");
+    }
+    BSIM_LAUNCH_CHECK();
+}
+
+void interpolate(const MeshView& m, int ncomp, FieldArg x, int method, const double* grad, double* out, int* err,
+                 hipStream_t s) {
+    if (method != kLinear && method != kCorrected) throw std::invalid_argument("unsupported interpolation method");
+    const double* g = method == kCorrected ? grad : nullptr;
+    if (ncomp == 1) kInterp<1><<<blocksFor(m.nf), kBlock, 0, s>>>(m, x, g, out, err);
+    else kInterp<3><<<blocksFor(m.nf), kBlock, 0, s>>>(m, x, g, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void seedFluxContext(const MeshView& m, const double* U, double* ctx, hipStream_t s) {
+    kSeedCtx<<<blocksFor(m.nf), kBlock, 0, s>>>(m, U, ctx);
+    BSIM_LAUNCH_CHECK();
+}
+
+void flux(const MeshView& m, FieldArg U, const double* gradU, double* out, int* err, hipStream_t s) {
+    kFlux<<<blocksFor(m.nf), kBlock, 0, s>>>(m, U, gradU, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void normalGradient(const MeshView& m, FieldArg x, const double* grad, int dmethod, double* out, int* err,
+                    hipStream_t s) {
+    kSnGrad<<<blocksFor(m.nf), kBlock, 0, s>>>(m, x, grad, dmethod, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void diffusionFlux(const MeshView& m, FieldArg x, const double* kf, const double* grad, int dmethod, double* out,
+                   int* err, hipStream_t s) {
+    kDiffusionFlux<<<blocksFor(m.nf), kBlock, 0, s>>>(m, x, kf, grad, dmethod, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void divergenceFace(const MeshView& m, const double* phi, double* out, hipStream_t s) {
+    kDivFace<<<blocksFor(m.nc), kBlock, 0, s>>>(m, phi, out);
+    BSIM_LAUNCH_CHECK();
+}
+
+void explicitLaplacian(const MeshView& m, FieldArg x, const double* kf, double k, const double* grad, int dmethod,
+                       double* out, int* err, hipStream_t s) {
+    kExplicitLaplacian<<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, kf, k, grad, dmethod, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void explicitConvection(const MeshView& m, int ncomp, FieldArg x, const double* phi, int conv, int interp,
+                        const double* grad, double* out, int* err, hipStream_t s) {
+    if (ncomp == 1) kExplicitConvection<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, out, err);
+    else kExplicitConvection<3><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, out, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void assembleConvection(const MeshView& m, int ncomp, FieldArg x, const double* phi, int conv, int interp,
+                        const double* grad, double scale, double* diag, double* off, double* src, int* err,
+                        hipStream_t s) {
+    if (conv != kUpwind && conv != kLinearUpwind && conv != kCentral) throw std::invalid_argument("unsupported convection method");
+    if (interp != kLinear && interp != kCorrected) throw std::invalid_argument("unsupported convection interpolation method");
+    if (!std::isfinite(scale)) throw std::invalid_argument("convection flux scale must be finite");
+    if (ncomp == 1) kAssembleConvection<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, scale, diag, off, src, err);
+    else kAssembleConvection<3><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, scale, diag, off, src, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void assembleDiffusion(const MeshView& m, int ncomp, FieldArg x, const double* kf, const double* grad, int dmethod,
+                       double factor, double* diag, double* off, double* src, int* err, hipStream_t s) {
+    if (dmethod != kOrthogonal && dmethod != kDiffCorrected && dmethod != kLimitedCorrected)
+        throw std::invalid_argument("unsupported diffusion method");
+    if (ncomp == 1) kAssembleDiffusion<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, kf, grad, dmethod, factor, diag, off, src, err);
+    else kAssembleDiffusion<3><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, kf, grad, dmethod, factor, diag, off, src, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void gradientTraces(const MeshView& m, FieldArg x, const double* grad, double* traces, hipStream_t s) {
+    if (m.nf > m.nfi) kGradTraces<<<blocksFor(m.nf - m.nfi), kBlock, 0, s>>>(m, x, grad, traces);
+    BSIM_LAUNCH_CHECK();
+}
+
+}  // namespace ops
+}  // namespace bsim

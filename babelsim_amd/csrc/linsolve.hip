@@ -1,0 +1,850 @@
+// Krylov solvers, preconditioners and the solve contract on the GPU.
+//
+// Every reduction is deterministic (fixed grid, fixed trees), every scalar of
+// the recurrences (alpha, beta, omega, ...) is computed on the device by the
+// last block of the reduction that produces it, and the host reads one
+// residual norm per iteration to run BabelSim's convergence test.
+#include "linsolve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace bsim {
+
+namespace {
+
+enum Scalar : int { S_RHO = 0, S_RHOOLD, S_ALPHA, S_OMEGA, S_OMEGAOLD, S_BETA, S_BETAOLD, S_DPI, S_A, S_DP, S_D1, S_D2,
+                    S_TR, S_TMP, S_BN, S_COUNT = 16 };
+enum Flag : int { F_BETA_ZERO = 1, F_INDEF_PC = 2, F_INDEF_MAT = 4, F_D1_ZERO = 8, F_D2_ZERO = 16, F_NONFINITE = 32 };
+
+using Clock = std::chrono::steady_clock;
+double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+__device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
+
+template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int i) {
+    double y = static_cast<double>(A.diag[i]) * static_cast<double>(x[i]);
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
+    return y;
+}
+// FP64 matrices keep the exact operation order of the oracle's spmv.
+__device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) {
+    double y = A.diag[i] * x[i];
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * x[A.col[k]];
+    return y;
+}
+
+__global__ void kFill(int n, double* x, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void kCopy(int n, const double* a, double* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void kAxpy(int n, double a, const double* x, double* y) {  // y += a x
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+// Sum of squares of b (optionally copied to r) -> sqrt into sc[idx].
+__global__ void kCopyNorm(int n, const double* b, double* r, double* sc, int idx, ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = b[i];
+        if (r) r[i] = v;
+        acc += v * v;
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = sqrt(out[0]);
+}
+
+// r = b - A x and its norm.
+__global__ void kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* sc, int idx,
+                              ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const double v = b[i] - rowDot64(A, x, i);
+        r[i] = v;
+        acc += v * v;
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = sqrt(out[0]);
+}
+
+__global__ void kDot(int n, const double* a, const double* b, double* sc, int idx, ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = out[0];
+}
+
+// ---- CG (cg.c KSPSolve_CG, unpreconditioned norm) ----
+__device__ inline void finishBeta(double beta, double* sc, int* fl, int checkSign) {
+    sc[S_BETA] = beta;
+    if (!isfinite(beta)) atomicOr(fl, F_NONFINITE);
+    else if (beta == 0.0) atomicOr(fl, F_BETA_ZERO);
+    else if (checkSign && beta * sc[S_BETAOLD] < 0.0) atomicOr(fl, F_INDEF_PC);
+}
+
+// z = r * (1/d) and beta = z . r
+__global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z, double* sc, int* fl, int checkSign,
+                           ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    if (fl[0]) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v = r[i] * jinv[i];
+        z[i] = v;
+        acc += v * r[i];
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) finishBeta(out[0], sc, fl, checkSign);
+}
+__global__ void kPlainDot(int n, const double* z, const double* r, double* sc, int* fl, int checkSign, ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    if (fl[0]) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += z[i] * r[i];
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) finishBeta(out[0], sc, fl, checkSign);
+}
+
+__global__ void kCgDirection(int n, const double* z, double* p, const double* sc, const int* fl, int first) {
+    if (fl[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (first) p[i] = z[i];
+    else {
+        const double bb = sc[S_BETA] / sc[S_BETAOLD];
+        p[i] = z[i] + bb * p[i];
+    }
+}
+
+// w = A p, dpi = p . w, a = beta / dpi
+__global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, double* sc, int* fl, int iter,
+                           ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    if (fl[0]) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const double y = rowDot64(A, p, i);
+        w[i] = y;
+        acc += p[i] * y;
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
+        const double dpiold = sc[S_DPI], dpi = out[0];
+        sc[S_DPI] = dpi;
+        if (!isfinite(dpi)) { atomicOr(fl, F_NONFINITE); return; }
+        sc[S_BETAOLD] = sc[S_BETA];
+        if (dpi == 0.0 || (iter > 0 && sgn(dpi) * sgn(dpiold) < 0)) atomicOr(fl, F_INDEF_MAT);
+        else sc[S_A] = sc[S_BETA] / dpi;
+    }
+}
+
+// x += a p, r += -a w, dp = ||r||
+__global__ void kCgUpdate(int n, double* x, const double* p, double* r, const double* w, double* sc, const int* fl,
+                          ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    if (fl[0]) return;
+    const double a = sc[S_A];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const double ri = r[i] + -a * w[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[S_DP] = sqrt(out[0]);
+}
+
+// ---- BiCGSTAB (bcgs.c:14-140, right preconditioning) ----
+__global__ void kBcgsRho(int n, const double* r, const double* rp, double* sc, ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += r[i] * rp[i];
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
+        const double rho = out[0];
+        sc[S_RHO] = rho;
+        sc[S_BETA] = (rho / sc[S_RHOOLD]) * (sc[S_ALPHA] / sc[S_OMEGAOLD]);
+    }
+}
+__global__ void kBcgsP(int n, double* p, const double* r, const double* v, const double* sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double beta = sc[S_BETA];
+    p[i] = r[i] + (-sc[S_OMEGAOLD] * beta) * v[i] + beta * p[i];
+}
+// v = A t, d1 = v . rp, alpha = rho / d1
+__global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* sc, int* fl,
+                            ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const double y = rowDot64(A, t, i);
+        v[i] = y;
+        acc += y * rp[i];
+    }
+    double vv[1] = {acc}, out[1];
+    if (reduceFinish<1>(vv, slot, smem, out) && threadIdx.x == 0) {
+        const double d1 = out[0];
+        sc[S_D1] = d1;
+        if (!isfinite(d1)) atomicOr(fl, F_NONFINITE);
+        else if (d1 == 0.0) atomicOr(fl, F_D1_ZERO);
+        else sc[S_ALPHA] = sc[S_RHO] / d1;
+    }
+}
+__global__ void kBcgsS(int n, double* s, const double* v, const double* r, const double* sc, const int* fl) {
+    if (fl[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = -sc[S_ALPHA] * v[i] + r[i];
+}
+// t = A rt, d1 = s . t, d2 = t . t, omega = d1 / d2
+__global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, const double* s, double* sc, int* fl,
+                            ReduceSlot slot) {
+    __shared__ double smem[2 * kBlock];
+    if (fl[0]) return;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const double y = rowDot64(A, rt, i);
+        t[i] = y;
+        a1 += s[i] * y;
+        a2 += y * y;
+    }
+    double v[2] = {a1, a2}, out[2];
+    if (reduceFinish<2>(v, slot, smem, out) && threadIdx.x == 0) {
+        sc[S_D1] = out[0];
+        sc[S_D2] = out[1];
+        if (out[1] == 0.0) atomicOr(fl, F_D2_ZERO);
+        else sc[S_OMEGA] = out[0] / out[1];
+    }
+}
+// x = alpha p + omega s + x, r = -omega t + s, dp = ||r||
+__global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, double* r, const double* t, double* sc,
+                            const int* fl, ReduceSlot slot) {
+    __shared__ double smem[kBlock];
+    if (fl[0]) return;
+    const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] = alpha * p[i] + omega * s[i] + x[i];
+        const double ri = -omega * t[i] + s[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    double v[1] = {acc}, out[1];
+    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
+        sc[S_DP] = sqrt(out[0]);
+        sc[S_RHOOLD] = sc[S_RHO];
+        sc[S_OMEGAOLD] = sc[S_OMEGA];
+    }
+}
+__global__ void kSpmvPlain(CsrView<double> A, const double* x, double* y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n) y[i] = rowDot64(A, x, i);
+}
+__global__ void kJacobiApply(int n, const double* r, const double* jinv, double* z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = r[i] * jinv[i];
+}
+__global__ void kInvDiag(int n, const double* d, double* inv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[i] = 1.0 / d[i];
+}
+__global__ void kCompare(int n, const double* a, const double* b, int* diff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(a[i] == b[i])) atomicOr(diff, 1);
+}
+
+// ---- AMG ----
+template <class T> __global__ void kJacobiFirst(int n, const T* d, const T* b, T* x, double omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = static_cast<T>(omega * static_cast<double>(b[i]) / static_cast<double>(d[i]));
+}
+template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const double ax = rowDot(A, x, i);
+    xo[i] = static_cast<T>(static_cast<double>(x[i]) + omega * (static_cast<double>(b[i]) - ax) / static_cast<double>(A.diag[i]));
+}
+template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n) r[i] = static_cast<T>(static_cast<double>(b[i]) - rowDot(A, x, i));
+}
+template <class T> __global__ void kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (int k = mp[I]; k < mp[I + 1]; ++k) s += static_cast<double>(r[mi[k]]);
+    bc[I] = static_cast<T>(s);
+}
+template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = static_cast<T>(static_cast<double>(x[i]) + alpha * static_cast<double>(xc[agg[i]]));
+}
+__global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
+                          double* cd, double* cv) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double s = 0.0;
+    for (int t = gp[e]; t < gp[e + 1]; ++t) {
+        const int src = gs[t];
+        s += src < n ? fd[src] : fv[src - n];
+    }
+    if (e < nc) cd[e] = s;
+    else cv[e - nc] = s;
+}
+__global__ void kToFloat(int n, const double* a, float* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = static_cast<float>(a[i]);
+}
+__global__ void kToDouble(int n, const float* a, double* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = static_cast<double>(a[i]);
+}
+// Dense fill + LU with partial pivoting of the coarsest matrix (one block).
+__global__ void kDenseLU(int n, const int* ptr, const int* col, const double* d, const double* v, double* a, int* piv,
+                         int* flag) {
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) a[e] = 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a[static_cast<long long>(i) * n + i] = d[i];
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) a[static_cast<long long>(i) * n + col[k]] += v[k];
+    }
+    __syncthreads();
+    __shared__ int sp;
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            int p = k;
+            for (int i = k + 1; i < n; ++i)
+                if (fabs(a[static_cast<long long>(i) * n + k]) > fabs(a[static_cast<long long>(p) * n + k])) p = i;
+            piv[k] = p;
+            sp = p;
+            if (!(fabs(a[static_cast<long long>(p) * n + k]) > 0.0)) *flag = 1;
+        }
+        __syncthreads();
+        const int p = sp;
+        if (p != k)
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                const double t = a[static_cast<long long>(k) * n + j];
+                a[static_cast<long long>(k) * n + j] = a[static_cast<long long>(p) * n + j];
+                a[static_cast<long long>(p) * n + j] = t;
+            }
+        __syncthreads();
+        const double pv = a[static_cast<long long>(k) * n + k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+            const double l = a[static_cast<long long>(i) * n + k] / pv;
+            a[static_cast<long long>(i) * n + k] = l;
+            for (int j = k + 1; j < n; ++j) a[static_cast<long long>(i) * n + j] -= l * a[static_cast<long long>(k) * n + j];
+        }
+        __syncthreads();
+    }
+}
+template <class T> __global__ void kDenseSolve(int n, const double* a, const int* piv, const T* b, T* xo, double* work) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) work[i] = static_cast<double>(b[i]);
+    for (int k = 0; k < n; ++k) { const double t = work[k]; work[k] = work[piv[k]]; work[piv[k]] = t; }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) work[i] -= a[static_cast<long long>(i) * n + j] * work[j];
+    for (int i = n - 1; i >= 0; --i) {
+        for (int j = i + 1; j < n; ++j) work[i] -= a[static_cast<long long>(i) * n + j] * work[j];
+        work[i] /= a[static_cast<long long>(i) * n + i];
+    }
+    for (int i = 0; i < n; ++i) xo[i] = static_cast<T>(work[i]);
+}
+
+}  // namespace
+
+bool SolverCfg::operator==(const SolverCfg& o) const {
+    return solver == o.solver && pc == o.pc && atol == o.atol && rtol == o.rtol && maxIter == o.maxIter &&
+           warmStart == o.warmStart && rebuildInterval == o.rebuildInterval && reuseGrowth == o.reuseGrowth &&
+           amgMaxLevels == o.amgMaxLevels && amgCoarseSize == o.amgCoarseSize && amgPre == o.amgPre &&
+           amgPost == o.amgPost && amgOmega == o.amgOmega && amgStrength == o.amgStrength &&
+           amgCorrection == o.amgCorrection && precision == o.precision && innerRtol == o.innerRtol;
+}
+
+// ------------------------------------------------------------------ AMG
+void AmgDevice::setup(const CsrView<double>& A, const std::vector<int>& hptr, const std::vector<int>& hcol,
+                      const SolverCfg& c) {
+    const auto t0 = Clock::now();
+    cfg = c;
+    mixed = c.precision == 1;
+    hipStream_t s = stream;
+    if (!aggregated) {
+        HostCsr A0;
+        A0.n = A.n;
+        A0.ptr = hptr;
+        A0.col = hcol;
+        A0.diag.resize(A.n);
+        A0.val.resize(hcol.size());
+        BSIM_HIP(hipMemcpyAsync(A0.diag.data(), A.diag, sizeof(double) * A.n, hipMemcpyDeviceToHost, s));
+        BSIM_HIP(hipMemcpyAsync(A0.val.data(), A.val, sizeof(double) * hcol.size(), hipMemcpyDeviceToHost, s));
+        BSIM_HIP(hipStreamSynchronize(s));
+        std::vector<HostLevel> H = buildHierarchy(A0, c.amgStrength, c.amgMaxLevels, c.amgCoarseSize);
+        L.clear();
+        for (std::size_t l = 0; l < H.size(); ++l) {
+            auto lv = std::make_unique<Level>();
+            lv->n = H[l].A.n;
+            lv->nnz = static_cast<int>(H[l].A.col.size());
+            if (l == 0) { lv->ptr = A.ptr; lv->col = A.col; }
+            else {
+                lv->ptrOwn.upload(H[l].A.ptr, s);
+                lv->colOwn.upload(H[l].A.col, s);
+                lv->ptr = lv->ptrOwn.p;
+                lv->col = lv->colOwn.p;
+            }
+            lv->nc = H[l].nc;
+            if (!H[l].agg.empty()) {
+                lv->agg.upload(H[l].agg, s);
+                lv->memPtr.upload(H[l].memPtr, s);
+                lv->memIdx.upload(H[l].memIdx, s);
+                lv->gPtr.upload(H[l].gPtr, s);
+                lv->gSrc.upload(H[l].gSrc, s);
+            }
+            lv->diag.alloc(lv->n);
+            lv->val.alloc(std::max(lv->nnz, 1));
+            lv->x.alloc(lv->n); lv->b.alloc(lv->n); lv->r.alloc(lv->n); lv->t.alloc(lv->n);
+            L.push_back(std::move(lv));
+        }
+        hostAgg0 = H[0].agg;
+        const int nco = L.back()->n;
+        lu.alloc(static_cast<std::size_t>(nco) * nco);
+        piv.alloc(nco);
+        luFlag.alloc(1);
+        aggregated = true;
+    }
+    if (mixed) {
+        for (auto& lv : L) {
+            lv->diag32.ensure(lv->n); lv->val32.ensure(std::max(lv->nnz, 1));
+            lv->x32.ensure(lv->n); lv->b32.ensure(lv->n); lv->r32.ensure(lv->n); lv->t32.ensure(lv->n);
+        }
+    }
+    // Numeric setup: the preconditioner keeps its own copy of the fine matrix.
+    BSIM_HIP(hipMemcpyAsync(L[0]->diag.p, A.diag, sizeof(double) * A.n, hipMemcpyDeviceToDevice, s));
+    if (L[0]->nnz) BSIM_HIP(hipMemcpyAsync(L[0]->val.p, A.val, sizeof(double) * L[0]->nnz, hipMemcpyDeviceToDevice, s));
+    for (std::size_t l = 0; l + 1 < L.size(); ++l) {
+        Level& f = *L[l];
+        Level& cc = *L[l + 1];
+        const int ne = cc.n + cc.nnz;
+        kGalerkin<<<blocksFor(ne), kBlock, 0, s>>>(f.n, cc.n, ne, f.gPtr.p, f.gSrc.p, f.diag.p, f.val.p, cc.diag.p, cc.val.p);
+    }
+    if (mixed)
+        for (auto& lv : L) {
+            kToFloat<<<blocksFor(lv->n), kBlock, 0, s>>>(lv->n, lv->diag.p, lv->diag32.p);
+            if (lv->nnz) kToFloat<<<blocksFor(lv->nnz), kBlock, 0, s>>>(lv->nnz, lv->val.p, lv->val32.p);
+        }
+    Level& co = *L.back();
+    luFlag.zero(s);
+    kDenseLU<<<1, 1024, 0, s>>>(co.n, co.ptr, co.col, co.diag.p, co.val.p, lu.p, piv.p, luFlag.p);
+    BSIM_LAUNCH_CHECK();
+    if (luFlag.toHost(s)[0]) throw std::runtime_error("AMG coarsest matrix is singular");
+    setupSeconds += since(t0);
+}
+
+std::vector<int> AmgDevice::sizes() const {
+    std::vector<int> v;
+    for (auto& lv : L) v.push_back(lv->n);
+    return v;
+}
+
+std::vector<double> AmgDevice::coarseDiag(int level) const { return L.at(level)->diag.toHost(stream); }
+
+template <class T> void AmgDevice::cycle(int l) {
+    Level& lv = *L[l];
+    hipStream_t s = stream;
+    T* x = mixed ? reinterpret_cast<T*>(lv.x32.p) : reinterpret_cast<T*>(lv.x.p);
+    T* b = mixed ? reinterpret_cast<T*>(lv.b32.p) : reinterpret_cast<T*>(lv.b.p);
+    T* r = mixed ? reinterpret_cast<T*>(lv.r32.p) : reinterpret_cast<T*>(lv.r.p);
+    T* t = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
+    const T* dg = mixed ? reinterpret_cast<const T*>(lv.diag32.p) : reinterpret_cast<const T*>(lv.diag.p);
+    const T* vl = mixed ? reinterpret_cast<const T*>(lv.val32.p) : reinterpret_cast<const T*>(lv.val.p);
+    const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl};
+    const int nb = blocksFor(lv.n);
+    if (l == static_cast<int>(L.size()) - 1) {
+        kDenseSolve<T><<<1, 64, 0, s>>>(lv.n, lu.p, piv.p, b, x, lv.r.p);
+        return;
+    }
+    const double w = cfg.amgOmega;
+    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, x, w);
+    for (int k = 1; k < cfg.amgPre; ++k) {
+        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, x, t, w);
+        std::swap(x, t);
+    }
+    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, x, r);
+    Level& cl = *L[l + 1];
+    T* bc = mixed ? reinterpret_cast<T*>(cl.b32.p) : reinterpret_cast<T*>(cl.b.p);
+    kRestrict<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, bc);
+    cycle<T>(l + 1);
+    const T* xc = mixed ? reinterpret_cast<const T*>(cl.x32.p) : reinterpret_cast<const T*>(cl.x.p);
+    kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, x, cfg.amgCorrection);
+    for (int k = 0; k < cfg.amgPost; ++k) {
+        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, x, t, w);
+        std::swap(x, t);
+    }
+    // Results live in the buffer named x for this level.
+    T* named = mixed ? reinterpret_cast<T*>(lv.x32.p) : reinterpret_cast<T*>(lv.x.p);
+    if (x != named) {
+        if (mixed) std::swap(lv.x32.p, lv.t32.p);
+        else std::swap(lv.x.p, lv.t.p);
+    }
+}
+
+void AmgDevice::apply(const double* r, double* z) {
+    Level& l0 = *L[0];
+    hipStream_t s = stream;
+    const int nb = blocksFor(l0.n);
+    if (mixed) {
+        kToFloat<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b32.p);
+        cycle<float>(0);
+        kToDouble<<<nb, kBlock, 0, s>>>(l0.n, l0.x32.p, z);
+    } else {
+        kCopy<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b.p);
+        cycle<double>(0);
+        kCopy<<<nb, kBlock, 0, s>>>(l0.n, l0.x.p, z);
+    }
+    BSIM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------- systems
+DeviceSystem::DeviceSystem(hipStream_t st, int n_, const int* dptr, const int* dcol, const std::vector<int>& hp,
+                           const std::vector<int>& hc)
+    : n(n_), nnz(static_cast<int>(hc.size())), stream(st), ptr(dptr), col(dcol), hptr(hp), hcol(hc) {
+    for (int k = 0; k < 2; ++k) { diagBuf[k].alloc(n); valBuf[k].alloc(std::max(nnz, 1)); }
+    sc.alloc(S_COUNT);
+    flags.alloc(1);
+    partials.alloc(4 * kMaxReduceBlocks);
+    counter.alloc(1);
+    counter.zero(stream);
+    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostScalars), sizeof(double) * (S_COUNT + 2), hipHostMallocDefault));
+    for (DBuf<double>* b : {&r, &z, &p, &w, &rp, &v, &t, &s, &tmp, &guess, &saved}) b->alloc(n);
+}
+
+DeviceSystem::~DeviceSystem() {
+    if (hostScalars) (void)hipHostFree(hostScalars);
+}
+
+bool DeviceSystem::update() {
+    bool changed = true;
+    if (ready) {
+        cmp.ensure(1);
+        cmp.zero(stream);
+        kCompare<<<blocksFor(n), kBlock, 0, stream>>>(n, diagBuf[0].p, diagBuf[1].p, cmp.p);
+        if (nnz) kCompare<<<blocksFor(nnz), kBlock, 0, stream>>>(nnz, valBuf[0].p, valBuf[1].p, cmp.p);
+        BSIM_LAUNCH_CHECK();
+        changed = cmp.toHost(stream, 1)[0] != 0;
+    }
+    if (changed) {
+        cand = 1 - cand;  // the candidate becomes the effective matrix
+        ready = true;
+        setupPending = true;
+        ++counters.matrixUpdates;
+    } else {
+        ++counters.rhsOnly;
+    }
+    return changed;
+}
+
+void DeviceSystem::prepare(const SolverCfg& c) {
+    if (!configured || !(c == cfg)) {
+        cfg = c;
+        configured = true;
+        setupPending = true;
+        forceRebuild = true;
+    }
+    if (!setupPending && !forceRebuild) return;
+    const bool reuse = cfg.pc != 0 && counters.pcSetups != 0 && !forceRebuild &&
+                       counters.matrixUpdates - pcMatrixUpdate < static_cast<std::uint64_t>(cfg.rebuildInterval) &&
+                       pcRhsSolves < static_cast<std::uint64_t>(cfg.rebuildInterval);
+    setup(reuse);
+}
+
+void DeviceSystem::setup(bool reuse) {
+    const auto t0 = Clock::now();
+    if (!reuse) {
+        if (cfg.pc == 1) {
+            jinv.ensure(n);
+            kInvDiag<<<blocksFor(n), kBlock, 0, stream>>>(n, diag(), jinv.p);
+            BSIM_LAUNCH_CHECK();
+        } else if (cfg.pc == 2) {
+            if (!amg) amg = std::make_unique<AmgDevice>(stream);
+            amg->setup(view(), hptr, hcol, cfg);
+        }
+        if (cfg.pc != 0) ++counters.pcSetups;
+        pcMatrixUpdate = counters.matrixUpdates;
+        pcRhsSolves = 0;
+        freshIterations[0] = freshIterations[1] = freshIterations[2] = 0;
+    } else {
+        ++counters.pcReuses;
+    }
+    setupPending = false;
+    forceRebuild = false;
+    counters.setupSeconds += since(t0);
+}
+
+void DeviceSystem::applyPC(const double* rr, double* zz) {
+    ++counters.pcApplications;
+    if (cfg.pc == 0) kCopy<<<blocksFor(n), kBlock, 0, stream>>>(n, rr, zz);
+    else if (cfg.pc == 1) kJacobiApply<<<blocksFor(n), kBlock, 0, stream>>>(n, rr, jinv.p, zz);
+    else amg->apply(rr, zz);
+    BSIM_LAUNCH_CHECK();
+}
+
+double DeviceSystem::readScalar(int idx, int* fl) {
+    BSIM_HIP(hipMemcpyAsync(hostScalars, sc.p + idx, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (fl) BSIM_HIP(hipMemcpyAsync(reinterpret_cast<int*>(hostScalars + S_COUNT), flags.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    BSIM_HIP(hipStreamSynchronize(stream));
+    if (fl) *fl = *reinterpret_cast<int*>(hostScalars + S_COUNT);
+    return hostScalars[0];
+}
+
+void DeviceSystem::spmv(const double* x, double* y) {
+    kFill<<<blocksFor(n), kBlock, 0, stream>>>(n, tmp.p, 0.0);
+    kResidualNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(view(), tmp.p, x, y, sc.p, S_TMP, {partials.p, counter.p});
+    // y currently holds -A x
+    kAxpy<<<blocksFor(n), kBlock, 0, stream>>>(n, -2.0, y, y);
+    BSIM_LAUNCH_CHECK();
+}
+
+double DeviceSystem::timeSpmv(int reps) {
+    if (reps <= 0) throw std::invalid_argument("reps must be positive");
+    kFill<<<blocksFor(n), kBlock, 0, stream>>>(n, p.p, 1.0);
+    hipEvent_t e0, e1;
+    BSIM_HIP(hipEventCreate(&e0));
+    BSIM_HIP(hipEventCreate(&e1));
+    kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);  // warm
+    BSIM_HIP(hipEventRecord(e0, stream));
+    for (int k = 0; k < reps; ++k) kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);
+    BSIM_HIP(hipEventRecord(e1, stream));
+    BSIM_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    BSIM_HIP(hipEventDestroy(e0));
+    BSIM_HIP(hipEventDestroy(e1));
+    return ms * 1e-3 / reps;
+}
+
+double DeviceSystem::trueResidual(const double* b, const double* x) {
+    kResidualNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(view(), b, x, tmp.p, sc.p, S_TR, {partials.p, counter.p});
+    BSIM_LAUNCH_CHECK();
+    ++counters.spmvs;
+    ++counters.trueResidualChecks;
+    return readScalar(S_TR);
+}
+
+void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason) {
+    const ReduceSlot slot{partials.p, counter.p};
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    flags.zero(stream);
+    {
+        const double init[S_COUNT] = {0, 1.0, 1.0, 0, 1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        BSIM_HIP(hipMemcpyAsync(sc.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    }
+    if (!guessZero) { kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, sc.p, S_DP, slot); ++counters.spmvs; }
+    else kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, sc.p, S_DP, slot);
+    BSIM_LAUNCH_CHECK();
+    double dp = readScalar(S_DP);
+    its = 0;
+    reason = 0;
+    hist.push_back(dp);
+    auto test = [&](double v) {
+        if (!std::isfinite(v)) { reason = -9; return true; }
+        if (v <= target) { reason = 2; return true; }
+        return false;
+    };
+    if (test(dp)) return;
+    auto pcDot = [&](int checkSign) {
+        if (cfg.pc == 1) {
+            kJacobiDot<<<rb, kBlock, 0, stream>>>(n, r.p, jinv.p, z.p, sc.p, flags.p, checkSign, slot);
+            ++counters.pcApplications;
+        } else {
+            applyPC(r.p, z.p);
+            kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, sc.p, flags.p, checkSign, slot);
+        }
+        BSIM_LAUNCH_CHECK();
+    };
+    pcDot(0);
+    int i = 0;
+    int fl = 0;
+    do {
+        its = i + 1;
+        kCgDirection<<<nb, kBlock, 0, stream>>>(n, z.p, p.p, sc.p, flags.p, i == 0);
+        kCgSpmvDot<<<rb, kBlock, 0, stream>>>(view(), p.p, w.p, sc.p, flags.p, i, slot);
+        kCgUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, r.p, w.p, sc.p, flags.p, slot);
+        BSIM_LAUNCH_CHECK();
+        ++counters.spmvs;
+        dp = readScalar(S_DP, &fl);
+        if (fl & F_BETA_ZERO) { reason = 3; break; }
+        if (fl & F_INDEF_PC) { reason = -8; break; }
+        if (fl & F_NONFINITE) { reason = -9; break; }
+        if (fl & F_INDEF_MAT) { reason = -10; break; }
+        hist.push_back(dp);
+        if (test(dp)) break;
+        pcDot(1);
+        ++i;
+    } while (i < maxIt);
+    if (i >= maxIt) reason = -3;
+}
+
+void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason) {
+    const ReduceSlot slot{partials.p, counter.p};
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    flags.zero(stream);
+    {
+        const double init[S_COUNT] = {0, 1.0, 1.0, 0, 1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        BSIM_HIP(hipMemcpyAsync(sc.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    }
+    bool haveGuess = false;
+    if (!guessZero) {
+        kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, sc.p, S_DP, slot);
+        ++counters.spmvs;
+        kCopy<<<nb, kBlock, 0, stream>>>(n, x, guess.p);
+        kFill<<<nb, kBlock, 0, stream>>>(n, x, 0.0);
+        haveGuess = true;
+    } else {
+        kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, sc.p, S_DP, slot);
+    }
+    BSIM_LAUNCH_CHECK();
+    double dp = readScalar(S_DP);
+    its = 0;
+    reason = 0;
+    hist.push_back(dp);
+    auto test = [&](double vv) {
+        if (!std::isfinite(vv)) { reason = -9; return true; }
+        if (vv <= target) { reason = 2; return true; }
+        return false;
+    };
+    if (test(dp)) {
+        if (haveGuess) kAxpy<<<nb, kBlock, 0, stream>>>(n, 1.0, guess.p, x);
+        return;
+    }
+    kCopy<<<nb, kBlock, 0, stream>>>(n, r.p, rp.p);
+    kFill<<<nb, kBlock, 0, stream>>>(n, p.p, 0.0);
+    kFill<<<nb, kBlock, 0, stream>>>(n, v.p, 0.0);
+    int i = 0, fl = 0;
+    do {
+        kBcgsRho<<<rb, kBlock, 0, stream>>>(n, r.p, rp.p, sc.p, slot);
+        kBcgsP<<<nb, kBlock, 0, stream>>>(n, p.p, r.p, v.p, sc.p);
+        applyPC(p.p, tmp.p);
+        kBcgsSpmvD1<<<rb, kBlock, 0, stream>>>(view(), tmp.p, v.p, rp.p, sc.p, flags.p, slot);
+        kBcgsS<<<nb, kBlock, 0, stream>>>(n, s.p, v.p, r.p, sc.p, flags.p);
+        BSIM_LAUNCH_CHECK();
+        applyPC(s.p, r.p);
+        kBcgsSpmvD2<<<rb, kBlock, 0, stream>>>(view(), r.p, t.p, s.p, sc.p, flags.p, slot);
+        kBcgsUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, s.p, r.p, t.p, sc.p, flags.p, slot);
+        BSIM_LAUNCH_CHECK();
+        counters.spmvs += 2;
+        const double newDp = readScalar(S_DP, &fl);
+        if (fl & F_NONFINITE) { reason = -9; break; }
+        if (fl & F_D1_ZERO) { reason = -5; break; }
+        if (fl & F_D2_ZERO) {
+            kDot<<<rb, kBlock, 0, stream>>>(n, s.p, s.p, sc.p, S_TMP, slot);
+            BSIM_LAUNCH_CHECK();
+            if (readScalar(S_TMP) != 0.0) { reason = -5; break; }
+            const double alpha = readScalar(S_ALPHA);
+            kAxpy<<<nb, kBlock, 0, stream>>>(n, alpha, p.p, x);
+            ++its;
+            reason = 2;
+            hist.push_back(dp);
+            break;
+        }
+        dp = newDp;
+        ++its;
+        hist.push_back(dp);
+        if (test(dp)) break;
+        if (readScalar(S_RHO) == 0.0) { reason = -5; break; }
+        ++i;
+    } while (i < maxIt);
+    if (i >= maxIt) reason = -3;
+    applyPC(x, tmp.p);  // unwind right preconditioning
+    kCopy<<<nb, kBlock, 0, stream>>>(n, tmp.p, x);
+    if (haveGuess) kAxpy<<<nb, kBlock, 0, stream>>>(n, 1.0, guess.p, x);
+    BSIM_LAUNCH_CHECK();
+}
+
+void DeviceSystem::bcgsMixed(const double* b, double* x, bool guessZero, int maxIt, double target, int& its,
+                             int& reason) {
+    (void)b; (void)x; (void)guessZero; (void)maxIt; (void)target; (void)its; (void)reason;
+    throw std::invalid_argument("mixed-precision BiCGSTAB is not available in this build");
+}
+
+SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
+    if (!ready || !configured || setupPending)
+        throw std::logic_error("linear system must update and prepare before solving a right-hand side");
+    const auto t0 = Clock::now();
+    const int nb = blocksFor(n);
+    hist.clear();
+    ++counters.solves;
+    // x0 = 0, or the current field when warm starting.
+    kCopy<<<nb, kBlock, 0, stream>>>(n, x, saved.p);
+    auto resetGuess = [&]() {
+        if (cfg.warmStart) kCopy<<<nb, kBlock, 0, stream>>>(n, saved.p, x);
+        else kFill<<<nb, kBlock, 0, stream>>>(n, x, 0.0);
+    };
+    resetGuess();
+    BSIM_LAUNCH_CHECK();
+    kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, sc.p, S_BN, {partials.p, counter.p});
+    const double rhsNorm = readScalar(S_BN);
+    const double initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
+    const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;
+    const double target = std::max(cfg.atol, cfg.rtol * scale);
+    const bool reused = cfg.pc != 0 && pcMatrixUpdate < counters.matrixUpdates;
+    SolveRes res;
+    if (std::isfinite(initial) && initial <= target) {
+        res.status = kConverged; res.iterations = 0; res.initial = initial; res.final_ = initial;
+        res.relative = initial / scale;
+        ++counters.trueResidualChecks;
+        counters.solveSeconds += since(t0);
+        return res;
+    }
+    int total = 0;
+    bool pcReused = reused;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        int attemptIts = 0, reason = 0;
+        double fin = 0.0, internalTarget = target;
+        bool guessZero = !cfg.warmStart;
+        for (int restart = 0; restart < 3; ++restart) {
+            int its = 0;
+            if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            else if (cfg.precision == 1) bcgsMixed(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            else bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            attemptIts += its;
+            total += its;
+            fin = trueResidual(b, x);
+            if ((std::isfinite(fin) && fin <= target) || reason <= 0 || !std::isfinite(fin) || attemptIts >= cfg.maxIter)
+                break;
+            internalTarget *= 0.5;
+            guessZero = false;
+        }
+        int status = kNumericalFailure;
+        if (std::isfinite(fin) && fin <= target) status = kConverged;
+        else if (reason == -3 || attemptIts >= cfg.maxIter) status = kMaxIterations;
+        res.status = status; res.iterations = total; res.initial = initial; res.final_ = fin; res.relative = fin / scale;
+        if (status != kConverged && attempt == 0 && pcReused) {
+            setup(false);
+            ++counters.retries;
+            resetGuess();
+            pcReused = false;
+            continue;
+        }
+        if (status == kConverged) {
+            int& fresh = freshIterations[rhsIndex];
+            if (pcMatrixUpdate == counters.matrixUpdates) fresh = std::max(fresh, attemptIts);
+            else if (attemptIts > cfg.reuseGrowth * static_cast<double>(std::max(fresh, 1))) forceRebuild = true;
+        }
+        break;
+    }
+    if (total > 0) ++pcRhsSolves;
+    counters.krylovIterations += total;
+    counters.solveSeconds += since(t0);
+    return res;
+}
+
+}  // namespace bsim

@@ -1,0 +1,135 @@
+// Linear systems behind BabelSim's ComputeBackend seam
+// (src/internal/compute_backend.h:30-31): CSR matrices in HBM, CG and
+// BiCGSTAB following PETSc's KSPSolve_CG / KSPSolve_BCGS recurrences, Jacobi
+// and aggregation-AMG preconditioners, and the true-residual / preconditioner
+// reuse contract of src/backend/petsc_linear_system.cpp:257-481.
+#pragma once
+
+#include "amg_host.hpp"
+#include "device_common.hpp"
+
+#include <memory>
+
+namespace bsim {
+
+struct SolverCfg {
+    int solver = 1, pc = 2;
+    double atol = 1e-12, rtol = 1e-8;
+    int maxIter = 1000, warmStart = 0, rebuildInterval = 8;
+    double reuseGrowth = 2.0;
+    int amgMaxLevels = 12, amgCoarseSize = 48, amgPre = 2, amgPost = 2;
+    double amgOmega = 0.8, amgStrength = 0.25, amgCorrection = 1.0;
+    int precision = 0;
+    double innerRtol = 1e-3;
+    bool operator==(const SolverCfg& o) const;
+};
+
+enum SolveStatus : int { kConverged = 0, kMaxIterations = 1, kNumericalFailure = 2 };
+struct SolveRes {
+    int status = kNumericalFailure, iterations = 0;
+    double initial = 0, final_ = 0, relative = 0;
+};
+
+// Matrix with the diagonal stored apart from the off-diagonal CSR entries.
+template <class T> struct CsrView {
+    int n;
+    const int* ptr;
+    const int* col;
+    const T* diag;
+    const T* val;
+};
+
+class AmgDevice;
+
+struct SystemCounters {
+    std::uint64_t matrixUpdates = 0, rhsOnly = 0, pcSetups = 0, pcReuses = 0, retries = 0, solves = 0,
+                  krylovIterations = 0, trueResidualChecks = 0, spmvs = 0, pcApplications = 0;
+    double setupSeconds = 0, solveSeconds = 0;
+};
+
+class DeviceSystem {
+public:
+    DeviceSystem(hipStream_t stream, int n, const int* dptr, const int* dcol, const std::vector<int>& hptr,
+                 const std::vector<int>& hcol);
+    ~DeviceSystem();
+    int n, nnz;
+    // Assembly writes the candidate matrix here; update() compares it with the
+    // effective matrix and swaps it in when any value differs.
+    double* candDiag() { return diagBuf[cand].p; }
+    double* candVal() { return valBuf[cand].p; }
+    const double* diag() const { return diagBuf[1 - cand].p; }
+    const double* val() const { return valBuf[1 - cand].p; }
+    bool update();                          // petsc_linear_system.cpp:257-282
+    void prepare(const SolverCfg& cfg);     // petsc_linear_system.cpp:284-301
+    SolveRes solve(const double* b, double* x, int rhsIndex);  // petsc_linear_system.cpp:355-481
+    void spmv(const double* x, double* y);   // y = A x with the effective matrix
+    double timeSpmv(int reps);               // mean seconds of the solver's SpMV kernel (HIP events)
+    std::vector<double> hist;               // unpreconditioned residual norm per Krylov iteration
+    SystemCounters counters;
+    SolverCfg cfg;
+    CsrView<double> view() const { return {n, ptr, col, diag(), val()}; }
+    hipStream_t stream;
+    const int* ptr;
+    const int* col;
+    std::vector<int> hptr, hcol;
+
+private:
+    void setup(bool reuse);
+    void applyPC(const double* r, double* z);
+    double trueResidual(const double* b, const double* x);
+    void cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
+    void bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
+    void bcgsMixed(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
+    double readScalar(int idx, int* flags = nullptr);
+    DBuf<double> diagBuf[2], valBuf[2];
+    int cand = 0;
+    bool ready = false, configured = false, setupPending = false, forceRebuild = false;
+    std::uint64_t pcMatrixUpdate = 0, pcRhsSolves = 0;
+    int freshIterations[3] = {0, 0, 0};
+    DBuf<double> jinv;          // Jacobi: 1/diag of the matrix at PC setup
+    std::unique_ptr<AmgDevice> amg;
+    DBuf<double> r, z, p, w, rp, v, t, s, tmp, guess, saved;
+    DBuf<float> r32, z32, p32, w32, rp32, v32, t32, s32, x32, tmp32, diag32, val32;
+    DBuf<double> sc;            // device scalars
+    DBuf<int> flags, cmp;
+    DBuf<double> partials;
+    DBuf<unsigned int> counter;
+    double* hostScalars = nullptr;  // pinned mirror
+};
+
+// Aggregation AMG V-cycle in FP64 or FP32 with an FP64 dense coarsest solve.
+class AmgDevice {
+public:
+    AmgDevice(hipStream_t s) : stream(s) {}
+    void setup(const CsrView<double>& A, const std::vector<int>& hptr, const std::vector<int>& hcol,
+               const SolverCfg& cfg);
+    void apply(const double* r, double* z);  // z = M^{-1} r (FP64 in/out)
+    int levels() const { return static_cast<int>(L.size()); }
+    std::vector<int> sizes() const;
+    std::vector<int> aggregates0() const { return hostAgg0; }
+    std::vector<double> coarseDiag(int level) const;
+    double setupSeconds = 0.0;
+
+private:
+    struct Level {
+        int n = 0, nnz = 0, nc = 0;
+        const int* ptr = nullptr;
+        const int* col = nullptr;
+        DBuf<int> ptrOwn, colOwn, agg, memPtr, memIdx, gPtr, gSrc;
+        DBuf<double> diag, val;
+        DBuf<float> diag32, val32;
+        DBuf<double> x, b, r, t;
+        DBuf<float> x32, b32, r32, t32;
+    };
+    template <class T> void cycle(int l);
+    hipStream_t stream;
+    std::vector<std::unique_ptr<Level>> L;
+    DBuf<double> lu;
+    DBuf<int> piv, luFlag;
+    bool aggregated = false;
+    bool mixed = false;
+    SolverCfg cfg;
+    std::vector<int> hostAgg0;
+};
+
+}  // namespace bsim

@@ -5,6 +5,7 @@
 #include "mesh_host.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <stdexcept>
 
@@ -73,18 +74,121 @@ MeshDesc generateHex(int nx, int ny, int nz, const double lo[3], const double hi
     // Split decisions and cell numbering: a split hex becomes two consecutive cells.
     std::vector<signed char> family(nh, -1);
     std::vector<int> base(nh);
-    int next = 0;
     const std::uint64_t splitSeed = mix(seed ^ 0x5b1175eedULL);
-    for (long long hx = 0; hx < nh; ++hx) {
+    for (long long hx = 0; hx < nh; ++hx)
         if (splitFraction > 0.0 && unit(splitSeed, static_cast<std::uint64_t>(hx), 0) < splitFraction)
             family[hx] = static_cast<signed char>(mix(splitSeed + static_cast<std::uint64_t>(hx)) % 3);
-        base[hx] = next;
-        next += family[hx] >= 0 ? 2 : 1;
-    }
     auto hexId = [&](int a, int b, int c) { return static_cast<long long>(a) + nx * (static_cast<long long>(b) + static_cast<long long>(ny) * c); };
     auto vid = [&](int a, int b, int c, int lv) {
         return static_cast<int>((a + kOff[lv][0]) + nvx * ((b + kOff[lv][1]) + nvy * (c + kOff[lv][2])));
     };
+    // A split is kept only if every face of the two prisms keeps its area
+    // vector within ~84 degrees of the owner-to-neighbour direction (vertex
+    // averages stand in for centroids); heavy jitter can otherwise produce a
+    // face with S_f . d <= 0, which Mesh::polyhedral rejects (mesh.cpp:515-520).
+    {
+        auto pt = [&](int v) { return &m.vert[3 * static_cast<std::size_t>(v)]; };
+        auto partCentre = [&](int a, int b, int c, int part, double* out) {  // part -1: whole hex
+            const long long hx = hexId(a, b, c);
+            out[0] = out[1] = out[2] = 0.0;
+            int cnt = 0;
+            for (int lv = 0; lv < 8; ++lv) {
+                if (part >= 0) {
+                    const int f = family[hx], first = kPlane[f][0], second = kPlane[f][1];
+                    const bool inA = kOff[lv][first] >= kOff[lv][second];
+                    const bool inB = kOff[lv][first] <= kOff[lv][second];
+                    if ((part == 0 && !inA) || (part == 1 && !inB)) continue;
+                }
+                const double* p = pt(vid(a, b, c, lv));
+                out[0] += p[0]; out[1] += p[1]; out[2] += p[2];
+                ++cnt;
+            }
+            out[0] /= cnt; out[1] /= cnt; out[2] /= cnt;
+        };
+        // Which part of hex (a,b,c) holds the lattice vertices g.
+        auto partOf = [&](int a, int b, int c, const int* g, int n) {
+            const long long hx = hexId(a, b, c);
+            if (family[hx] < 0) return -1;
+            const int first = kPlane[family[hx]][0], second = kPlane[family[hx]][1];
+            for (int t = 0; t < n; ++t) {
+                const long long v = g[t];
+                const int d[3] = {static_cast<int>(v % nvx) - a, static_cast<int>((v / nvx) % nvy) - b, static_cast<int>(v / (nvx * nvy)) - c};
+                if (d[first] < d[second]) return 1;
+            }
+            return 0;
+        };
+        auto faceOk = [&](const int* ring, int n, const double* co, const double* cn) {
+            double S[3] = {0, 0, 0}, avg[3] = {0, 0, 0};
+            for (int t = 0; t < n; ++t) {
+                const double* p = pt(ring[t]);
+                const double* q = pt(ring[(t + 1) % n]);
+                S[0] += 0.5 * (p[1] * q[2] - p[2] * q[1]);
+                S[1] += 0.5 * (p[2] * q[0] - p[0] * q[2]);
+                S[2] += 0.5 * (p[0] * q[1] - p[1] * q[0]);
+                for (int k = 0; k < 3; ++k) avg[k] += p[k] / n;
+            }
+            double sgn = 0.0;
+            for (int k = 0; k < 3; ++k) sgn += S[k] * (avg[k] - co[k]);
+            const double d[3] = {cn[0] - co[0], cn[1] - co[1], cn[2] - co[2]};
+            const double sd = (sgn < 0 ? -1.0 : 1.0) * (S[0] * d[0] + S[1] * d[1] + S[2] * d[2]);
+            const double ns = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+            const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            return sd > 0.1 * ns * nd;
+        };
+        auto hexOk = [&](int a, int b, int c) {
+            const long long hx = hexId(a, b, c);
+            double cA[3], cB[3], cn[3];
+            partCentre(a, b, c, 0, cA);
+            partCentre(a, b, c, 1, cB);
+            int ring[4];
+            for (int t = 0; t < 4; ++t) ring[t] = vid(a, b, c, kInterior[family[hx]][t]);
+            if (!faceOk(ring, 4, cA, cB)) return false;
+            const int cnt[3] = {nx, ny, nz};
+            for (int lf = 0; lf < 6; ++lf) {
+                const int dir = lf / 2;
+                int nb[3] = {a, b, c};
+                nb[dir] += lf % 2 == 1 ? 1 : -1;
+                if (nb[dir] < 0 || nb[dir] >= cnt[dir]) continue;
+                for (int t = 0; t < 4; ++t) ring[t] = vid(a, b, c, kFace[lf][t]);
+                const bool cut = family[hx] == dir || family[hexId(nb[0], nb[1], nb[2])] == dir;
+                int subs[2][4], sn[2], ns = 0;
+                if (!cut) { for (int t = 0; t < 4; ++t) subs[0][t] = ring[t]; sn[0] = 4; ns = 1; }
+                else {
+                    int p0 = 0;
+                    for (int t = 0; t < 4; ++t) {
+                        const int lv = kFace[lf][t];
+                        if (kOff[lv][(dir + 1) % 3] == 0 && kOff[lv][(dir + 2) % 3] == 0) { p0 = t; break; }
+                    }
+                    subs[0][0] = ring[p0]; subs[0][1] = ring[(p0 + 1) % 4]; subs[0][2] = ring[(p0 + 2) % 4];
+                    subs[1][0] = ring[(p0 + 2) % 4]; subs[1][1] = ring[(p0 + 3) % 4]; subs[1][2] = ring[p0];
+                    sn[0] = sn[1] = 3;
+                    ns = 2;
+                }
+                for (int k = 0; k < ns; ++k) {
+                    const int po = partOf(a, b, c, subs[k], sn[k]);
+                    const int pn = partOf(nb[0], nb[1], nb[2], subs[k], sn[k]);
+                    partCentre(nb[0], nb[1], nb[2], pn, cn);
+                    if (!faceOk(subs[k], sn[k], po == 1 ? cB : cA, cn)) return false;
+                }
+            }
+            return true;
+        };
+        for (int pass = 0; pass < 8; ++pass) {
+            bool changed = false;
+            for (int c = 0; c < nz; ++c)
+                for (int b = 0; b < ny; ++b)
+                    for (int a = 0; a < nx; ++a) {
+                        const long long hx = hexId(a, b, c);
+                        if (family[hx] >= 0 && !hexOk(a, b, c)) { family[hx] = -1; changed = true; }
+                    }
+            if (!changed) break;
+        }
+    }
+    int next = 0;
+    for (long long hx = 0; hx < nh; ++hx) {
+        base[hx] = next;
+        next += family[hx] >= 0 ? 2 : 1;
+    }
     // Cell of hex (a,b,c) containing the lattice vertices `g` (global ids).
     auto cellOf = [&](int a, int b, int c, const int* g, int n) {
         const long long hx = hexId(a, b, c);

@@ -1,0 +1,441 @@
+// Steady SIMPLE on the GPU (src/physics/simple/main.cpp:47-127).  Each step
+// names the reference line it follows; fused kernels keep BabelSim's
+// per-element arithmetic (relax-then-scale rows, V/aP, Rhie-Chow face flux,
+// frozen pressure flux) while gathering by row instead of scattering by face.
+#include "fvm_device.hpp"
+#include "simple.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace bsim {
+namespace {
+
+using namespace dev;
+using Clock = std::chrono::steady_clock;
+double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+constexpr int kRedK = 6;
+
+__global__ void kFillD(long long n, double* x, double v) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void kCopyD(long long n, const double* a, double* b) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void kExtract(int n, const double* v3, int comp, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v3[3 * static_cast<long long>(i) + comp];
+}
+__global__ void kInsert(int n, const double* in, int comp, double* v3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v3[3 * static_cast<long long>(i) + comp] = in[i];
+}
+
+// equ::source(momentum, -grad(p)): source += (1*V) * (-1*gradP)  (procedural_equation.cpp:192-196)
+__global__ void kMomentumPressureSource(MeshView m, const double* gradP, double* src) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.nc) return;
+    const double v = 1.0 * m.V[i];
+    for (int k = 0; k < 3; ++k) src[3 * static_cast<long long>(i) + k] += v * (-1.0 * gradP[3 * static_cast<long long>(i) + k]);
+}
+
+// diagnostics::relativeResidual partial sums of |b-Ax|^2, |Ax|^2, |b|^2
+// (procedural_equation.cpp:429-433) on the unrelaxed momentum rows.
+__global__ void kMomentumResidual(MeshView m, const double* diag, const double* off, const double* src, const double* U,
+                                  double* red, ReduceSlot slot) {
+    __shared__ double smem[3 * kBlock];
+    double a[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.nc; i += gridDim.x * blockDim.x) {
+        double ax[3];
+        for (int k = 0; k < 3; ++k) ax[k] = diag[i] * U[3 * static_cast<long long>(i) + k];
+        for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e)
+            for (int k = 0; k < 3; ++k) ax[k] += off[e] * U[3 * static_cast<long long>(m.mcol[e]) + k];
+        double r2 = 0, a2 = 0, b2 = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double b = src[3 * static_cast<long long>(i) + k];
+            const double r = b - ax[k];
+            r2 += r * r; a2 += ax[k] * ax[k]; b2 += b * b;
+        }
+        a[0] += r2; a[1] += a2; a[2] += b2;
+    }
+    double out[3];
+    if (reduceFinish<3>(a, slot, smem, out) && threadIdx.x == 0) { red[0] = out[0]; red[1] = out[1]; red[2] = out[2]; }
+}
+
+// relax (procedural_equation.cpp:298-305), scale (240-247) and rAU = V/aP (main.cpp:62-66).
+__global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, const double* Uprev, double alpha,
+                            double* rAU) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.nc) return;
+    const double extra = diag[i] * (1 / alpha - 1);
+    double d = diag[i] + extra;
+    for (int k = 0; k < 3; ++k) {
+        const long long j = 3 * static_cast<long long>(i) + k;
+        const double s = src[j] + extra * Uprev[j];
+        src[j] = alpha * s;
+    }
+    d *= alpha;
+    diag[i] = d;
+    for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e) off[e] *= alpha;
+    rAU[i] = m.V[i] / d;
+}
+
+// Calculated-field traces of V/aP: V_o / aP_o on boundary faces.
+__global__ void kOwnerTrace(MeshView m, const double* cell, double* trace) {
+    const int f = m.nfi + blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < m.nf) trace[f] = cell[m.own[f]];
+}
+
+// X = rAU * gradP with traces rAU_trace * gradP_trace.
+__global__ void kScaleVector(MeshView m, const double* s, const double* g, double* X, const double* str,
+                             const double* gtr, double* Xtr) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < m.nc)
+        for (int k = 0; k < 3; ++k) X[3 * i + k] = s[i] * g[3 * i + k];
+    const long long f = m.nfi + i;
+    if (f < m.nf)
+        for (int k = 0; k < 3; ++k) Xtr[3 * f + k] = str[f] * gtr[3 * f + k];
+}
+
+// Rhie-Chow on interior faces (main.cpp:71-79): phiH += (rAU gradP)_f . Sf,
+// phiH += -1 * ((rAU_f * snGrad p) * |Sf|).
+__global__ void kRhieChow(MeshView m, double* phiH, const double* Xf, const double* rAUf, const double* sn) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nfi) return;
+    double v = phiH[f] + dot(ld3(Xf, f), ld3(m.Sf, f));
+    v += -1.0 * ((rAUf[f] * sn[f]) * m.Af[f]);
+    phiH[f] = v;
+}
+
+// equ::source(pEq, -div(phiH)) (procedural_equation.cpp:192-196)
+__global__ void kScalarSource(MeshView m, const double* divPhiH, double* src) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.nc) src[i] += (1.0 * m.V[i]) * (-1.0 * divPhiH[i]);
+}
+
+// referenceIfUnanchored constant-null-mode test (procedural_equation.cpp:316-333).
+__global__ void kReferenceTest(MeshView m, const double* diag, const double* off, int* notConstant) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.nc) return;
+    double rowSum = diag[i], mag = fabs(diag[i]);
+    for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e) { rowSum += off[e]; mag += fabs(off[e]); }
+    if (!(isfinite(rowSum) && fabs(rowSum) <= 64 * 2.220446049250313e-16 * mag)) atomicOr(notConstant, 1);
+}
+// reference(global cell 0, value) (procedural_equation.cpp:306-315).
+__global__ void kReferenceApply(double* diag, double* src, int cell, double value, const int* notConstant, int* err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || notConstant[0]) return;
+    if (!(diag[cell] > 0)) { atomicOr(err, kErrNonFinite); return; }
+    src[cell] += diag[cell] * value;
+    diag[cell] *= 2;
+}
+
+__global__ void kCorrection(int nf, const double* full, const double* orth, double* corr) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nf) corr[f] = full[f] + (-1.0) * orth[f];
+}
+
+// equ::faceFlux (procedural_equation.cpp:335-352) with phi = phiH + pressureFlux (main.cpp:98).
+__global__ void kPressureFluxPhi(MeshView m, FieldArg pp, const double* kf, const double* corr, const double* phiH,
+                                 double* phi, int* err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const double k = kf[f];
+    if (!(k >= 0.0) || !isfinite(k)) atomicOr(err, kErrNegativeDiffusivity);
+    double g[1];
+    integratedNormalGradient<1>(pp, nullptr, m, f, kOrthogonal, g, err);
+    double inc = k * g[0];
+    inc += 1.0 * corr[f];
+    const double pf = 0.0 + -1.0 * inc;
+    phi[f] = phiH[f] + pf;
+}
+
+// p += alphaP p'; U -= rAU grad(p')  (main.cpp:95-97)
+__global__ void kCorrectUP(MeshView m, double* p, const double* pp, double alphaP, double* U, const double* rAU,
+                           const double* gpp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.nc) return;
+    p[i] += 1.0 * (alphaP * pp[i]);
+    for (int k = 0; k < 3; ++k) {
+        const long long j = 3 * static_cast<long long>(i) + k;
+        U[j] += -1.0 * (rAU[i] * gpp[j]);
+    }
+}
+
+// relativeChange, relativeMagnitude and fluxBalance sums (fvm_execution.cpp:91-177).
+__global__ void kConvergence(MeshView m, const double* U, const double* Up, const double* pp, const double* p,
+                             const double* phi, double* red, ReduceSlot slot) {
+    __shared__ double smem[kRedK * kBlock];
+    double a[kRedK] = {0, 0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nc; c += gridDim.x * blockDim.x) {
+        const D3 u = ld3(U, c), up = ld3(Up, c);
+        const D3 d = u - up;
+        a[0] += dot(d, d);
+        a[1] += dot(u, u);
+        a[2] += pp[c] * pp[c];
+        a[3] += p[c] * p[c];
+        double imb = 0.0, sc = 0.0;
+        for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+            const int f = m.mface[e];
+            const double out = m.own[f] == c ? phi[f] : -phi[f];
+            imb += out;
+            sc += fabs(out);
+        }
+        for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+            const double out = phi[m.bface[e]];
+            imb += out;
+            sc += fabs(out);
+        }
+        a[4] += fabs(imb);
+        a[5] += sc;
+    }
+    double out[kRedK];
+    if (reduceFinish<kRedK>(a, slot, smem, out) && threadIdx.x == 0)
+        for (int k = 0; k < kRedK; ++k) red[k] = out[k];
+}
+
+}  // namespace
+
+SimpleSolver::SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, const double* ubcv, const int* pbct,
+                           const double* pbcv)
+    : C(ctx), c(cfg) {
+    const DeviceMesh& M = *C.mesh;
+    hipStream_t s = C.stream;
+    const int nc = M.nc, nf = M.nf, np = M.np;
+    if (!(c.alphaU > 0 && c.alphaU <= 1) || !(c.alphaP > 0 && c.alphaP <= 1))
+        throw std::invalid_argument("relaxation must be in (0,1]");
+    if (!(c.rho > 0) || !(c.mu > 0)) throw std::invalid_argument("density and viscosity must be positive");
+    std::vector<int> ppt(np), zg(np, kZeroGradient);
+    std::vector<double> ppv(np, 0.0), zgv(3 * np, 0.0);
+    for (int k = 0; k < np; ++k) {
+        if (ubct[k] < 0 || ubct[k] > 4 || pbct[k] < 0 || pbct[k] > 4) throw std::invalid_argument("invalid boundary type");
+        // field.h:382-409: homogeneous correction boundaries
+        ppt[k] = pbct[k] == kFixedValue ? kFixedValue : pbct[k] == kSymmetry ? kSymmetry : kZeroGradient;
+    }
+    uType.upload(ubct, np, s); uVal.upload(ubcv, 3 * np, s);
+    pType.upload(pbct, np, s); pVal.upload(pbcv, np, s);
+    ppType.upload(ppt, s); ppVal.upload(ppv, s);
+    zgType.upload(zg, s); zgVal.upload(zgv, s);
+    for (DBuf<double>* b : {&p, &rAU, &divPhiH, &pPrime, &psrc, &bcomp, &xcomp, &mu}) b->alloc(nc);
+    for (DBuf<double>* b : {&U, &Uprev, &gradP, &gradPP, &X, &src}) b->alloc(3 * static_cast<std::size_t>(nc));
+    for (DBuf<double>* b : {&gradU, &work9, &gradX}) b->alloc(9 * static_cast<std::size_t>(nc));
+    for (DBuf<double>* b : {&phi, &phiH, &rAUtr, &rAUf, &sn, &kfMom, &kfP, &corr, &full, &ctxSeed}) b->alloc(nf);
+    for (DBuf<double>* b : {&gtr, &Xtr, &Xf}) b->alloc(3 * static_cast<std::size_t>(nf));
+    red.alloc(kRedK * kMaxReduceBlocks + 16);
+    redCounter.alloc(1);
+    redCounter.zero(s);
+    U.zero(s); p.zero(s); phi.zero(s); corr.zero(s);
+    const MeshView mv = M.view();
+    momSys = std::make_unique<DeviceSystem>(s, nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
+    pSys = std::make_unique<DeviceSystem>(s, nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
+    refCell = C.cellOldToNew.at(0);
+    // muEffective: constant cell field with default (zeroGradient) patches
+    // (src/physics/RANS/model.cpp:58-63), interpolated with the momentum
+    // diffusion coefficient scheme once: it never changes for laminar flow.
+    kFillD<<<blocksFor(nc), kBlock, 0, s>>>(nc, mu.p, c.mu);
+    FieldArg muF{mu.p, zgType.p, zgVal.p, nullptr, nullptr};
+    if (c.mCoefInterp == kCorrected) {
+        ops::gradient(mv, M.orthogonal, 1, muF, c.mCoefGrad, gradP.p, work9.p, C.err.p, s);
+        ops::interpolate(mv, 1, muF, kCorrected, gradP.p, kfMom.p, C.err.p, s);
+    } else {
+        ops::interpolate(mv, 1, muF, kLinear, nullptr, kfMom.p, C.err.p, s);
+    }
+    C.checkErrors();
+}
+
+FieldArg SimpleSolver::fieldU(const double* ctx) const { return {U.p, uType.p, uVal.p, ctx, nullptr}; }
+FieldArg SimpleSolver::fieldP() const { return {p.p, pType.p, pVal.p, nullptr, nullptr}; }
+FieldArg SimpleSolver::fieldPP() const { return {pPrime.p, ppType.p, ppVal.p, nullptr, nullptr}; }
+FieldArg SimpleSolver::fieldRAU() const { return {rAU.p, zgType.p, zgVal.p, nullptr, rAUtr.p}; }
+
+void SimpleSolver::setFields(const double* Uc, const double* pc) {
+    hipStream_t s = C.stream;
+    if (Uc) { auto v = C.cellsIn(Uc, 3); U.upload(v, s); }
+    if (pc) { auto v = C.cellsIn(pc, 1); p.upload(v, s); }
+    BSIM_HIP(hipStreamSynchronize(s));
+    started = false;
+}
+
+void SimpleSolver::getFields(double* Uc, double* pc, double* phic) {
+    hipStream_t s = C.stream;
+    if (Uc) C.cellsOut(U.toHost(s), Uc, 3);
+    if (pc) C.cellsOut(p.toHost(s), pc, 1);
+    if (phic) C.facesOut(phi.toHost(s), phic, 1);
+}
+
+double SimpleSolver::sync() {
+    BSIM_HIP(hipStreamSynchronize(C.stream));
+    return 0.0;
+}
+
+void SimpleSolver::start() {
+    // main.cpp:41: phi = flux(U); the first use seeds inletOutlet directions
+    // from owner velocities (operators.cpp:977-987).
+    const DeviceMesh& M = *C.mesh;
+    const MeshView mv = M.view();
+    hipStream_t s = C.stream;
+    ops::seedFluxContext(mv, U.p, ctxSeed.p, s);
+    const double* g = nullptr;
+    if (c.mInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, fieldU(ctxSeed.p), c.mGrad, gradU.p, work9.p, C.err.p, s); g = gradU.p; }
+    ops::flux(mv, fieldU(ctxSeed.p), g, phi.p, C.err.p, s);
+    C.checkErrors();
+    started = true;
+}
+
+IterRep SimpleSolver::iterate() {
+    if (!started) start();
+    const DeviceMesh& M = *C.mesh;
+    const MeshView mv = M.view();
+    hipStream_t s = C.stream;
+    const int nc = M.nc, nf = M.nf;
+    const int nb = blocksFor(nc);
+    const ReduceSlot slot{red.p + 16, redCounter.p};
+    IterRep rep;
+    auto t0 = Clock::now();
+    // U_previous = U (main.cpp:48)
+    kCopyD<<<blocksFor(3LL * nc), kBlock, 0, s>>>(3LL * nc, U.p, Uprev.p);
+    // Momentum assembly (main.cpp:51-57): convection with U's flux context = phi.
+    double* diag = momSys->candDiag();
+    double* off = momSys->candVal();
+    kFillD<<<nb, kBlock, 0, s>>>(nc, diag, 0.0);
+    kFillD<<<blocksFor(std::max(M.nnzOff, 1)), kBlock, 0, s>>>(M.nnzOff, off, 0.0);
+    kFillD<<<blocksFor(3LL * nc), kBlock, 0, s>>>(3LL * nc, src.p, 0.0);
+    const FieldArg Uf = fieldU(phi.p);
+    const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
+                           c.mDiff != kOrthogonal;
+    if (needGradU) ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s);
+    ops::assembleConvection(mv, 3, Uf, phi.p, c.mConv, c.mInterp, gradU.p, c.rho, diag, off, src.p, C.err.p, s);
+    ops::assembleDiffusion(mv, 3, Uf, kfMom.p, gradU.p, c.mDiff, -(-1.0), diag, off, src.p, C.err.p, s);
+    ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
+    kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
+    // rU before relaxation (main.cpp:60)
+    kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, red.p, slot);
+    kRelaxScale<<<nb, kBlock, 0, s>>>(mv, diag, off, src.p, Uprev.p, c.alphaU, rAU.p);
+    if (nf > M.nfi) kOwnerTrace<<<blocksFor(nf - M.nfi), kBlock, 0, s>>>(mv, rAU.p, rAUtr.p);
+    BSIM_LAUNCH_CHECK();
+    double hr[3];
+    BSIM_HIP(hipMemcpyAsync(hr, red.p, sizeof(hr), hipMemcpyDeviceToHost, s));
+    C.checkErrors();
+    rep.rU = std::sqrt(hr[0]) / std::max(std::sqrt(hr[1]) + std::sqrt(hr[2]), 1e-30);
+    timing.assembly += since(t0);
+    t0 = Clock::now();
+    // Momentum solve: one matrix, three right-hand sides (petsc_backend.cpp:156-198).
+    momSys->update();
+    momSys->prepare(c.mom);
+    rep.momStatus = kConverged;
+    for (int comp = 0; comp < 3; ++comp) {
+        kExtract<<<nb, kBlock, 0, s>>>(nc, src.p, comp, bcomp.p);
+        kExtract<<<nb, kBlock, 0, s>>>(nc, U.p, comp, xcomp.p);
+        const SolveRes r = momSys->solve(bcomp.p, xcomp.p, comp);
+        kInsert<<<nb, kBlock, 0, s>>>(nc, xcomp.p, comp, U.p);
+        momHist[comp] = momSys->hist;
+        rep.momIters[comp] = r.iterations;
+        if (r.status == kNumericalFailure) rep.momStatus = kNumericalFailure;
+        else if (r.status != kConverged && rep.momStatus != kNumericalFailure) rep.momStatus = kMaxIterations;
+    }
+    BSIM_LAUNCH_CHECK();
+    timing.momentumSolve += since(t0);
+    t0 = Clock::now();
+    // Rhie-Chow (main.cpp:71-79).  grad(p) is unchanged since assembly.
+    const double* gU = nullptr;
+    if (c.mInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s); gU = gradU.p; }
+    ops::flux(mv, Uf, gU, phiH.p, C.err.p, s);
+    ops::gradientTraces(mv, fieldP(), gradP.p, gtr.p, s);
+    kScaleVector<<<blocksFor(std::max(nc, nf - M.nfi)), kBlock, 0, s>>>(mv, rAU.p, gradP.p, X.p, rAUtr.p, gtr.p, Xtr.p);
+    {
+        const FieldArg Xa{X.p, zgType.p, zgVal.p, nullptr, Xtr.p};
+        const double* gX = nullptr;
+        if (c.pInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, Xa, c.pGrad, gradX.p, work9.p, C.err.p, s); gX = gradX.p; }
+        ops::interpolate(mv, 3, Xa, c.pInterp, gX, Xf.p, C.err.p, s);
+    }
+    {
+        const double* gR = nullptr;
+        if (c.pCoefInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 1, fieldRAU(), c.pCoefGrad, gradPP.p, work9.p, C.err.p, s); gR = gradPP.p; }
+        ops::interpolate(mv, 1, fieldRAU(), c.pCoefInterp, gR, rAUf.p, C.err.p, s);
+    }
+    ops::normalGradient(mv, fieldP(), gradP.p, c.pDiff, sn.p, C.err.p, s);
+    kRhieChow<<<blocksFor(std::max(M.nfi, 1)), kBlock, 0, s>>>(mv, phiH.p, Xf.p, rAUf.p, sn.p);
+    ops::divergenceFace(mv, phiH.p, divPhiH.p, s);
+    BSIM_LAUNCH_CHECK();
+    timing.rhieChow += since(t0);
+    // Pressure correction (main.cpp:81-93).
+    kFillD<<<nb, kBlock, 0, s>>>(nc, pPrime.p, 0.0);
+    const int solves = c.pDiff == kOrthogonal ? 1 : c.nonOrthCorr + 1;
+    bool pConverged = true;
+    rep.pStatus = kConverged;
+    // Coefficient interpolation of rAU (procedural_equation.cpp:149-176).
+    {
+        const double* gR = nullptr;
+        if (c.pCoefInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 1, fieldRAU(), c.pCoefGrad, gradPP.p, work9.p, C.err.p, s); gR = gradPP.p; }
+        ops::interpolate(mv, 1, fieldRAU(), c.pCoefInterp, gR, kfP.p, C.err.p, s);
+    }
+    DBuf<int>& refFlag = C.err;  // reuse: bit layout below uses a dedicated word
+    (void)refFlag;
+    for (int k = 0; k < solves; ++k) {
+        t0 = Clock::now();
+        double* pd = pSys->candDiag();
+        double* po = pSys->candVal();
+        kFillD<<<nb, kBlock, 0, s>>>(nc, pd, 0.0);
+        kFillD<<<blocksFor(std::max(M.nnzOff, 1)), kBlock, 0, s>>>(M.nnzOff, po, 0.0);
+        kFillD<<<nb, kBlock, 0, s>>>(nc, psrc.p, 0.0);
+        if (c.pDiff != kOrthogonal) {
+            ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradPP.p, work9.p, C.err.p, s);
+            ops::assembleDiffusion(mv, 1, fieldPP(), kfP.p, gradPP.p, c.pDiff, -(-1.0), pd, po, psrc.p, C.err.p, s);
+            // diffusionCorrection (procedural_equation.cpp:77-87)
+            ops::diffusionFlux(mv, fieldPP(), kfP.p, gradPP.p, c.pDiff, full.p, C.err.p, s);
+            ops::diffusionFlux(mv, fieldPP(), kfP.p, gradPP.p, kOrthogonal, sn.p, C.err.p, s);
+            kCorrection<<<blocksFor(nf), kBlock, 0, s>>>(nf, full.p, sn.p, corr.p);
+        } else {
+            ops::assembleDiffusion(mv, 1, fieldPP(), kfP.p, nullptr, kOrthogonal, -(-1.0), pd, po, psrc.p, C.err.p, s);
+            kFillD<<<blocksFor(nf), kBlock, 0, s>>>(nf, corr.p, 0.0);
+        }
+        kScalarSource<<<nb, kBlock, 0, s>>>(mv, divPhiH.p, psrc.p);
+        int* notConst = reinterpret_cast<int*>(red.p + 8);
+        BSIM_HIP(hipMemsetAsync(notConst, 0, sizeof(int), s));
+        kReferenceTest<<<nb, kBlock, 0, s>>>(mv, pd, po, notConst);
+        kReferenceApply<<<1, 64, 0, s>>>(pd, psrc.p, refCell, 0.0, notConst, C.err.p);
+        BSIM_LAUNCH_CHECK();
+        int nconst = 0;
+        BSIM_HIP(hipMemcpyAsync(&nconst, notConst, sizeof(int), hipMemcpyDeviceToHost, s));
+        C.checkErrors();
+        rep.referenced = nconst == 0;
+        timing.pressureAssembly += since(t0);
+        t0 = Clock::now();
+        pSys->update();
+        pSys->prepare(c.pre);
+        const SolveRes r = pSys->solve(psrc.p, pPrime.p, 0);
+        pHist = pSys->hist;
+        rep.pIters += r.iterations;
+        if (r.status == kNumericalFailure) rep.pStatus = kNumericalFailure;
+        pConverged = pConverged && r.status == kConverged;
+        rep.linP = r.relative;
+        timing.pressureSolve += since(t0);
+    }
+    if (!pConverged && rep.pStatus != kNumericalFailure) rep.pStatus = kMaxIterations;
+    t0 = Clock::now();
+    // Corrections (main.cpp:95-98).
+    kPressureFluxPhi<<<blocksFor(nf), kBlock, 0, s>>>(mv, fieldPP(), kfP.p, corr.p, phiH.p, phi.p, C.err.p);
+    ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradPP.p, work9.p, C.err.p, s);
+    kCorrectUP<<<nb, kBlock, 0, s>>>(mv, p.p, pPrime.p, c.alphaP, U.p, rAU.p, gradPP.p);
+    BSIM_LAUNCH_CHECK();
+    timing.correction += since(t0);
+    t0 = Clock::now();
+    // Convergence measures (main.cpp:112-126).
+    kConvergence<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uprev.p, pPrime.p, p.p, phi.p, red.p, slot);
+    BSIM_LAUNCH_CHECK();
+    double h[kRedK];
+    BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    C.checkErrors();
+    rep.dU = std::sqrt(h[0]) / std::max(std::sqrt(h[1]), 1e-30);
+    rep.dP = std::sqrt(h[2]) / std::max(std::sqrt(h[3]), 1e-30);
+    rep.mass = h[4] / std::max(h[5], 1e-30);
+    const bool linearConverged = rep.momStatus == kConverged && pConverged;
+    rep.converged = linearConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP && rep.mass <= c.tolMass;
+    timing.convergence += since(t0);
+    ++timing.iterations;
+    return rep;
+}
+
+}  // namespace bsim

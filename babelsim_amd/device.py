@@ -1,0 +1,256 @@
+"""Python handles over the device library: a case context (mesh in HBM), its
+face operators, persistent linear systems and the SIMPLE driver.
+
+The method signatures mirror the operator set of BabelSim's
+``include/babelsim/operators.h`` (one field at a time, boundary conditions per
+patch), so a test can run the same call against the CPU oracle and the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._capi import (IterReport, MeshData, SimpleConfig, SolveResult, SolverConfig, check, dp, ip, lib, ptr)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+class Context:
+    """One device, one stream, the mesh resident in HBM."""
+
+    def __init__(self, mesh: MeshData | dict, renumber: bool = True, device: int = 0):
+        if isinstance(mesh, dict):
+            mesh = MeshData.from_arrays(mesh)
+        self._mesh = mesh
+        h = C.c_void_p()
+        check(lib().bsim_ctx_create(mesh.handle, int(renumber), int(device), C.byref(h)))
+        self.h = h
+        vals = [C.c_int() for _ in range(5)]
+        check(lib().bsim_ctx_sizes(self.h, *[C.byref(v) for v in vals]))
+        self.nc, self.nf, self.nfi, self.nnz_off, orth = (v.value for v in vals)
+        self.orthogonal = bool(orth)
+        self.npatch = mesh.sizes()[3]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bsim_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geometry(self) -> dict:
+        nc, nf = self.nc, self.nf
+        g = dict(C=np.zeros(3 * nc), V=np.zeros(nc), Cf=np.zeros(3 * nf), Sf=np.zeros(3 * nf), Af=np.zeros(nf),
+                 n=np.zeros(3 * nf), w=np.zeros(nf), orth=np.zeros(nf), k=np.zeros(3 * nf), skew=np.zeros(3 * nf))
+        check(lib().bsim_ctx_geometry(self.h, ptr(g["C"]), ptr(g["V"]), ptr(g["Cf"]), ptr(g["Sf"]), ptr(g["Af"]),
+                                      ptr(g["n"]), ptr(g["w"]), ptr(g["orth"]), ptr(g["k"]), ptr(g["skew"])))
+        for key in ("C", "Cf", "Sf", "n", "k", "skew"):
+            g[key] = g[key].reshape(-1, 3)
+        return g
+
+    # ---- operators -----------------------------------------------------
+    def gradient(self, x, bct, bcv, method, ctx=None):
+        x = _f64(x)
+        ncomp = 1 if x.ndim == 1 else 3
+        out = np.zeros(self.nc * 3 * ncomp)
+        c = None if ctx is None else _f64(ctx)
+        check(lib().bsim_op_gradient(self.h, ncomp, ptr(x), ptr(_i32(bct), ip), ptr(_f64(bcv).reshape(-1)), ptr(c),
+                                     method, ptr(out)))
+        return out.reshape(self.nc, 3) if ncomp == 1 else out.reshape(self.nc, 3, 3)
+
+    def interpolate(self, x, bct, bcv, method, gmethod, ctx=None):
+        x = _f64(x)
+        ncomp = 1 if x.ndim == 1 else 3
+        out = np.zeros(self.nf * ncomp)
+        c = None if ctx is None else _f64(ctx)
+        check(lib().bsim_op_interpolate(self.h, ncomp, ptr(x), ptr(_i32(bct), ip), ptr(_f64(bcv).reshape(-1)),
+                                        ptr(c), method, gmethod, ptr(out)))
+        return out if ncomp == 1 else out.reshape(self.nf, 3)
+
+    def flux(self, U, bct, bcv, method, gmethod, ctx=None):
+        out = np.zeros(self.nf)
+        c = None if ctx is None else _f64(ctx)
+        check(lib().bsim_op_flux(self.h, ptr(_f64(U)), ptr(_i32(bct), ip), ptr(_f64(bcv).reshape(-1)), ptr(c), method,
+                                 gmethod, ptr(out)))
+        return out
+
+    def normal_gradient(self, x, bct, bcv, gmethod, dmethod):
+        out = np.zeros(self.nf)
+        check(lib().bsim_op_normal_gradient(self.h, ptr(_f64(x)), ptr(_i32(bct), ip), ptr(_f64(bcv).reshape(-1)),
+                                            gmethod, dmethod, ptr(out)))
+        return out
+
+    def divergence_face(self, phi):
+        out = np.zeros(self.nc)
+        check(lib().bsim_op_divergence_face(self.h, ptr(_f64(phi)), ptr(out)))
+        return out
+
+    def explicit_laplacian(self, x, bct, bcv, kf, k, gmethod, dmethod):
+        out = np.zeros(self.nc)
+        kfa = None if kf is None else _f64(kf)
+        check(lib().bsim_op_explicit_laplacian(self.h, ptr(_f64(x)), ptr(_i32(bct), ip), ptr(_f64(bcv).reshape(-1)),
+                                               ptr(kfa), float(k), gmethod, dmethod, ptr(out)))
+        return out
+
+    def explicit_convection(self, x, bct, bcv, phi, conv, interp, gmethod):
+        x = _f64(x)
+        ncomp = 1 if x.ndim == 1 else 3
+        out = np.zeros(self.nc * ncomp)
+        check(lib().bsim_op_explicit_convection(self.h, ncomp, ptr(x), ptr(_i32(bct), ip),
+                                                ptr(_f64(bcv).reshape(-1)), ptr(_f64(phi)), conv, interp, gmethod,
+                                                ptr(out)))
+        return out if ncomp == 1 else out.reshape(self.nc, 3)
+
+    def assemble(self, kind, x, bct, bcv, phi=None, kf=None, conv=1, interp=0, gmethod=0, dmethod=0, scale=1.0,
+                 ctx=None):
+        x = _f64(x)
+        ncomp = 1 if x.ndim == 1 else 3
+        diag, up, lo, src = np.zeros(self.nc), np.zeros(self.nf), np.zeros(self.nf), np.zeros(self.nc * ncomp)
+        if kind == "convection":
+            check(lib().bsim_op_assemble_convection(self.h, ncomp, ptr(x), ptr(_i32(bct), ip),
+                                                    ptr(_f64(bcv).reshape(-1)), ptr(_f64(phi)), conv, interp, gmethod,
+                                                    float(scale), ptr(diag), ptr(up), ptr(lo), ptr(src)))
+        else:
+            c = None if ctx is None else _f64(ctx)
+            check(lib().bsim_op_assemble_diffusion(self.h, ncomp, ptr(x), ptr(_i32(bct), ip),
+                                                   ptr(_f64(bcv).reshape(-1)), ptr(c), ptr(_f64(kf)), gmethod,
+                                                   dmethod, ptr(diag), ptr(up), ptr(lo), ptr(src)))
+        return dict(diag=diag, upper=up, lower=lo, source=src if ncomp == 1 else src.reshape(self.nc, 3))
+
+
+class System:
+    """A persistent linear system (ComputeBackend::solve for one equation)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().bsim_system_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bsim_system_destroy(self.h)
+            self.h = None
+
+    def solve(self, diag, upper, lower, source, x, cfg: SolverConfig):
+        source = _f64(source)
+        nrhs = 1 if source.ndim == 1 else 3
+        xx = _f64(x).copy()
+        res = (SolveResult * nrhs)()
+        check(lib().bsim_system_solve_ldu(self.h, ptr(_f64(diag)), ptr(_f64(upper)), ptr(_f64(lower)), nrhs,
+                                          ptr(source), ptr(xx), C.byref(cfg), res))
+        return xx, list(res)
+
+    def history(self):
+        n = C.c_int()
+        check(lib().bsim_system_history(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        check(lib().bsim_system_history(self.h, ptr(out), n.value, C.byref(n)))
+        return out
+
+    def counters(self) -> dict:
+        out = (C.c_uint64 * 10)()
+        check(lib().bsim_system_counters(self.h, out))
+        keys = ["matrix_updates", "rhs_only", "pc_setups", "pc_reuses", "retries", "solves", "krylov_iterations",
+                "true_residual_checks", "spmvs", "pc_applications"]
+        return dict(zip(keys, list(out)))
+
+
+def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, device=0, hist_cap=100000):
+    n = len(diag)
+    x = _f64(x0).copy()
+    res = SolveResult()
+    hist = np.zeros(hist_cap)
+    ln = C.c_int()
+    check(lib().bsim_csr_solve(device, n, ptr(_i32(ptr_), ip), ptr(_i32(col), ip), ptr(_f64(diag)), ptr(_f64(val)),
+                               ptr(_f64(b)), ptr(x), C.byref(cfg), C.byref(res), ptr(hist), hist_cap, C.byref(ln)))
+    return x, res, hist[:ln.value].copy()
+
+
+def csr_spmv(ptr_, col, diag, val, x, device=0):
+    y = np.zeros(len(diag))
+    check(lib().bsim_csr_spmv(device, len(diag), ptr(_i32(ptr_), ip), ptr(_i32(col), ip), ptr(_f64(diag)),
+                              ptr(_f64(val)), ptr(_f64(x)), ptr(y)))
+    return y
+
+
+def amg_hierarchy(ptr_, col, diag, val, cfg: SolverConfig, device=0):
+    n = len(diag)
+    nl = C.c_int()
+    sizes = np.zeros(64, np.int32)
+    agg0 = np.full(n, -1, np.int32)
+    cd1 = np.zeros(n)
+    check(lib().bsim_amg_hierarchy(device, n, ptr(_i32(ptr_), ip), ptr(_i32(col), ip), ptr(_f64(diag)),
+                                   ptr(_f64(val)), C.byref(cfg), C.byref(nl), ptr(sizes, ip), ptr(agg0, ip), ptr(cd1)))
+    sizes = sizes[:nl.value]
+    return sizes, agg0, cd1[:sizes[1]] if len(sizes) > 1 else None
+
+
+class Simple:
+    """Steady SIMPLE resident on the GPU (src/physics/simple/main.cpp)."""
+
+    def __init__(self, ctx: Context, cfg: SimpleConfig, u_bc, p_bc):
+        self.ctx = ctx
+        self.cfg = cfg
+        ut, uv = u_bc
+        pt, pv = p_bc
+        h = C.c_void_p()
+        check(lib().bsim_simple_create(ctx.h, C.byref(cfg), ptr(_i32(ut), ip), ptr(_f64(uv).reshape(-1)),
+                                       ptr(_i32(pt), ip), ptr(_f64(pv).reshape(-1)), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bsim_simple_destroy(self.h)
+            self.h = None
+
+    def set_fields(self, U=None, p=None):
+        check(lib().bsim_simple_set_fields(self.h, ptr(None if U is None else _f64(U)),
+                                           ptr(None if p is None else _f64(p))))
+
+    def start(self):
+        check(lib().bsim_simple_start(self.h))
+
+    def iterate(self, n=1):
+        reps = (IterReport * n)()
+        check(lib().bsim_simple_iterate(self.h, n, reps))
+        return list(reps)
+
+    def fields(self):
+        U = np.zeros((self.ctx.nc, 3))
+        p = np.zeros(self.ctx.nc)
+        phi = np.zeros(self.ctx.nf)
+        check(lib().bsim_simple_get_fields(self.h, ptr(U), ptr(p), ptr(phi)))
+        return U, p, phi
+
+    def history(self, which):
+        n = C.c_int()
+        check(lib().bsim_simple_history(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        check(lib().bsim_simple_history(self.h, which, ptr(out), n.value, C.byref(n)))
+        return out
+
+    def timing(self) -> dict:
+        out = np.zeros(8)
+        check(lib().bsim_simple_timing(self.h, ptr(out)))
+        keys = ["assembly", "momentum_solve", "rhie_chow", "pressure_assembly", "pressure_solve", "correction",
+                "convergence", "iterations"]
+        return dict(zip(keys, out.tolist()))
+
+    def time_spmv(self, which=1, reps=20):
+        s = C.c_double()
+        n = C.c_int()
+        nnz = C.c_int()
+        check(lib().bsim_simple_time_spmv(self.h, which, reps, C.byref(s), C.byref(n), C.byref(nnz)))
+        return s.value, n.value, nnz.value

@@ -1,0 +1,137 @@
+#!/usr/bin/env python
+"""SIMPLE outer-iteration throughput of the MI355X-native hot path.
+
+Workload: the 3-D lid-driven cavity at Re=400 on a 126^3 hex lattice with
+interior vertices jittered by +-20% of the spacing and ~10% of the hexes cut
+into prism pairs (about 2.2M polyhedral cells), laminar steady SIMPLE with
+least-squares gradients, corrected diffusion, linearUpwind convection;
+momentum by BiCGSTAB+Jacobi, pressure correction by CG with the aggregation
+AMG, FP64.  One step is one SIMPLE outer iteration with every field, matrix
+and preconditioner resident in HBM.
+
+Prints one JSON line (rank 0, one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=126, help="hex lattice cells per side")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-iters", type=int, default=1, help="timed oracle iterations for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rebuild-interval", type=int, default=8)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+    import cases
+    from babelsim_amd import solver_config
+    from babelsim_amd.device import Context, Simple
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    n = args.n
+    t0 = time.time()
+    mesh = cases.cavity3d_mesh(n, jitter=0.2, split=0.1, seed=20261203)
+    nv, nf_in, _, _, nc_in = mesh.sizes()
+    log(f"mesh generated: {nc_in} cells, {nf_in} faces ({time.time() - t0:.1f}s)")
+    prec = 1 if args.precision == "mixed" else 0
+    cfg = cases.c2_config(
+        mom=solver_config(solver=1, pc=1, rtol=1e-6, atol=1e-14, precision=prec,
+                          rebuild_interval=args.rebuild_interval),
+        pre=solver_config(solver=0, pc=2, rtol=1e-6, atol=1e-14, precision=prec,
+                          rebuild_interval=args.rebuild_interval))
+    ub, pb = cases.cavity_bcs(two_d=False)
+    ctx = Context(mesh, renumber=True)
+    log(f"context ready: N={ctx.nc} F={ctx.nf} nnz_off={ctx.nnz_off} orthogonal={ctx.orthogonal} "
+        f"({time.time() - t0:.1f}s)")
+    sim = Simple(ctx, cfg, ub, pb)
+    sim.start()
+    reps = []
+    for k in range(args.warmup):
+        reps += sim.iterate(1)
+        r = reps[-1]
+        log(f"warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters}")
+    t1 = time.time()
+    timed = sim.iterate(args.steps)
+    t2 = time.time()
+    sec = t2 - t1
+    its_per_s = args.steps / sec
+    last = timed[-1]
+    log(f"timed {args.steps} its in {sec:.3f}s -> {its_per_s:.3f} it/s; last rU={last.rU:.3e} "
+        f"mom_its={list(last.mom_iters)} p_its={last.p_iters} mass={last.mass:.2e}")
+    timing = sim.timing()
+    # Roofline of the pressure-solve SpMV (CSR, FP64): 12*nnz + 4*(N+1) + 8*N + 8*N bytes per launch.
+    spmv_s, N, nnz_off = sim.time_spmv(1, reps=50)
+    nnz = N + nnz_off
+    spmv_bytes = 12 * nnz + 4 * (N + 1) + 16 * N
+    achieved = spmv_bytes / spmv_s / 1e9
+    log(f"spmv: {spmv_s * 1e6:.1f} us per launch, {achieved:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu and args.cpu_iters > 0:
+        import oracle_lib as orc
+        arrays = mesh.arrays()
+        t3 = time.time()
+        om = orc.oracle_mesh(arrays)
+        so = orc.OracleSimple(om, cfg, ub, pb)
+        so.start()
+        so.iterate(1)  # the first outer iteration is outside the timing window (runtime.cpp:117-130)
+        t4 = time.time()
+        so.iterate(args.cpu_iters)
+        t5 = time.time()
+        cpu = {"value": args.cpu_iters / (t5 - t4), "unit": "outer_iterations/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} timed FP64 oracle SIMPLE outer iteration(s) on the same "
+                         f"{nc_in}-cell mesh after one untimed iteration (setup {t4 - t3:.1f}s excluded)"}
+        log(f"cpu baseline: {cpu['value']:.4f} it/s")
+    line = {
+        "metric": "SIMPLE outer iterations/s at 2M polyhedral cells (SpMV HBM GB/s in roofline)",
+        "value": its_per_s,
+        "unit": "outer_iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"3-D lid-driven cavity Re=400, {n}^3 hex lattice, +-20% jitter, ~10% prism splits",
+                   "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
+                   "pressure": "CG + aggregation AMG", "momentum": "BiCGSTAB + Jacobi",
+                   "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
+                   "phase_seconds": timing},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "pressure CSR SpMV (FP64)", "bytes_per_launch": spmv_bytes,
+                     "seconds_per_launch": spmv_s},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -73,6 +73,109 @@ typedef struct {
     double initial, final_residual, relative;
 } bsim_solve_result;
 
+/* ------------------------------------------------------------- contexts
+ * One device, one stream, the mesh resident in HBM.  Replaces
+ * detail::makeComputeBackend(const Mesh&, ParallelContext)
+ * (src/internal/compute_backend.h:39-41) together with the geometry of
+ * Mesh::polyhedral (src/core/mesh.cpp:321-578), computed on the GPU. */
+typedef struct bsim_ctx bsim_ctx;
+int bsim_ctx_create(const bsim_mesh* mesh, int renumber, int device, bsim_ctx** out);
+void bsim_ctx_destroy(bsim_ctx* ctx);
+int bsim_ctx_sizes(bsim_ctx* ctx, int* ncells, int* nfaces, int* ninterior, int* nnz_off, int* orthogonal);
+/* Mesh::Storage arrays (include/babelsim/mesh.h:138-177) in caller numbering. */
+int bsim_ctx_geometry(bsim_ctx* ctx, double* centres, double* volumes, double* face_centres, double* sf,
+                      double* areas, double* normals, double* owner_weights, double* orth, double* non_orth,
+                      double* skew);
+
+/* ------------------------------------------------------------ operators
+ * Host-array forms of the face operators (include/babelsim/operators.h:9-186),
+ * one field at a time: values (ncomp per cell), per-patch boundary type and
+ * value (BoundaryCondition, field.h:34-55) and an optional face flux context
+ * for inletOutlet (Field::setBoundaryFlux, field.h:277-282). */
+int bsim_op_gradient(bsim_ctx* ctx, int ncomp, const double* x, const int* bc_type, const double* bc_value,
+                     const double* flux_ctx, int method, double* out);                 /* operators.h gradient */
+int bsim_op_interpolate(bsim_ctx* ctx, int ncomp, const double* x, const int* bc_type, const double* bc_value,
+                        const double* flux_ctx, int method, int gmethod, double* out);  /* operators.h interpolate */
+int bsim_op_flux(bsim_ctx* ctx, const double* U, const int* bc_type, const double* bc_value, const double* flux_ctx,
+                 int method, int gmethod, double* out);                                 /* operators.h flux */
+int bsim_op_normal_gradient(bsim_ctx* ctx, const double* x, const int* bc_type, const double* bc_value,
+                            int gmethod, int dmethod, double* out);  /* fvm_execution.cpp:522-543 */
+int bsim_op_divergence_face(bsim_ctx* ctx, const double* phi, double* out);             /* operators.h divergence */
+int bsim_op_explicit_laplacian(bsim_ctx* ctx, const double* x, const int* bc_type, const double* bc_value,
+                               const double* kf, double k, int gmethod, int dmethod, double* out); /* laplacian */
+int bsim_op_explicit_convection(bsim_ctx* ctx, int ncomp, const double* x, const int* bc_type, const double* bc_value,
+                                const double* phi, int conv, int interp, int gmethod, double* out); /* convection */
+/* Implicit terms into an LDU system (operators.h addConvection/addDiffusion);
+ * the outputs are accumulated into the caller's diag/upper/lower/source. */
+int bsim_op_assemble_convection(bsim_ctx* ctx, int ncomp, const double* x, const int* bc_type, const double* bc_value,
+                                const double* phi, int conv, int interp, int gmethod, double scale, double* diag,
+                                double* upper, double* lower, double* source);
+int bsim_op_assemble_diffusion(bsim_ctx* ctx, int ncomp, const double* x, const int* bc_type, const double* bc_value,
+                               const double* flux_ctx, const double* kf, int gmethod, int dmethod, double* diag,
+                               double* upper, double* lower, double* source);
+
+/* ------------------------------------------------------- linear systems
+ * One persistent system per equation identity: ComputeBackend::solve for a
+ * scalar (nrhs = 1) or vector (nrhs = 3, sources and unknowns interleaved per
+ * cell like Vec3) DiscreteEquation (compute_backend.h:30-31,
+ * src/backend/petsc_backend.cpp:123-198). */
+typedef struct bsim_system bsim_system;
+int bsim_system_create(bsim_ctx* ctx, bsim_system** out);
+int bsim_system_solve_ldu(bsim_system* sys, const double* diag, const double* upper, const double* lower, int nrhs,
+                          const double* source, double* x, const bsim_solver_config* cfg, bsim_solve_result* results);
+/* Residual norm history of the last right-hand side solved. */
+int bsim_system_history(bsim_system* sys, double* out, int cap, int* len);
+/* PerformanceCounters subset (solver_control.h:36-163): matrix updates,
+ * rhs-only solves, PC setups, PC reuses, retries, solves, Krylov iterations,
+ * true-residual checks, SpMVs, PC applications. */
+int bsim_system_counters(bsim_system* sys, uint64_t* out10);
+void bsim_system_destroy(bsim_system* sys);
+
+/* Stand-alone CSR solve / SpMV / AMG hierarchy (diagonal stored apart from the
+ * off-diagonal entries); the minimum slice of the seam above. */
+int bsim_csr_solve(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                   const double* b, double* x, const bsim_solver_config* cfg, bsim_solve_result* res, double* hist,
+                   int cap, int* len);
+int bsim_csr_spmv(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                  const double* x, double* y);
+int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
+                       const bsim_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
+
+/* ----------------------------------------------------------------- SIMPLE
+ * Steady SIMPLE (src/physics/simple/main.cpp:11-131) resident on the GPU. */
+typedef struct {
+    double rho, mu;
+    int m_interp, m_grad, m_conv, m_diff, m_coef_interp, m_coef_grad;
+    int p_interp, p_grad, p_diff, p_coef_interp, p_coef_grad;
+    double alpha_u, alpha_p;
+    int non_orth_corr;
+    double tol_mass, tol_u, tol_mom, tol_p;
+    bsim_solver_config mom, pre;
+} bsim_simple_config;
+
+typedef struct {
+    double rU, dU, dP, mass, linP;
+    int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
+} bsim_iter_report;
+
+typedef struct bsim_simple bsim_simple;
+int bsim_simple_create(bsim_ctx* ctx, const bsim_simple_config* cfg, const int* u_bc_type, const double* u_bc_value,
+                       const int* p_bc_type, const double* p_bc_value, bsim_simple** out);
+int bsim_simple_set_fields(bsim_simple* s, const double* U, const double* p);
+int bsim_simple_start(bsim_simple* s);
+int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports);
+int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi);
+/* which: 0..2 momentum components, 3 pressure correction (last solve). */
+int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* len);
+/* Seconds per phase: assembly, momentum solve, Rhie-Chow, pressure assembly,
+ * pressure solve, correction, convergence, iterations. */
+int bsim_simple_timing(bsim_simple* s, double* out8);
+/* Times `reps` launches of the solver's fused SpMV on the current pressure
+ * matrix with HIP events on the context stream; returns mean seconds and the
+ * matrix size. */
+int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, int* n, int* nnz_off);
+void bsim_simple_destroy(bsim_simple* s);
+
 #ifdef __cplusplus
 }
 #endif

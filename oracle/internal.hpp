@@ -219,6 +219,7 @@ struct Amg {
     std::vector<Index> coarsePiv;
     SolverConfig cfg;
     std::vector<std::vector<double>> wx, wb, wr;
+    bool aggregated = false;  // aggregates are built at the first setup and kept
     void setup(const Csr& A, const SolverConfig& c);
     void apply(const double* r, double* z);
     void vcycle(int l);

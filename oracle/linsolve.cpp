@@ -64,10 +64,36 @@ void spmv(const Csr& A, const double* x, double* y) {
 
 namespace {
 
+// Inner products are summed in a fixed blocked order: 256 strided partial sums
+// per block over at most 1024 blocks, pairwise trees inside each block, then
+// the same over the block sums.  Both sides of the parity check use this order
+// (it is the deterministic reduction of the device library), so FP64 Krylov
+// histories can be compared bit for bit.
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
-    double s = 0.0;
-    for (std::size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
-    return s;
+    constexpr long B = 256, Gmax = 1024;
+    const long n = static_cast<long>(a.size());
+    long G = (n + B - 1) / B;
+    G = G < 1 ? 1 : (G > Gmax ? Gmax : G);
+    std::vector<double> partial(G);
+    double acc[B];
+    for (long g = 0; g < G; ++g) {
+        for (long t = 0; t < B; ++t) {
+            double s = 0.0;
+            for (long i = g * B + t; i < n; i += G * B) s += a[i] * b[i];
+            acc[t] = s;
+        }
+        for (long s = B / 2; s > 0; s >>= 1)
+            for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
+        partial[g] = acc[0];
+    }
+    for (long t = 0; t < B; ++t) {
+        double s = 0.0;
+        for (long g = t; g < G; g += B) s += partial[g];
+        acc[t] = s;
+    }
+    for (long s = B / 2; s > 0; s >>= 1)
+        for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
+    return acc[0];
 }
 double nrm(const std::vector<double>& a) { return std::sqrt(dotv(a, a)); }
 
@@ -159,18 +185,27 @@ void galerkin(const Csr& A, const std::vector<Index>& agg, Index nc, Csr& Ac) {
 }
 
 void Amg::setup(const Csr& A, const SolverConfig& c) {
+    // The first setup aggregates every level; later setups (preconditioner
+    // rebuilds) keep the aggregates and recompute the Galerkin values.
     cfg = c;
-    levels.clear();
-    levels.push_back(AmgLevel{A, {}, 0});
-    while (static_cast<int>(levels.size()) < cfg.amgMaxLevels && levels.back().A.n > cfg.amgCoarseSize) {
-        AmgLevel& L = levels.back();
-        Index nc = 0;
-        aggregate(L.A, cfg.amgStrength, L.agg, nc);
-        if (nc == 0 || nc >= L.A.n) { L.agg.clear(); break; }
-        L.nc = nc;
-        Csr Ac;
-        galerkin(L.A, L.agg, nc, Ac);
-        levels.push_back(AmgLevel{std::move(Ac), {}, 0});
+    if (!aggregated) {
+        levels.clear();
+        levels.push_back(AmgLevel{A, {}, 0});
+        while (static_cast<int>(levels.size()) < cfg.amgMaxLevels && levels.back().A.n > cfg.amgCoarseSize) {
+            AmgLevel& L = levels.back();
+            Index nc = 0;
+            aggregate(L.A, cfg.amgStrength, L.agg, nc);
+            if (nc == 0 || nc >= L.A.n) { L.agg.clear(); break; }
+            L.nc = nc;
+            Csr Ac;
+            galerkin(L.A, L.agg, nc, Ac);
+            levels.push_back(AmgLevel{std::move(Ac), {}, 0});
+        }
+        aggregated = true;
+    } else {
+        levels[0].A = A;
+        for (std::size_t l = 0; l + 1 < levels.size(); ++l)
+            galerkin(levels[l].A, levels[l].agg, levels[l].nc, levels[l + 1].A);
     }
     const Csr& Cc = levels.back().A;
     const Index n = Cc.n;

@@ -207,3 +207,109 @@ def oracle_mesh(arrays: dict) -> CpuMesh:
 def reference_mesh(arrays: dict) -> CpuMesh | None:
     L = reference()
     return None if L is None else CpuMesh(L, "ref_", arrays)
+
+
+class OracleSimple:
+    """The oracle's SIMPLE driver with the same interface as babelsim_amd.device.Simple."""
+
+    def __init__(self, mesh: CpuMesh, cfg: SimpleConfig, u_bc, p_bc):
+        L = oracle()
+        self.m = mesh
+        ut, uv = u_bc
+        pt, pv = p_bc
+        ut = np.ascontiguousarray(ut, np.int32)
+        pt = np.ascontiguousarray(pt, np.int32)
+        uv = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+        pv = np.ascontiguousarray(np.asarray(pv, np.float64).reshape(-1))
+        h = L.orc_simple_new(mesh.h, C.byref(cfg), ptr(ut, ip), ptr(uv), ptr(pt, ip), ptr(pv))
+        if not h:
+            raise ValueError(L.orc_last_error().decode())
+        self.h = vp(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle().orc_simple_free(self.h)
+            self.h = None
+
+    def _call(self, name, *args):
+        if getattr(oracle(), name)(self.h, *args):
+            raise ValueError(oracle().orc_last_error().decode())
+
+    def set_fields(self, U, p):
+        self._call("orc_simple_set_fields", ptr(np.ascontiguousarray(U, np.float64)),
+                   ptr(np.ascontiguousarray(p, np.float64)))
+
+    def start(self):
+        self._call("orc_simple_start")
+
+    def iterate(self, n=1):
+        reps = (IterReport * n)()
+        self._call("orc_simple_iterate", n, reps)
+        return list(reps)
+
+    def fields(self):
+        U = np.zeros((self.m.nc, 3))
+        p = np.zeros(self.m.nc)
+        phi = np.zeros(self.m.nf)
+        self._call("orc_simple_get_fields", ptr(U), ptr(p), ptr(phi))
+        return U, p, phi
+
+    def history(self, which):
+        n = C.c_int()
+        out = np.zeros(200000)
+        self._call("orc_simple_history", which, ptr(out), len(out), C.byref(n))
+        return out[:n.value].copy()
+
+
+def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, hist_cap=200000):
+    L = oracle()
+    n = len(diag)
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    res = SolveResult()
+    hist = np.zeros(hist_cap)
+    ln = C.c_int()
+    rc = L.orc_csr_solve(n, ptr(np.ascontiguousarray(ptr_, np.int32), ip), ptr(np.ascontiguousarray(col, np.int32), ip),
+                         ptr(np.ascontiguousarray(diag, np.float64)), ptr(np.ascontiguousarray(val, np.float64)),
+                         ptr(np.ascontiguousarray(b, np.float64)), ptr(x), C.byref(cfg), C.byref(res), ptr(hist),
+                         hist_cap, C.byref(ln))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    return x, res, hist[:ln.value].copy()
+
+
+def amg_hierarchy(ptr_, col, diag, val, cfg: SolverConfig):
+    L = oracle()
+    n = len(diag)
+    nl = C.c_int()
+    sizes = np.zeros(64, np.int32)
+    agg0 = np.full(n, -1, np.int32)
+    cd1 = np.zeros(n)
+    rc = L.orc_amg_hierarchy(n, ptr(np.ascontiguousarray(ptr_, np.int32), ip),
+                             ptr(np.ascontiguousarray(col, np.int32), ip), ptr(np.ascontiguousarray(diag, np.float64)),
+                             ptr(np.ascontiguousarray(val, np.float64)), C.byref(cfg), C.byref(nl), ptr(sizes, ip),
+                             ptr(agg0, ip), ptr(cd1))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    sizes = sizes[:nl.value]
+    return sizes, agg0, cd1[:sizes[1]] if len(sizes) > 1 else None
+
+
+def csr_from_ldu(arrays: dict, diag, upper, lower):
+    """CSR (diagonal apart) with rows = cells, off-diagonal entries sorted by
+    (column, face): the layout both the oracle and the device use."""
+    own, nei = arrays["owner"], arrays["neighbour"]
+    nc = len(diag)
+    rows = [[] for _ in range(nc)]
+    for f in range(len(own)):
+        if nei[f] < 0:
+            continue
+        rows[own[f]].append((nei[f], f, upper[f]))
+        rows[nei[f]].append((own[f], f, lower[f]))
+    ptr_ = [0]
+    col, val = [], []
+    for r in rows:
+        r.sort()
+        col += [e[0] for e in r]
+        val += [e[2] for e in r]
+        ptr_.append(len(col))
+    return np.array(ptr_, np.int32), np.array(col, np.int32), np.asarray(diag, np.float64), np.array(val)
