@@ -59,8 +59,9 @@ void lduOut(Context& C, const std::vector<double>& dg, const std::vector<double>
     for (int i = 0; i < M.nc; ++i)
         for (int k = M.hmptr[i]; k < M.hmptr[i + 1]; ++k) {
             const int f = M.hmface[k];
-            if (M.hown[f] == i) upper[M.faceNewToOld[f]] += off[k];
-            else lower[M.faceNewToOld[f]] += off[k];
+            const double v = off[M.csrToSliced[k]];
+            if (M.hown[f] == i) upper[M.faceNewToOld[f]] += v;
+            else lower[M.faceNewToOld[f]] += v;
         }
 }
 
@@ -225,7 +226,7 @@ int bsim_op_assemble_convection(bsim_ctx* ctx, int ncomp, const double* x, const
         const DeviceMesh& M = *C.mesh;
         OpField f = makeField(C, ncomp, x, bct, bcv, phi);
         DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
-        DBuf<double> dg(M.nc), off(std::max(M.nnzOff, 1)), src(static_cast<std::size_t>(ncomp) * M.nc);
+        DBuf<double> dg(M.nc), off(std::max(M.nnzSliced, 1)), src(static_cast<std::size_t>(ncomp) * M.nc);
         dg.zero(C.stream); off.zero(C.stream); src.zero(C.stream);
         if (conv == kLinearUpwind || (conv == kCentral && interp == kCorrected))
             ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
@@ -247,7 +248,7 @@ int bsim_op_assemble_diffusion(bsim_ctx* ctx, int ncomp, const double* x, const 
         const DeviceMesh& M = *C.mesh;
         OpField f = makeField(C, ncomp, x, bct, bcv, flux_ctx);
         DBuf<double> g(3 * ncomp * static_cast<std::size_t>(M.nc)), work(3 * ncomp * static_cast<std::size_t>(M.nc));
-        DBuf<double> dg(M.nc), off(std::max(M.nnzOff, 1)), src(static_cast<std::size_t>(ncomp) * M.nc), kd;
+        DBuf<double> dg(M.nc), off(std::max(M.nnzSliced, 1)), src(static_cast<std::size_t>(ncomp) * M.nc), kd;
         dg.zero(C.stream); off.zero(C.stream); src.zero(C.stream);
         kd.upload(C.facesIn(kf, 1), C.stream);
         if (dmethod != kOrthogonal) ops::gradient(M.view(), M.orthogonal, ncomp, f.arg, gmethod, g.p, work.p, C.err.p, C.stream);
@@ -265,7 +266,7 @@ int bsim_system_create(bsim_ctx* ctx, bsim_system** out) {
         const DeviceMesh& M = *ctx->c->mesh;
         auto* s = new bsim_system;
         s->ctx = ctx;
-        s->sys = std::make_unique<DeviceSystem>(ctx->c->stream, M.nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
+        s->sys = std::make_unique<DeviceSystem>(ctx->c->stream, meshPattern(M));
         *out = s;
     });
 }
@@ -280,14 +281,14 @@ int bsim_system_solve_ldu(bsim_system* sys, const double* diag, const double* up
         const SolverCfg sc = toSolverCfg(cfg);
         DeviceSystem& D = *sys->sys;
         // LDU -> CSR values (petsc_assembly_plan.cpp:62-87, row owner holds upper).
-        std::vector<double> dg = C.cellsIn(diag, 1), off(M.nnzOff);
+        std::vector<double> dg = C.cellsIn(diag, 1), off(M.nnzSliced, 0.0);
         for (int i = 0; i < M.nc; ++i)
             for (int k = M.hmptr[i]; k < M.hmptr[i + 1]; ++k) {
                 const int f = M.hmface[k];
-                off[k] = M.hown[f] == i ? upper[M.faceNewToOld[f]] : lower[M.faceNewToOld[f]];
+                off[M.csrToSliced[k]] = M.hown[f] == i ? upper[M.faceNewToOld[f]] : lower[M.faceNewToOld[f]];
             }
         BSIM_HIP(hipMemcpyAsync(D.candDiag(), dg.data(), sizeof(double) * M.nc, hipMemcpyHostToDevice, s));
-        if (M.nnzOff) BSIM_HIP(hipMemcpyAsync(D.candVal(), off.data(), sizeof(double) * M.nnzOff, hipMemcpyHostToDevice, s));
+        if (M.nnzSliced) BSIM_HIP(hipMemcpyAsync(D.candVal(), off.data(), sizeof(double) * M.nnzSliced, hipMemcpyHostToDevice, s));
         D.update();
         D.prepare(sc);
         const std::vector<double> b = C.cellsIn(source, nrhs), x0 = C.cellsIn(x, nrhs);
@@ -342,7 +343,16 @@ struct StandaloneCsr {
             if (hc[k] < 0 || hc[k] >= n) throw std::invalid_argument("column index out of range");
         ptr.upload(hp, stream);
         col.upload(hc, stream);
-        sys = std::make_unique<DeviceSystem>(stream, n, ptr.p, col.p, hp, hc);
+        Pattern P;
+        P.n = n;
+        P.nnzStore = p[n];
+        P.K = 0;
+        P.ptr = ptr.p;
+        P.col = col.p;
+        P.sliceOff = nullptr;
+        P.hptr = hp;
+        P.hcol = hc;
+        sys = std::make_unique<DeviceSystem>(stream, P);
         BSIM_HIP(hipMemcpyAsync(sys->candDiag(), d, sizeof(double) * n, hipMemcpyHostToDevice, stream));
         if (p[n]) BSIM_HIP(hipMemcpyAsync(sys->candVal(), v, sizeof(double) * p[n], hipMemcpyHostToDevice, stream));
         sys->update();
@@ -393,7 +403,7 @@ int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const 
         SolverCfg c = toSolverCfg(cfg);
         c.pc = 2;
         AmgDevice amg(S.stream);
-        amg.setup(S.sys->view(), S.sys->hptr, S.sys->hcol, c);
+        amg.setup(S.sys->view(), S.sys->pat, c);
         const std::vector<int> sz = amg.sizes();
         *nlevels = static_cast<int>(sz.size());
         std::memcpy(sizes, sz.data(), sizeof(int) * sz.size());

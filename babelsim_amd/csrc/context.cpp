@@ -61,6 +61,20 @@ template void Context::cellsOut(const std::vector<double>&, double*, int) const;
 template std::vector<double> Context::facesIn(const double*, int) const;
 template void Context::facesOut(const std::vector<double>&, double*, int) const;
 
+Pattern meshPattern(const DeviceMesh& M) {
+    Pattern P;
+    P.n = M.nc;
+    P.nnzStore = M.nnzSliced;
+    P.K = M.K;
+    P.ptr = nullptr;
+    P.col = M.mcol.p;
+    P.sliceOff = M.sliceOff.p;
+    P.hptr = M.hmptr;
+    P.hcol = M.hmcol;
+    P.csrToStore = M.csrToSliced;
+    return P;
+}
+
 SolverCfg toSolverCfg(const void* p) {
     const auto* c = static_cast<const bsim_solver_config*>(p);
     SolverCfg s;

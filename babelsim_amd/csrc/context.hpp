@@ -31,4 +31,7 @@ struct Context {
 
 SolverCfg toSolverCfg(const void* cfg /* bsim_solver_config* */);
 
+// Matrix storage pattern of a cell-centred operator on the mesh (sliced rows).
+Pattern meshPattern(const DeviceMesh& mesh);
+
 }  // namespace bsim

@@ -23,7 +23,7 @@ namespace bsim {
 #define BSIM_LAUNCH_CHECK() BSIM_HIP(hipGetLastError())
 
 constexpr int kBlock = 256;          // 4 wavefronts of 64
-constexpr int kMaxReduceBlocks = 1024;
+constexpr int kMaxReduceBlocks = 4096;
 
 inline int blocksFor(long long n, int block = kBlock) {
     long long b = (n + block - 1) / block;
@@ -74,6 +74,21 @@ template <class T> struct DBuf {
     }
 };
 
+// Row layout shared by the cell-gather kernels and the fine-level matrices:
+// rows are grouped in slices of 64 (one wavefront); the k-th off-diagonal
+// entry of the rows of slice s is stored at sliceOff[s*(K+1)+k] + lane, so a
+// wavefront reading level k touches one contiguous run.  Rows in a slice are
+// ordered by decreasing length when the mesh is renumbered; otherwise the gaps
+// are padding entries with face and column -1.
+struct RowRange {
+    const int* so;
+    int lane, K;
+    __host__ __device__ RowRange(const int* sliceOff, int K_, int c)
+        : so(sliceOff + static_cast<long long>(c >> 6) * (K_ + 1)), lane(c & 63), K(K_) {}
+    __host__ __device__ bool valid(int k) const { return k < K && lane < so[k + 1] - so[k]; }
+    __host__ __device__ int at(int k) const { return so[k] + lane; }
+};
+
 // ---- device vector helpers (component-wise, AoS double[3]) ----
 struct D3 {
     double x, y, z;
@@ -93,9 +108,12 @@ __device__ inline D3 ld3(const double* p, long long i) { return {p[3 * i], p[3 *
 __device__ inline void st3(double* p, long long i, D3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
 
 // ---- deterministic reductions ----
-// Stage 1: each block writes its partial sums (fixed grid, fixed in-block tree).
-// Stage 2: the last block to finish sums the partials in index order with a
-// fixed tree, so the result does not depend on scheduling.
+// Stage 1 (the producing kernel): each block reduces its values with a fixed
+// in-block tree and stores the K sums at partials[k * kMaxReduceBlocks + block].
+// Stage 2 (kReduceTail, one block launched after it on the same stream): sums
+// the partials in block order with a fixed tree and hands them to a functor.
+// The kernel boundary orders the two stages, so no device-scope fence or
+// completion counter is needed and the result does not depend on scheduling.
 template <int K> __device__ inline void blockSum(double (&v)[K], double* smem /* K*kBlock */) {
     const int t = threadIdx.x;
     for (int k = 0; k < K; ++k) smem[k * kBlock + t] = v[k];
@@ -109,37 +127,30 @@ template <int K> __device__ inline void blockSum(double (&v)[K], double* smem /*
     __syncthreads();
 }
 
-// Workspace for one reduction family: partials[K*maxBlocks], a completion counter.
-struct ReduceSlot {
-    double* partials;
-    unsigned int* counter;
-};
-
-// Returns true in exactly one block (the last); that block then holds the
-// final K sums in `out` (valid for all its threads).
-template <int K>
-__device__ inline bool reduceFinish(double (&v)[K], ReduceSlot slot, double* smem, double (&out)[K]) {
+template <int K> __device__ inline void blockPartial(double (&v)[K], double* partials, double* smem) {
     blockSum<K>(v, smem);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < K; ++k) slot.partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned int done = atomicAdd(slot.counter, 1u);
-        last = done == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
+}
+
+// F: device functor called by thread 0 with the K totals.
+template <int K, class F> __global__ void __launch_bounds__(kBlock) kReduceTail(int nb, const double* partials, F f) {
+    __shared__ double smem[K * kBlock];
     double acc[K];
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock)
-        for (int k = 0; k < K; ++k) acc[k] += __hip_atomic_load(&slot.partials[k * kMaxReduceBlocks + b], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+        for (int k = 0; k < K; ++k) acc[k] += partials[k * kMaxReduceBlocks + b];
     blockSum<K>(acc, smem);
-    for (int k = 0; k < K; ++k) out[k] = acc[k];
-    if (threadIdx.x == 0) *slot.counter = 0u;  // re-arm for the next launch on this slot
-    return true;
+    if (threadIdx.x == 0) f(acc);
 }
+
+// Stores the K totals to out[0..K-1].
+template <int K> struct StoreSums {
+    double* out;
+    __device__ void operator()(const double (&v)[K]) const {
+        for (int k = 0; k < K; ++k) out[k] = v[k];
+    }
+};
 
 inline int reduceBlocks(long long n) {
     const int b = blocksFor(n);

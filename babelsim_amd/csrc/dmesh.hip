@@ -109,7 +109,13 @@ __global__ void kCellReference(MeshView m, const double* psum, const int* pcnt, 
     if (c >= m.nc) return;
     D3 s = d3(0, 0, 0);
     int cnt = 0;
-    for (int k = m.mptr[c]; k < m.mptr[c + 1]; ++k) { s = s + ld3(psum, m.mface[k]); cnt += pcnt[m.mface[k]]; }
+    const RowRange R = m.row(c);
+    for (int k = 0; R.valid(k); ++k) {
+        const int f = m.mface[R.at(k)];
+        if (f < 0) continue;
+        s = s + ld3(psum, f);
+        cnt += pcnt[f];
+    }
     for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) { s = s + ld3(psum, m.bface[k]); cnt += pcnt[m.bface[k]]; }
     st3(ref, c, s / static_cast<double>(cnt > 1 ? cnt : 1));
 }
@@ -179,7 +185,11 @@ __global__ void kCellGeometry(MeshView m, const int* fpo, const int* fpi, const 
     const D3 r = ld3(ref, c);
     double vol = 0.0, atot = 0.0;
     D3 mom = d3(0, 0, 0), closure = d3(0, 0, 0);
-    for (int k = m.mptr[c]; k < m.mptr[c + 1]; ++k) cellFace(m, fpo, fpi, vx, c, m.mface[k], r, vol, mom, closure, atot, err);
+    const RowRange R = m.row(c);
+    for (int k = 0; R.valid(k); ++k) {
+        const int f = m.mface[R.at(k)];
+        if (f >= 0) cellFace(m, fpo, fpi, vx, c, f, r, vol, mom, closure, atot, err);
+    }
     for (int k = m.bptr[c]; k < m.bptr[c + 1]; ++k) cellFace(m, fpo, fpi, vx, c, m.bface[k], r, vol, mom, closure, atot, err);
     if (!(vol > 0.0) || !isfinite(vol)) atomicOr(err, kErrVolume);
     if (!(sqrt(dot(closure, closure)) <= 1e-9 * fmax(atot, 1.0))) atomicOr(err, kErrClosure);
@@ -256,14 +266,44 @@ DeviceMesh::DeviceMesh(const MeshDesc& caller, bool renumber, hipStream_t s) {
     }
     hmcol.resize(nnzOff);
     hmface.resize(nnzOff);
-    std::vector<int> hslotOwn(nfi), hslotNei(nfi);
     for (int i = 0; i < nc; ++i) {
         std::sort(entries.begin() + hmptr[i], entries.begin() + hmptr[i + 1]);
         for (int k = hmptr[i]; k < hmptr[i + 1]; ++k) {
-            const int f = entries[k].second;
-            hmface[k] = f;
+            hmface[k] = entries[k].second;
             hmcol[k] = static_cast<int>(entries[k].first / nf);
-            if (d.own[f] == i) hslotOwn[f] = k; else hslotNei[f] = k;
+        }
+    }
+    // Sliced layout: level k of slice s holds the k-th entries of the slice's
+    // rows; the level's width is one past the last lane that still has one.
+    K = 0;
+    for (int i = 0; i < nc; ++i) K = std::max(K, cnt[i]);
+    const int nslices = (nc + 63) / 64;
+    hsliceOff.assign(static_cast<std::size_t>(nslices) * (K + 1), 0);
+    int base = 0;
+    for (int s0 = 0; s0 < nslices; ++s0) {
+        int* so = &hsliceOff[static_cast<std::size_t>(s0) * (K + 1)];
+        so[0] = base;
+        for (int k = 0; k < K; ++k) {
+            int width = 0;
+            for (int l = 0; l < 64 && s0 * 64 + l < nc; ++l)
+                if (cnt[s0 * 64 + l] > k) width = l + 1;
+            so[k + 1] = so[k] + width;
+        }
+        base = so[K];
+    }
+    nnzSliced = base;
+    std::vector<int> scol(std::max(nnzSliced, 1), -1), sface(std::max(nnzSliced, 1), -1);
+    csrToSliced.assign(nnzOff, 0);
+    std::vector<int> hslotOwn(nfi), hslotNei(nfi);
+    for (int i = 0; i < nc; ++i) {
+        const int* so = &hsliceOff[static_cast<std::size_t>(i / 64) * (K + 1)];
+        for (int k = 0; k < cnt[i]; ++k) {
+            const int e = so[k] + (i % 64), q = hmptr[i] + k;
+            csrToSliced[q] = e;
+            scol[e] = hmcol[q];
+            sface[e] = hmface[q];
+            const int f = hmface[q];
+            if (d.own[f] == i) hslotOwn[f] = e; else hslotNei[f] = e;
         }
     }
     std::vector<int> hbface(nf - nfi);
@@ -276,7 +316,7 @@ DeviceMesh::DeviceMesh(const MeshDesc& caller, bool renumber, hipStream_t s) {
         if (d.patch[f] < d.patch[f - 1]) throw std::logic_error("boundary faces are not grouped by patch");
 
     own.upload(d.own, s); nei.upload(d.nei, s); patch.upload(d.patch, s);
-    mptr.upload(hmptr, s); mcol.upload(hmcol, s); mface.upload(hmface, s);
+    sliceOff.upload(hsliceOff, s); mcol.upload(scol, s); mface.upload(sface, s);
     bptr.upload(hbptr, s); bface.upload(hbface, s);
     slotOwn.upload(hslotOwn, s); slotNei.upload(hslotNei, s);
     patchStart.upload(hpatchStart, s);
@@ -308,11 +348,11 @@ DeviceMesh::DeviceMesh(const MeshDesc& caller, bool renumber, hipStream_t s) {
 
 MeshView DeviceMesh::view() const {
     MeshView m;
-    m.nc = nc; m.nf = nf; m.nfi = nfi; m.np = np;
+    m.nc = nc; m.nf = nf; m.nfi = nfi; m.np = np; m.K = K;
     m.own = own.p; m.nei = nei.p; m.patch = patch.p;
     m.Cf = Cf.p; m.Sf = Sf.p; m.nrm = nrm.p; m.Af = Af.p; m.w = w.p; m.orth = orth.p; m.kf = kf.p; m.skew = skew.p;
     m.C = C.p; m.V = V.p; m.invV = invV.p;
-    m.mptr = mptr.p; m.mcol = mcol.p; m.mface = mface.p; m.bptr = bptr.p; m.bface = bface.p;
+    m.sliceOff = sliceOff.p; m.mcol = mcol.p; m.mface = mface.p; m.bptr = bptr.p; m.bface = bface.p;
     m.slotOwn = slotOwn.p; m.slotNei = slotNei.p; m.patchStart = patchStart.p;
     return m;
 }

@@ -18,7 +18,8 @@ __global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, i
     for (int k = 0; k < NC; ++k) acc[k] = d3(0, 0, 0);
     const double iv = m.invV[c];
     double val[NC];
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         if (PASS2) correctedInternal<NC>(x, g0, m, f, val);
         else internalValue<NC>(x, m, f, val);
@@ -58,7 +59,8 @@ template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, i
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) A[i][j] += wv[i] * dv[j];
         for (int i = 0; i < 3; ++i) for (int k = 0; k < NC; ++k) rhs[i * NC + k] += wv[i] * diff[k];
     };
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int other = m.mcol[e];
         double diff[NC];
         cellValue<NC>(x, other, val);
@@ -140,7 +142,8 @@ __global__ void kDivFace(MeshView m, const double* phi, double* out) {
     if (c >= m.nc) return;
     const double iv = m.invV[c];
     double acc = 0.0;
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         acc = m.own[f] == c ? acc + phi[f] * iv : acc - phi[f] * iv;
     }
@@ -154,7 +157,8 @@ __global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, dou
     if (c >= m.nc) return;
     const double iv = m.invV[c];
     double acc = 0.0;
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         double q[1];
         if (correctedDiffusion(dm)) integratedNormalGradient<1>(x, g, m, f, dm, q, err);
@@ -188,7 +192,8 @@ __global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, i
     const double iv = m.invV[c];
     double acc[NC], fv[NC], corr[NC];
     for (int k = 0; k < NC; ++k) acc[k] = 0.0;
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         const double q = phi[f];
         if (conv == kUpwind || conv == kLinearUpwind) {
@@ -235,7 +240,8 @@ __global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, i
     const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
     double dAcc = 0.0, sAcc[NC], corr[NC], a[NC], b[NC];
     for (int k = 0; k < NC; ++k) sAcc[k] = 0.0;
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         const int o = m.own[f], n = m.nei[f];
         const bool owner = o == c;
@@ -305,7 +311,8 @@ __global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, con
     const bool corrected = correctedDiffusion(dm);
     double dAcc = 0.0, sAcc[NC], ing[NC], bv[NC];
     for (int k = 0; k < NC; ++k) sAcc[k] = 0.0;
-    for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
         const int f = m.mface[e];
         const double gamma = kf[f];
         if (gamma < 0.0 || !isfinite(gamma)) { atomicOr(err, kErrNegativeDiffusivity); continue; }

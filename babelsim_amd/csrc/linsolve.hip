@@ -23,17 +23,22 @@ double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::
 
 __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 
+// Row i of A x: the diagonal first, then the off-diagonal entries in row order
+// (the operation order of the oracle's spmv); padding entries are skipped.
 template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int i) {
     double y = static_cast<double>(A.diag[i]) * static_cast<double>(x[i]);
-    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
+    if (A.sliceOff) {
+        const RowRange R(A.sliceOff, A.K, i);
+        for (int k = 0; R.valid(k); ++k) {
+            const int e = R.at(k), c = A.col[e];
+            if (c >= 0) y += static_cast<double>(A.val[e]) * static_cast<double>(x[c]);
+        }
+    } else {
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
+    }
     return y;
 }
-// FP64 matrices keep the exact operation order of the oracle's spmv.
-__device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) {
-    double y = A.diag[i] * x[i];
-    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * x[A.col[k]];
-    return y;
-}
+__device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i); }
 
 __global__ void kFill(int n, double* x, double v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,8 +53,8 @@ __global__ void kAxpy(int n, double a, const double* x, double* y) {  // y += a 
     if (i < n) y[i] += a * x[i];
 }
 
-// Sum of squares of b (optionally copied to r) -> sqrt into sc[idx].
-__global__ void kCopyNorm(int n, const double* b, double* r, double* sc, int idx, ReduceSlot slot) {
+// Sum of squares of b (optionally copied to r); partial sums per block.
+__global__ void kCopyNorm(int n, const double* b, double* r, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -57,13 +62,12 @@ __global__ void kCopyNorm(int n, const double* b, double* r, double* sc, int idx
         if (r) r[i] = v;
         acc += v * v;
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = sqrt(out[0]);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
 
-// r = b - A x and its norm.
-__global__ void kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* sc, int idx,
-                              ReduceSlot slot) {
+// r = b - A x and the partial sums of its squares.
+__global__ void kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
@@ -71,29 +75,48 @@ __global__ void kResidualNorm(CsrView<double> A, const double* b, const double* 
         r[i] = v;
         acc += v * v;
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = sqrt(out[0]);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
 
-__global__ void kDot(int n, const double* a, const double* b, double* sc, int idx, ReduceSlot slot) {
+__global__ void kDot(int n, const double* a, const double* b, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[idx] = out[0];
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
+
+struct StoreSqrt {  // sc[idx] = sqrt(sum)
+    double* sc;
+    int idx;
+    __device__ void operator()(const double (&v)[1]) const { sc[idx] = sqrt(v[0]); }
+};
+struct StoreValue {  // sc[idx] = sum
+    double* sc;
+    int idx;
+    __device__ void operator()(const double (&v)[1]) const { sc[idx] = v[0]; }
+};
 
 // ---- CG (cg.c KSPSolve_CG, unpreconditioned norm) ----
-__device__ inline void finishBeta(double beta, double* sc, int* fl, int checkSign) {
-    sc[S_BETA] = beta;
-    if (!isfinite(beta)) atomicOr(fl, F_NONFINITE);
-    else if (beta == 0.0) atomicOr(fl, F_BETA_ZERO);
-    else if (checkSign && beta * sc[S_BETAOLD] < 0.0) atomicOr(fl, F_INDEF_PC);
-}
+// Stage-1 kernels return at once when a breakdown flag is set; their tails
+// check the same flag so that no stale partials are consumed.
+struct CgBeta {  // beta = z . r
+    double* sc;
+    int* fl;
+    int checkSign;
+    __device__ void operator()(const double (&v)[1]) const {
+        if (fl[0]) return;
+        const double beta = v[0];
+        sc[S_BETA] = beta;
+        if (!isfinite(beta)) fl[0] |= F_NONFINITE;
+        else if (beta == 0.0) fl[0] |= F_BETA_ZERO;
+        else if (checkSign && beta * sc[S_BETAOLD] < 0.0) fl[0] |= F_INDEF_PC;
+    }
+};
 
-// z = r * (1/d) and beta = z . r
-__global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z, double* sc, int* fl, int checkSign,
-                           ReduceSlot slot) {
+// z = r * (1/d) and the partial sums of z . r
+__global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z, const int* fl, double* part) {
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
@@ -102,16 +125,16 @@ __global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z
         z[i] = v;
         acc += v * r[i];
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) finishBeta(out[0], sc, fl, checkSign);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
-__global__ void kPlainDot(int n, const double* z, const double* r, double* sc, int* fl, int checkSign, ReduceSlot slot) {
+__global__ void kPlainDot(int n, const double* z, const double* r, const int* fl, double* part) {
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += z[i] * r[i];
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) finishBeta(out[0], sc, fl, checkSign);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
 
 __global__ void kCgDirection(int n, const double* z, double* p, const double* sc, const int* fl, int first) {
@@ -125,9 +148,8 @@ __global__ void kCgDirection(int n, const double* z, double* p, const double* sc
     }
 }
 
-// w = A p, dpi = p . w, a = beta / dpi
-__global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, double* sc, int* fl, int iter,
-                           ReduceSlot slot) {
+// w = A p and the partial sums of p . w
+__global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, const int* fl, double* part) {
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
@@ -136,20 +158,27 @@ __global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, double
         w[i] = y;
         acc += p[i] * y;
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
-        const double dpiold = sc[S_DPI], dpi = out[0];
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
+}
+struct CgAlpha {  // dpi = p . w, a = beta / dpi
+    double* sc;
+    int* fl;
+    int iter;
+    __device__ void operator()(const double (&v)[1]) const {
+        if (fl[0]) return;
+        const double dpiold = sc[S_DPI], dpi = v[0];
         sc[S_DPI] = dpi;
-        if (!isfinite(dpi)) { atomicOr(fl, F_NONFINITE); return; }
+        if (!isfinite(dpi)) { fl[0] |= F_NONFINITE; return; }
         sc[S_BETAOLD] = sc[S_BETA];
-        if (dpi == 0.0 || (iter > 0 && sgn(dpi) * sgn(dpiold) < 0)) atomicOr(fl, F_INDEF_MAT);
+        if (dpi == 0.0 || (iter > 0 && sgn(dpi) * sgn(dpiold) < 0)) fl[0] |= F_INDEF_MAT;
         else sc[S_A] = sc[S_BETA] / dpi;
     }
-}
+};
 
-// x += a p, r += -a w, dp = ||r||
-__global__ void kCgUpdate(int n, double* x, const double* p, double* r, const double* w, double* sc, const int* fl,
-                          ReduceSlot slot) {
+// x += a p, r += -a w and the partial sums of r . r
+__global__ void kCgUpdate(int n, double* x, const double* p, double* r, const double* w, const double* sc, const int* fl,
+                          double* part) {
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     const double a = sc[S_A];
@@ -160,31 +189,42 @@ __global__ void kCgUpdate(int n, double* x, const double* p, double* r, const do
         r[i] = ri;
         acc += ri * ri;
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) sc[S_DP] = sqrt(out[0]);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
 }
+struct CgNorm {  // dp = ||r||
+    double* sc;
+    const int* fl;
+    __device__ void operator()(const double (&v)[1]) const {
+        if (fl[0]) return;
+        sc[S_DP] = sqrt(v[0]);
+    }
+};
 
 // ---- BiCGSTAB (bcgs.c:14-140, right preconditioning) ----
-__global__ void kBcgsRho(int n, const double* r, const double* rp, double* sc, ReduceSlot slot) {
+__global__ void kBcgsRho(int n, const double* r, const double* rp, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += r[i] * rp[i];
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
-        const double rho = out[0];
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
+}
+struct BcgsRho {  // rho = r . rp, beta = (rho / rhoold) (alpha / omegaold)
+    double* sc;
+    __device__ void operator()(const double (&v)[1]) const {
+        const double rho = v[0];
         sc[S_RHO] = rho;
         sc[S_BETA] = (rho / sc[S_RHOOLD]) * (sc[S_ALPHA] / sc[S_OMEGAOLD]);
     }
-}
+};
 __global__ void kBcgsP(int n, double* p, const double* r, const double* v, const double* sc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double beta = sc[S_BETA];
     p[i] = r[i] + (-sc[S_OMEGAOLD] * beta) * v[i] + beta * p[i];
 }
-// v = A t, d1 = v . rp, alpha = rho / d1
-__global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* sc, int* fl,
-                            ReduceSlot slot) {
+// v = A t and the partial sums of v . rp
+__global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
@@ -192,24 +232,29 @@ __global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const
         v[i] = y;
         acc += y * rp[i];
     }
-    double vv[1] = {acc}, out[1];
-    if (reduceFinish<1>(vv, slot, smem, out) && threadIdx.x == 0) {
-        const double d1 = out[0];
+    double vv[1] = {acc};
+    blockPartial<1>(vv, part, smem);
+}
+struct BcgsAlpha {  // d1 = v . rp, alpha = rho / d1
+    double* sc;
+    int* fl;
+    __device__ void operator()(const double (&v)[1]) const {
+        const double d1 = v[0];
         sc[S_D1] = d1;
-        if (!isfinite(d1)) atomicOr(fl, F_NONFINITE);
-        else if (d1 == 0.0) atomicOr(fl, F_D1_ZERO);
+        if (!isfinite(d1)) fl[0] |= F_NONFINITE;
+        else if (d1 == 0.0) fl[0] |= F_D1_ZERO;
         else sc[S_ALPHA] = sc[S_RHO] / d1;
     }
-}
+};
 __global__ void kBcgsS(int n, double* s, const double* v, const double* r, const double* sc, const int* fl) {
     if (fl[0]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     s[i] = -sc[S_ALPHA] * v[i] + r[i];
 }
-// t = A rt, d1 = s . t, d2 = t . t, omega = d1 / d2
-__global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, const double* s, double* sc, int* fl,
-                            ReduceSlot slot) {
+// t = A rt and the partial sums of s . t and t . t
+__global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, const double* s, const int* fl,
+                            double* part) {
     __shared__ double smem[2 * kBlock];
     if (fl[0]) return;
     double a1 = 0.0, a2 = 0.0;
@@ -219,17 +264,23 @@ __global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, cons
         a1 += s[i] * y;
         a2 += y * y;
     }
-    double v[2] = {a1, a2}, out[2];
-    if (reduceFinish<2>(v, slot, smem, out) && threadIdx.x == 0) {
-        sc[S_D1] = out[0];
-        sc[S_D2] = out[1];
-        if (out[1] == 0.0) atomicOr(fl, F_D2_ZERO);
-        else sc[S_OMEGA] = out[0] / out[1];
-    }
+    double v[2] = {a1, a2};
+    blockPartial<2>(v, part, smem);
 }
-// x = alpha p + omega s + x, r = -omega t + s, dp = ||r||
-__global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, double* r, const double* t, double* sc,
-                            const int* fl, ReduceSlot slot) {
+struct BcgsOmega {  // omega = (s . t) / (t . t)
+    double* sc;
+    int* fl;
+    __device__ void operator()(const double (&v)[2]) const {
+        if (fl[0]) return;
+        sc[S_D1] = v[0];
+        sc[S_D2] = v[1];
+        if (v[1] == 0.0) fl[0] |= F_D2_ZERO;
+        else sc[S_OMEGA] = v[0] / v[1];
+    }
+};
+// x = alpha p + omega s + x, r = -omega t + s and the partial sums of r . r
+__global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, double* r, const double* t,
+                            const double* sc, const int* fl, double* part) {
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
@@ -240,13 +291,19 @@ __global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, 
         r[i] = ri;
         acc += ri * ri;
     }
-    double v[1] = {acc}, out[1];
-    if (reduceFinish<1>(v, slot, smem, out) && threadIdx.x == 0) {
-        sc[S_DP] = sqrt(out[0]);
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
+}
+struct BcgsNorm {  // dp = ||r||; rho and omega become the previous values
+    double* sc;
+    const int* fl;
+    __device__ void operator()(const double (&v)[1]) const {
+        if (fl[0]) return;
+        sc[S_DP] = sqrt(v[0]);
         sc[S_RHOOLD] = sc[S_RHO];
         sc[S_OMEGAOLD] = sc[S_OMEGA];
     }
-}
+};
 __global__ void kSpmvPlain(CsrView<double> A, const double* x, double* y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < A.n) y[i] = rowDot64(A, x, i);
@@ -259,9 +316,18 @@ __global__ void kInvDiag(int n, const double* d, double* inv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) inv[i] = 1.0 / d[i];
 }
-__global__ void kCompare(int n, const double* a, const double* b, int* diff) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !(a[i] == b[i])) atomicOr(diff, 1);
+// Counts entries where (a0, a1) and (b0, b1) differ, as partial sums per block.
+__global__ void kCompare(int n0, const double* a0, const double* b0, int n1, const double* a1, const double* b1,
+                         double* part) {
+    __shared__ double smem[kBlock];
+    double cnt = 0.0;
+    const int n = n0 + n1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const bool same = i < n0 ? a0[i] == b0[i] : a1[i - n0] == b1[i - n0];
+        if (!same) cnt += 1.0;
+    }
+    double v[1] = {cnt};
+    blockPartial<1>(v, part, smem);
 }
 
 // ---- AMG ----
@@ -348,17 +414,28 @@ __global__ void kDenseLU(int n, const int* ptr, const int* col, const double* d,
         __syncthreads();
     }
 }
-template <class T> __global__ void kDenseSolve(int n, const double* a, const int* piv, const T* b, T* xo, double* work) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int i = 0; i < n; ++i) work[i] = static_cast<double>(b[i]);
-    for (int k = 0; k < n; ++k) { const double t = work[k]; work[k] = work[piv[k]]; work[piv[k]] = t; }
+// Explicit inverse of the coarsest matrix from its LU factors: column j is
+// the LU solve of the unit vector e_j (thread j works in column j of inv).
+__global__ void kDenseInverse(int n, const double* a, const int* piv, double* inv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    auto w = [&](int i) -> double& { return inv[static_cast<long long>(i) * n + j]; };
+    for (int i = 0; i < n; ++i) w(i) = i == j ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k) { const double t = w(k); w(k) = w(piv[k]); w(piv[k]) = t; }
     for (int i = 0; i < n; ++i)
-        for (int j = 0; j < i; ++j) work[i] -= a[static_cast<long long>(i) * n + j] * work[j];
+        for (int q = 0; q < i; ++q) w(i) -= a[static_cast<long long>(i) * n + q] * w(q);
     for (int i = n - 1; i >= 0; --i) {
-        for (int j = i + 1; j < n; ++j) work[i] -= a[static_cast<long long>(i) * n + j] * work[j];
-        work[i] /= a[static_cast<long long>(i) * n + i];
+        for (int q = i + 1; q < n; ++q) w(i) -= a[static_cast<long long>(i) * n + q] * w(q);
+        w(i) /= a[static_cast<long long>(i) * n + i];
     }
-    for (int i = 0; i < n; ++i) xo[i] = static_cast<T>(work[i]);
+}
+// Coarsest solve as a dense product x = inv * b (FP64 accumulation).
+template <class T> __global__ void kDenseApply(int n, const double* inv, const T* b, T* xo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += inv[static_cast<long long>(i) * n + j] * static_cast<double>(b[j]);
+    xo[i] = static_cast<T>(s);
 }
 
 }  // namespace
@@ -372,29 +449,35 @@ bool SolverCfg::operator==(const SolverCfg& o) const {
 }
 
 // ------------------------------------------------------------------ AMG
-void AmgDevice::setup(const CsrView<double>& A, const std::vector<int>& hptr, const std::vector<int>& hcol,
-                      const SolverCfg& c) {
+void AmgDevice::setup(const CsrView<double>& A, const Pattern& P, const SolverCfg& c) {
     const auto t0 = Clock::now();
     cfg = c;
     mixed = c.precision == 1;
     hipStream_t s = stream;
     if (!aggregated) {
+        // Aggregation runs once, on the row-major CSR of the first matrix.
         HostCsr A0;
         A0.n = A.n;
-        A0.ptr = hptr;
-        A0.col = hcol;
+        A0.ptr = P.hptr;
+        A0.col = P.hcol;
         A0.diag.resize(A.n);
-        A0.val.resize(hcol.size());
+        std::vector<double> stored(std::max(P.nnzStore, 1));
         BSIM_HIP(hipMemcpyAsync(A0.diag.data(), A.diag, sizeof(double) * A.n, hipMemcpyDeviceToHost, s));
-        BSIM_HIP(hipMemcpyAsync(A0.val.data(), A.val, sizeof(double) * hcol.size(), hipMemcpyDeviceToHost, s));
+        if (P.nnzStore) BSIM_HIP(hipMemcpyAsync(stored.data(), A.val, sizeof(double) * P.nnzStore, hipMemcpyDeviceToHost, s));
         BSIM_HIP(hipStreamSynchronize(s));
+        A0.val.resize(P.hcol.size());
+        for (std::size_t q = 0; q < P.hcol.size(); ++q) A0.val[q] = stored[P.csrToStore.empty() ? q : P.csrToStore[q]];
         std::vector<HostLevel> H = buildHierarchy(A0, c.amgStrength, c.amgMaxLevels, c.amgCoarseSize);
+        // Level-0 Galerkin sources address the stored (possibly sliced) values.
+        if (!P.csrToStore.empty() && !H[0].gSrc.empty())
+            for (int& src : H[0].gSrc)
+                if (src >= A.n) src = A.n + P.csrToStore[src - A.n];
         L.clear();
         for (std::size_t l = 0; l < H.size(); ++l) {
             auto lv = std::make_unique<Level>();
             lv->n = H[l].A.n;
             lv->nnz = static_cast<int>(H[l].A.col.size());
-            if (l == 0) { lv->ptr = A.ptr; lv->col = A.col; }
+            if (l == 0) { lv->ptr = A.ptr; lv->col = A.col; lv->sliceOff = A.sliceOff; lv->K = A.K; lv->nnz = P.nnzStore; }
             else {
                 lv->ptrOwn.upload(H[l].A.ptr, s);
                 lv->colOwn.upload(H[l].A.col, s);
@@ -446,6 +529,9 @@ void AmgDevice::setup(const CsrView<double>& A, const std::vector<int>& hptr, co
     kDenseLU<<<1, 1024, 0, s>>>(co.n, co.ptr, co.col, co.diag.p, co.val.p, lu.p, piv.p, luFlag.p);
     BSIM_LAUNCH_CHECK();
     if (luFlag.toHost(s)[0]) throw std::runtime_error("AMG coarsest matrix is singular");
+    inv.ensure(static_cast<std::size_t>(co.n) * co.n);
+    kDenseInverse<<<blocksFor(co.n), kBlock, 0, s>>>(co.n, lu.p, piv.p, inv.p);
+    BSIM_LAUNCH_CHECK();
     setupSeconds += since(t0);
 }
 
@@ -466,10 +552,10 @@ template <class T> void AmgDevice::cycle(int l) {
     T* t = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
     const T* dg = mixed ? reinterpret_cast<const T*>(lv.diag32.p) : reinterpret_cast<const T*>(lv.diag.p);
     const T* vl = mixed ? reinterpret_cast<const T*>(lv.val32.p) : reinterpret_cast<const T*>(lv.val.p);
-    const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl};
+    const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K};
     const int nb = blocksFor(lv.n);
     if (l == static_cast<int>(L.size()) - 1) {
-        kDenseSolve<T><<<1, 64, 0, s>>>(lv.n, lu.p, piv.p, b, x, lv.r.p);
+        kDenseApply<T><<<blocksFor(lv.n), kBlock, 0, s>>>(lv.n, inv.p, b, x);
         return;
     }
     const double w = cfg.amgOmega;
@@ -514,15 +600,12 @@ void AmgDevice::apply(const double* r, double* z) {
 }
 
 // ------------------------------------------------------------- systems
-DeviceSystem::DeviceSystem(hipStream_t st, int n_, const int* dptr, const int* dcol, const std::vector<int>& hp,
-                           const std::vector<int>& hc)
-    : n(n_), nnz(static_cast<int>(hc.size())), stream(st), ptr(dptr), col(dcol), hptr(hp), hcol(hc) {
+DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
+    : n(pattern.n), nnz(pattern.nnzStore), stream(st), pat(std::move(pattern)) {
     for (int k = 0; k < 2; ++k) { diagBuf[k].alloc(n); valBuf[k].alloc(std::max(nnz, 1)); }
     sc.alloc(S_COUNT);
     flags.alloc(1);
     partials.alloc(4 * kMaxReduceBlocks);
-    counter.alloc(1);
-    counter.zero(stream);
     BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostScalars), sizeof(double) * (S_COUNT + 2), hipHostMallocDefault));
     for (DBuf<double>* b : {&r, &z, &p, &w, &rp, &v, &t, &s, &tmp, &guess, &saved}) b->alloc(n);
 }
@@ -534,12 +617,11 @@ DeviceSystem::~DeviceSystem() {
 bool DeviceSystem::update() {
     bool changed = true;
     if (ready) {
-        cmp.ensure(1);
-        cmp.zero(stream);
-        kCompare<<<blocksFor(n), kBlock, 0, stream>>>(n, diagBuf[0].p, diagBuf[1].p, cmp.p);
-        if (nnz) kCompare<<<blocksFor(nnz), kBlock, 0, stream>>>(nnz, valBuf[0].p, valBuf[1].p, cmp.p);
+        const int rb = reduceBlocks(static_cast<long long>(n) + nnz);
+        kCompare<<<rb, kBlock, 0, stream>>>(n, diagBuf[0].p, diagBuf[1].p, nnz, valBuf[0].p, valBuf[1].p, partials.p);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, partials.p, StoreValue{sc.p, S_TMP});
         BSIM_LAUNCH_CHECK();
-        changed = cmp.toHost(stream, 1)[0] != 0;
+        changed = readScalar(S_TMP) != 0.0;
     }
     if (changed) {
         cand = 1 - cand;  // the candidate becomes the effective matrix
@@ -575,7 +657,7 @@ void DeviceSystem::setup(bool reuse) {
             BSIM_LAUNCH_CHECK();
         } else if (cfg.pc == 2) {
             if (!amg) amg = std::make_unique<AmgDevice>(stream);
-            amg->setup(view(), hptr, hcol, cfg);
+            amg->setup(view(), pat, cfg);
         }
         if (cfg.pc != 0) ++counters.pcSetups;
         pcMatrixUpdate = counters.matrixUpdates;
@@ -606,10 +688,7 @@ double DeviceSystem::readScalar(int idx, int* fl) {
 }
 
 void DeviceSystem::spmv(const double* x, double* y) {
-    kFill<<<blocksFor(n), kBlock, 0, stream>>>(n, tmp.p, 0.0);
-    kResidualNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(view(), tmp.p, x, y, sc.p, S_TMP, {partials.p, counter.p});
-    // y currently holds -A x
-    kAxpy<<<blocksFor(n), kBlock, 0, stream>>>(n, -2.0, y, y);
+    kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), x, y);
     BSIM_LAUNCH_CHECK();
 }
 
@@ -632,7 +711,9 @@ double DeviceSystem::timeSpmv(int reps) {
 }
 
 double DeviceSystem::trueResidual(const double* b, const double* x) {
-    kResidualNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(view(), b, x, tmp.p, sc.p, S_TR, {partials.p, counter.p});
+    const int rb = reduceBlocks(n);
+    kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, tmp.p, partials.p);
+    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, partials.p, StoreSqrt{sc.p, S_TR});
     BSIM_LAUNCH_CHECK();
     ++counters.spmvs;
     ++counters.trueResidualChecks;
@@ -640,15 +721,16 @@ double DeviceSystem::trueResidual(const double* b, const double* x) {
 }
 
 void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason) {
-    const ReduceSlot slot{partials.p, counter.p};
+    double* part = partials.p;
     const int rb = reduceBlocks(n), nb = blocksFor(n);
     flags.zero(stream);
     {
         const double init[S_COUNT] = {0, 1.0, 1.0, 0, 1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         BSIM_HIP(hipMemcpyAsync(sc.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
     }
-    if (!guessZero) { kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, sc.p, S_DP, slot); ++counters.spmvs; }
-    else kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, sc.p, S_DP, slot);
+    if (!guessZero) { kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, part); ++counters.spmvs; }
+    else kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, part);
+    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
     BSIM_LAUNCH_CHECK();
     double dp = readScalar(S_DP);
     its = 0;
@@ -662,12 +744,13 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
     if (test(dp)) return;
     auto pcDot = [&](int checkSign) {
         if (cfg.pc == 1) {
-            kJacobiDot<<<rb, kBlock, 0, stream>>>(n, r.p, jinv.p, z.p, sc.p, flags.p, checkSign, slot);
+            kJacobiDot<<<rb, kBlock, 0, stream>>>(n, r.p, jinv.p, z.p, flags.p, part);
             ++counters.pcApplications;
         } else {
             applyPC(r.p, z.p);
-            kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, sc.p, flags.p, checkSign, slot);
+            kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, flags.p, part);
         }
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgBeta{sc.p, flags.p, checkSign});
         BSIM_LAUNCH_CHECK();
     };
     pcDot(0);
@@ -676,8 +759,10 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
     do {
         its = i + 1;
         kCgDirection<<<nb, kBlock, 0, stream>>>(n, z.p, p.p, sc.p, flags.p, i == 0);
-        kCgSpmvDot<<<rb, kBlock, 0, stream>>>(view(), p.p, w.p, sc.p, flags.p, i, slot);
-        kCgUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, r.p, w.p, sc.p, flags.p, slot);
+        kCgSpmvDot<<<rb, kBlock, 0, stream>>>(view(), p.p, w.p, flags.p, part);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgAlpha{sc.p, flags.p, i});
+        kCgUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, r.p, w.p, sc.p, flags.p, part);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgNorm{sc.p, flags.p});
         BSIM_LAUNCH_CHECK();
         ++counters.spmvs;
         dp = readScalar(S_DP, &fl);
@@ -694,7 +779,7 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
 }
 
 void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason) {
-    const ReduceSlot slot{partials.p, counter.p};
+    double* part = partials.p;
     const int rb = reduceBlocks(n), nb = blocksFor(n);
     flags.zero(stream);
     {
@@ -703,14 +788,15 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     }
     bool haveGuess = false;
     if (!guessZero) {
-        kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, sc.p, S_DP, slot);
+        kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, part);
         ++counters.spmvs;
         kCopy<<<nb, kBlock, 0, stream>>>(n, x, guess.p);
         kFill<<<nb, kBlock, 0, stream>>>(n, x, 0.0);
         haveGuess = true;
     } else {
-        kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, sc.p, S_DP, slot);
+        kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, part);
     }
+    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
     BSIM_LAUNCH_CHECK();
     double dp = readScalar(S_DP);
     its = 0;
@@ -730,22 +816,27 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     kFill<<<nb, kBlock, 0, stream>>>(n, v.p, 0.0);
     int i = 0, fl = 0;
     do {
-        kBcgsRho<<<rb, kBlock, 0, stream>>>(n, r.p, rp.p, sc.p, slot);
+        kBcgsRho<<<rb, kBlock, 0, stream>>>(n, r.p, rp.p, part);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsRho{sc.p});
         kBcgsP<<<nb, kBlock, 0, stream>>>(n, p.p, r.p, v.p, sc.p);
         applyPC(p.p, tmp.p);
-        kBcgsSpmvD1<<<rb, kBlock, 0, stream>>>(view(), tmp.p, v.p, rp.p, sc.p, flags.p, slot);
+        kBcgsSpmvD1<<<rb, kBlock, 0, stream>>>(view(), tmp.p, v.p, rp.p, part);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsAlpha{sc.p, flags.p});
         kBcgsS<<<nb, kBlock, 0, stream>>>(n, s.p, v.p, r.p, sc.p, flags.p);
         BSIM_LAUNCH_CHECK();
         applyPC(s.p, r.p);
-        kBcgsSpmvD2<<<rb, kBlock, 0, stream>>>(view(), r.p, t.p, s.p, sc.p, flags.p, slot);
-        kBcgsUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, s.p, r.p, t.p, sc.p, flags.p, slot);
+        kBcgsSpmvD2<<<rb, kBlock, 0, stream>>>(view(), r.p, t.p, s.p, flags.p, part);
+        kReduceTail<2><<<1, kBlock, 0, stream>>>(rb, part, BcgsOmega{sc.p, flags.p});
+        kBcgsUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, s.p, r.p, t.p, sc.p, flags.p, part);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsNorm{sc.p, flags.p});
         BSIM_LAUNCH_CHECK();
         counters.spmvs += 2;
         const double newDp = readScalar(S_DP, &fl);
         if (fl & F_NONFINITE) { reason = -9; break; }
         if (fl & F_D1_ZERO) { reason = -5; break; }
         if (fl & F_D2_ZERO) {
-            kDot<<<rb, kBlock, 0, stream>>>(n, s.p, s.p, sc.p, S_TMP, slot);
+            kDot<<<rb, kBlock, 0, stream>>>(n, s.p, s.p, part);
+            kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreValue{sc.p, S_TMP});
             BSIM_LAUNCH_CHECK();
             if (readScalar(S_TMP) != 0.0) { reason = -5; break; }
             const double alpha = readScalar(S_ALPHA);
@@ -790,7 +881,8 @@ SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
     };
     resetGuess();
     BSIM_LAUNCH_CHECK();
-    kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, sc.p, S_BN, {partials.p, counter.p});
+    kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, partials.p);
+    kReduceTail<1><<<1, kBlock, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
     const double rhsNorm = readScalar(S_BN);
     const double initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
     const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;

@@ -30,13 +30,27 @@ struct SolveRes {
     double initial = 0, final_ = 0, relative = 0;
 };
 
-// Matrix with the diagonal stored apart from the off-diagonal CSR entries.
+// Matrix with the diagonal stored apart from the off-diagonal entries, which
+// are either row-major CSR (ptr) or the mesh's sliced layout (sliceOff, K; see
+// RowRange in dmesh.hpp; padding entries carry column -1).
 template <class T> struct CsrView {
     int n;
     const int* ptr;
     const int* col;
     const T* diag;
     const T* val;
+    const int* sliceOff;
+    int K;
+};
+
+// Storage pattern of a linear system's off-diagonal values.
+struct Pattern {
+    int n = 0, nnzStore = 0, K = 0;
+    const int* ptr = nullptr;       // device CSR row pointers, or nullptr when sliced
+    const int* col = nullptr;       // device columns in storage order
+    const int* sliceOff = nullptr;  // device slice offsets when sliced
+    std::vector<int> hptr, hcol;    // host row-major CSR
+    std::vector<int> csrToStore;    // row-major entry -> storage position (empty: identity)
 };
 
 class AmgDevice;
@@ -49,10 +63,9 @@ struct SystemCounters {
 
 class DeviceSystem {
 public:
-    DeviceSystem(hipStream_t stream, int n, const int* dptr, const int* dcol, const std::vector<int>& hptr,
-                 const std::vector<int>& hcol);
+    DeviceSystem(hipStream_t stream, Pattern pattern);
     ~DeviceSystem();
-    int n, nnz;
+    int n, nnz;  // rows, stored off-diagonal entries
     // Assembly writes the candidate matrix here; update() compares it with the
     // effective matrix and swaps it in when any value differs.
     double* candDiag() { return diagBuf[cand].p; }
@@ -67,11 +80,9 @@ public:
     std::vector<double> hist;               // unpreconditioned residual norm per Krylov iteration
     SystemCounters counters;
     SolverCfg cfg;
-    CsrView<double> view() const { return {n, ptr, col, diag(), val()}; }
+    CsrView<double> view() const { return {n, pat.ptr, pat.col, diag(), val(), pat.sliceOff, pat.K}; }
     hipStream_t stream;
-    const int* ptr;
-    const int* col;
-    std::vector<int> hptr, hcol;
+    Pattern pat;
 
 private:
     void setup(bool reuse);
@@ -91,9 +102,8 @@ private:
     DBuf<double> r, z, p, w, rp, v, t, s, tmp, guess, saved;
     DBuf<float> r32, z32, p32, w32, rp32, v32, t32, s32, x32, tmp32, diag32, val32;
     DBuf<double> sc;            // device scalars
-    DBuf<int> flags, cmp;
-    DBuf<double> partials;
-    DBuf<unsigned int> counter;
+    DBuf<int> flags;
+    DBuf<double> partials;      // per-block reduction partials
     double* hostScalars = nullptr;  // pinned mirror
 };
 
@@ -101,8 +111,7 @@ private:
 class AmgDevice {
 public:
     AmgDevice(hipStream_t s) : stream(s) {}
-    void setup(const CsrView<double>& A, const std::vector<int>& hptr, const std::vector<int>& hcol,
-               const SolverCfg& cfg);
+    void setup(const CsrView<double>& A, const Pattern& pattern, const SolverCfg& cfg);
     void apply(const double* r, double* z);  // z = M^{-1} r (FP64 in/out)
     int levels() const { return static_cast<int>(L.size()); }
     std::vector<int> sizes() const;
@@ -112,9 +121,10 @@ public:
 
 private:
     struct Level {
-        int n = 0, nnz = 0, nc = 0;
+        int n = 0, nnz = 0, nc = 0, K = 0;
         const int* ptr = nullptr;
         const int* col = nullptr;
+        const int* sliceOff = nullptr;
         DBuf<int> ptrOwn, colOwn, agg, memPtr, memIdx, gPtr, gSrc;
         DBuf<double> diag, val;
         DBuf<float> diag32, val32;
@@ -124,7 +134,7 @@ private:
     template <class T> void cycle(int l);
     hipStream_t stream;
     std::vector<std::unique_ptr<Level>> L;
-    DBuf<double> lu;
+    DBuf<double> lu, inv;
     DBuf<int> piv, luFlag;
     bool aggregated = false;
     bool mixed = false;

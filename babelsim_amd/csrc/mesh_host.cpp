@@ -312,6 +312,14 @@ MeshDesc prepare(const MeshDesc& in, bool renumber, std::vector<int>& cellNewToO
             }
         }
         std::reverse(order.begin(), order.end());
+        // Within each 64-cell slice (one wavefront of row threads) order the
+        // rows by decreasing number of interior faces, so the k-th neighbours
+        // of a slice's rows form one contiguous, unpadded run in HBM.
+        std::vector<int> nfaces(nc, 0);
+        for (int f = 0; f < nf; ++f) if (in.nei[f] >= 0) { ++nfaces[in.own[f]]; ++nfaces[in.nei[f]]; }
+        for (int s0 = 0; s0 < nc; s0 += 64)
+            std::stable_sort(order.begin() + s0, order.begin() + std::min(nc, s0 + 64),
+                             [&](int x, int y) { return nfaces[x] > nfaces[y]; });
         for (int i = 0; i < nc; ++i) { cellNewToOld[i] = order[i]; oldToNew[order[i]] = i; }
     } else {
         std::iota(cellNewToOld.begin(), cellNewToOld.end(), 0);

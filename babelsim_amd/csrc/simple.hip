@@ -45,14 +45,16 @@ __global__ void kMomentumPressureSource(MeshView m, const double* gradP, double*
 // diagnostics::relativeResidual partial sums of |b-Ax|^2, |Ax|^2, |b|^2
 // (procedural_equation.cpp:429-433) on the unrelaxed momentum rows.
 __global__ void kMomentumResidual(MeshView m, const double* diag, const double* off, const double* src, const double* U,
-                                  double* red, ReduceSlot slot) {
+                                  double* part) {
     __shared__ double smem[3 * kBlock];
     double a[3] = {0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.nc; i += gridDim.x * blockDim.x) {
         double ax[3];
         for (int k = 0; k < 3; ++k) ax[k] = diag[i] * U[3 * static_cast<long long>(i) + k];
-        for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e)
+        for (int k_ = 0, e = 0; m.row(i).valid(k_) && (e = m.row(i).at(k_), true); ++k_) {
+            if (m.mcol[e] < 0) continue;
             for (int k = 0; k < 3; ++k) ax[k] += off[e] * U[3 * static_cast<long long>(m.mcol[e]) + k];
+        }
         double r2 = 0, a2 = 0, b2 = 0;
         for (int k = 0; k < 3; ++k) {
             const double b = src[3 * static_cast<long long>(i) + k];
@@ -61,8 +63,7 @@ __global__ void kMomentumResidual(MeshView m, const double* diag, const double* 
         }
         a[0] += r2; a[1] += a2; a[2] += b2;
     }
-    double out[3];
-    if (reduceFinish<3>(a, slot, smem, out) && threadIdx.x == 0) { red[0] = out[0]; red[1] = out[1]; red[2] = out[2]; }
+    blockPartial<3>(a, part, smem);
 }
 
 // relax (procedural_equation.cpp:298-305), scale (240-247) and rAU = V/aP (main.cpp:62-66).
@@ -79,7 +80,7 @@ __global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, 
     }
     d *= alpha;
     diag[i] = d;
-    for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e) off[e] *= alpha;
+    for (int k_ = 0, e = 0; m.row(i).valid(k_) && (e = m.row(i).at(k_), true); ++k_) off[e] *= alpha;
     rAU[i] = m.V[i] / d;
 }
 
@@ -121,7 +122,11 @@ __global__ void kReferenceTest(MeshView m, const double* diag, const double* off
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.nc) return;
     double rowSum = diag[i], mag = fabs(diag[i]);
-    for (int e = m.mptr[i]; e < m.mptr[i + 1]; ++e) { rowSum += off[e]; mag += fabs(off[e]); }
+    for (int k_ = 0, e = 0; m.row(i).valid(k_) && (e = m.row(i).at(k_), true); ++k_) {
+        if (m.mcol[e] < 0) continue;
+        rowSum += off[e];
+        mag += fabs(off[e]);
+    }
     if (!(isfinite(rowSum) && fabs(rowSum) <= 64 * 2.220446049250313e-16 * mag)) atomicOr(notConstant, 1);
 }
 // reference(global cell 0, value) (procedural_equation.cpp:306-315).
@@ -166,7 +171,7 @@ __global__ void kCorrectUP(MeshView m, double* p, const double* pp, double alpha
 
 // relativeChange, relativeMagnitude and fluxBalance sums (fvm_execution.cpp:91-177).
 __global__ void kConvergence(MeshView m, const double* U, const double* Up, const double* pp, const double* p,
-                             const double* phi, double* red, ReduceSlot slot) {
+                             const double* phi, double* part) {
     __shared__ double smem[kRedK * kBlock];
     double a[kRedK] = {0, 0, 0, 0, 0, 0};
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nc; c += gridDim.x * blockDim.x) {
@@ -177,8 +182,9 @@ __global__ void kConvergence(MeshView m, const double* U, const double* Up, cons
         a[2] += pp[c] * pp[c];
         a[3] += p[c] * p[c];
         double imb = 0.0, sc = 0.0;
-        for (int e = m.mptr[c]; e < m.mptr[c + 1]; ++e) {
+        for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
             const int f = m.mface[e];
+            if (f < 0) continue;
             const double out = m.own[f] == c ? phi[f] : -phi[f];
             imb += out;
             sc += fabs(out);
@@ -191,9 +197,7 @@ __global__ void kConvergence(MeshView m, const double* U, const double* Up, cons
         a[4] += fabs(imb);
         a[5] += sc;
     }
-    double out[kRedK];
-    if (reduceFinish<kRedK>(a, slot, smem, out) && threadIdx.x == 0)
-        for (int k = 0; k < kRedK; ++k) red[k] = out[k];
+    blockPartial<kRedK>(a, part, smem);
 }
 
 }  // namespace
@@ -224,12 +228,10 @@ SimpleSolver::SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, 
     for (DBuf<double>* b : {&phi, &phiH, &rAUtr, &rAUf, &sn, &kfMom, &kfP, &corr, &full, &ctxSeed}) b->alloc(nf);
     for (DBuf<double>* b : {&gtr, &Xtr, &Xf}) b->alloc(3 * static_cast<std::size_t>(nf));
     red.alloc(kRedK * kMaxReduceBlocks + 16);
-    redCounter.alloc(1);
-    redCounter.zero(s);
     U.zero(s); p.zero(s); phi.zero(s); corr.zero(s);
     const MeshView mv = M.view();
-    momSys = std::make_unique<DeviceSystem>(s, nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
-    pSys = std::make_unique<DeviceSystem>(s, nc, M.mptr.p, M.mcol.p, M.hmptr, M.hmcol);
+    momSys = std::make_unique<DeviceSystem>(s, meshPattern(M));
+    pSys = std::make_unique<DeviceSystem>(s, meshPattern(M));
     refCell = C.cellOldToNew.at(0);
     // muEffective: constant cell field with default (zeroGradient) patches
     // (src/physics/RANS/model.cpp:58-63), interpolated with the momentum
@@ -291,7 +293,7 @@ IterRep SimpleSolver::iterate() {
     hipStream_t s = C.stream;
     const int nc = M.nc, nf = M.nf;
     const int nb = blocksFor(nc);
-    const ReduceSlot slot{red.p + 16, redCounter.p};
+    double* part = red.p + 16;
     IterRep rep;
     auto t0 = Clock::now();
     // U_previous = U (main.cpp:48)
@@ -300,7 +302,7 @@ IterRep SimpleSolver::iterate() {
     double* diag = momSys->candDiag();
     double* off = momSys->candVal();
     kFillD<<<nb, kBlock, 0, s>>>(nc, diag, 0.0);
-    kFillD<<<blocksFor(std::max(M.nnzOff, 1)), kBlock, 0, s>>>(M.nnzOff, off, 0.0);
+    kFillD<<<blocksFor(std::max(M.nnzSliced, 1)), kBlock, 0, s>>>(M.nnzSliced, off, 0.0);
     kFillD<<<blocksFor(3LL * nc), kBlock, 0, s>>>(3LL * nc, src.p, 0.0);
     const FieldArg Uf = fieldU(phi.p);
     const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
@@ -311,7 +313,8 @@ IterRep SimpleSolver::iterate() {
     ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
     kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
     // rU before relaxation (main.cpp:60)
-    kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, red.p, slot);
+    kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, part);
+    kReduceTail<3><<<1, kBlock, 0, s>>>(reduceBlocks(nc), part, StoreSums<3>{red.p});
     kRelaxScale<<<nb, kBlock, 0, s>>>(mv, diag, off, src.p, Uprev.p, c.alphaU, rAU.p);
     if (nf > M.nfi) kOwnerTrace<<<blocksFor(nf - M.nfi), kBlock, 0, s>>>(mv, rAU.p, rAUtr.p);
     BSIM_LAUNCH_CHECK();
@@ -378,7 +381,7 @@ IterRep SimpleSolver::iterate() {
         double* pd = pSys->candDiag();
         double* po = pSys->candVal();
         kFillD<<<nb, kBlock, 0, s>>>(nc, pd, 0.0);
-        kFillD<<<blocksFor(std::max(M.nnzOff, 1)), kBlock, 0, s>>>(M.nnzOff, po, 0.0);
+        kFillD<<<blocksFor(std::max(M.nnzSliced, 1)), kBlock, 0, s>>>(M.nnzSliced, po, 0.0);
         kFillD<<<nb, kBlock, 0, s>>>(nc, psrc.p, 0.0);
         if (c.pDiff != kOrthogonal) {
             ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradPP.p, work9.p, C.err.p, s);
@@ -423,7 +426,8 @@ IterRep SimpleSolver::iterate() {
     timing.correction += since(t0);
     t0 = Clock::now();
     // Convergence measures (main.cpp:112-126).
-    kConvergence<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uprev.p, pPrime.p, p.p, phi.p, red.p, slot);
+    kConvergence<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uprev.p, pPrime.p, p.p, phi.p, part);
+    kReduceTail<kRedK><<<1, kBlock, 0, s>>>(reduceBlocks(nc), part, StoreSums<kRedK>{red.p});
     BSIM_LAUNCH_CHECK();
     double h[kRedK];
     BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -57,7 +57,6 @@ private:
     DBuf<double> uVal, pVal, ppVal, zgVal;
     DBuf<double> U, p, phi, phiH, Uprev, gradU, gradP, gradPP, work9, rAU, rAUtr, gtr, X, Xtr, Xf, gradX, rAUf, sn,
         kfMom, mu, kfP, divPhiH, pPrime, corr, full, src, psrc, bcomp, xcomp, ctxSeed, red;
-    DBuf<unsigned int> redCounter;
     std::unique_ptr<DeviceSystem> momSys, pSys;
     bool started = false;
 };

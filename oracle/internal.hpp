@@ -216,6 +216,7 @@ struct AmgLevel {
 struct Amg {
     std::vector<AmgLevel> levels;
     std::vector<double> coarseLU;    // dense LU of coarsest, row-major
+    std::vector<double> coarseInv;   // its explicit inverse, row-major
     std::vector<Index> coarsePiv;
     SolverConfig cfg;
     std::vector<std::vector<double>> wx, wb, wr;

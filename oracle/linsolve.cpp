@@ -65,12 +65,12 @@ void spmv(const Csr& A, const double* x, double* y) {
 namespace {
 
 // Inner products are summed in a fixed blocked order: 256 strided partial sums
-// per block over at most 1024 blocks, pairwise trees inside each block, then
+// per block over at most 4096 blocks, pairwise trees inside each block, then
 // the same over the block sums.  Both sides of the parity check use this order
 // (it is the deterministic reduction of the device library), so FP64 Krylov
 // histories can be compared bit for bit.
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
-    constexpr long B = 256, Gmax = 1024;
+    constexpr long B = 256, Gmax = 4096;
     const long n = static_cast<long>(a.size());
     long G = (n + B - 1) / B;
     G = G < 1 ? 1 : (G > Gmax ? Gmax : G);
@@ -232,6 +232,20 @@ void Amg::setup(const Csr& A, const SolverConfig& c) {
             for (Index j = k + 1; j < n; ++j) coarseLU[static_cast<std::size_t>(i) * n + j] -= l * coarseLU[static_cast<std::size_t>(k) * n + j];
         }
     }
+    // Explicit inverse: column j is the LU solve of e_j.
+    coarseInv.assign(static_cast<std::size_t>(n) * n, 0.0);
+    std::vector<double> w(n);
+    for (Index j = 0; j < n; ++j) {
+        for (Index i = 0; i < n; ++i) w[i] = i == j ? 1.0 : 0.0;
+        for (Index k = 0; k < n; ++k) std::swap(w[k], w[coarsePiv[k]]);
+        for (Index i = 0; i < n; ++i)
+            for (Index q = 0; q < i; ++q) w[i] -= coarseLU[static_cast<std::size_t>(i) * n + q] * w[q];
+        for (Index i = n - 1; i >= 0; --i) {
+            for (Index q = i + 1; q < n; ++q) w[i] -= coarseLU[static_cast<std::size_t>(i) * n + q] * w[q];
+            w[i] /= coarseLU[static_cast<std::size_t>(i) * n + i];
+        }
+        for (Index i = 0; i < n; ++i) coarseInv[static_cast<std::size_t>(i) * n + j] = w[i];
+    }
     wx.assign(levels.size(), {}); wb.assign(levels.size(), {}); wr.assign(levels.size(), {});
     for (std::size_t l = 0; l < levels.size(); ++l) {
         wx[l].assign(levels[l].A.n, 0.0); wb[l].assign(levels[l].A.n, 0.0); wr[l].assign(levels[l].A.n, 0.0);
@@ -244,14 +258,11 @@ void Amg::vcycle(int l) {
     std::vector<double>& b = wb[l];
     std::vector<double>& r = wr[l];
     const Index n = A.n;
-    if (l == static_cast<int>(levels.size()) - 1) {  // dense LU solve
-        for (Index i = 0; i < n; ++i) x[i] = b[i];
-        for (Index k = 0; k < n; ++k) std::swap(x[k], x[coarsePiv[k]]);
-        for (Index i = 0; i < n; ++i)
-            for (Index j = 0; j < i; ++j) x[i] -= coarseLU[static_cast<std::size_t>(i) * n + j] * x[j];
-        for (Index i = n - 1; i >= 0; --i) {
-            for (Index j = i + 1; j < n; ++j) x[i] -= coarseLU[static_cast<std::size_t>(i) * n + j] * x[j];
-            x[i] /= coarseLU[static_cast<std::size_t>(i) * n + i];
+    if (l == static_cast<int>(levels.size()) - 1) {  // dense solve with the explicit inverse
+        for (Index i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (Index j = 0; j < n; ++j) s += coarseInv[static_cast<std::size_t>(i) * n + j] * b[j];
+            x[i] = s;
         }
         return;
     }
