@@ -137,4 +137,63 @@ std::vector<HostLevel> buildHierarchy(const HostCsr& A0, double theta, int maxLe
     return L;
 }
 
+SlicedPattern sliceRows(const HostCsr& A) {
+    SlicedPattern S;
+    const int n = A.n, nslices = (n + 63) / 64;
+    for (int i = 0; i < n; ++i) S.K = std::max(S.K, A.ptr[i + 1] - A.ptr[i]);
+    S.perm.resize(n);
+    for (int i = 0; i < n; ++i) S.perm[i] = i;
+    auto len = [&](int i) { return A.ptr[i + 1] - A.ptr[i]; };
+    for (int s = 0; s < nslices; ++s)
+        std::stable_sort(S.perm.begin() + 64 * s, S.perm.begin() + std::min(n, 64 * s + 64),
+                         [&](int a, int b) { return len(a) > len(b); });
+    S.sliceOff.assign(static_cast<std::size_t>(nslices) * (S.K + 1), 0);
+    int base = 0;
+    for (int s = 0; s < nslices; ++s) {
+        int* so = &S.sliceOff[static_cast<std::size_t>(s) * (S.K + 1)];
+        const int lanes = std::min(64, n - 64 * s);
+        so[0] = base;
+        for (int k = 0; k < S.K; ++k) {
+            int width = 0;
+            while (width < lanes && len(S.perm[64 * s + width]) > k) ++width;
+            so[k + 1] = so[k] + width;
+        }
+        base = so[S.K];
+    }
+    S.nnz = base;
+    S.col.assign(std::max(S.nnz, 1), -1);
+    S.csrToStore.assign(A.ptr[n], 0);
+    for (int q = 0; q < n; ++q) {
+        const int i = S.perm[q];
+        const int* so = &S.sliceOff[static_cast<std::size_t>(q / 64) * (S.K + 1)];
+        for (int k = 0; k < len(i); ++k) {
+            const int e = so[k] + q % 64;
+            S.csrToStore[A.ptr[i] + k] = e;
+            S.col[e] = A.col[A.ptr[i] + k];
+        }
+    }
+    return S;
+}
+
+void remapGatherDest(std::vector<int>& gPtr, std::vector<int>& gSrc, int nc, const std::vector<int>& csrToStore) {
+    const int ne = static_cast<int>(gPtr.size()) - 1;
+    std::vector<int> owner(ne);  // destination -> original list
+    for (int e = 0; e < nc; ++e) owner[e] = e;
+    for (int k = 0; k + nc < ne; ++k) owner[nc + csrToStore[k]] = nc + k;
+    std::vector<int> np(1, 0), ns;
+    ns.reserve(gSrc.size());
+    for (int e = 0; e < ne; ++e) {
+        const int o = owner[e];
+        ns.insert(ns.end(), gSrc.begin() + gPtr[o], gSrc.begin() + gPtr[o + 1]);
+        np.push_back(static_cast<int>(ns.size()));
+    }
+    gPtr.swap(np);
+    gSrc.swap(ns);
+}
+
+void remapGatherSrc(std::vector<int>& gSrc, int n, const std::vector<int>& csrToStore) {
+    for (int& src : gSrc)
+        if (src >= n) src = n + csrToStore[src - n];
+}
+
 }  // namespace bsim

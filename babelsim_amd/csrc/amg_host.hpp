@@ -34,4 +34,20 @@ void galerkinSymbolic(const HostCsr& A, const std::vector<int>& agg, int nc, Hos
 // Builds the level structure (aggregation on each level) from the fine matrix.
 std::vector<HostLevel> buildHierarchy(const HostCsr& A0, double theta, int maxLevels, int coarseSize);
 
+// Sliced storage of a CSR pattern for wavefront-wide row kernels: storage
+// slot q (slice q / 64, lane q % 64) holds row perm[q]; the rows of a slice
+// are ordered by decreasing length, and the k-th entry of the row in lane l
+// sits at sliceOff[slice * (K + 1) + k] + l.  Entries keep their CSR order.
+struct SlicedPattern {
+    int K = 0, nnz = 0;
+    std::vector<int> sliceOff, col, perm, csrToStore;
+};
+SlicedPattern sliceRows(const HostCsr& A);
+
+// Rewrites a Galerkin gather whose destination level is stored sliced:
+// coarse value nc + k (CSR entry k) moves to nc + csrToStore[k].
+void remapGatherDest(std::vector<int>& gPtr, std::vector<int>& gSrc, int nc, const std::vector<int>& csrToStore);
+// Rewrites gather sources that address a sliced level: n + k -> n + csrToStore[k].
+void remapGatherSrc(std::vector<int>& gSrc, int n, const std::vector<int>& csrToStore);
+
 }  // namespace bsim

@@ -80,13 +80,17 @@ template <class T> struct DBuf {
 // wavefront reading level k touches one contiguous run.  Rows in a slice are
 // ordered by decreasing length when the mesh is renumbered; otherwise the gaps
 // are padding entries with face and column -1.
+// Every kernel that walks rows runs blocks of whole wavefronts over
+// consecutive rows (or strides by a multiple of 64), so all lanes of a
+// wavefront share one slice and its offsets are read with scalar loads.
+static_assert(kBlock % 64 == 0, "row kernels need blocks of whole wavefronts");
 struct RowRange {
     const int* so;
     int lane, K;
-    __host__ __device__ RowRange(const int* sliceOff, int K_, int c)
-        : so(sliceOff + static_cast<long long>(c >> 6) * (K_ + 1)), lane(c & 63), K(K_) {}
-    __host__ __device__ bool valid(int k) const { return k < K && lane < so[k + 1] - so[k]; }
-    __host__ __device__ int at(int k) const { return so[k] + lane; }
+    __device__ RowRange(const int* sliceOff, int K_, int c)
+        : so(sliceOff + static_cast<long long>(__builtin_amdgcn_readfirstlane(c >> 6)) * (K_ + 1)), lane(c & 63), K(K_) {}
+    __device__ bool valid(int k) const { return k < K && lane < so[k + 1] - so[k]; }
+    __device__ int at(int k) const { return so[k] + lane; }
 };
 
 // ---- device vector helpers (component-wise, AoS double[3]) ----

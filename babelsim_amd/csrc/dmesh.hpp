@@ -19,7 +19,7 @@ struct MeshView {
     const int *bptr, *bface;             // boundary faces per cell
     const int *slotOwn, *slotNei;        // interior face -> off-diagonal slot in owner / neighbour row
     const int* patchStart;               // boundary faces of patch p: [nfi + patchStart[p], nfi + patchStart[p+1])
-    __host__ __device__ RowRange row(int c) const { return RowRange(sliceOff, K, c); }
+    __device__ RowRange row(int c) const { return RowRange(sliceOff, K, c); }
 };
 
 class DeviceMesh {

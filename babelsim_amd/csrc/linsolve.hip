@@ -23,22 +23,40 @@ double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::
 
 __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 
-// Row i of A x: the diagonal first, then the off-diagonal entries in row order
-// (the operation order of the oracle's spmv); padding entries are skipped.
-template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int i) {
+// Row i (held in storage slot q) of A x: the diagonal first, then the
+// off-diagonal entries in row order (the operation order of the oracle's
+// spmv); padding entries are skipped.
+template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int q, int i) {
     double y = static_cast<double>(A.diag[i]) * static_cast<double>(x[i]);
     if (A.sliceOff) {
-        const RowRange R(A.sliceOff, A.K, i);
-        for (int k = 0; R.valid(k); ++k) {
-            const int e = R.at(k), c = A.col[e];
-            if (c >= 0) y += static_cast<double>(A.val[e]) * static_cast<double>(x[c]);
+        // Entries are fetched four levels at a time so that their column,
+        // value and x loads overlap; the sum still runs in entry order.
+        const RowRange R(A.sliceOff, A.K, q);
+        constexpr int kC = 4;
+        for (int k0 = 0; k0 < A.K; k0 += kC) {
+            if (R.so[k0 + 1] == R.so[k0]) break;  // no row of the slice has more entries
+            int e[kC], c[kC];
+            T v[kC], xv[kC];
+#pragma unroll
+            for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
+#pragma unroll
+            for (int j = 0; j < kC; ++j) {
+                c[j] = e[j] >= 0 ? A.col[e[j]] : -1;
+                v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
+#pragma unroll
+            for (int j = 0; j < kC; ++j)
+                if (c[j] >= 0) y += static_cast<double>(v[j]) * static_cast<double>(xv[j]);
         }
     } else {
         for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
     }
     return y;
 }
-__device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i); }
+// Krylov kernels run on matrices whose slot q holds row q.
+__device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
 
 __global__ void kFill(int n, double* x, double v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,14 +354,17 @@ template <class T> __global__ void kJacobiFirst(int n, const T* d, const T* b, T
     if (i < n) x[i] = static_cast<T>(omega * static_cast<double>(b[i]) / static_cast<double>(d[i]));
 }
 template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
-    const double ax = rowDot(A, x, i);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.n) return;
+    const int i = A.perm ? A.perm[q] : q;
+    const double ax = rowDot(A, x, q, i);
     xo[i] = static_cast<T>(static_cast<double>(x[i]) + omega * (static_cast<double>(b[i]) - ax) / static_cast<double>(A.diag[i]));
 }
 template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < A.n) r[i] = static_cast<T>(static_cast<double>(b[i]) - rowDot(A, x, i));
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.n) return;
+    const int i = A.perm ? A.perm[q] : q;
+    r[i] = static_cast<T>(static_cast<double>(b[i]) - rowDot(A, x, q, i));
 }
 template <class T> __global__ void kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,16 +490,33 @@ void AmgDevice::setup(const CsrView<double>& A, const Pattern& P, const SolverCf
         for (std::size_t q = 0; q < P.hcol.size(); ++q) A0.val[q] = stored[P.csrToStore.empty() ? q : P.csrToStore[q]];
         std::vector<HostLevel> H = buildHierarchy(A0, c.amgStrength, c.amgMaxLevels, c.amgCoarseSize);
         // Level-0 Galerkin sources address the stored (possibly sliced) values.
-        if (!P.csrToStore.empty() && !H[0].gSrc.empty())
-            for (int& src : H[0].gSrc)
-                if (src >= A.n) src = A.n + P.csrToStore[src - A.n];
+        if (!P.csrToStore.empty() && !H[0].gSrc.empty()) remapGatherSrc(H[0].gSrc, A.n, P.csrToStore);
+        // Smoothed coarse levels are stored sliced (the coarsest stays CSR for
+        // the dense factorisation); the gathers that write and read their
+        // values are rewritten to the sliced positions.
+        std::vector<SlicedPattern> sliced(H.size());
+        for (std::size_t l = 1; l + 1 < H.size(); ++l) {
+            sliced[l] = sliceRows(H[l].A);
+            remapGatherDest(H[l - 1].gPtr, H[l - 1].gSrc, H[l].A.n, sliced[l].csrToStore);
+            remapGatherSrc(H[l].gSrc, H[l].A.n, sliced[l].csrToStore);
+        }
         L.clear();
         for (std::size_t l = 0; l < H.size(); ++l) {
             auto lv = std::make_unique<Level>();
             lv->n = H[l].A.n;
             lv->nnz = static_cast<int>(H[l].A.col.size());
             if (l == 0) { lv->ptr = A.ptr; lv->col = A.col; lv->sliceOff = A.sliceOff; lv->K = A.K; lv->nnz = P.nnzStore; }
-            else {
+            else if (l + 1 < H.size()) {
+                const SlicedPattern& S = sliced[l];
+                lv->nnz = S.nnz;
+                lv->K = S.K;
+                lv->colOwn.upload(S.col, s);
+                lv->sliceOwn.upload(S.sliceOff, s);
+                lv->permOwn.upload(S.perm, s);
+                lv->col = lv->colOwn.p;
+                lv->sliceOff = lv->sliceOwn.p;
+                lv->perm = lv->permOwn.p;
+            } else {
                 lv->ptrOwn.upload(H[l].A.ptr, s);
                 lv->colOwn.upload(H[l].A.col, s);
                 lv->ptr = lv->ptrOwn.p;
@@ -552,7 +590,7 @@ template <class T> void AmgDevice::cycle(int l) {
     T* t = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
     const T* dg = mixed ? reinterpret_cast<const T*>(lv.diag32.p) : reinterpret_cast<const T*>(lv.diag.p);
     const T* vl = mixed ? reinterpret_cast<const T*>(lv.val32.p) : reinterpret_cast<const T*>(lv.val.p);
-    const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K};
+    const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K, lv.perm};
     const int nb = blocksFor(lv.n);
     if (l == static_cast<int>(L.size()) - 1) {
         kDenseApply<T><<<blocksFor(lv.n), kBlock, 0, s>>>(lv.n, inv.p, b, x);

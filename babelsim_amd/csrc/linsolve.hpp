@@ -31,8 +31,9 @@ struct SolveRes {
 };
 
 // Matrix with the diagonal stored apart from the off-diagonal entries, which
-// are either row-major CSR (ptr) or the mesh's sliced layout (sliceOff, K; see
-// RowRange in dmesh.hpp; padding entries carry column -1).
+// are either row-major CSR (ptr) or a sliced layout (sliceOff, K; see RowRange
+// in device_common.hpp; padding entries carry column -1).  In the sliced
+// layout, storage slot q holds row perm[q] (perm == nullptr: row q).
 template <class T> struct CsrView {
     int n;
     const int* ptr;
@@ -41,6 +42,7 @@ template <class T> struct CsrView {
     const T* val;
     const int* sliceOff;
     int K;
+    const int* perm;
 };
 
 // Storage pattern of a linear system's off-diagonal values.
@@ -80,7 +82,7 @@ public:
     std::vector<double> hist;               // unpreconditioned residual norm per Krylov iteration
     SystemCounters counters;
     SolverCfg cfg;
-    CsrView<double> view() const { return {n, pat.ptr, pat.col, diag(), val(), pat.sliceOff, pat.K}; }
+    CsrView<double> view() const { return {n, pat.ptr, pat.col, diag(), val(), pat.sliceOff, pat.K, nullptr}; }
     hipStream_t stream;
     Pattern pat;
 
@@ -125,7 +127,8 @@ private:
         const int* ptr = nullptr;
         const int* col = nullptr;
         const int* sliceOff = nullptr;
-        DBuf<int> ptrOwn, colOwn, agg, memPtr, memIdx, gPtr, gSrc;
+        const int* perm = nullptr;
+        DBuf<int> ptrOwn, colOwn, sliceOwn, permOwn, agg, memPtr, memIdx, gPtr, gSrc;
         DBuf<double> diag, val;
         DBuf<float> diag32, val32;
         DBuf<double> x, b, r, t;
