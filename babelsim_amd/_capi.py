@@ -146,6 +146,8 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_simple_history": [vp, C.c_int, dp, C.c_int, ip],
     "bsim_simple_timing": [vp, dp],
     "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, dp, ip, ip],
+    "bsim_simple_probe_start": [vp, C.c_int],
+    "bsim_simple_probe_stop": [vp, ip, dp, dp],
 }
 _VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp]}
 

@@ -481,6 +481,16 @@ int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, 
         *nnz = D.nnz;
     });
 }
+int bsim_simple_probe_start(bsim_simple* s, int capacity) {
+    return guarded([&] { s->s->pressureSystem().probe.start(capacity); });
+}
+int bsim_simple_probe_stop(bsim_simple* s, int* launches, double* seconds, double* bytes_per_launch) {
+    return guarded([&] {
+        KernelProbe& P = s->s->pressureSystem().probe;
+        *seconds = P.stop(*launches);
+        *bytes_per_launch = P.bytesPerLaunch;
+    });
+}
 void bsim_simple_destroy(bsim_simple* s) { delete s; }
 
 }  // extern "C"

@@ -74,6 +74,18 @@ template <class T> struct DBuf {
     }
 };
 
+// Logical workgroup index.  The hardware deals workgroups round-robin over
+// the eight XCDs, each with its own L2; renumbering them so that XCD x runs
+// one contiguous range of blocks keeps neighbouring rows, which gather the
+// same cache lines, in one L2.  A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ inline int xcdBlock() {
+    const int b = blockIdx.x, q = gridDim.x / kXcds, r = gridDim.x % kXcds;
+    const int x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+__device__ inline int globalThread() { return xcdBlock() * blockDim.x + threadIdx.x; }
+
 // Row layout shared by the cell-gather kernels and the fine-level matrices:
 // rows are grouped in slices of 64 (one wavefront); the k-th off-diagonal
 // entry of the rows of slice s is stored at sliceOff[s*(K+1)+k] + lane, so a
@@ -113,7 +125,8 @@ __device__ inline void st3(double* p, long long i, D3 v) { p[3 * i] = v.x; p[3 *
 
 // ---- deterministic reductions ----
 // Stage 1 (the producing kernel): each block reduces its values with a fixed
-// in-block tree and stores the K sums at partials[k * kMaxReduceBlocks + block].
+// in-block tree and stores the K sums at partials[k * kMaxReduceBlocks + block]
+// (logical block index, see xcdBlock).
 // Stage 2 (kReduceTail, one block launched after it on the same stream): sums
 // the partials in block order with a fixed tree and hands them to a functor.
 // The kernel boundary orders the two stages, so no device-scope fence or
@@ -131,10 +144,14 @@ template <int K> __device__ inline void blockSum(double (&v)[K], double* smem /*
     __syncthreads();
 }
 
+// Partials are stored under the logical block index, so the final sum runs
+// over the rows in the same order whatever the hardware placement.
 template <int K> __device__ inline void blockPartial(double (&v)[K], double* partials, double* smem) {
     blockSum<K>(v, smem);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < K; ++k) partials[k * kMaxReduceBlocks + blockIdx.x] = v[k];
+    if (threadIdx.x == 0) {
+        const int blk = xcdBlock();
+        for (int k = 0; k < K; ++k) partials[k * kMaxReduceBlocks + blk] = v[k];
+    }
 }
 
 // F: device functor called by thread 0 with the K totals.

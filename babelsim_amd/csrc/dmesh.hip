@@ -96,7 +96,7 @@ __device__ D3 areaOf(const D3* p, int n, const unsigned char* tri) {
 }
 
 __global__ void kFacePoints(int nf, const int* fpo, const int* fpi, const double* vx, double* psum, int* pcnt) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= nf) return;
     D3 s = d3(0, 0, 0);
     for (int k = fpo[f]; k < fpo[f + 1]; ++k) s = s + ld3(vx, fpi[k]);
@@ -105,7 +105,7 @@ __global__ void kFacePoints(int nf, const int* fpo, const int* fpi, const double
 }
 
 __global__ void kCellReference(MeshView m, const double* psum, const int* pcnt, double* ref) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     D3 s = d3(0, 0, 0);
     int cnt = 0;
@@ -123,7 +123,7 @@ __global__ void kCellReference(MeshView m, const double* psum, const int* pcnt, 
 // Orientation from owner toward exterior, area vector, area-weighted centre.
 __global__ void kFaceGeometry(int nf, const int* fpo, int* fpi, const double* vx, const int* own, const double* psum,
                               const double* ref, double* Cf, double* Sf, double* Af, double* nrm, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= nf) return;
     const int b = fpo[f], n = fpo[f + 1] - b;
     if (n < 3 || n > kMaxRing) { atomicOr(err, kErrRing); return; }
@@ -180,7 +180,7 @@ __device__ void cellFace(MeshView m, const int* fpo, const int* fpi, const doubl
 
 __global__ void kCellGeometry(MeshView m, const int* fpo, const int* fpi, const double* vx, const double* ref,
                               double* C, double* V, double* invV, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const D3 r = ld3(ref, c);
     double vol = 0.0, atot = 0.0;
@@ -199,7 +199,7 @@ __global__ void kCellGeometry(MeshView m, const int* fpo, const int* fpi, const 
 }
 
 __global__ void kFaceFactors(MeshView m, double* w, double* orth, double* kf, double* skew, int* nonOrth, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     const int o = m.own[f], n = m.nei[f];
     const D3 Cf = ld3(m.Cf, f), Sf = ld3(m.Sf, f), nr = ld3(m.nrm, f), Co = ld3(m.C, o);

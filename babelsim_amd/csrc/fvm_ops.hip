@@ -12,7 +12,7 @@ using namespace dev;
 // ---------------------------------------------------------------- gradients
 template <int NC, bool PASS2>
 __global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     D3 acc[NC];
     for (int k = 0; k < NC; ++k) acc[k] = d3(0, 0, 0);
@@ -42,7 +42,7 @@ __global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, i
 
 
 template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     double rhs[3 * NC];
@@ -85,7 +85,7 @@ template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, i
 
 // -------------------------------------------------------- face quantities
 template <int NC> __global__ void kInterp(MeshView m, FieldArg x, const double* g, double* out, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     double v[NC];
     if (f < m.nfi) {
@@ -99,13 +99,13 @@ template <int NC> __global__ void kInterp(MeshView m, FieldArg x, const double* 
 }
 
 __global__ void kSeedCtx(MeshView m, const double* U, double* ctx) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     ctx[f] = f < m.nfi ? 0.0 : dot(ld3(U, m.own[f]), ld3(m.Sf, f));
 }
 
 __global__ void kFlux(MeshView m, FieldArg U, const double* g, double* out, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     double v[3];
     if (f < m.nfi) {
@@ -119,7 +119,7 @@ __global__ void kFlux(MeshView m, FieldArg U, const double* g, double* out, int*
 }
 
 __global__ void kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double* out, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     double v[1];
     integratedNormalGradient<1>(x, g, m, f, dm, v, err);
@@ -128,7 +128,7 @@ __global__ void kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double*
 
 __global__ void kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double* out,
                                int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     const double k = kf[f];
     if (!(k >= 0.0) || !isfinite(k)) atomicOr(err, kErrNegativeDiffusivity);
@@ -138,7 +138,7 @@ __global__ void kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const d
 }
 
 __global__ void kDivFace(MeshView m, const double* phi, double* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const double iv = m.invV[c];
     double acc = 0.0;
@@ -153,7 +153,7 @@ __global__ void kDivFace(MeshView m, const double* phi, double* out) {
 
 __global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, double kc, const double* g, int dm,
                                    double* out, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const double iv = m.invV[c];
     double acc = 0.0;
@@ -186,7 +186,7 @@ __global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, dou
 template <int NC>
 __global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
                                     double* out, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
     const double iv = m.invV[c];
@@ -235,7 +235,7 @@ __global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, i
 template <int NC>
 __global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
                                     double scale, double* diag, double* off, double* src, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
     double dAcc = 0.0, sAcc[NC], corr[NC], a[NC], b[NC];
@@ -306,7 +306,7 @@ __global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, i
 template <int NC>
 __global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double factor,
                                    double* diag, double* off, double* src, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = globalThread();
     if (c >= m.nc) return;
     const bool corrected = correctedDiffusion(dm);
     double dAcc = 0.0, sAcc[NC], ing[NC], bv[NC];
@@ -359,7 +359,7 @@ __global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, con
 }
 
 __global__ void kGradTraces(MeshView m, FieldArg x, const double* g, double* traces) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = globalThread();
     const int f = m.nfi + b;
     if (f >= m.nf) return;
     int dummy = 0;

@@ -59,15 +59,15 @@ template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T*
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
 
 __global__ void kFill(int n, double* x, double v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) x[i] = v;
 }
 __global__ void kCopy(int n, const double* a, double* b) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) b[i] = a[i];
 }
 __global__ void kAxpy(int n, double a, const double* x, double* y) {  // y += a x
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) y[i] += a * x[i];
 }
 
@@ -75,7 +75,7 @@ __global__ void kAxpy(int n, double a, const double* x, double* y) {  // y += a 
 __global__ void kCopyNorm(int n, const double* b, double* r, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const double v = b[i];
         if (r) r[i] = v;
         acc += v * v;
@@ -88,7 +88,7 @@ __global__ void kCopyNorm(int n, const double* b, double* r, double* part) {
 __global__ void kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double v = b[i] - rowDot64(A, x, i);
         r[i] = v;
         acc += v * v;
@@ -100,7 +100,7 @@ __global__ void kResidualNorm(CsrView<double> A, const double* b, const double* 
 __global__ void kDot(int n, const double* a, const double* b, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
@@ -138,7 +138,7 @@ __global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const double v = r[i] * jinv[i];
         z[i] = v;
         acc += v * r[i];
@@ -150,14 +150,14 @@ __global__ void kPlainDot(int n, const double* z, const double* r, const int* fl
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += z[i] * r[i];
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += z[i] * r[i];
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
 
 __global__ void kCgDirection(int n, const double* z, double* p, const double* sc, const int* fl, int first) {
     if (fl[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= n) return;
     if (first) p[i] = z[i];
     else {
@@ -171,7 +171,7 @@ __global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, const 
     __shared__ double smem[kBlock];
     if (fl[0]) return;
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double y = rowDot64(A, p, i);
         w[i] = y;
         acc += p[i] * y;
@@ -201,7 +201,7 @@ __global__ void kCgUpdate(int n, double* x, const double* p, double* r, const do
     if (fl[0]) return;
     const double a = sc[S_A];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         x[i] += a * p[i];
         const double ri = r[i] + -a * w[i];
         r[i] = ri;
@@ -223,7 +223,7 @@ struct CgNorm {  // dp = ||r||
 __global__ void kBcgsRho(int n, const double* r, const double* rp, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += r[i] * rp[i];
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += r[i] * rp[i];
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
@@ -236,7 +236,7 @@ struct BcgsRho {  // rho = r . rp, beta = (rho / rhoold) (alpha / omegaold)
     }
 };
 __global__ void kBcgsP(int n, double* p, const double* r, const double* v, const double* sc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= n) return;
     const double beta = sc[S_BETA];
     p[i] = r[i] + (-sc[S_OMEGAOLD] * beta) * v[i] + beta * p[i];
@@ -245,7 +245,7 @@ __global__ void kBcgsP(int n, double* p, const double* r, const double* v, const
 __global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* part) {
     __shared__ double smem[kBlock];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double y = rowDot64(A, t, i);
         v[i] = y;
         acc += y * rp[i];
@@ -266,7 +266,7 @@ struct BcgsAlpha {  // d1 = v . rp, alpha = rho / d1
 };
 __global__ void kBcgsS(int n, double* s, const double* v, const double* r, const double* sc, const int* fl) {
     if (fl[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= n) return;
     s[i] = -sc[S_ALPHA] * v[i] + r[i];
 }
@@ -276,7 +276,7 @@ __global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, cons
     __shared__ double smem[2 * kBlock];
     if (fl[0]) return;
     double a1 = 0.0, a2 = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double y = rowDot64(A, rt, i);
         t[i] = y;
         a1 += s[i] * y;
@@ -303,7 +303,7 @@ __global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, 
     if (fl[0]) return;
     const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         x[i] = alpha * p[i] + omega * s[i] + x[i];
         const double ri = -omega * t[i] + s[i];
         r[i] = ri;
@@ -323,15 +323,15 @@ struct BcgsNorm {  // dp = ||r||; rho and omega become the previous values
     }
 };
 __global__ void kSpmvPlain(CsrView<double> A, const double* x, double* y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < A.n) y[i] = rowDot64(A, x, i);
 }
 __global__ void kJacobiApply(int n, const double* r, const double* jinv, double* z) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) z[i] = r[i] * jinv[i];
 }
 __global__ void kInvDiag(int n, const double* d, double* inv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) inv[i] = 1.0 / d[i];
 }
 // Counts entries where (a0, a1) and (b0, b1) differ, as partial sums per block.
@@ -340,7 +340,7 @@ __global__ void kCompare(int n0, const double* a0, const double* b0, int n1, con
     __shared__ double smem[kBlock];
     double cnt = 0.0;
     const int n = n0 + n1;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const bool same = i < n0 ? a0[i] == b0[i] : a1[i - n0] == b1[i - n0];
         if (!same) cnt += 1.0;
     }
@@ -350,36 +350,36 @@ __global__ void kCompare(int n0, const double* a0, const double* b0, int n1, con
 
 // ---- AMG ----
 template <class T> __global__ void kJacobiFirst(int n, const T* d, const T* b, T* x, double omega) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) x[i] = static_cast<T>(omega * static_cast<double>(b[i]) / static_cast<double>(d[i]));
 }
 template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
     const double ax = rowDot(A, x, q, i);
     xo[i] = static_cast<T>(static_cast<double>(x[i]) + omega * (static_cast<double>(b[i]) - ax) / static_cast<double>(A.diag[i]));
 }
 template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
     r[i] = static_cast<T>(static_cast<double>(b[i]) - rowDot(A, x, q, i));
 }
 template <class T> __global__ void kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = globalThread();
     if (I >= nc) return;
     double s = 0.0;
     for (int k = mp[I]; k < mp[I + 1]; ++k) s += static_cast<double>(r[mi[k]]);
     bc[I] = static_cast<T>(s);
 }
 template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) x[i] = static_cast<T>(static_cast<double>(x[i]) + alpha * static_cast<double>(xc[agg[i]]));
 }
 __global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
                           double* cd, double* cv) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = globalThread();
     if (e >= ne) return;
     double s = 0.0;
     for (int t = gp[e]; t < gp[e + 1]; ++t) {
@@ -390,11 +390,11 @@ __global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, c
     else cv[e - nc] = s;
 }
 __global__ void kToFloat(int n, const double* a, float* b) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) b[i] = static_cast<float>(a[i]);
 }
 __global__ void kToDouble(int n, const float* a, double* b) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) b[i] = static_cast<double>(a[i]);
 }
 // Dense fill + LU with partial pivoting of the coarsest matrix (one block).
@@ -438,7 +438,7 @@ __global__ void kDenseLU(int n, const int* ptr, const int* col, const double* d,
 // Explicit inverse of the coarsest matrix from its LU factors: column j is
 // the LU solve of the unit vector e_j (thread j works in column j of inv).
 __global__ void kDenseInverse(int n, const double* a, const int* piv, double* inv) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = globalThread();
     if (j >= n) return;
     auto w = [&](int i) -> double& { return inv[static_cast<long long>(i) * n + j]; };
     for (int i = 0; i < n; ++i) w(i) = i == j ? 1.0 : 0.0;
@@ -452,7 +452,7 @@ __global__ void kDenseInverse(int n, const double* a, const int* piv, double* in
 }
 // Coarsest solve as a dense product x = inv * b (FP64 accumulation).
 template <class T> __global__ void kDenseApply(int n, const double* inv, const T* b, T* xo) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= n) return;
     double s = 0.0;
     for (int j = 0; j < n; ++j) s += inv[static_cast<long long>(i) * n + j] * static_cast<double>(b[j]);
@@ -467,6 +467,46 @@ bool SolverCfg::operator==(const SolverCfg& o) const {
            amgMaxLevels == o.amgMaxLevels && amgCoarseSize == o.amgCoarseSize && amgPre == o.amgPre &&
            amgPost == o.amgPost && amgOmega == o.amgOmega && amgStrength == o.amgStrength &&
            amgCorrection == o.amgCorrection && precision == o.precision && innerRtol == o.innerRtol;
+}
+
+// ------------------------------------------------------------ probe
+KernelProbe::~KernelProbe() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+void KernelProbe::start(int capacity) {
+    if (capacity <= 0) throw std::invalid_argument("probe capacity must be positive");
+    while (static_cast<int>(ev.size()) < 2 * capacity) {
+        hipEvent_t e;
+        BSIM_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+    }
+    used = 0;
+    on = true;
+}
+void KernelProbe::begin(hipStream_t s) {
+    pending = 0;
+    if (!on || 2 * (used + 1) > static_cast<int>(ev.size())) return;
+    BSIM_HIP(hipEventRecord(ev[2 * used], s));
+    pending = 1;
+}
+void KernelProbe::end(hipStream_t s) {
+    if (!pending) return;
+    BSIM_HIP(hipEventRecord(ev[2 * used + 1], s));
+    ++used;
+    pending = 0;
+}
+double KernelProbe::stop(int& launches) {
+    on = false;
+    double total = 0.0;
+    for (int k = 0; k < used; ++k) {
+        BSIM_HIP(hipEventSynchronize(ev[2 * k + 1]));
+        float ms = 0.0f;
+        BSIM_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        total += 1e-3 * ms;
+    }
+    launches = used;
+    used = 0;
+    return total;
 }
 
 // ------------------------------------------------------------------ AMG
@@ -597,11 +637,15 @@ template <class T> void AmgDevice::cycle(int l) {
         return;
     }
     const double w = cfg.amgOmega;
-    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, x, w);
-    for (int k = 1; k < cfg.amgPre; ++k) {
+    KernelProbe* pr = l == 0 ? probe : nullptr;
+    auto sweep = [&]() {
+        if (pr) pr->begin(s);
         kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, x, t, w);
+        if (pr) pr->end(s);
         std::swap(x, t);
-    }
+    };
+    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, x, w);
+    for (int k = 1; k < cfg.amgPre; ++k) sweep();
     kResidual<T><<<nb, kBlock, 0, s>>>(A, b, x, r);
     Level& cl = *L[l + 1];
     T* bc = mixed ? reinterpret_cast<T*>(cl.b32.p) : reinterpret_cast<T*>(cl.b.p);
@@ -609,10 +653,7 @@ template <class T> void AmgDevice::cycle(int l) {
     cycle<T>(l + 1);
     const T* xc = mixed ? reinterpret_cast<const T*>(cl.x32.p) : reinterpret_cast<const T*>(cl.x.p);
     kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, x, cfg.amgCorrection);
-    for (int k = 0; k < cfg.amgPost; ++k) {
-        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, x, t, w);
-        std::swap(x, t);
-    }
+    for (int k = 0; k < cfg.amgPost; ++k) sweep();
     // Results live in the buffer named x for this level.
     T* named = mixed ? reinterpret_cast<T*>(lv.x32.p) : reinterpret_cast<T*>(lv.x.p);
     if (x != named) {
@@ -695,7 +736,12 @@ void DeviceSystem::setup(bool reuse) {
             BSIM_LAUNCH_CHECK();
         } else if (cfg.pc == 2) {
             if (!amg) amg = std::make_unique<AmgDevice>(stream);
+            amg->probe = &probe;
             amg->setup(view(), pat, cfg);
+            // Fine-level sweep: off-diagonal values and columns, then the
+            // diagonal, b, x (each read once) and the result.
+            const double vb = cfg.precision == 1 ? 4.0 : 8.0;
+            probe.bytesPerLaunch = (vb + 4.0) * nnz + 4.0 * vb * n;
         }
         if (cfg.pc != 0) ++counters.pcSetups;
         pcMatrixUpdate = counters.matrixUpdates;

@@ -57,6 +57,23 @@ struct Pattern {
 
 class AmgDevice;
 
+// HIP-event timing of one kernel's launches: each timed launch is bracketed
+// by a pair of events recorded on the launching stream.  The events are
+// created up front so that recording allocates nothing.
+class KernelProbe {
+public:
+    ~KernelProbe();
+    void start(int capacity);        // creates the event pool, clears the count
+    void begin(hipStream_t s);       // before a launch (no-op when stopped or full)
+    void end(hipStream_t s);         // after it
+    double stop(int& launches);      // waits, stops, returns the summed seconds
+    bool on = false;
+    double bytesPerLaunch = 0.0;     // algorithmic bytes of the probed kernel
+private:
+    std::vector<hipEvent_t> ev;
+    int used = 0, pending = 0;
+};
+
 struct SystemCounters {
     std::uint64_t matrixUpdates = 0, rhsOnly = 0, pcSetups = 0, pcReuses = 0, retries = 0, solves = 0,
                   krylovIterations = 0, trueResidualChecks = 0, spmvs = 0, pcApplications = 0;
@@ -101,6 +118,9 @@ private:
     int freshIterations[3] = {0, 0, 0};
     DBuf<double> jinv;          // Jacobi: 1/diag of the matrix at PC setup
     std::unique_ptr<AmgDevice> amg;
+public:
+    KernelProbe probe;  // times the fine-level AMG smoother sweeps
+private:
     DBuf<double> r, z, p, w, rp, v, t, s, tmp, guess, saved;
     DBuf<float> r32, z32, p32, w32, rp32, v32, t32, s32, x32, tmp32, diag32, val32;
     DBuf<double> sc;            // device scalars
@@ -120,6 +140,8 @@ public:
     std::vector<int> aggregates0() const { return hostAgg0; }
     std::vector<double> coarseDiag(int level) const;
     double setupSeconds = 0.0;
+
+    KernelProbe* probe = nullptr;
 
 private:
     struct Level {

@@ -26,17 +26,17 @@ __global__ void kCopyD(long long n, const double* a, double* b) {
     if (i < n) b[i] = a[i];
 }
 __global__ void kExtract(int n, const double* v3, int comp, double* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) out[i] = v3[3 * static_cast<long long>(i) + comp];
 }
 __global__ void kInsert(int n, const double* in, int comp, double* v3) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < n) v3[3 * static_cast<long long>(i) + comp] = in[i];
 }
 
 // equ::source(momentum, -grad(p)): source += (1*V) * (-1*gradP)  (procedural_equation.cpp:192-196)
 __global__ void kMomentumPressureSource(MeshView m, const double* gradP, double* src) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= m.nc) return;
     const double v = 1.0 * m.V[i];
     for (int k = 0; k < 3; ++k) src[3 * static_cast<long long>(i) + k] += v * (-1.0 * gradP[3 * static_cast<long long>(i) + k]);
@@ -48,7 +48,7 @@ __global__ void kMomentumResidual(MeshView m, const double* diag, const double* 
                                   double* part) {
     __shared__ double smem[3 * kBlock];
     double a[3] = {0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.nc; i += gridDim.x * blockDim.x) {
+    for (int i = globalThread(); i < m.nc; i += gridDim.x * blockDim.x) {
         double ax[3];
         for (int k = 0; k < 3; ++k) ax[k] = diag[i] * U[3 * static_cast<long long>(i) + k];
         for (int k_ = 0, e = 0; m.row(i).valid(k_) && (e = m.row(i).at(k_), true); ++k_) {
@@ -69,7 +69,7 @@ __global__ void kMomentumResidual(MeshView m, const double* diag, const double* 
 // relax (procedural_equation.cpp:298-305), scale (240-247) and rAU = V/aP (main.cpp:62-66).
 __global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, const double* Uprev, double alpha,
                             double* rAU) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= m.nc) return;
     const double extra = diag[i] * (1 / alpha - 1);
     double d = diag[i] + extra;
@@ -86,7 +86,7 @@ __global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, 
 
 // Calculated-field traces of V/aP: V_o / aP_o on boundary faces.
 __global__ void kOwnerTrace(MeshView m, const double* cell, double* trace) {
-    const int f = m.nfi + blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = m.nfi + globalThread();
     if (f < m.nf) trace[f] = cell[m.own[f]];
 }
 
@@ -104,7 +104,7 @@ __global__ void kScaleVector(MeshView m, const double* s, const double* g, doubl
 // Rhie-Chow on interior faces (main.cpp:71-79): phiH += (rAU gradP)_f . Sf,
 // phiH += -1 * ((rAU_f * snGrad p) * |Sf|).
 __global__ void kRhieChow(MeshView m, double* phiH, const double* Xf, const double* rAUf, const double* sn) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nfi) return;
     double v = phiH[f] + dot(ld3(Xf, f), ld3(m.Sf, f));
     v += -1.0 * ((rAUf[f] * sn[f]) * m.Af[f]);
@@ -113,13 +113,13 @@ __global__ void kRhieChow(MeshView m, double* phiH, const double* Xf, const doub
 
 // equ::source(pEq, -div(phiH)) (procedural_equation.cpp:192-196)
 __global__ void kScalarSource(MeshView m, const double* divPhiH, double* src) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i < m.nc) src[i] += (1.0 * m.V[i]) * (-1.0 * divPhiH[i]);
 }
 
 // referenceIfUnanchored constant-null-mode test (procedural_equation.cpp:316-333).
 __global__ void kReferenceTest(MeshView m, const double* diag, const double* off, int* notConstant) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= m.nc) return;
     double rowSum = diag[i], mag = fabs(diag[i]);
     for (int k_ = 0, e = 0; m.row(i).valid(k_) && (e = m.row(i).at(k_), true); ++k_) {
@@ -138,14 +138,14 @@ __global__ void kReferenceApply(double* diag, double* src, int cell, double valu
 }
 
 __global__ void kCorrection(int nf, const double* full, const double* orth, double* corr) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f < nf) corr[f] = full[f] + (-1.0) * orth[f];
 }
 
 // equ::faceFlux (procedural_equation.cpp:335-352) with phi = phiH + pressureFlux (main.cpp:98).
 __global__ void kPressureFluxPhi(MeshView m, FieldArg pp, const double* kf, const double* corr, const double* phiH,
                                  double* phi, int* err) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = globalThread();
     if (f >= m.nf) return;
     const double k = kf[f];
     if (!(k >= 0.0) || !isfinite(k)) atomicOr(err, kErrNegativeDiffusivity);
@@ -160,7 +160,7 @@ __global__ void kPressureFluxPhi(MeshView m, FieldArg pp, const double* kf, cons
 // p += alphaP p'; U -= rAU grad(p')  (main.cpp:95-97)
 __global__ void kCorrectUP(MeshView m, double* p, const double* pp, double alphaP, double* U, const double* rAU,
                            const double* gpp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = globalThread();
     if (i >= m.nc) return;
     p[i] += 1.0 * (alphaP * pp[i]);
     for (int k = 0; k < 3; ++k) {
@@ -174,7 +174,7 @@ __global__ void kConvergence(MeshView m, const double* U, const double* Up, cons
                              const double* phi, double* part) {
     __shared__ double smem[kRedK * kBlock];
     double a[kRedK] = {0, 0, 0, 0, 0, 0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nc; c += gridDim.x * blockDim.x) {
+    for (int c = globalThread(); c < m.nc; c += gridDim.x * blockDim.x) {
         const D3 u = ld3(U, c), up = ld3(Up, c);
         const D3 d = u - up;
         a[0] += dot(d, d);

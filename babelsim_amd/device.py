@@ -254,3 +254,15 @@ class Simple:
         nnz = C.c_int()
         check(lib().bsim_simple_time_spmv(self.h, which, reps, C.byref(s), C.byref(n), C.byref(nnz)))
         return s.value, n.value, nnz.value
+
+    def probe_start(self, capacity=8192):
+        """Start HIP-event timing of the pressure AMG's fine-level smoother sweeps."""
+        check(lib().bsim_simple_probe_start(self.h, capacity))
+
+    def probe_stop(self):
+        """Stop the probe: (launches, summed kernel seconds, algorithmic bytes per launch)."""
+        n = C.c_int()
+        s = C.c_double()
+        b = C.c_double()
+        check(lib().bsim_simple_probe_stop(self.h, C.byref(n), C.byref(s), C.byref(b)))
+        return n.value, s.value, b.value

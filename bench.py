@@ -27,6 +27,26 @@ sys.path.insert(0, str(ROOT / "tests"))
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic(cells):
+    """HBM bytes per fine-level sweep launch from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken on a mesh of
+    the same size; None otherwise."""
+    path = ROOT / "profiles" / "kJacobiSweep_fine_cavity126_pmc.md"
+    if not path.exists():
+        return None
+    vals = {}
+    for line in path.read_text().splitlines():
+        if line.startswith("- ") and ":" in line:
+            k, v = line[2:].split(":", 1)
+            vals[k.strip()] = v.strip().split()[0]
+    try:
+        if int(vals["cells"]) != cells:
+            return None
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,11 +57,29 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=1, help="timed oracle iterations for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rebuild-interval", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 plain 1 GiB device-to-device copies (the known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": 5}), flush=True)
+    return 0
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
@@ -74,21 +112,28 @@ def main():
         reps += sim.iterate(1)
         r = reps[-1]
         log(f"warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters}")
+    sim.probe_start(capacity=512 * args.steps)
     t1 = time.time()
     timed = sim.iterate(args.steps)
     t2 = time.time()
+    launches, sweep_s, sweep_bytes = sim.probe_stop()
     sec = t2 - t1
     its_per_s = args.steps / sec
     last = timed[-1]
     log(f"timed {args.steps} its in {sec:.3f}s -> {its_per_s:.3f} it/s; last rU={last.rU:.3e} "
         f"mom_its={list(last.mom_iters)} p_its={last.p_iters} mass={last.mass:.2e}")
     timing = sim.timing()
-    # Roofline of the pressure-solve SpMV (CSR, FP64): 12*nnz + 4*(N+1) + 8*N + 8*N bytes per launch.
+    # Dominant kernel: the fine-level smoother sweep of the pressure AMG,
+    # timed with HIP events around every launch inside the timed region.
+    sweep_avg = sweep_s / max(launches, 1)
+    achieved = sweep_bytes / sweep_avg / 1e9 if launches else 0.0
+    log(f"fine-level sweep: {launches} launches, {sweep_avg * 1e6:.1f} us each, {achieved:.0f} GB/s")
+    # Auxiliary: the pressure matrix SpMV alone (after the timed region).
     spmv_s, N, nnz_off = sim.time_spmv(1, reps=50)
     nnz = N + nnz_off
-    spmv_bytes = 12 * nnz + 4 * (N + 1) + 16 * N
-    achieved = spmv_bytes / spmv_s / 1e9
-    log(f"spmv: {spmv_s * 1e6:.1f} us per launch, {achieved:.0f} GB/s")
+    spmv_bytes = 12 * nnz_off + 24 * N
+    log(f"spmv: {spmv_s * 1e6:.1f} us per launch, {spmv_bytes / spmv_s / 1e9:.0f} GB/s")
+    traffic = measured_traffic(ctx.nc)
     cpu = None
     if not args.no_cpu and args.cpu_iters > 0:
         import oracle_lib as orc
@@ -124,9 +169,12 @@ def main():
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
                    "phase_seconds": timing},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "pressure CSR SpMV (FP64)", "bytes_per_launch": spmv_bytes,
-                     "seconds_per_launch": spmv_s},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "kJacobiSweep (pressure AMG fine-level smoother sweep, sliced rows)",
+                     "launches": launches, "seconds_per_launch": sweep_avg,
+                     "algorithmic_bytes_per_launch": sweep_bytes,
+                     "spmv": {"seconds_per_launch": spmv_s, "algorithmic_bytes_per_launch": spmv_bytes,
+                              "GB/s": spmv_bytes / spmv_s / 1e9}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -174,6 +174,14 @@ int bsim_simple_timing(bsim_simple* s, double* out8);
  * matrix with HIP events on the context stream; returns mean seconds and the
  * matrix size. */
 int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, int* n, int* nnz_off);
+/* Diagnostics with no counterpart in the reference: HIP-event timing of the
+ * fine-level AMG smoother sweeps of the pressure solve.  start() creates a
+ * pool of `capacity` event pairs and times every later sweep launch (up to
+ * the pool size) by events recorded on the context stream around it; stop()
+ * waits for them and returns the launch count, the summed kernel seconds and
+ * the kernel's algorithmic bytes per launch. */
+int bsim_simple_probe_start(bsim_simple* s, int capacity);
+int bsim_simple_probe_stop(bsim_simple* s, int* launches, double* seconds, double* bytes_per_launch);
 void bsim_simple_destroy(bsim_simple* s);
 
 #ifdef __cplusplus
