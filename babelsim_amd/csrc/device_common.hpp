@@ -23,7 +23,7 @@ namespace bsim {
 #define BSIM_LAUNCH_CHECK() BSIM_HIP(hipGetLastError())
 
 constexpr int kBlock = 256;          // 4 wavefronts of 64
-constexpr int kMaxReduceBlocks = 4096;
+constexpr int kMaxReduceBlocks = 32768;  // one row per thread up to 8.4M rows
 
 inline int blocksFor(long long n, int block = kBlock) {
     long long b = (n + block - 1) / block;

@@ -476,8 +476,10 @@ KernelProbe::~KernelProbe() {
 void KernelProbe::start(int capacity) {
     if (capacity <= 0) throw std::invalid_argument("probe capacity must be positive");
     while (static_cast<int>(ev.size()) < 2 * capacity) {
+        // Timing-only events: no system-scope fence (cache writeback and
+        // invalidate) when they complete, so they do not perturb the kernels.
         hipEvent_t e;
-        BSIM_HIP(hipEventCreate(&e));
+        BSIM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         ev.push_back(e);
     }
     used = 0;
@@ -497,6 +499,7 @@ void KernelProbe::end(hipStream_t s) {
 }
 double KernelProbe::stop(int& launches) {
     on = false;
+    BSIM_HIP(hipDeviceSynchronize());
     double total = 0.0;
     for (int k = 0; k < used; ++k) {
         BSIM_HIP(hipEventSynchronize(ev[2 * k + 1]));

@@ -65,12 +65,12 @@ void spmv(const Csr& A, const double* x, double* y) {
 namespace {
 
 // Inner products are summed in a fixed blocked order: 256 strided partial sums
-// per block over at most 4096 blocks, pairwise trees inside each block, then
+// per block over at most 32768 blocks, pairwise trees inside each block, then
 // the same over the block sums.  Both sides of the parity check use this order
 // (it is the deterministic reduction of the device library), so FP64 Krylov
 // histories can be compared bit for bit.
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
-    constexpr long B = 256, Gmax = 4096;
+    constexpr long B = 256, Gmax = 32768;
     const long n = static_cast<long>(a.size());
     long G = (n + B - 1) / B;
     G = G < 1 ? 1 : (G > Gmax ? Gmax : G);
