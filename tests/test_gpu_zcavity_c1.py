@@ -22,9 +22,9 @@ def test_c1_cavity_converges_to_ghia():
     sim = Simple(ctx, cases.c1_config(), ub, pb)
     sim.start()
     its, done = 0, False
-    while its < 8000 and not done:
-        reps = sim.iterate(100)
-        its += 100
+    while its < 20000 and not done:
+        reps = sim.iterate(500)
+        its += 500
         done = any(r.converged for r in reps)
         print(f"[c1] {its} its rU={reps[-1].rU:.3e} mass={reps[-1].mass:.3e}", flush=True)
     assert done
