@@ -47,6 +47,19 @@ def measured_traffic(cells):
         return None
 
 
+def device_memory_used_gb():
+    """HBM in use on the current device (hipMemGetInfo), in GB."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return None
+    return (total.value - free.value) / 1e9
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +125,7 @@ def main():
         reps += sim.iterate(1)
         r = reps[-1]
         log(f"warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters}")
+    before = sim.timing()
     sim.probe_start(capacity=512 * args.steps)
     t1 = time.time()
     timed = sim.iterate(args.steps)
@@ -122,7 +136,9 @@ def main():
     last = timed[-1]
     log(f"timed {args.steps} its in {sec:.3f}s -> {its_per_s:.3f} it/s; last rU={last.rU:.3e} "
         f"mom_its={list(last.mom_iters)} p_its={last.p_iters} mass={last.mass:.2e}")
-    timing = sim.timing()
+    after = sim.timing()
+    timing = {k: (after[k] - before[k]) / args.steps for k in after if k != "iterations"}
+    used_gb = device_memory_used_gb()
     # Dominant kernel: the fine-level smoother sweep of the pressure AMG,
     # timed with HIP events around every launch inside the timed region.
     sweep_avg = sweep_s / max(launches, 1)
@@ -167,7 +183,7 @@ def main():
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
                    "pressure": "CG + aggregation AMG", "momentum": "BiCGSTAB + Jacobi",
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
-                   "phase_seconds": timing},
+                   "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "kJacobiSweep (pressure AMG fine-level smoother sweep, sliced rows)",

@@ -1,8 +1,9 @@
 /* bsim.h - C ABI of the MI355X-native BabelSim hot path (libbsim.so).
  *
- * Plain pointers and sizes only; every entry point returns 0 on success and a
- * non-zero code on failure, with the message available from bsim_last_error()
- * (thread-local).  Host arrays are in the caller's numbering: cells and faces
+ * Plain pointers and sizes only; every entry point returns 0 on success, 2 for
+ * an invalid argument (std::invalid_argument in the reference) and 1 for any
+ * other failure (std::runtime_error / std::logic_error), with the message
+ * available from bsim_last_error() (thread-local).  Host arrays are in the caller's numbering: cells and faces
  * are renumbered internally and permuted back on every upload/download.
  *
  * Each declaration names the reference interface it replaces.  The reference is

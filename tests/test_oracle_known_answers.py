@@ -163,6 +163,37 @@ def test_least_squares_gradient_exact_for_affine(name):
     np.testing.assert_allclose(g[keep], np.broadcast_to(AFFINE_B, g[keep].shape), atol=1e-11)
 
 
+def test_boundary_rows_and_gradient_traces_exact_for_affine():
+    """Least-squares boundary mirror rows (operators.cpp:335-343) and computed-
+    gradient boundary traces (fvm_execution.cpp:221-235): on a box with
+    jittered interior vertices and prism splits (flat walls), phi = a + c x
+    with fixedValue a on x = 0, fixedGradient c on x = 1, zeroGradient on the
+    y walls and symmetry on the z walls has the exact gradient (c, 0, 0) in
+    every cell, and every boundary trace equals it."""
+    from oracle_lib import ptr, ip
+    arrays = meshes.jittered((6, 5, 4), jitter=0.2, split=0.3, seed=7)
+    m = orc.oracle_mesh(arrays)
+    g = m.geometry()
+    a, c = 0.4, -1.7
+    x = a + c * g["C"][:, 0]
+    bct = np.array([FV, FG, ZG, ZG, SYM, SYM], np.int32)
+    bcv = np.array([a, c, 0.0, 0.0, 0.0, 0.0])
+    grad = m.gradient(x, bct, bcv, LS)
+    np.testing.assert_allclose(grad, np.broadcast_to([c, 0.0, 0.0], grad.shape), atol=1e-12)
+    traces = np.zeros((m.nf, 3))
+    m.call("gradient_traces", ptr(np.ascontiguousarray(x)), ptr(bct, ip), ptr(bcv),
+           ptr(np.ascontiguousarray(grad.reshape(-1))), ptr(traces))
+    bnd = arrays["neighbour"] < 0
+    np.testing.assert_allclose(traces[bnd], np.broadcast_to([c, 0.0, 0.0], traces[bnd].shape), atol=1e-12)
+    # A wrong fixedValue trace (dropping the tangential gradient term) is caught:
+    # on the jittered walls d is not parallel to n.
+    d = g["Cf"] - g["C"][arrays["owner"]]
+    nrm = g["n"]
+    dn = np.einsum("ij,ij->i", d, nrm)
+    tang = d - dn[:, None] * nrm
+    assert np.abs(tang[bnd & (arrays["patch"] == 0)]).max() > 1e-3
+
+
 def test_green_gauss_gradient_exact_for_affine_on_uniform_box():
     arrays = meshes.box((5, 5, 5), (0, 0, 0), (1, 1, 1))
     m = orc.oracle_mesh(arrays)
