@@ -167,34 +167,64 @@ __device__ inline void integratedNormalGradient(const FieldArg& x, const double*
     }
 }
 
-// Symmetric 3x3 solve by pivoted LDL^T; the same elimination as the oracle.
-template <int NR> __device__ inline bool solveSym3(double A[3][3], const double* b, double* x) {
-    int perm[3] = {0, 1, 2};
-    double M[3][3], L[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, D[3];
+// Symmetric pivoted LDL^T solve of a 3x3 system with NR right-hand sides
+// (b and x row-major 3 x NR).  Pivots swap rows/columns through constant-index
+// branches and permute the right-hand side as they happen, so every array
+// stays in registers; the arithmetic is the textbook loop order.
+template <int A_, int B_, int K_, int NR>
+__device__ inline void symSwap(double (&M)[3][3], double (&L)[3][3], double (&bp)[3][NR]) {
+    for (int j = 0; j < 3; ++j) { const double q = M[A_][j]; M[A_][j] = M[B_][j]; M[B_][j] = q; }
+    for (int i = 0; i < 3; ++i) { const double q = M[i][A_]; M[i][A_] = M[i][B_]; M[i][B_] = q; }
+    for (int j = 0; j < K_; ++j) { const double q = L[A_][j]; L[A_][j] = L[B_][j]; L[B_][j] = q; }
+    for (int r = 0; r < NR; ++r) { const double q = bp[A_][r]; bp[A_][r] = bp[B_][r]; bp[B_][r] = q; }
+}
+template <int NR> __device__ inline bool solveSym3(const double (&A)[3][3], const double* b, double* x) {
+    double M[3][3], L[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, D[3], bp[3][NR];
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) M[i][j] = A[i][j];
-    for (int k = 0; k < 3; ++k) {
-        int p = k;
-        for (int i = k + 1; i < 3; ++i) if (fabs(M[i][i]) > fabs(M[p][p])) p = i;
-        if (p != k) {
-            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
-            for (int j = 0; j < 3; ++j) { double q = M[k][j]; M[k][j] = M[p][j]; M[p][j] = q; }
-            for (int i = 0; i < 3; ++i) { double q = M[i][k]; M[i][k] = M[i][p]; M[i][p] = q; }
-            for (int j = 0; j < k; ++j) { double q = L[k][j]; L[k][j] = L[p][j]; L[p][j] = q; }
-        }
-        D[k] = M[k][k];
-        if (!(fabs(D[k]) > 0.0) || !isfinite(D[k])) return false;
-        for (int i = k + 1; i < 3; ++i) {
-            L[i][k] = M[i][k] / D[k];
-            for (int j = k + 1; j < 3; ++j) M[i][j] -= L[i][k] * M[k][j];
-        }
-    }
+    for (int i = 0; i < 3; ++i) for (int r = 0; r < NR; ++r) bp[i][r] = b[i * NR + r];
+    auto ok = [](double d) { return fabs(d) > 0.0 && isfinite(d); };
+    // step 0: largest diagonal of rows 0..2
+    int p0 = 0;
+    double best = fabs(M[0][0]);
+    if (fabs(M[1][1]) > best) { p0 = 1; best = fabs(M[1][1]); }
+    if (fabs(M[2][2]) > best) p0 = 2;
+    if (p0 == 1) symSwap<0, 1, 0, NR>(M, L, bp);
+    else if (p0 == 2) symSwap<0, 2, 0, NR>(M, L, bp);
+    D[0] = M[0][0];
+    if (!ok(D[0])) return false;
+    L[1][0] = M[1][0] / D[0];
+    M[1][1] -= L[1][0] * M[0][1];
+    M[1][2] -= L[1][0] * M[0][2];
+    L[2][0] = M[2][0] / D[0];
+    M[2][1] -= L[2][0] * M[0][1];
+    M[2][2] -= L[2][0] * M[0][2];
+    // step 1: rows 1..2
+    const int p1 = fabs(M[2][2]) > fabs(M[1][1]) ? 2 : 1;
+    if (p1 == 2) symSwap<1, 2, 1, NR>(M, L, bp);
+    D[1] = M[1][1];
+    if (!ok(D[1])) return false;
+    L[2][1] = M[2][1] / D[1];
+    M[2][2] -= L[2][1] * M[1][2];
+    D[2] = M[2][2];
+    if (!ok(D[2])) return false;
     for (int r = 0; r < NR; ++r) {
-        double y[3];
-        for (int i = 0; i < 3; ++i) y[i] = b[perm[i] * NR + r];
-        for (int i = 0; i < 3; ++i) for (int j = 0; j < i; ++j) y[i] -= L[i][j] * y[j];
-        for (int i = 0; i < 3; ++i) y[i] /= D[i];
-        for (int i = 2; i >= 0; --i) for (int j = i + 1; j < 3; ++j) y[i] -= L[j][i] * y[j];
-        for (int i = 0; i < 3; ++i) x[perm[i] * NR + r] = y[i];
+        double y0 = bp[0][r], y1 = bp[1][r], y2 = bp[2][r];
+        y1 -= L[1][0] * y0;
+        y2 -= L[2][0] * y0;
+        y2 -= L[2][1] * y1;
+        y0 /= D[0];
+        y1 /= D[1];
+        y2 /= D[2];
+        y1 -= L[2][1] * y2;
+        y0 -= L[1][0] * y1;
+        y0 -= L[2][0] * y2;
+        // undo the pivots in reverse order
+        if (p1 == 2) { const double q = y1; y1 = y2; y2 = q; }
+        if (p0 == 1) { const double q = y0; y0 = y1; y1 = q; }
+        else if (p0 == 2) { const double q = y0; y0 = y2; y2 = q; }
+        x[0 * NR + r] = y0;
+        x[1 * NR + r] = y1;
+        x[2 * NR + r] = y2;
     }
     for (int i = 0; i < 3 * NR; ++i) if (!isfinite(x[i])) return false;
     return true;

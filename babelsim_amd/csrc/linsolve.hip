@@ -26,17 +26,18 @@ __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 // Row i (held in storage slot q) of A x: the diagonal first, then the
 // off-diagonal entries in row order (the operation order of the oracle's
 // spmv); padding entries are skipped.
-template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int q, int i) {
-    double y = static_cast<double>(A.diag[i]) * static_cast<double>(x[i]);
+template <class T, class XF> __device__ inline double rowDotF(const CsrView<T>& A, XF xf, int q, int i) {
+    double y = static_cast<double>(A.diag[i]) * xf(i);
     if (A.sliceOff) {
-        // Entries are fetched four levels at a time so that their column,
-        // value and x loads overlap; the sum still runs in entry order.
+        // Entries are fetched kC levels at a time so that their column, value
+        // and x loads overlap; the sum still runs in entry order.
         const RowRange R(A.sliceOff, A.K, q);
-        constexpr int kC = 4;
+        constexpr int kC = 8;
         for (int k0 = 0; k0 < A.K; k0 += kC) {
             if (R.so[k0 + 1] == R.so[k0]) break;  // no row of the slice has more entries
             int e[kC], c[kC];
-            T v[kC], xv[kC];
+            T v[kC];
+            double xv[kC];
 #pragma unroll
             for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
 #pragma unroll
@@ -45,15 +46,21 @@ template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T*
                 v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
             }
 #pragma unroll
-            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
+            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? xf(c[j]) : 0.0;
 #pragma unroll
             for (int j = 0; j < kC; ++j)
-                if (c[j] >= 0) y += static_cast<double>(v[j]) * static_cast<double>(xv[j]);
+                if (c[j] >= 0) y += static_cast<double>(v[j]) * xv[j];
         }
     } else {
-        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * xf(A.col[k]);
     }
     return y;
+}
+// Row i (held in storage slot q) of A x: the diagonal first, then the
+// off-diagonal entries in row order (the operation order of the oracle's
+// spmv); padding entries are skipped.
+template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int q, int i) {
+    return rowDotF(A, [x](int c) { return static_cast<double>(x[c]); }, q, i);
 }
 // Krylov kernels run on matrices whose slot q holds row q.
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
@@ -624,45 +631,46 @@ std::vector<int> AmgDevice::sizes() const {
 
 std::vector<double> AmgDevice::coarseDiag(int level) const { return L.at(level)->diag.toHost(stream); }
 
-template <class T> void AmgDevice::cycle(int l) {
+// One V-cycle on level l: b is the right-hand side, and the result is left
+// in xOut.  Sweeps alternate between xOut and the level's spare vector; the
+// first write goes to whichever of the two makes the last one land in xOut.
+template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut) {
     Level& lv = *L[l];
     hipStream_t s = stream;
-    T* x = mixed ? reinterpret_cast<T*>(lv.x32.p) : reinterpret_cast<T*>(lv.x.p);
-    T* b = mixed ? reinterpret_cast<T*>(lv.b32.p) : reinterpret_cast<T*>(lv.b.p);
     T* r = mixed ? reinterpret_cast<T*>(lv.r32.p) : reinterpret_cast<T*>(lv.r.p);
-    T* t = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
+    T* spare = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
     const T* dg = mixed ? reinterpret_cast<const T*>(lv.diag32.p) : reinterpret_cast<const T*>(lv.diag.p);
     const T* vl = mixed ? reinterpret_cast<const T*>(lv.val32.p) : reinterpret_cast<const T*>(lv.val.p);
     const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K, lv.perm};
     const int nb = blocksFor(lv.n);
     if (l == static_cast<int>(L.size()) - 1) {
-        kDenseApply<T><<<blocksFor(lv.n), kBlock, 0, s>>>(lv.n, inv.p, b, x);
+        kDenseApply<T><<<blocksFor(lv.n), kBlock, 0, s>>>(lv.n, inv.p, b, xOut);
         return;
     }
     const double w = cfg.amgOmega;
+    const int pre = cfg.amgPre, post = cfg.amgPost;
+    // Every sweep after the first damped-Jacobi step writes the other vector.
+    const int later = pre - 1 + post;
+    T* cur = later % 2 == 0 ? xOut : spare;
+    T* nxt = later % 2 == 0 ? spare : xOut;
     KernelProbe* pr = l == 0 ? probe : nullptr;
-    auto sweep = [&]() {
+    auto sweep = [&]() {  // cur -> nxt (the probed kernel)
         if (pr) pr->begin(s);
-        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, x, t, w);
+        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, cur, nxt, w);
         if (pr) pr->end(s);
-        std::swap(x, t);
+        std::swap(cur, nxt);
     };
-    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, x, w);
-    for (int k = 1; k < cfg.amgPre; ++k) sweep();
-    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, x, r);
+    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, cur, w);
+    for (int k = 1; k < pre; ++k) sweep();
+    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, cur, r);
     Level& cl = *L[l + 1];
     T* bc = mixed ? reinterpret_cast<T*>(cl.b32.p) : reinterpret_cast<T*>(cl.b.p);
+    T* xc = mixed ? reinterpret_cast<T*>(cl.x32.p) : reinterpret_cast<T*>(cl.x.p);
     kRestrict<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, bc);
-    cycle<T>(l + 1);
-    const T* xc = mixed ? reinterpret_cast<const T*>(cl.x32.p) : reinterpret_cast<const T*>(cl.x.p);
-    kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, x, cfg.amgCorrection);
-    for (int k = 0; k < cfg.amgPost; ++k) sweep();
-    // Results live in the buffer named x for this level.
-    T* named = mixed ? reinterpret_cast<T*>(lv.x32.p) : reinterpret_cast<T*>(lv.x.p);
-    if (x != named) {
-        if (mixed) std::swap(lv.x32.p, lv.t32.p);
-        else std::swap(lv.x.p, lv.t.p);
-    }
+    cycle<T>(l + 1, bc, xc);
+    kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, cur, cfg.amgCorrection);
+    for (int k = 0; k < post; ++k) sweep();
+    if (cur != xOut) throw std::logic_error("V-cycle result left outside its output vector");
 }
 
 void AmgDevice::apply(const double* r, double* z) {
@@ -671,12 +679,10 @@ void AmgDevice::apply(const double* r, double* z) {
     const int nb = blocksFor(l0.n);
     if (mixed) {
         kToFloat<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b32.p);
-        cycle<float>(0);
+        cycle<float>(0, l0.b32.p, l0.x32.p);
         kToDouble<<<nb, kBlock, 0, s>>>(l0.n, l0.x32.p, z);
     } else {
-        kCopy<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b.p);
-        cycle<double>(0);
-        kCopy<<<nb, kBlock, 0, s>>>(l0.n, l0.x.p, z);
+        cycle<double>(0, r, z);
     }
     BSIM_LAUNCH_CHECK();
 }

@@ -156,7 +156,7 @@ private:
         DBuf<double> x, b, r, t;
         DBuf<float> x32, b32, r32, t32;
     };
-    template <class T> void cycle(int l);
+    template <class T> void cycle(int l, const T* b, T* xOut);
     hipStream_t stream;
     std::vector<std::unique_ptr<Level>> L;
     DBuf<double> lu, inv;
