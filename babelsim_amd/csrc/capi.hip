@@ -15,6 +15,8 @@ struct bsim_ctx {
 struct bsim_system {
     bsim_ctx* ctx;
     std::unique_ptr<bsim::DeviceSystem> sys;
+    std::vector<double> hist[3];  // per right-hand side of the last solve
+    int nrhs = 1;
 };
 struct bsim_simple {
     bsim_ctx* ctx;
@@ -293,23 +295,35 @@ int bsim_system_solve_ldu(bsim_system* sys, const double* diag, const double* up
         D.prepare(sc);
         const std::vector<double> b = C.cellsIn(source, nrhs), x0 = C.cellsIn(x, nrhs);
         std::vector<double> bc(M.nc), xc(M.nc), xout(x0.size());
-        DBuf<double> db(M.nc), dx(M.nc);
+        DBuf<double> db[3], dx[3];
         for (int comp = 0; comp < nrhs; ++comp) {
             for (int i = 0; i < M.nc; ++i) { bc[i] = b[static_cast<std::size_t>(i) * nrhs + comp]; xc[i] = x0[static_cast<std::size_t>(i) * nrhs + comp]; }
-            db.upload(bc, s);
-            dx.upload(xc, s);
-            const SolveRes r = D.solve(db.p, dx.p, comp);
-            const std::vector<double> h = dx.toHost(s);
-            for (int i = 0; i < M.nc; ++i) xout[static_cast<std::size_t>(i) * nrhs + comp] = h[i];
-            results[comp] = bsim_solve_result{r.status, r.iterations, r.initial, r.final_, r.relative};
+            db[comp].upload(bc, s);
+            dx[comp].upload(xc, s);
         }
+        SolveRes r[3];
+        if (nrhs == 1) {
+            r[0] = D.solve(db[0].p, dx[0].p, 0);
+            sys->hist[0] = D.hist;
+        } else {
+            const double* const b3[3] = {db[0].p, db[1].p, db[2].p};
+            double* const x3[3] = {dx[0].p, dx[1].p, dx[2].p};
+            D.solve3(b3, x3, r, sys->hist);
+        }
+        for (int comp = 0; comp < nrhs; ++comp) {
+            const std::vector<double> h = dx[comp].toHost(s);
+            for (int i = 0; i < M.nc; ++i) xout[static_cast<std::size_t>(i) * nrhs + comp] = h[i];
+            results[comp] = bsim_solve_result{r[comp].status, r[comp].iterations, r[comp].initial, r[comp].final_,
+                                              r[comp].relative};
+        }
+        sys->nrhs = nrhs;
         C.cellsOut(xout, x, nrhs);
     });
 }
 
 int bsim_system_history(bsim_system* sys, double* out, int cap, int* len) {
     return guarded([&] {
-        const auto& h = sys->sys->hist;
+        const auto& h = sys->hist[sys->nrhs - 1];
         *len = static_cast<int>(h.size());
         if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<std::size_t>(cap, h.size()));
     });

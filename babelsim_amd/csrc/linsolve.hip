@@ -329,6 +329,226 @@ struct BcgsNorm {  // dp = ||r||; rho and omega become the previous values
         sc[S_OMEGAOLD] = sc[S_OMEGA];
     }
 };
+// ---- BiCGSTAB on the three components of a vector equation in lockstep ----
+// Component c runs the recurrence of the kernels above on its own vectors,
+// scalars (sc3 + c * S_COUNT) and flags (fl3[c]); act[c] == 0 freezes it.  Each
+// component's arithmetic and reduction order are those of a solo solve, so
+// the three results equal three sequential solves bit for bit, while the
+// matrix is streamed once per product.
+struct Vec3p {
+    double* c[3];
+};
+struct CVec3p {
+    const double* c[3];
+};
+// Row i of A x_c for the components with m[c] set: the chunk's columns and
+// values are loaded once for all of them, and each sum runs in the solo order.
+__device__ inline void rowDot3(const CsrView<double>& A, const CVec3p& x, const bool (&m)[3], int i, double (&y)[3]) {
+    for (int c = 0; c < 3; ++c) y[c] = m[c] ? A.diag[i] * x.c[c][i] : 0.0;
+    if (A.sliceOff) {
+        const RowRange R(A.sliceOff, A.K, i);
+        constexpr int kC = 8;
+        for (int k0 = 0; k0 < A.K; k0 += kC) {
+            if (R.so[k0 + 1] == R.so[k0]) break;
+            int e[kC], col[kC];
+            double v[kC];
+#pragma unroll
+            for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
+#pragma unroll
+            for (int j = 0; j < kC; ++j) {
+                col[j] = e[j] >= 0 ? A.col[e[j]] : -1;
+                v[j] = e[j] >= 0 ? A.val[e[j]] : 0.0;
+            }
+            for (int c = 0; c < 3; ++c) {
+                if (!m[c]) continue;
+                double xv[kC];
+#pragma unroll
+                for (int j = 0; j < kC; ++j) xv[j] = col[j] >= 0 ? x.c[c][col[j]] : 0.0;
+#pragma unroll
+                for (int j = 0; j < kC; ++j)
+                    if (col[j] >= 0) y[c] += v[j] * xv[j];
+            }
+        }
+    } else {
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+            for (int c = 0; c < 3; ++c)
+                if (m[c]) y[c] += A.val[k] * x.c[c][A.col[k]];
+    }
+}
+
+__global__ void k3CopyNorm(int n, CVec3p b, Vec3p r, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) {
+            if (!act[c]) continue;
+            const double v = b.c[c][i];
+            if (r.c[c]) r.c[c][i] = v;
+            acc[c] += v * v;
+        }
+    blockPartial<3>(acc, part, smem);
+}
+__global__ void k3ResidualNorm(CsrView<double> A, CVec3p b, CVec3p x, Vec3p r, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
+        double ax[3];
+        rowDot3(A, x, m, i, ax);
+        for (int c = 0; c < 3; ++c) {
+            if (!m[c]) continue;
+            const double v = b.c[c][i] - ax[c];
+            r.c[c][i] = v;
+            acc[c] += v * v;
+        }
+    }
+    blockPartial<3>(acc, part, smem);
+}
+__global__ void k3Dot(int n, CVec3p a, CVec3p b, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c)
+            if (act[c]) acc[c] += a.c[c][i] * b.c[c][i];
+    blockPartial<3>(acc, part, smem);
+}
+__global__ void k3Copy(int n, CVec3p a, Vec3p b, const int* act) {
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) b.c[c][i] = a.c[c][i];
+}
+__global__ void k3Fill(int n, Vec3p a, double v, const int* act) {
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) a.c[c][i] = v;
+}
+__global__ void k3Axpy(int n, double a, CVec3p x, Vec3p y, const int* act) {  // y += a x
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) y.c[c][i] += a * x.c[c][i];
+}
+// z = r * (1/d), or a copy without a preconditioner
+__global__ void k3Jacobi(int n, CVec3p r, const double* jinv, Vec3p z, const int* act) {
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) z.c[c][i] = jinv ? r.c[c][i] * jinv[i] : r.c[c][i];
+}
+__global__ void k3BcgsP(int n, Vec3p p, CVec3p r, CVec3p v, const double* sc3, const int* act) {
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c) {
+        if (!act[c]) continue;
+        const double* sc = sc3 + c * S_COUNT;
+        const double beta = sc[S_BETA];
+        p.c[c][i] = r.c[c][i] + (-sc[S_OMEGAOLD] * beta) * v.c[c][i] + beta * p.c[c][i];
+    }
+}
+__global__ void k3SpmvD1(CsrView<double> A, CVec3p t, Vec3p v, CVec3p rp, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
+        double y[3];
+        rowDot3(A, t, m, i, y);
+        for (int c = 0; c < 3; ++c) {
+            if (!m[c]) continue;
+            v.c[c][i] = y[c];
+            acc[c] += y[c] * rp.c[c][i];
+        }
+    }
+    blockPartial<3>(acc, part, smem);
+}
+__global__ void k3BcgsS(int n, Vec3p s, CVec3p v, CVec3p r, const double* sc3, const int* fl3, const int* act) {
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c] && !fl3[c]) s.c[c][i] = -sc3[c * S_COUNT + S_ALPHA] * v.c[c][i] + r.c[c][i];
+}
+// Partials: s . t of the three components at k = 0..2, t . t at k = 3..5.
+__global__ void k3SpmvD2(CsrView<double> A, CVec3p rt, Vec3p t, CVec3p s, const int* fl3, const int* act,
+                         double* part) {
+    __shared__ double smem[6 * kBlock];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool m[3] = {act[0] && !fl3[0], act[1] && !fl3[1], act[2] && !fl3[2]};
+    for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
+        double y[3];
+        rowDot3(A, rt, m, i, y);
+        for (int c = 0; c < 3; ++c) {
+            if (!m[c]) continue;
+            t.c[c][i] = y[c];
+            acc[c] += s.c[c][i] * y[c];
+            acc[3 + c] += y[c] * y[c];
+        }
+    }
+    blockPartial<6>(acc, part, smem);
+}
+__global__ void k3BcgsUpdate(int n, Vec3p x, CVec3p p, CVec3p s, Vec3p r, CVec3p t, const double* sc3, const int* fl3,
+                             const int* act, double* part) {
+    __shared__ double smem[3 * kBlock];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) {
+            if (!act[c] || fl3[c]) continue;
+            const double alpha = sc3[c * S_COUNT + S_ALPHA], omega = sc3[c * S_COUNT + S_OMEGA];
+            x.c[c][i] = alpha * p.c[c][i] + omega * s.c[c][i] + x.c[c][i];
+            const double ri = -omega * t.c[c][i] + s.c[c][i];
+            r.c[c][i] = ri;
+            acc[c] += ri * ri;
+        }
+    blockPartial<3>(acc, part, smem);
+}
+struct Tail3Sqrt {  // sc[idx] = sqrt(sum) per active component
+    double* sc3;
+    int idx;
+    const int* act;
+    __device__ void operator()(const double (&v)[3]) const {
+        for (int c = 0; c < 3; ++c)
+            if (act[c]) sc3[c * S_COUNT + idx] = sqrt(v[c]);
+    }
+};
+struct Tail3Rho {
+    double* sc3;
+    const int* act;
+    __device__ void operator()(const double (&v)[3]) const {
+        for (int c = 0; c < 3; ++c)
+            if (act[c]) { const double one[1] = {v[c]}; BcgsRho{sc3 + c * S_COUNT}(one); }
+    }
+};
+struct Tail3Alpha {
+    double* sc3;
+    int* fl3;
+    const int* act;
+    __device__ void operator()(const double (&v)[3]) const {
+        for (int c = 0; c < 3; ++c)
+            if (act[c]) { const double one[1] = {v[c]}; BcgsAlpha{sc3 + c * S_COUNT, fl3 + c}(one); }
+    }
+};
+struct Tail3Omega {
+    double* sc3;
+    int* fl3;
+    const int* act;
+    __device__ void operator()(const double (&v)[6]) const {
+        for (int c = 0; c < 3; ++c) {
+            if (!act[c]) continue;
+            const double pair[2] = {v[c], v[3 + c]};
+            BcgsOmega{sc3 + c * S_COUNT, fl3 + c}(pair);
+        }
+    }
+};
+struct Tail3Norm {
+    double* sc3;
+    const int* fl3;
+    const int* act;
+    __device__ void operator()(const double (&v)[3]) const {
+        for (int c = 0; c < 3; ++c)
+            if (act[c]) { const double one[1] = {v[c]}; BcgsNorm{sc3 + c * S_COUNT, fl3 + c}(one); }
+    }
+};
+
 __global__ void kSpmvPlain(CsrView<double> A, const double* x, double* y) {
     const int i = globalThread();
     if (i < A.n) y[i] = rowDot64(A, x, i);
@@ -693,13 +913,20 @@ DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
     for (int k = 0; k < 2; ++k) { diagBuf[k].alloc(n); valBuf[k].alloc(std::max(nnz, 1)); }
     sc.alloc(S_COUNT);
     flags.alloc(1);
-    partials.alloc(4 * kMaxReduceBlocks);
+    partials.alloc(6 * kMaxReduceBlocks);
     BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostScalars), sizeof(double) * (S_COUNT + 2), hipHostMallocDefault));
     for (DBuf<double>* b : {&r, &z, &p, &w, &rp, &v, &t, &s, &tmp, &guess, &saved}) b->alloc(n);
+    sc3.alloc(3 * S_COUNT);
+    fl3.alloc(3);
+    act3.alloc(3);
+    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc3), sizeof(double) * (3 * S_COUNT + 2), hipHostMallocDefault));
+    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hact3), sizeof(int) * 4 * kActSlots, hipHostMallocDefault));
 }
 
 DeviceSystem::~DeviceSystem() {
     if (hostScalars) (void)hipHostFree(hostScalars);
+    if (hsc3) (void)hipHostFree(hsc3);
+    if (hact3) (void)hipHostFree(hact3);
 }
 
 bool DeviceSystem::update() {
@@ -959,25 +1186,41 @@ void DeviceSystem::bcgsMixed(const double* b, double* x, bool guessZero, int max
     throw std::invalid_argument("mixed-precision BiCGSTAB is not available in this build");
 }
 
-SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
+SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) { return solveFrom(b, x, rhsIndex, nullptr); }
+
+// The solve contract of petsc_linear_system.cpp:355-481.  With `first`, the
+// prelude (norms) and the first Krylov call of attempt 0 were already run on
+// (b, x) by the lockstep solver; their results, history and counters are
+// taken from it, and `saved` must hold the component's initial guess.
+SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const FirstRun* first) {
     if (!ready || !configured || setupPending)
         throw std::logic_error("linear system must update and prepare before solving a right-hand side");
     const auto t0 = Clock::now();
     const int nb = blocksFor(n);
     hist.clear();
     ++counters.solves;
-    // x0 = 0, or the current field when warm starting.
-    kCopy<<<nb, kBlock, 0, stream>>>(n, x, saved.p);
     auto resetGuess = [&]() {
         if (cfg.warmStart) kCopy<<<nb, kBlock, 0, stream>>>(n, saved.p, x);
         else kFill<<<nb, kBlock, 0, stream>>>(n, x, 0.0);
     };
-    resetGuess();
-    BSIM_LAUNCH_CHECK();
-    kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, partials.p);
-    kReduceTail<1><<<1, kBlock, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
-    const double rhsNorm = readScalar(S_BN);
-    const double initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
+    double rhsNorm, initial;
+    if (first) {
+        hist = first->hist;
+        counters.spmvs += first->delta.spmvs;
+        counters.trueResidualChecks += first->delta.trueResidualChecks;
+        counters.pcApplications += first->delta.pcApplications;
+        rhsNorm = first->rhsNorm;
+        initial = first->initial;
+    } else {
+        // x0 = 0, or the current field when warm starting.
+        kCopy<<<nb, kBlock, 0, stream>>>(n, x, saved.p);
+        resetGuess();
+        BSIM_LAUNCH_CHECK();
+        kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, partials.p);
+        kReduceTail<1><<<1, kBlock, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
+        rhsNorm = readScalar(S_BN);
+        initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
+    }
     const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;
     const double target = std::max(cfg.atol, cfg.rtol * scale);
     const bool reused = cfg.pc != 0 && pcMatrixUpdate < counters.matrixUpdates;
@@ -997,12 +1240,14 @@ SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
         bool guessZero = !cfg.warmStart;
         for (int restart = 0; restart < 3; ++restart) {
             int its = 0;
-            if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            const bool given = first && attempt == 0 && restart == 0;
+            if (given) { its = first->its; reason = first->reason; }
+            else if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             else if (cfg.precision == 1) bcgsMixed(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             else bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             attemptIts += its;
             total += its;
-            fin = trueResidual(b, x);
+            fin = given ? first->fin : trueResidual(b, x);
             if ((std::isfinite(fin) && fin <= target) || reason <= 0 || !std::isfinite(fin) || attemptIts >= cfg.maxIter)
                 break;
             internalTarget *= 0.5;
@@ -1030,6 +1275,231 @@ SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
     counters.krylovIterations += total;
     counters.solveSeconds += since(t0);
     return res;
+}
+
+// The active mask goes to the device from a ring of pinned slots: a slot is
+// only rewritten after the stream has synchronised since its last copy.
+void DeviceSystem::uploadAct(const bool (&m)[3]) {
+    int* slot = hact3 + 4 * (actSlot++ % kActSlots);
+    for (int c = 0; c < 3; ++c) slot[c] = m[c] ? 1 : 0;
+    BSIM_HIP(hipMemcpyAsync(act3.p, slot, sizeof(int) * 3, hipMemcpyHostToDevice, stream));
+}
+
+void DeviceSystem::syncScalars3() {
+    BSIM_HIP(hipMemcpyAsync(hsc3, sc3.p, sizeof(double) * 3 * S_COUNT, hipMemcpyDeviceToHost, stream));
+    BSIM_HIP(hipMemcpyAsync(reinterpret_cast<int*>(hsc3 + 3 * S_COUNT), fl3.p, sizeof(int) * 3, hipMemcpyDeviceToHost,
+                            stream));
+    BSIM_HIP(hipStreamSynchronize(stream));
+}
+
+namespace {
+Vec3p vec3(DBuf<double> (&a)[3]) { return Vec3p{{a[0].p, a[1].p, a[2].p}}; }
+CVec3p cvec3(DBuf<double> (&a)[3]) { return CVec3p{{a[0].p, a[1].p, a[2].p}}; }
+}  // namespace
+
+// One BiCGSTAB call (bcgs() above, bcgs.c:14-140) on every component with
+// run[c], in lockstep.  Each component keeps the solo control flow: its own
+// convergence test, breakdown exits and iteration count; a component that
+// stops is frozen through act3 while the others go on.  Counters go to dl[c].
+void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const bool (&run)[3], bool guessZero, int maxIt,
+                         const double (&target)[3], int (&its)[3], int (&reason)[3], std::vector<double> (&h)[3],
+                         SystemCounters (&dl)[3]) {
+    double* part = partials.p;
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    const CsrView<double> A = view();
+    const CVec3p B{{b[0], b[1], b[2]}};
+    const Vec3p X{{x[0], x[1], x[2]}};
+    const CVec3p cX{{x[0], x[1], x[2]}};
+    const double* jv = cfg.pc == 1 ? jinv.p : nullptr;
+    int* act = act3.p;
+    auto upload = [&](const bool (&m)[3]) { uploadAct(m); };
+    bool on[3] = {run[0], run[1], run[2]};
+    upload(on);
+    BSIM_HIP(hipMemsetAsync(fl3.p, 0, sizeof(int) * 3, stream));
+    {
+        double init[3 * S_COUNT];
+        const double one[S_COUNT] = {0, 1.0, 1.0, 0, 1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 3; ++c) std::memcpy(init + c * S_COUNT, one, sizeof(one));
+        std::memcpy(hsc3, init, sizeof(init));
+        BSIM_HIP(hipMemcpyAsync(sc3.p, hsc3, sizeof(init), hipMemcpyHostToDevice, stream));
+    }
+    const bool haveGuess = !guessZero;
+    if (!guessZero) {
+        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(A, B, cX, vec3(r3), act, part);
+        for (int c = 0; c < 3; ++c) if (on[c]) ++dl[c].spmvs;
+        k3Copy<<<nb, kBlock, 0, stream>>>(n, cX, vec3(guess3), act);
+        k3Fill<<<nb, kBlock, 0, stream>>>(n, X, 0.0, act);
+    } else {
+        k3CopyNorm<<<rb, kBlock, 0, stream>>>(n, B, vec3(r3), act, part);
+    }
+    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_DP, act});
+    BSIM_LAUNCH_CHECK();
+    syncScalars3();
+    double dp[3];
+    bool loop[3] = {false, false, false}, atStart[3] = {false, false, false};
+    auto test = [&](int c, double vv) {
+        if (!std::isfinite(vv)) { reason[c] = -9; return true; }
+        if (vv <= target[c]) { reason[c] = 2; return true; }
+        return false;
+    };
+    for (int c = 0; c < 3; ++c) {
+        if (!on[c]) continue;
+        dp[c] = hsc3[c * S_COUNT + S_DP];
+        its[c] = 0;
+        reason[c] = 0;
+        h[c].push_back(dp[c]);
+        if (test(c, dp[c])) atStart[c] = true;
+        else loop[c] = true;
+    }
+    if (haveGuess && (atStart[0] || atStart[1] || atStart[2])) {  // x = guess for those that stop at once
+        upload(atStart);
+        k3Axpy<<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(guess3), X, act);
+    }
+    bool active[3] = {loop[0], loop[1], loop[2]};
+    upload(active);
+    k3Copy<<<nb, kBlock, 0, stream>>>(n, cvec3(r3), vec3(rp3), act);
+    k3Fill<<<nb, kBlock, 0, stream>>>(n, vec3(p3), 0.0, act);
+    k3Fill<<<nb, kBlock, 0, stream>>>(n, vec3(vv3), 0.0, act);
+    int iter[3] = {0, 0, 0};
+    while (active[0] || active[1] || active[2]) {
+        k3Dot<<<rb, kBlock, 0, stream>>>(n, cvec3(r3), cvec3(rp3), act, part);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Rho{sc3.p, act});
+        k3BcgsP<<<nb, kBlock, 0, stream>>>(n, vec3(p3), cvec3(r3), cvec3(vv3), sc3.p, act);
+        k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cvec3(p3), jv, vec3(tmp3), act);
+        k3SpmvD1<<<rb, kBlock, 0, stream>>>(A, cvec3(tmp3), vec3(vv3), cvec3(rp3), act, part);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Alpha{sc3.p, fl3.p, act});
+        k3BcgsS<<<nb, kBlock, 0, stream>>>(n, vec3(s3), cvec3(vv3), cvec3(r3), sc3.p, fl3.p, act);
+        k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cvec3(s3), jv, vec3(r3), act);
+        k3SpmvD2<<<rb, kBlock, 0, stream>>>(A, cvec3(r3), vec3(t3), cvec3(s3), fl3.p, act, part);
+        kReduceTail<6><<<1, kBlock, 0, stream>>>(rb, part, Tail3Omega{sc3.p, fl3.p, act});
+        k3BcgsUpdate<<<rb, kBlock, 0, stream>>>(n, X, cvec3(p3), cvec3(s3), vec3(r3), cvec3(t3), sc3.p, fl3.p, act,
+                                                part);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Norm{sc3.p, fl3.p, act});
+        BSIM_LAUNCH_CHECK();
+        syncScalars3();
+        bool changed = false;
+        for (int c = 0; c < 3; ++c) {
+            if (!active[c]) continue;
+            dl[c].spmvs += 2;
+            dl[c].pcApplications += 2;
+            const double* hs = hsc3 + c * S_COUNT;
+            const int fl = reinterpret_cast<const int*>(hsc3 + 3 * S_COUNT)[c];
+            bool stop = false;
+            if (fl & F_NONFINITE) { reason[c] = -9; stop = true; }
+            else if (fl & F_D1_ZERO) { reason[c] = -5; stop = true; }
+            else if (fl & F_D2_ZERO) {
+                stop = true;
+                kDot<<<rb, kBlock, 0, stream>>>(n, s3[c].p, s3[c].p, part);
+                kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreValue{sc.p, S_TMP});
+                BSIM_LAUNCH_CHECK();
+                if (readScalar(S_TMP) != 0.0) reason[c] = -5;
+                else {
+                    kAxpy<<<nb, kBlock, 0, stream>>>(n, hs[S_ALPHA], p3[c].p, x[c]);
+                    ++its[c];
+                    reason[c] = 2;
+                    h[c].push_back(dp[c]);
+                }
+            } else {
+                dp[c] = hs[S_DP];
+                ++its[c];
+                h[c].push_back(dp[c]);
+                if (test(c, dp[c])) stop = true;
+                else if (hs[S_RHO] == 0.0) { reason[c] = -5; stop = true; }
+                else if (++iter[c] >= maxIt) { reason[c] = -3; stop = true; }
+            }
+            if (stop) { active[c] = false; changed = true; }
+        }
+        if (changed) upload(active);
+    }
+    // Unwind the right preconditioning for the components that iterated.
+    upload(loop);
+    k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cX, jv, vec3(tmp3), act);
+    k3Copy<<<nb, kBlock, 0, stream>>>(n, cvec3(tmp3), X, act);
+    if (haveGuess) k3Axpy<<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(guess3), X, act);
+    BSIM_LAUNCH_CHECK();
+    for (int c = 0; c < 3; ++c) if (loop[c]) ++dl[c].pcApplications;
+}
+
+void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3]) {
+    if (!ready || !configured || setupPending)
+        throw std::logic_error("linear system must update and prepare before solving a right-hand side");
+    const bool lockstep = cfg.solver == 1 && cfg.pc != 2 && cfg.precision == 0;
+    if (!lockstep) {
+        for (int c = 0; c < 3; ++c) {
+            res[c] = solve(b[c], x[c], c);
+            hists[c] = hist;
+        }
+        return;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (DBuf<double>* d : {&r3[c], &p3[c], &vv3[c], &t3[c], &s3[c], &rp3[c], &tmp3[c], &guess3[c], &saved3[c]})
+            d->ensure(n);
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    const CVec3p B{{b[0], b[1], b[2]}};
+    const Vec3p X{{x[0], x[1], x[2]}};
+    const CVec3p cX{{x[0], x[1], x[2]}};
+    // Prelude of every component (solveFrom): keep the guess, reset, norms.
+    uploadAct({true, true, true});
+    k3Copy<<<nb, kBlock, 0, stream>>>(n, cX, vec3(saved3), act3.p);
+    if (!cfg.warmStart) k3Fill<<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
+    k3CopyNorm<<<rb, kBlock, 0, stream>>>(n, B, Vec3p{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
+    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
+    if (cfg.warmStart) {
+        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(tmp3), act3.p, partials.p);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+    }
+    BSIM_LAUNCH_CHECK();
+    syncScalars3();
+    FirstRun fr[3];
+    bool run[3];
+    double target[3];
+    for (int c = 0; c < 3; ++c) {
+        fr[c].rhsNorm = hsc3[c * S_COUNT + S_BN];
+        fr[c].initial = cfg.warmStart ? hsc3[c * S_COUNT + S_TR] : fr[c].rhsNorm;
+        if (cfg.warmStart) { ++fr[c].delta.spmvs; ++fr[c].delta.trueResidualChecks; }
+        const double scale = std::max(fr[c].initial, fr[c].rhsNorm) > 0.0 ? std::max(fr[c].initial, fr[c].rhsNorm) : 1.0;
+        target[c] = std::max(cfg.atol, cfg.rtol * scale);
+        run[c] = !(std::isfinite(fr[c].initial) && fr[c].initial <= target[c]);
+    }
+    // First Krylov call of attempt 0 for every component that needs one.
+    int its[3] = {0, 0, 0}, reason[3] = {0, 0, 0};
+    std::vector<double> h[3];
+    SystemCounters dl[3];
+    if (run[0] || run[1] || run[2]) {
+        bcgs3(b, x, run, !cfg.warmStart, cfg.maxIter, target, its, reason, h, dl);
+        uploadAct(run);
+        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(tmp3), act3.p, partials.p);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+        BSIM_LAUNCH_CHECK();
+        syncScalars3();
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (!run[c]) continue;
+        fr[c].its = its[c];
+        fr[c].reason = reason[c];
+        fr[c].fin = hsc3[c * S_COUNT + S_TR];
+        fr[c].hist = std::move(h[c]);
+        fr[c].delta.spmvs += dl[c].spmvs + 1;
+        fr[c].delta.pcApplications += dl[c].pcApplications;
+        ++fr[c].delta.trueResidualChecks;
+    }
+    // Commit in component order.  A preconditioner rebuild during one
+    // component's retry changes what the later components would have seen,
+    // so they are solved again from their initial guesses.
+    bool diverged = false;
+    for (int c = 0; c < 3; ++c) {
+        if (diverged) {
+            kCopy<<<nb, kBlock, 0, stream>>>(n, saved3[c].p, x[c]);
+            res[c] = solve(b[c], x[c], c);
+        } else {
+            kCopy<<<nb, kBlock, 0, stream>>>(n, saved3[c].p, saved.p);
+            const std::uint64_t retries = counters.retries;
+            res[c] = solveFrom(b[c], x[c], c, &fr[c]);
+            diverged = counters.retries != retries;
+        }
+        hists[c] = hist;
+    }
+    BSIM_LAUNCH_CHECK();
 }
 
 }  // namespace bsim

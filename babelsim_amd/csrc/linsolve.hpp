@@ -94,6 +94,10 @@ public:
     bool update();                          // petsc_linear_system.cpp:257-282
     void prepare(const SolverCfg& cfg);     // petsc_linear_system.cpp:284-301
     SolveRes solve(const double* b, double* x, int rhsIndex);  // petsc_linear_system.cpp:355-481
+    // The three right-hand sides of a vector equation (petsc_backend.cpp:156-198):
+    // the results, histories and counters of solve() on components 0, 1, 2 in
+    // order, with BiCGSTAB (Jacobi or no preconditioner, FP64) run in lockstep.
+    void solve3(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3]);
     void spmv(const double* x, double* y);   // y = A x with the effective matrix
     double timeSpmv(int reps);               // mean seconds of the solver's SpMV kernel (HIP events)
     std::vector<double> hist;               // unpreconditioned residual norm per Krylov iteration
@@ -111,6 +115,27 @@ private:
     void bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     void bcgsMixed(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     double readScalar(int idx, int* flags = nullptr);
+    // The prelude and first Krylov call of one component, already executed.
+    struct FirstRun {
+        double rhsNorm = 0, initial = 0, fin = 0;
+        int its = 0, reason = 0;
+        std::vector<double> hist;
+        SystemCounters delta;
+    };
+    SolveRes solveFrom(const double* b, double* x, int rhsIndex, const FirstRun* first);
+    void bcgs3(const double* const b[3], double* const x[3], const bool (&run)[3], bool guessZero, int maxIt,
+               const double (&target)[3], int (&its)[3], int (&reason)[3], std::vector<double> (&h)[3],
+               SystemCounters (&dl)[3]);
+    void syncScalars3();
+    void uploadAct(const bool (&m)[3]);
+    // Lockstep work vectors, one per component.
+    DBuf<double> r3[3], p3[3], vv3[3], t3[3], s3[3], rp3[3], tmp3[3], guess3[3], saved3[3];
+    DBuf<double> sc3;           // 3 x S_COUNT device scalars
+    DBuf<int> fl3, act3;
+    double* hsc3 = nullptr;     // pinned: 3 x S_COUNT scalars then 3 flags
+    static constexpr int kActSlots = 16;
+    int* hact3 = nullptr;       // pinned ring of active masks (kActSlots x 4 ints)
+    unsigned actSlot = 0;
     DBuf<double> diagBuf[2], valBuf[2];
     int cand = 0;
     bool ready = false, configured = false, setupPending = false, forceRebuild = false;

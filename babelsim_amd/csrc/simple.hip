@@ -25,13 +25,20 @@ __global__ void kCopyD(long long n, const double* a, double* b) {
     const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
     if (i < n) b[i] = a[i];
 }
-__global__ void kExtract(int n, const double* v3, int comp, double* out) {
+// Vec3 cell arrays <-> one array per component.
+__global__ void kSplit(int n, const double* v3, double* c0, double* c1, double* c2) {
     const int i = globalThread();
-    if (i < n) out[i] = v3[3 * static_cast<long long>(i) + comp];
+    if (i >= n) return;
+    c0[i] = v3[3 * static_cast<long long>(i)];
+    c1[i] = v3[3 * static_cast<long long>(i) + 1];
+    c2[i] = v3[3 * static_cast<long long>(i) + 2];
 }
-__global__ void kInsert(int n, const double* in, int comp, double* v3) {
+__global__ void kMerge(int n, const double* c0, const double* c1, const double* c2, double* v3) {
     const int i = globalThread();
-    if (i < n) v3[3 * static_cast<long long>(i) + comp] = in[i];
+    if (i >= n) return;
+    v3[3 * static_cast<long long>(i)] = c0[i];
+    v3[3 * static_cast<long long>(i) + 1] = c1[i];
+    v3[3 * static_cast<long long>(i) + 2] = c2[i];
 }
 
 // equ::source(momentum, -grad(p)): source += (1*V) * (-1*gradP)  (procedural_equation.cpp:192-196)
@@ -222,7 +229,8 @@ SimpleSolver::SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, 
     pType.upload(pbct, np, s); pVal.upload(pbcv, np, s);
     ppType.upload(ppt, s); ppVal.upload(ppv, s);
     zgType.upload(zg, s); zgVal.upload(zgv, s);
-    for (DBuf<double>* b : {&p, &rAU, &divPhiH, &pPrime, &psrc, &bcomp, &xcomp, &mu}) b->alloc(nc);
+    for (DBuf<double>* b : {&p, &rAU, &divPhiH, &pPrime, &psrc, &mu}) b->alloc(nc);
+    for (int c = 0; c < 3; ++c) { bcomp[c].alloc(nc); xcomp[c].alloc(nc); }
     for (DBuf<double>* b : {&U, &Uprev, &gradP, &gradPP, &X, &src}) b->alloc(3 * static_cast<std::size_t>(nc));
     for (DBuf<double>* b : {&gradU, &work9, &gradX}) b->alloc(9 * static_cast<std::size_t>(nc));
     for (DBuf<double>* b : {&phi, &phiH, &rAUtr, &rAUf, &sn, &kfMom, &kfP, &corr, &full, &ctxSeed}) b->alloc(nf);
@@ -328,16 +336,21 @@ IterRep SimpleSolver::iterate() {
     momSys->update();
     momSys->prepare(c.mom);
     rep.momStatus = kConverged;
-    for (int comp = 0; comp < 3; ++comp) {
-        kExtract<<<nb, kBlock, 0, s>>>(nc, src.p, comp, bcomp.p);
-        kExtract<<<nb, kBlock, 0, s>>>(nc, U.p, comp, xcomp.p);
-        const SolveRes r = momSys->solve(bcomp.p, xcomp.p, comp);
-        kInsert<<<nb, kBlock, 0, s>>>(nc, xcomp.p, comp, U.p);
-        momHist[comp] = momSys->hist;
-        rep.momIters[comp] = r.iterations;
-        if (r.status == kNumericalFailure) rep.momStatus = kNumericalFailure;
-        else if (r.status != kConverged && rep.momStatus != kNumericalFailure) rep.momStatus = kMaxIterations;
+    kSplit<<<nb, kBlock, 0, s>>>(nc, src.p, bcomp[0].p, bcomp[1].p, bcomp[2].p);
+    kSplit<<<nb, kBlock, 0, s>>>(nc, U.p, xcomp[0].p, xcomp[1].p, xcomp[2].p);
+    {
+        const double* const b3[3] = {bcomp[0].p, bcomp[1].p, bcomp[2].p};
+        double* const x3[3] = {xcomp[0].p, xcomp[1].p, xcomp[2].p};
+        SolveRes r3[3];
+        momSys->solve3(b3, x3, r3, momHist);
+        for (int comp = 0; comp < 3; ++comp) {
+            const SolveRes& r = r3[comp];
+            rep.momIters[comp] = r.iterations;
+            if (r.status == kNumericalFailure) rep.momStatus = kNumericalFailure;
+            else if (r.status != kConverged && rep.momStatus != kNumericalFailure) rep.momStatus = kMaxIterations;
+        }
     }
+    kMerge<<<nb, kBlock, 0, s>>>(nc, xcomp[0].p, xcomp[1].p, xcomp[2].p, U.p);
     BSIM_LAUNCH_CHECK();
     timing.momentumSolve += since(t0);
     t0 = Clock::now();

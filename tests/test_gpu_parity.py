@@ -202,10 +202,21 @@ def run_pair(mesh_data, cfg, two_d, iters):
     return so, sg, ro, rg
 
 
-@pytest.mark.parametrize("kind", ["c1", "c2"])
+@pytest.mark.parametrize("kind", ["c1", "c2", "c1_retry", "c1_warm"])
 def test_simple_parity(kind):
+    """Device SIMPLE against the oracle.  c1_retry caps the momentum solves at 3
+    BiCGSTAB iterations so that lagged-preconditioner failures trigger the
+    rebuild-and-retry path of the three-component solve; c1_warm warm-starts
+    every solve."""
     if kind == "c1":
         mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(), True, 12
+    elif kind == "c1_retry":
+        mom = solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14, max_iter=3)
+        mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(mom=mom), True, 12
+    elif kind == "c1_warm":
+        mom = solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14, warm_start=1)
+        pre = solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14, warm_start=1)
+        mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(mom=mom, pre=pre), True, 12
     else:
         mesh, cfg, two_d, iters = cases.cavity3d_mesh(7, split=0.15, seed=3), cases.c2_config(), False, 8
     so, sg, ro, rg = run_pair(mesh, cfg, two_d, iters)
