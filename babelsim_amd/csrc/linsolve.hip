@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 
 namespace bsim {
 
@@ -587,6 +588,22 @@ template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const 
     const double ax = rowDot(A, x, q, i);
     xo[i] = static_cast<T>(static_cast<double>(x[i]) + omega * (static_cast<double>(b[i]) - ax) / static_cast<double>(A.diag[i]));
 }
+// The last fine-level sweep of a preconditioner application inside CG, with
+// the partial sums of z . r (z the result, r = b) that kPlainDot would form.
+__global__ void kJacobiSweepDot(CsrView<double> A, const double* b, const double* x, double* xo, double omega,
+                                double* part) {
+    __shared__ double smem[kBlock];
+    double acc = 0.0;
+    for (int q = globalThread(); q < A.n; q += gridDim.x * blockDim.x) {
+        const int i = A.perm ? A.perm[q] : q;
+        const double ax = rowDot(A, x, q, i);
+        const double z = x[i] + omega * (b[i] - ax) / A.diag[i];
+        xo[i] = z;
+        acc += z * b[i];
+    }
+    double v[1] = {acc};
+    blockPartial<1>(v, part, smem);
+}
 template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
     const int q = globalThread();
     if (q >= A.n) return;
@@ -625,13 +642,25 @@ __global__ void kToDouble(int n, const float* a, double* b) {
     if (i < n) b[i] = static_cast<double>(a[i]);
 }
 // Dense fill + LU with partial pivoting of the coarsest matrix (one block).
-__global__ void kDenseLU(int n, const int* ptr, const int* col, const double* d, const double* v, double* a, int* piv,
-                         int* flag) {
+// The coarsest level can be stored as CSR or sliced (a one-level hierarchy is
+// the fine matrix itself); rows are filled in entry order either way.
+__global__ void kDenseLU(CsrView<double> A, double* a, int* piv, int* flag) {
+    const int n = A.n;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) a[e] = 0.0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        a[static_cast<long long>(i) * n + i] = d[i];
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k) a[static_cast<long long>(i) * n + col[k]] += v[k];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int i = A.perm ? A.perm[q] : q;
+        double* row = a + static_cast<long long>(i) * n;
+        row[i] = A.diag[i];
+        if (A.sliceOff) {
+            const RowRange R(A.sliceOff, A.K, q);
+            for (int k = 0; R.valid(k); ++k) {
+                const int e = R.at(k), c = A.col[e];
+                if (c >= 0) row[c] += A.val[e];
+            }
+        } else {
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) row[A.col[k]] += A.val[k];
+        }
     }
     __syncthreads();
     __shared__ int sp;
@@ -664,10 +693,12 @@ __global__ void kDenseLU(int n, const int* ptr, const int* col, const double* d,
 }
 // Explicit inverse of the coarsest matrix from its LU factors: column j is
 // the LU solve of the unit vector e_j (thread j works in column j of inv).
+// The inverse is stored column-major (thread j owns column j), so that the
+// dense apply below reads it with consecutive lanes on consecutive rows.
 __global__ void kDenseInverse(int n, const double* a, const int* piv, double* inv) {
     const int j = globalThread();
     if (j >= n) return;
-    auto w = [&](int i) -> double& { return inv[static_cast<long long>(i) * n + j]; };
+    auto w = [&](int i) -> double& { return inv[static_cast<long long>(j) * n + i]; };
     for (int i = 0; i < n; ++i) w(i) = i == j ? 1.0 : 0.0;
     for (int k = 0; k < n; ++k) { const double t = w(k); w(k) = w(piv[k]); w(piv[k]) = t; }
     for (int i = 0; i < n; ++i)
@@ -682,7 +713,18 @@ template <class T> __global__ void kDenseApply(int n, const double* inv, const T
     const int i = globalThread();
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < n; ++j) s += inv[static_cast<long long>(i) * n + j] * static_cast<double>(b[j]);
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {  // eight independent loads in flight, summed in order
+        double v[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = inv[static_cast<long long>(j + u) * n + i];
+            w[u] = static_cast<double>(b[j + u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u] * w[u];
+    }
+    for (; j < n; ++j) s += inv[static_cast<long long>(j) * n + i] * static_cast<double>(b[j]);
     xo[i] = static_cast<T>(s);
 }
 
@@ -834,7 +876,9 @@ void AmgDevice::setup(const CsrView<double>& A, const Pattern& P, const SolverCf
         }
     Level& co = *L.back();
     luFlag.zero(s);
-    kDenseLU<<<1, 1024, 0, s>>>(co.n, co.ptr, co.col, co.diag.p, co.val.p, lu.p, piv.p, luFlag.p);
+    if (!co.ptr && !co.sliceOff) throw std::logic_error("AMG coarsest level has no row layout");
+    const CsrView<double> Aco{co.n, co.ptr, co.col, co.diag.p, co.val.p, co.sliceOff, co.K, co.perm};
+    kDenseLU<<<1, 1024, 0, s>>>(Aco, lu.p, piv.p, luFlag.p);
     BSIM_LAUNCH_CHECK();
     if (luFlag.toHost(s)[0]) throw std::runtime_error("AMG coarsest matrix is singular");
     inv.ensure(static_cast<std::size_t>(co.n) * co.n);
@@ -889,22 +933,36 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut) {
     kRestrict<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, bc);
     cycle<T>(l + 1, bc, xc);
     kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, cur, cfg.amgCorrection);
-    for (int k = 0; k < post; ++k) sweep();
+    for (int k = 0; k < post; ++k) {
+        if constexpr (std::is_same<T, double>::value) {
+            if (l == 0 && k == post - 1 && dotPartials) {  // last sweep also forms z . r
+                kJacobiSweepDot<<<reduceBlocks(lv.n), kBlock, 0, s>>>(A, b, cur, nxt, w, dotPartials);
+                std::swap(cur, nxt);
+                continue;
+            }
+        }
+        sweep();
+    }
     if (cur != xOut) throw std::logic_error("V-cycle result left outside its output vector");
 }
 
-void AmgDevice::apply(const double* r, double* z) {
+bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
     Level& l0 = *L[0];
     hipStream_t s = stream;
     const int nb = blocksFor(l0.n);
+    bool fused = false;
     if (mixed) {
         kToFloat<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b32.p);
         cycle<float>(0, l0.b32.p, l0.x32.p);
         kToDouble<<<nb, kBlock, 0, s>>>(l0.n, l0.x32.p, z);
     } else {
+        fused = zrPartials && L.size() > 1 && cfg.amgPost > 0;
+        dotPartials = fused ? zrPartials : nullptr;
         cycle<double>(0, r, z);
+        dotPartials = nullptr;
     }
     BSIM_LAUNCH_CHECK();
+    return fused;
 }
 
 // ------------------------------------------------------------- systems
@@ -1066,6 +1124,9 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
         if (cfg.pc == 1) {
             kJacobiDot<<<rb, kBlock, 0, stream>>>(n, r.p, jinv.p, z.p, flags.p, part);
             ++counters.pcApplications;
+        } else if (cfg.pc == 2) {
+            ++counters.pcApplications;
+            if (!amg->apply(r.p, z.p, part)) kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, flags.p, part);
         } else {
             applyPC(r.p, z.p);
             kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, flags.p, part);

@@ -159,7 +159,10 @@ class AmgDevice {
 public:
     AmgDevice(hipStream_t s) : stream(s) {}
     void setup(const CsrView<double>& A, const Pattern& pattern, const SolverCfg& cfg);
-    void apply(const double* r, double* z);  // z = M^{-1} r (FP64 in/out)
+    // z = M^{-1} r (FP64 in/out).  With zrPartials, the FP64 cycle's last
+    // fine-level sweep also stores the per-block partials of z . r there and
+    // apply() returns true.
+    bool apply(const double* r, double* z, double* zrPartials = nullptr);
     int levels() const { return static_cast<int>(L.size()); }
     std::vector<int> sizes() const;
     std::vector<int> aggregates0() const { return hostAgg0; }
@@ -167,6 +170,7 @@ public:
     double setupSeconds = 0.0;
 
     KernelProbe* probe = nullptr;
+    double* dotPartials = nullptr;  // set by apply() for the fused last sweep
 
 private:
     struct Level {

@@ -157,6 +157,20 @@ def test_csr_solver_histories(solver, pc):
     assert rel_l2(x2, x1) < 1e-9
 
 
+def test_one_level_amg_on_the_mesh_layout():
+    """A hierarchy with a single level (coarse size above the row count) puts
+    the dense factorisation on the fine matrix in its sliced mesh layout."""
+    arrays, om, diag, up, lo, b = laplace_system(n=(6, 5, 4), anchor=True)
+    assert len(diag) < 200
+    ctx = Context(arrays, renumber=True)
+    sys = System(ctx)
+    cfg = solver_config(solver=0, pc=2, rtol=1e-10, atol=1e-30, amg_coarse_size=len(diag) + 1)
+    x, res = sys.solve(diag, up, lo, b, np.zeros_like(b), cfg)
+    assert res[0].status == 0 and res[0].iterations <= 2
+    xo, ro, _ = orc.csr_solve(*orc.csr_from_ldu(arrays, diag, up, lo), b, np.zeros(len(b)), cfg)
+    assert rel_l2(x, xo) < 1e-9
+
+
 def test_manufactured_solution():
     # Manufactured SPD system with x = 1 + 0.01 * gid (petsc_backend_test.cpp:23-53), to 2e-11.
     arrays, om, diag, up, lo, _ = laplace_system()
