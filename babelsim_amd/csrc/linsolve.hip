@@ -27,8 +27,12 @@ __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 // Row i (held in storage slot q) of A x: the diagonal first, then the
 // off-diagonal entries in row order (the operation order of the oracle's
 // spmv); padding entries are skipped.
-template <class T, class XF> __device__ inline double rowDotF(const CsrView<T>& A, XF xf, int q, int i) {
-    double y = static_cast<double>(A.diag[i]) * xf(i);
+// Row i (held in storage slot q) of A x: the diagonal first, then the
+// off-diagonal entries in row order (the operation order of the oracle's
+// spmv); padding entries are skipped.  The sum runs in the storage type T:
+// FP64 on the FP64 path, FP32 in the FP32 AMG levels.
+template <class T> __device__ inline T rowDot(const CsrView<T>& A, const T* x, int q, int i) {
+    T y = A.diag[i] * x[i];
     if (A.sliceOff) {
         // Entries are fetched kC levels at a time so that their column, value
         // and x loads overlap; the sum still runs in entry order.
@@ -37,8 +41,7 @@ template <class T, class XF> __device__ inline double rowDotF(const CsrView<T>& 
         for (int k0 = 0; k0 < A.K; k0 += kC) {
             if (R.so[k0 + 1] == R.so[k0]) break;  // no row of the slice has more entries
             int e[kC], c[kC];
-            T v[kC];
-            double xv[kC];
+            T v[kC], xv[kC];
 #pragma unroll
             for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
 #pragma unroll
@@ -47,21 +50,15 @@ template <class T, class XF> __device__ inline double rowDotF(const CsrView<T>& 
                 v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
             }
 #pragma unroll
-            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? xf(c[j]) : 0.0;
+            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
 #pragma unroll
             for (int j = 0; j < kC; ++j)
-                if (c[j] >= 0) y += static_cast<double>(v[j]) * xv[j];
+                if (c[j] >= 0) y += v[j] * xv[j];
         }
     } else {
-        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += static_cast<double>(A.val[k]) * xf(A.col[k]);
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * x[A.col[k]];
     }
     return y;
-}
-// Row i (held in storage slot q) of A x: the diagonal first, then the
-// off-diagonal entries in row order (the operation order of the oracle's
-// spmv); padding entries are skipped.
-template <class T> __device__ inline double rowDot(const CsrView<T>& A, const T* x, int q, int i) {
-    return rowDotF(A, [x](int c) { return static_cast<double>(x[c]); }, q, i);
 }
 // Krylov kernels run on matrices whose slot q holds row q.
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
@@ -335,36 +332,41 @@ struct BcgsNorm {  // dp = ||r||; rho and omega become the previous values
 // scalars (sc3 + c * S_COUNT) and flags (fl3[c]); act[c] == 0 freezes it.  Each
 // component's arithmetic and reduction order are those of a solo solve, so
 // the three results equal three sequential solves bit for bit, while the
-// matrix is streamed once per product.
-struct Vec3p {
-    double* c[3];
+// matrix is streamed once per product.  T is the storage type of the vectors
+// and the matrix (double, or float inside the mixed-precision refinement);
+// arithmetic is done in double and every sum accumulates in double.
+template <class T> struct Vec3p {
+    T* c[3];
 };
-struct CVec3p {
-    const double* c[3];
+template <class T> struct CVec3p {
+    const T* c[3];
 };
+
 // Row i of A x_c for the components with m[c] set: the chunk's columns and
-// values are loaded once for all of them, and each sum runs in the solo order.
-__device__ inline void rowDot3(const CsrView<double>& A, const CVec3p& x, const bool (&m)[3], int i, double (&y)[3]) {
-    for (int c = 0; c < 3; ++c) y[c] = m[c] ? A.diag[i] * x.c[c][i] : 0.0;
+// values are loaded once for all of them, and each sum runs in the solo order
+// (in the storage type, like rowDot).
+template <class T>
+__device__ inline void rowDot3(const CsrView<T>& A, const CVec3p<T>& x, const bool (&m)[3], int i, T (&y)[3]) {
+    for (int c = 0; c < 3; ++c) y[c] = m[c] ? A.diag[i] * x.c[c][i] : T(0);
     if (A.sliceOff) {
         const RowRange R(A.sliceOff, A.K, i);
         constexpr int kC = 8;
         for (int k0 = 0; k0 < A.K; k0 += kC) {
             if (R.so[k0 + 1] == R.so[k0]) break;
             int e[kC], col[kC];
-            double v[kC];
+            T v[kC];
 #pragma unroll
             for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
 #pragma unroll
             for (int j = 0; j < kC; ++j) {
                 col[j] = e[j] >= 0 ? A.col[e[j]] : -1;
-                v[j] = e[j] >= 0 ? A.val[e[j]] : 0.0;
+                v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
             }
             for (int c = 0; c < 3; ++c) {
                 if (!m[c]) continue;
-                double xv[kC];
+                T xv[kC];
 #pragma unroll
-                for (int j = 0; j < kC; ++j) xv[j] = col[j] >= 0 ? x.c[c][col[j]] : 0.0;
+                for (int j = 0; j < kC; ++j) xv[j] = col[j] >= 0 ? x.c[c][col[j]] : T(0);
 #pragma unroll
                 for (int j = 0; j < kC; ++j)
                     if (col[j] >= 0) y[c] += v[j] * xv[j];
@@ -377,128 +379,141 @@ __device__ inline void rowDot3(const CsrView<double>& A, const CVec3p& x, const 
     }
 }
 
-__global__ void k3CopyNorm(int n, CVec3p b, Vec3p r, const int* act, double* part) {
+template <class T> __global__ void k3CopyNorm(int n, CVec3p<T> b, Vec3p<T> r, const int* act, double* part) {
     __shared__ double smem[3 * kBlock];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
             if (!act[c]) continue;
-            const double v = b.c[c][i];
+            const T v = b.c[c][i];
             if (r.c[c]) r.c[c][i] = v;
-            acc[c] += v * v;
+            acc[c] += static_cast<double>(v) * static_cast<double>(v);
         }
     blockPartial<3>(acc, part, smem);
 }
-__global__ void k3ResidualNorm(CsrView<double> A, CVec3p b, CVec3p x, Vec3p r, const int* act, double* part) {
+template <class T>
+__global__ void k3ResidualNorm(CsrView<T> A, CVec3p<T> b, CVec3p<T> x, Vec3p<T> r, const int* act, double* part) {
     __shared__ double smem[3 * kBlock];
     double acc[3] = {0.0, 0.0, 0.0};
     const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
-        double ax[3];
+        T ax[3];
         rowDot3(A, x, m, i, ax);
         for (int c = 0; c < 3; ++c) {
             if (!m[c]) continue;
-            const double v = b.c[c][i] - ax[c];
+            const T v = b.c[c][i] - ax[c];
             r.c[c][i] = v;
-            acc[c] += v * v;
+            acc[c] += static_cast<double>(v) * static_cast<double>(v);
         }
     }
     blockPartial<3>(acc, part, smem);
 }
-__global__ void k3Dot(int n, CVec3p a, CVec3p b, const int* act, double* part) {
+template <class T> __global__ void k3Dot(int n, CVec3p<T> a, CVec3p<T> b, const int* act, double* part) {
     __shared__ double smem[3 * kBlock];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c)
-            if (act[c]) acc[c] += a.c[c][i] * b.c[c][i];
+            if (act[c]) acc[c] += static_cast<double>(a.c[c][i]) * static_cast<double>(b.c[c][i]);
     blockPartial<3>(acc, part, smem);
 }
-__global__ void k3Copy(int n, CVec3p a, Vec3p b, const int* act) {
+template <class T> __global__ void k3Copy(int n, CVec3p<T> a, Vec3p<T> b, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c]) b.c[c][i] = a.c[c][i];
 }
-__global__ void k3Fill(int n, Vec3p a, double v, const int* act) {
+template <class T> __global__ void k3Fill(int n, Vec3p<T> a, double v, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
-        if (act[c]) a.c[c][i] = v;
+        if (act[c]) a.c[c][i] = static_cast<T>(v);
 }
-__global__ void k3Axpy(int n, double a, CVec3p x, Vec3p y, const int* act) {  // y += a x
+template <class T> __global__ void k3Axpy(int n, double a, CVec3p<T> x, Vec3p<T> y, const int* act) {  // y += a x
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
-        if (act[c]) y.c[c][i] += a * x.c[c][i];
+        if (act[c]) y.c[c][i] = static_cast<T>(static_cast<double>(y.c[c][i]) + a * static_cast<double>(x.c[c][i]));
 }
 // z = r * (1/d), or a copy without a preconditioner
-__global__ void k3Jacobi(int n, CVec3p r, const double* jinv, Vec3p z, const int* act) {
+template <class T> __global__ void k3Jacobi(int n, CVec3p<T> r, const T* jinv, Vec3p<T> z, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
-        if (act[c]) z.c[c][i] = jinv ? r.c[c][i] * jinv[i] : r.c[c][i];
+        if (act[c])
+            z.c[c][i] = jinv ? static_cast<T>(static_cast<double>(r.c[c][i]) * static_cast<double>(jinv[i])) : r.c[c][i];
 }
-__global__ void k3BcgsP(int n, Vec3p p, CVec3p r, CVec3p v, const double* sc3, const int* act) {
+template <class T> __global__ void k3BcgsP(int n, Vec3p<T> p, CVec3p<T> r, CVec3p<T> v, const double* sc3, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c) {
         if (!act[c]) continue;
         const double* sc = sc3 + c * S_COUNT;
         const double beta = sc[S_BETA];
-        p.c[c][i] = r.c[c][i] + (-sc[S_OMEGAOLD] * beta) * v.c[c][i] + beta * p.c[c][i];
+        p.c[c][i] = static_cast<T>(static_cast<double>(r.c[c][i]) + (-sc[S_OMEGAOLD] * beta) * static_cast<double>(v.c[c][i]) +
+                                   beta * static_cast<double>(p.c[c][i]));
     }
 }
-__global__ void k3SpmvD1(CsrView<double> A, CVec3p t, Vec3p v, CVec3p rp, const int* act, double* part) {
+template <class T>
+__global__ void k3SpmvD1(CsrView<T> A, CVec3p<T> t, Vec3p<T> v, CVec3p<T> rp, const int* act, double* part) {
     __shared__ double smem[3 * kBlock];
     double acc[3] = {0.0, 0.0, 0.0};
     const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
-        double y[3];
+        T y[3];
         rowDot3(A, t, m, i, y);
         for (int c = 0; c < 3; ++c) {
             if (!m[c]) continue;
-            v.c[c][i] = y[c];
-            acc[c] += y[c] * rp.c[c][i];
+            const T yv = y[c];
+            v.c[c][i] = yv;
+            acc[c] += static_cast<double>(yv) * static_cast<double>(rp.c[c][i]);
         }
     }
     blockPartial<3>(acc, part, smem);
 }
-__global__ void k3BcgsS(int n, Vec3p s, CVec3p v, CVec3p r, const double* sc3, const int* fl3, const int* act) {
+template <class T>
+__global__ void k3BcgsS(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const double* sc3, const int* fl3, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
-        if (act[c] && !fl3[c]) s.c[c][i] = -sc3[c * S_COUNT + S_ALPHA] * v.c[c][i] + r.c[c][i];
+        if (act[c] && !fl3[c])
+            s.c[c][i] = static_cast<T>(-sc3[c * S_COUNT + S_ALPHA] * static_cast<double>(v.c[c][i]) +
+                                       static_cast<double>(r.c[c][i]));
 }
 // Partials: s . t of the three components at k = 0..2, t . t at k = 3..5.
-__global__ void k3SpmvD2(CsrView<double> A, CVec3p rt, Vec3p t, CVec3p s, const int* fl3, const int* act,
+template <class T>
+__global__ void k3SpmvD2(CsrView<T> A, CVec3p<T> rt, Vec3p<T> t, CVec3p<T> s, const int* fl3, const int* act,
                          double* part) {
     __shared__ double smem[6 * kBlock];
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const bool m[3] = {act[0] && !fl3[0], act[1] && !fl3[1], act[2] && !fl3[2]};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
-        double y[3];
+        T y[3];
         rowDot3(A, rt, m, i, y);
         for (int c = 0; c < 3; ++c) {
             if (!m[c]) continue;
-            t.c[c][i] = y[c];
-            acc[c] += s.c[c][i] * y[c];
-            acc[3 + c] += y[c] * y[c];
+            const T yv = y[c];
+            const double yd = static_cast<double>(yv);
+            t.c[c][i] = yv;
+            acc[c] += static_cast<double>(s.c[c][i]) * yd;
+            acc[3 + c] += yd * yd;
         }
     }
     blockPartial<6>(acc, part, smem);
 }
-__global__ void k3BcgsUpdate(int n, Vec3p x, CVec3p p, CVec3p s, Vec3p r, CVec3p t, const double* sc3, const int* fl3,
-                             const int* act, double* part) {
+template <class T>
+__global__ void k3BcgsUpdate(int n, Vec3p<T> x, CVec3p<T> p, CVec3p<T> s, Vec3p<T> r, CVec3p<T> t, const double* sc3,
+                             const int* fl3, const int* act, double* part) {
     __shared__ double smem[3 * kBlock];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
             if (!act[c] || fl3[c]) continue;
             const double alpha = sc3[c * S_COUNT + S_ALPHA], omega = sc3[c * S_COUNT + S_OMEGA];
-            x.c[c][i] = alpha * p.c[c][i] + omega * s.c[c][i] + x.c[c][i];
-            const double ri = -omega * t.c[c][i] + s.c[c][i];
+            x.c[c][i] = static_cast<T>(alpha * static_cast<double>(p.c[c][i]) + omega * static_cast<double>(s.c[c][i]) +
+                                       static_cast<double>(x.c[c][i]));
+            const T ri = static_cast<T>(-omega * static_cast<double>(t.c[c][i]) + static_cast<double>(s.c[c][i]));
             r.c[c][i] = ri;
-            acc[c] += ri * ri;
+            acc[c] += static_cast<double>(ri) * static_cast<double>(ri);
         }
     blockPartial<3>(acc, part, smem);
 }
@@ -577,16 +592,17 @@ __global__ void kCompare(int n0, const double* a0, const double* b0, int n1, con
 }
 
 // ---- AMG ----
+// The AMG kernels below compute in the storage type T of their level.
 template <class T> __global__ void kJacobiFirst(int n, const T* d, const T* b, T* x, double omega) {
     const int i = globalThread();
-    if (i < n) x[i] = static_cast<T>(omega * static_cast<double>(b[i]) / static_cast<double>(d[i]));
+    if (i < n) x[i] = static_cast<T>(omega) * b[i] / d[i];
 }
 template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
-    const double ax = rowDot(A, x, q, i);
-    xo[i] = static_cast<T>(static_cast<double>(x[i]) + omega * (static_cast<double>(b[i]) - ax) / static_cast<double>(A.diag[i]));
+    const T ax = rowDot(A, x, q, i);
+    xo[i] = x[i] + static_cast<T>(omega) * (b[i] - ax) / A.diag[i];
 }
 // The last fine-level sweep of a preconditioner application inside CG, with
 // the partial sums of z . r (z the result, r = b) that kPlainDot would form.
@@ -608,18 +624,18 @@ template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* 
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
-    r[i] = static_cast<T>(static_cast<double>(b[i]) - rowDot(A, x, q, i));
+    r[i] = b[i] - rowDot(A, x, q, i);
 }
 template <class T> __global__ void kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
     const int I = globalThread();
     if (I >= nc) return;
-    double s = 0.0;
-    for (int k = mp[I]; k < mp[I + 1]; ++k) s += static_cast<double>(r[mi[k]]);
-    bc[I] = static_cast<T>(s);
+    T s = 0;
+    for (int k = mp[I]; k < mp[I + 1]; ++k) s += r[mi[k]];
+    bc[I] = s;
 }
 template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
     const int i = globalThread();
-    if (i < n) x[i] = static_cast<T>(static_cast<double>(x[i]) + alpha * static_cast<double>(xc[agg[i]]));
+    if (i < n) x[i] = x[i] + static_cast<T>(alpha) * xc[agg[i]];
 }
 __global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
                           double* cd, double* cv) {
@@ -1241,13 +1257,19 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     BSIM_LAUNCH_CHECK();
 }
 
-void DeviceSystem::bcgsMixed(const double* b, double* x, bool guessZero, int maxIt, double target, int& its,
-                             int& reason) {
-    (void)b; (void)x; (void)guessZero; (void)maxIt; (void)target; (void)its; (void)reason;
-    throw std::invalid_argument("mixed-precision BiCGSTAB is not available in this build");
-}
 
-SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) { return solveFrom(b, x, rhsIndex, nullptr); }
+SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
+    if (ready && configured && !setupPending && cfg.solver == 1 && cfg.precision == 1 && cfg.pc != 2) {
+        const double* const b3[3] = {b, b, b};
+        double* const x3[3] = {x, x, x};
+        SolveRes res[3];
+        std::vector<double> h[3];
+        solve3Mixed(b3, x3, res, h, 1);
+        hist = h[0];
+        return res[0];
+    }
+    return solveFrom(b, x, rhsIndex, nullptr);
+}
 
 // The solve contract of petsc_linear_system.cpp:355-481.  With `first`, the
 // prelude (norms) and the first Krylov call of attempt 0 were already run on
@@ -1304,7 +1326,6 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
             const bool given = first && attempt == 0 && restart == 0;
             if (given) { its = first->its; reason = first->reason; }
             else if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
-            else if (cfg.precision == 1) bcgsMixed(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             else bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             attemptIts += its;
             total += its;
@@ -1354,49 +1375,63 @@ void DeviceSystem::syncScalars3() {
 }
 
 namespace {
-Vec3p vec3(DBuf<double> (&a)[3]) { return Vec3p{{a[0].p, a[1].p, a[2].p}}; }
-CVec3p cvec3(DBuf<double> (&a)[3]) { return CVec3p{{a[0].p, a[1].p, a[2].p}}; }
+template <class T> Vec3p<T> vec3(DBuf<T> (&a)[3]) { return Vec3p<T>{{a[0].p, a[1].p, a[2].p}}; }
+template <class T> CVec3p<T> cvec3(DBuf<T> (&a)[3]) { return CVec3p<T>{{a[0].p, a[1].p, a[2].p}}; }
+__global__ void k3Narrow(int n, CVec3p<double> a, Vec3p<float> b, const int* act) {  // b = float(a)
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) b.c[c][i] = static_cast<float>(a.c[c][i]);
+}
+__global__ void k3AddWide(int n, CVec3p<float> d, Vec3p<double> x, const int* act) {  // x += double(d)
+    const int i = globalThread();
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c)
+        if (act[c]) x.c[c][i] += static_cast<double>(d.c[c][i]);
+}
 }  // namespace
+
+template <class T> void DeviceSystem::Work3<T>::ensure(int n) {
+    for (int c = 0; c < 3; ++c)
+        for (DBuf<T>* d : {&r[c], &p[c], &v[c], &t[c], &s[c], &rp[c], &tmp[c], &guess[c]}) d->ensure(n);
+}
 
 // One BiCGSTAB call (bcgs() above, bcgs.c:14-140) on every component with
 // run[c], in lockstep.  Each component keeps the solo control flow: its own
 // convergence test, breakdown exits and iteration count; a component that
 // stops is frozen through act3 while the others go on.  Counters go to dl[c].
-void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const bool (&run)[3], bool guessZero, int maxIt,
-                         const double (&target)[3], int (&its)[3], int (&reason)[3], std::vector<double> (&h)[3],
-                         SystemCounters (&dl)[3]) {
+template <class T>
+void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T* const b[3], T* const x[3],
+                         const bool (&run)[3], bool guessZero, int maxIt, const double (&target)[3], int (&its)[3],
+                         int (&reason)[3], std::vector<double> (&h)[3], SystemCounters (&dl)[3]) {
     double* part = partials.p;
     const int rb = reduceBlocks(n), nb = blocksFor(n);
-    const CsrView<double> A = view();
-    const CVec3p B{{b[0], b[1], b[2]}};
-    const Vec3p X{{x[0], x[1], x[2]}};
-    const CVec3p cX{{x[0], x[1], x[2]}};
-    const double* jv = cfg.pc == 1 ? jinv.p : nullptr;
+    const CVec3p<T> B{{b[0], b[1], b[2]}};
+    const Vec3p<T> X{{x[0], x[1], x[2]}};
+    const CVec3p<T> cX{{x[0], x[1], x[2]}};
     int* act = act3.p;
     auto upload = [&](const bool (&m)[3]) { uploadAct(m); };
     bool on[3] = {run[0], run[1], run[2]};
     upload(on);
     BSIM_HIP(hipMemsetAsync(fl3.p, 0, sizeof(int) * 3, stream));
     {
-        double init[3 * S_COUNT];
         const double one[S_COUNT] = {0, 1.0, 1.0, 0, 1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = 0; c < 3; ++c) std::memcpy(init + c * S_COUNT, one, sizeof(one));
-        std::memcpy(hsc3, init, sizeof(init));
-        BSIM_HIP(hipMemcpyAsync(sc3.p, hsc3, sizeof(init), hipMemcpyHostToDevice, stream));
+        for (int c = 0; c < 3; ++c) std::memcpy(hsc3 + c * S_COUNT, one, sizeof(one));
+        BSIM_HIP(hipMemcpyAsync(sc3.p, hsc3, sizeof(double) * 3 * S_COUNT, hipMemcpyHostToDevice, stream));
     }
     const bool haveGuess = !guessZero;
     if (!guessZero) {
-        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(A, B, cX, vec3(r3), act, part);
+        k3ResidualNorm<T><<<rb, kBlock, 0, stream>>>(A, B, cX, vec3(w.r), act, part);
         for (int c = 0; c < 3; ++c) if (on[c]) ++dl[c].spmvs;
-        k3Copy<<<nb, kBlock, 0, stream>>>(n, cX, vec3(guess3), act);
-        k3Fill<<<nb, kBlock, 0, stream>>>(n, X, 0.0, act);
+        k3Copy<T><<<nb, kBlock, 0, stream>>>(n, cX, vec3(w.guess), act);
+        k3Fill<T><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act);
     } else {
-        k3CopyNorm<<<rb, kBlock, 0, stream>>>(n, B, vec3(r3), act, part);
+        k3CopyNorm<T><<<rb, kBlock, 0, stream>>>(n, B, vec3(w.r), act, part);
     }
     kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_DP, act});
     BSIM_LAUNCH_CHECK();
     syncScalars3();
-    double dp[3];
+    double dp[3] = {0.0, 0.0, 0.0};
     bool loop[3] = {false, false, false}, atStart[3] = {false, false, false};
     auto test = [&](int c, double vv) {
         if (!std::isfinite(vv)) { reason[c] = -9; return true; }
@@ -1414,27 +1449,27 @@ void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const boo
     }
     if (haveGuess && (atStart[0] || atStart[1] || atStart[2])) {  // x = guess for those that stop at once
         upload(atStart);
-        k3Axpy<<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(guess3), X, act);
+        k3Axpy<T><<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(w.guess), X, act);
     }
     bool active[3] = {loop[0], loop[1], loop[2]};
     upload(active);
-    k3Copy<<<nb, kBlock, 0, stream>>>(n, cvec3(r3), vec3(rp3), act);
-    k3Fill<<<nb, kBlock, 0, stream>>>(n, vec3(p3), 0.0, act);
-    k3Fill<<<nb, kBlock, 0, stream>>>(n, vec3(vv3), 0.0, act);
+    k3Copy<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.r), vec3(w.rp), act);
+    k3Fill<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.p), 0.0, act);
+    k3Fill<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.v), 0.0, act);
     int iter[3] = {0, 0, 0};
     while (active[0] || active[1] || active[2]) {
-        k3Dot<<<rb, kBlock, 0, stream>>>(n, cvec3(r3), cvec3(rp3), act, part);
+        k3Dot<T><<<rb, kBlock, 0, stream>>>(n, cvec3(w.r), cvec3(w.rp), act, part);
         kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Rho{sc3.p, act});
-        k3BcgsP<<<nb, kBlock, 0, stream>>>(n, vec3(p3), cvec3(r3), cvec3(vv3), sc3.p, act);
-        k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cvec3(p3), jv, vec3(tmp3), act);
-        k3SpmvD1<<<rb, kBlock, 0, stream>>>(A, cvec3(tmp3), vec3(vv3), cvec3(rp3), act, part);
+        k3BcgsP<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.p), cvec3(w.r), cvec3(w.v), sc3.p, act);
+        k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.p), jv, vec3(w.tmp), act);
+        k3SpmvD1<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.tmp), vec3(w.v), cvec3(w.rp), act, part);
         kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Alpha{sc3.p, fl3.p, act});
-        k3BcgsS<<<nb, kBlock, 0, stream>>>(n, vec3(s3), cvec3(vv3), cvec3(r3), sc3.p, fl3.p, act);
-        k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cvec3(s3), jv, vec3(r3), act);
-        k3SpmvD2<<<rb, kBlock, 0, stream>>>(A, cvec3(r3), vec3(t3), cvec3(s3), fl3.p, act, part);
+        k3BcgsS<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.s), cvec3(w.v), cvec3(w.r), sc3.p, fl3.p, act);
+        k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.s), jv, vec3(w.r), act);
+        k3SpmvD2<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.r), vec3(w.t), cvec3(w.s), fl3.p, act, part);
         kReduceTail<6><<<1, kBlock, 0, stream>>>(rb, part, Tail3Omega{sc3.p, fl3.p, act});
-        k3BcgsUpdate<<<rb, kBlock, 0, stream>>>(n, X, cvec3(p3), cvec3(s3), vec3(r3), cvec3(t3), sc3.p, fl3.p, act,
-                                                part);
+        k3BcgsUpdate<T><<<rb, kBlock, 0, stream>>>(n, X, cvec3(w.p), cvec3(w.s), vec3(w.r), cvec3(w.t), sc3.p, fl3.p,
+                                                   act, part);
         kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Norm{sc3.p, fl3.p, act});
         BSIM_LAUNCH_CHECK();
         syncScalars3();
@@ -1450,16 +1485,21 @@ void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const boo
             else if (fl & F_D1_ZERO) { reason[c] = -5; stop = true; }
             else if (fl & F_D2_ZERO) {
                 stop = true;
-                kDot<<<rb, kBlock, 0, stream>>>(n, s3[c].p, s3[c].p, part);
-                kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreValue{sc.p, S_TMP});
+                const bool only[3] = {c == 0, c == 1, c == 2};
+                upload(only);
+                k3Dot<T><<<rb, kBlock, 0, stream>>>(n, cvec3(w.s), cvec3(w.s), act, part);
+                kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_TMP, act});
                 BSIM_LAUNCH_CHECK();
-                if (readScalar(S_TMP) != 0.0) reason[c] = -5;
+                syncScalars3();
+                const double ss = hsc3[c * S_COUNT + S_TMP];
+                if (ss != 0.0) reason[c] = -5;
                 else {
-                    kAxpy<<<nb, kBlock, 0, stream>>>(n, hs[S_ALPHA], p3[c].p, x[c]);
+                    k3Axpy<T><<<nb, kBlock, 0, stream>>>(n, hs[S_ALPHA], cvec3(w.p), X, act);
                     ++its[c];
                     reason[c] = 2;
                     h[c].push_back(dp[c]);
                 }
+                changed = true;  // the mask was replaced
             } else {
                 dp[c] = hs[S_DP];
                 ++its[c];
@@ -1474,9 +1514,9 @@ void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const boo
     }
     // Unwind the right preconditioning for the components that iterated.
     upload(loop);
-    k3Jacobi<<<nb, kBlock, 0, stream>>>(n, cX, jv, vec3(tmp3), act);
-    k3Copy<<<nb, kBlock, 0, stream>>>(n, cvec3(tmp3), X, act);
-    if (haveGuess) k3Axpy<<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(guess3), X, act);
+    k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cX, jv, vec3(w.tmp), act);
+    k3Copy<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.tmp), X, act);
+    if (haveGuess) k3Axpy<T><<<nb, kBlock, 0, stream>>>(n, 1.0, cvec3(w.guess), X, act);
     BSIM_LAUNCH_CHECK();
     for (int c = 0; c < 3; ++c) if (loop[c]) ++dl[c].pcApplications;
 }
@@ -1484,6 +1524,10 @@ void DeviceSystem::bcgs3(const double* const b[3], double* const x[3], const boo
 void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3]) {
     if (!ready || !configured || setupPending)
         throw std::logic_error("linear system must update and prepare before solving a right-hand side");
+    if (cfg.solver == 1 && cfg.pc != 2 && cfg.precision == 1) {
+        solve3Mixed(b, x, res, hists, 3);
+        return;
+    }
     const bool lockstep = cfg.solver == 1 && cfg.pc != 2 && cfg.precision == 0;
     if (!lockstep) {
         for (int c = 0; c < 3; ++c) {
@@ -1492,21 +1536,20 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
         }
         return;
     }
-    for (int c = 0; c < 3; ++c)
-        for (DBuf<double>* d : {&r3[c], &p3[c], &vv3[c], &t3[c], &s3[c], &rp3[c], &tmp3[c], &guess3[c], &saved3[c]})
-            d->ensure(n);
+    w64.ensure(n);
+    for (int c = 0; c < 3; ++c) saved3[c].ensure(n);
     const int rb = reduceBlocks(n), nb = blocksFor(n);
-    const CVec3p B{{b[0], b[1], b[2]}};
-    const Vec3p X{{x[0], x[1], x[2]}};
-    const CVec3p cX{{x[0], x[1], x[2]}};
+    const CVec3p<double> B{{b[0], b[1], b[2]}};
+    const Vec3p<double> X{{x[0], x[1], x[2]}};
+    const CVec3p<double> cX{{x[0], x[1], x[2]}};
     // Prelude of every component (solveFrom): keep the guess, reset, norms.
     uploadAct({true, true, true});
-    k3Copy<<<nb, kBlock, 0, stream>>>(n, cX, vec3(saved3), act3.p);
-    if (!cfg.warmStart) k3Fill<<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
-    k3CopyNorm<<<rb, kBlock, 0, stream>>>(n, B, Vec3p{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
+    k3Copy<double><<<nb, kBlock, 0, stream>>>(n, cX, vec3(saved3), act3.p);
+    if (!cfg.warmStart) k3Fill<double><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
+    k3CopyNorm<double><<<rb, kBlock, 0, stream>>>(n, B, Vec3p<double>{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
     kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
     if (cfg.warmStart) {
-        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(tmp3), act3.p, partials.p);
+        k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.tmp), act3.p, partials.p);
         kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
     }
     BSIM_LAUNCH_CHECK();
@@ -1527,9 +1570,10 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
     std::vector<double> h[3];
     SystemCounters dl[3];
     if (run[0] || run[1] || run[2]) {
-        bcgs3(b, x, run, !cfg.warmStart, cfg.maxIter, target, its, reason, h, dl);
+        const double* jv = cfg.pc == 1 ? jinv.p : nullptr;
+        bcgs3<double>(w64, view(), jv, b, x, run, !cfg.warmStart, cfg.maxIter, target, its, reason, h, dl);
         uploadAct(run);
-        k3ResidualNorm<<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(tmp3), act3.p, partials.p);
+        k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.tmp), act3.p, partials.p);
         kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
         BSIM_LAUNCH_CHECK();
         syncScalars3();
@@ -1561,6 +1605,120 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
         hists[c] = hist;
     }
     BSIM_LAUNCH_CHECK();
+}
+
+// Mixed precision for a vector equation: FP64 iterative refinement around an
+// FP32 lockstep BiCGSTAB (FP32 matrix, Jacobi preconditioner and vectors,
+// FP64 inner products).  Each sweep forms the true residual r = b - A x in
+// FP64, solves A d = r in FP32 to inner_rtol * ||r|| and adds d in FP64; a
+// component stops when its true residual meets the target of the solve
+// contract (max(atol, rtol * max(||r0||, ||b||))).  The history holds the
+// true residual before every sweep.
+void DeviceSystem::solve3Mixed(const double* const b[3], double* const x[3], SolveRes res[3],
+                               std::vector<double> hists[3], int ncomp) {
+    const auto t0 = Clock::now();
+    w64.ensure(n);
+    w32.ensure(n);
+    for (int c = 0; c < 3; ++c) { saved3[c].ensure(n); d32[c].ensure(n); r32[c].ensure(n); }
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    // FP32 copies of the effective matrix and of the Jacobi inverse.
+    const std::uint64_t stamp = counters.matrixUpdates * 1000003ull + counters.pcSetups;
+    if (stamp != f32Stamp) {
+        diag32.ensure(n);
+        val32.ensure(std::max(nnz, 1));
+        kToFloat<<<nb, kBlock, 0, stream>>>(n, diag(), diag32.p);
+        if (nnz) kToFloat<<<blocksFor(nnz), kBlock, 0, stream>>>(nnz, val(), val32.p);
+        if (cfg.pc == 1) {
+            jinv32.ensure(n);
+            kToFloat<<<nb, kBlock, 0, stream>>>(n, jinv.p, jinv32.p);
+        }
+        f32Stamp = stamp;
+    }
+    const CsrView<float> A32{n, pat.ptr, pat.col, diag32.p, val32.p, pat.sliceOff, pat.K, nullptr};
+    const float* jv32 = cfg.pc == 1 ? jinv32.p : nullptr;
+    const CVec3p<double> B{{b[0], b[1], b[2]}};
+    const Vec3p<double> X{{x[0], x[1], x[2]}};
+    const CVec3p<double> cX{{x[0], x[1], x[2]}};
+    counters.solves += ncomp;
+    const bool use[3] = {true, ncomp > 1, ncomp > 2};
+    uploadAct(use);
+    if (!cfg.warmStart) k3Fill<double><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
+    k3CopyNorm<double><<<rb, kBlock, 0, stream>>>(n, B, Vec3p<double>{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
+    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
+    BSIM_LAUNCH_CHECK();
+    syncScalars3();
+    double rhsNorm[3], initial[3] = {0, 0, 0}, target[3], scale[3], dp[3] = {0, 0, 0}, prev[3] = {0, 0, 0};
+    int total[3] = {0, 0, 0}, stalled[3] = {0, 0, 0};
+    bool done[3] = {!use[0], !use[1], !use[2]}, failed[3] = {false, false, false};
+    for (int c = 0; c < 3; ++c) { rhsNorm[c] = hsc3[c * S_COUNT + S_BN]; hists[c].clear(); }
+    for (int sweep = 0;; ++sweep) {
+        bool live[3];
+        for (int c = 0; c < 3; ++c) live[c] = !done[c];
+        uploadAct(live);
+        k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.r), act3.p, partials.p);
+        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+        BSIM_LAUNCH_CHECK();
+        syncScalars3();
+        bool run[3] = {false, false, false};
+        double inner[3] = {0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            if (done[c]) continue;
+            ++counters.spmvs;
+            ++counters.trueResidualChecks;
+            dp[c] = hsc3[c * S_COUNT + S_TR];
+            if (sweep == 0) {
+                initial[c] = cfg.warmStart ? dp[c] : rhsNorm[c];
+                scale[c] = std::max(initial[c], rhsNorm[c]) > 0.0 ? std::max(initial[c], rhsNorm[c]) : 1.0;
+                target[c] = std::max(cfg.atol, cfg.rtol * scale[c]);
+            }
+            hists[c].push_back(dp[c]);
+            if (!std::isfinite(dp[c])) { failed[c] = true; done[c] = true; continue; }
+            // Refinement that stops reducing the residual (two sweeps without a
+            // tenth of reduction) ends as a numerical failure.
+            if (sweep > 0) stalled[c] = dp[c] > 0.9 * prev[c] ? stalled[c] + 1 : 0;
+            prev[c] = dp[c];
+            if (stalled[c] >= 2 && dp[c] > target[c]) { failed[c] = true; done[c] = true; continue; }
+            if (dp[c] <= target[c] || total[c] >= cfg.maxIter || failed[c]) { done[c] = true; continue; }
+            run[c] = true;
+            inner[c] = std::max(cfg.innerRtol * dp[c], 0.5 * target[c]);
+        }
+        if (!run[0] && !run[1] && !run[2]) break;
+        // d = A32^{-1} float(r) from a zero guess, then x += d.
+        uploadAct(run);
+        k3Narrow<<<nb, kBlock, 0, stream>>>(n, cvec3(w64.r), vec3(r32), act3.p);
+        k3Fill<float><<<nb, kBlock, 0, stream>>>(n, vec3(d32), 0.0, act3.p);
+        const float* rb32[3] = {r32[0].p, r32[1].p, r32[2].p};
+        float* db32[3] = {d32[0].p, d32[1].p, d32[2].p};
+        int maxIt = cfg.maxIter;
+        for (int c = 0; c < 3; ++c) if (run[c]) maxIt = std::min(maxIt, cfg.maxIter - total[c]);
+        int its[3] = {0, 0, 0}, reason[3] = {0, 0, 0};
+        std::vector<double> h[3];
+        SystemCounters dl[3];
+        bcgs3<float>(w32, A32, jv32, rb32, db32, run, true, std::max(maxIt, 1), inner, its, reason, h, dl);
+        uploadAct(run);
+        k3AddWide<<<nb, kBlock, 0, stream>>>(n, cvec3(d32), X, act3.p);
+        BSIM_LAUNCH_CHECK();
+        for (int c = 0; c < 3; ++c) {
+            if (!run[c]) continue;
+            total[c] += its[c];
+            counters.spmvs += dl[c].spmvs;
+            counters.pcApplications += dl[c].pcApplications;
+            if (reason[c] < 0 && reason[c] != -3 && its[c] == 0) failed[c] = true;  // breakdown without progress
+        }
+    }
+    for (int c = 0; c < ncomp; ++c) {
+        SolveRes& r = res[c];
+        r.iterations = total[c];
+        r.initial = initial[c];
+        r.final_ = dp[c];
+        r.relative = dp[c] / scale[c];
+        if (std::isfinite(dp[c]) && dp[c] <= target[c]) r.status = kConverged;
+        else if (!failed[c] && total[c] >= cfg.maxIter) r.status = kMaxIterations;
+        else r.status = kNumericalFailure;
+        if (total[c] > 0) ++pcRhsSolves;
+        counters.krylovIterations += total[c];
+    }
+    counters.solveSeconds += since(t0);
 }
 
 }  // namespace bsim

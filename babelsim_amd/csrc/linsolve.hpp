@@ -113,7 +113,6 @@ private:
     double trueResidual(const double* b, const double* x);
     void cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     void bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
-    void bcgsMixed(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     double readScalar(int idx, int* flags = nullptr);
     // The prelude and first Krylov call of one component, already executed.
     struct FirstRun {
@@ -123,13 +122,24 @@ private:
         SystemCounters delta;
     };
     SolveRes solveFrom(const double* b, double* x, int rhsIndex, const FirstRun* first);
-    void bcgs3(const double* const b[3], double* const x[3], const bool (&run)[3], bool guessZero, int maxIt,
-               const double (&target)[3], int (&its)[3], int (&reason)[3], std::vector<double> (&h)[3],
-               SystemCounters (&dl)[3]);
+    // Work vectors of the lockstep solver, one per component.
+    template <class T> struct Work3 {
+        DBuf<T> r[3], p[3], v[3], t[3], s[3], rp[3], tmp[3], guess[3];
+        void ensure(int n);
+    };
+    template <class T>
+    void bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T* const b[3], T* const x[3], const bool (&run)[3],
+               bool guessZero, int maxIt, const double (&target)[3], int (&its)[3], int (&reason)[3],
+               std::vector<double> (&h)[3], SystemCounters (&dl)[3]);
+    void solve3Mixed(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3],
+                     int ncomp);
+    Work3<double> w64;
+    Work3<float> w32;
+    DBuf<float> d32[3], r32[3], jinv32;
+    std::uint64_t f32Stamp = ~0ull;  // matrix/PC state the FP32 copies were made from
     void syncScalars3();
     void uploadAct(const bool (&m)[3]);
-    // Lockstep work vectors, one per component.
-    DBuf<double> r3[3], p3[3], vv3[3], t3[3], s3[3], rp3[3], tmp3[3], guess3[3], saved3[3];
+    DBuf<double> saved3[3];     // initial guesses of the three components
     DBuf<double> sc3;           // 3 x S_COUNT device scalars
     DBuf<int> fl3, act3;
     double* hsc3 = nullptr;     // pinned: 3 x S_COUNT scalars then 3 flags
@@ -147,7 +157,7 @@ public:
     KernelProbe probe;  // times the fine-level AMG smoother sweeps
 private:
     DBuf<double> r, z, p, w, rp, v, t, s, tmp, guess, saved;
-    DBuf<float> r32, z32, p32, w32, rp32, v32, t32, s32, x32, tmp32, diag32, val32;
+    DBuf<float> diag32, val32;  // FP32 copy of the effective matrix (mixed precision)
     DBuf<double> sc;            // device scalars
     DBuf<int> flags;
     DBuf<double> partials;      // per-block reduction partials
