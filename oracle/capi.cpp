@@ -1,6 +1,8 @@
 // Oracle C ABI wrappers (test infrastructure only).
 #include "oracle.h"
 
+#include <omp.h>
+
 #include "simple.hpp"
 
 #include <cstring>
@@ -77,6 +79,7 @@ Csr makeCsr(int n, const int* ptr, const int* col, const double* diag, const dou
 extern "C" {
 
 const char* orc_last_error(void) { return g_error.c_str(); }
+int orc_threads(void) { return omp_get_max_threads(); }
 
 void* orc_mesh_new(int nv, const double* vertices, int nf, const int* fpo, const int* fpi, const int* owner,
                    const int* neighbour, const int* patch, int npatch) {

@@ -192,7 +192,7 @@ template <class T> void leastSquares(const Field<T>& x, std::vector<typename Gra
     const Mesh& m = *x.m;
     constexpr int nc = ncomp<T>();
     out.assign(m.nc, G{});
-    for (Index cell = 0; cell < m.nc; ++cell) {
+    parallelFor(m.nc, [&](Index cell) {
         double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         double rhs[3 * 3] = {0};
         for (Index k = m.cfo[cell]; k < m.cfo[cell + 1]; ++k) {
@@ -220,7 +220,7 @@ template <class T> void leastSquares(const Field<T>& x, std::vector<typename Gra
         double sol[9];
         if (!solveSym3(A, rhs, nc, sol)) throw std::runtime_error("least-squares gradient matrix is singular");
         for (int c = 0; c < nc; ++c) gradRow(out[cell], c) = V3{sol[0 * nc + c], sol[1 * nc + c], sol[2 * nc + c]};
-    }
+    });
 }
 
 template <class T> void gradient(const Field<T>& f, std::vector<typename GradOf<T>::type>& out, int method) {
@@ -234,13 +234,13 @@ template <class T> void interpolate(const Field<T>& f, std::vector<T>& out, int 
     const Mesh& m = *f.m;
     out.assign(m.nf, T{});
     if (method == Linear) {
-        for (Index i = 0; i < m.nf; ++i) out[i] = interpolatedFaceValue(f, i);
+        parallelFor(m.nf, [&](Index i) { out[i] = interpolatedFaceValue(f, i); });
         return;
     }
     if (method != Corrected) throw std::invalid_argument("unsupported interpolation method");
     std::vector<typename GradOf<T>::type> g;
     gradient(f, g, gmethod);
-    for (Index i = 0; i < m.nf; ++i) out[i] = correctedFaceValue(f, g.data(), i);
+    parallelFor(m.nf, [&](Index i) { out[i] = correctedFaceValue(f, g.data(), i); });
 }
 
 template <class T> T integratedNormalGradient(const Field<T>& x, const std::vector<typename GradOf<T>::type>& g,
@@ -295,10 +295,10 @@ void flux(VField& U, std::vector<double>& out, int interp, int gmethod) {  // op
     std::vector<T3> g;
     if (interp == Corrected) gradient(U, g, gmethod);
     out.assign(m.nf, 0.0);
-    for (Index f = 0; f < m.nf; ++f) {
+    parallelFor(m.nf, [&](Index f) {
         const V3 fv = interp == Corrected ? correctedFaceValue(U, g.data(), f) : interpolatedFaceValue(U, f);
         out[f] = dot(fv, m.Sf[f]);
-    }
+    });
     U.fluxCtx = out;  // fvm_execution.cpp:277-279
 }
 
@@ -331,18 +331,18 @@ void normalGradient(const SField& p, const std::vector<V3>* g, int gmethod, int 
     std::vector<V3> work;
     if (!g) { gradient(p, work, gmethod); g = &work; }
     out.assign(m.nf, 0.0);
-    for (Index f = 0; f < m.nf; ++f) out[f] = integratedNormalGradient(p, *g, f, dm) / m.Af[f];
+    parallelFor(m.nf, [&](Index f) { out[f] = integratedNormalGradient(p, *g, f, dm) / m.Af[f]; });
 }
 
 void diffusionFlux(const std::vector<double>& kf, const SField& x, const std::vector<V3>& g, int dm,
                    std::vector<double>& out) {  // operators.cpp:1004-1024
     const Mesh& m = *x.m;
     out.assign(m.nf, 0.0);
-    for (Index f = 0; f < m.nf; ++f) {
+    parallelFor(m.nf, [&](Index f) {
         const double k = kf[f];
         if (!(k >= 0.0) || !std::isfinite(k)) throw std::invalid_argument("face diffusivity must be non-negative and finite");
         out[f] = k * integratedNormalGradient(x, g, f, dm);
-    }
+    });
 }
 
 template <class T> void addConvection(Ldu<T>& eq, const std::vector<double>& phi, Field<T>& x, int conv,

@@ -15,12 +15,30 @@
 
 #include <cmath>
 #include <cstdint>
+#include <exception>
 #include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 namespace orc {
+
+// Runs body(i) for i in [0, n) on OpenMP threads.  Every iteration writes only
+// its own entries, so the result does not depend on the thread count; the
+// first exception an iteration throws is rethrown after the loop.
+template <class F> void parallelFor(std::int32_t n, F&& body) {
+    std::exception_ptr err;
+#pragma omp parallel for schedule(static)
+    for (std::int32_t i = 0; i < n; ++i) {
+        try {
+            body(i);
+        } catch (...) {
+#pragma omp critical(orc_parallel_for)
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
+}
 
 using Index = std::int32_t;
 constexpr Index kNone = -1;

@@ -55,6 +55,7 @@ void lduToCsr(const Mesh& m, const std::vector<double>& diag, const std::vector<
 }
 
 void spmv(const Csr& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < A.n; ++i) {
         double s = A.diag[i] * x[i];
         for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k) s += A.val[k] * x[A.col[k]];
@@ -75,8 +76,9 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b) {
     long G = (n + B - 1) / B;
     G = G < 1 ? 1 : (G > Gmax ? Gmax : G);
     std::vector<double> partial(G);
-    double acc[B];
+#pragma omp parallel for schedule(static)
     for (long g = 0; g < G; ++g) {
+        double acc[B];
         for (long t = 0; t < B; ++t) {
             double s = 0.0;
             for (long i = g * B + t; i < n; i += G * B) s += a[i] * b[i];
@@ -86,6 +88,7 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b) {
             for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
         partial[g] = acc[0];
     }
+    double acc[B];
     for (long t = 0; t < B; ++t) {
         double s = 0.0;
         for (long g = t; g < G; g += B) s += partial[g];
@@ -269,18 +272,22 @@ void Amg::vcycle(int l) {
     const double w = cfg.amgOmega;
     auto sweep = [&]() {  // x <- x + w (b - A x) / d
         spmv(A, x.data(), r.data());
+#pragma omp parallel for schedule(static)
         for (Index i = 0; i < n; ++i) r[i] = x[i] + w * (b[i] - r[i]) / A.diag[i];
         x.swap(r);
     };
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < n; ++i) x[i] = w * b[i] / A.diag[i];
     for (int s = 1; s < cfg.amgPreSweeps; ++s) sweep();
     spmv(A, x.data(), r.data());
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
     std::vector<double>& bc = wb[l + 1];
     std::fill(bc.begin(), bc.end(), 0.0);
     for (Index i = 0; i < n; ++i) bc[levels[l].agg[i]] += r[i];
     vcycle(l + 1);
     const std::vector<double>& xc = wx[l + 1];
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < n; ++i) x[i] += cfg.amgCorrection * xc[levels[l].agg[i]];
     for (int s = 0; s < cfg.amgPostSweeps; ++s) sweep();
 }
@@ -360,6 +367,7 @@ struct Ksp {
         if (cfg.pc == 0) { z = r; return; }
         if (cfg.pc == 1) {
             const Csr& P = pc->levels[0].A;
+#pragma omp parallel for schedule(static)
             for (Index i = 0; i < n; ++i) z[i] = r[i] * (1.0 / P.diag[i]);
             return;
         }
@@ -388,7 +396,11 @@ struct Ksp {
             if (beta == 0.0) { reason = 3; break; }
             if (i > 0 && beta * betaold < 0.0) { reason = -8; break; }
             if (i == 0) p = z;
-            else { const double bb = beta / betaold; for (Index k = 0; k < n; ++k) p[k] = z[k] + bb * p[k]; }
+            else {
+                const double bb = beta / betaold;
+#pragma omp parallel for schedule(static)
+                for (Index k = 0; k < n; ++k) p[k] = z[k] + bb * p[k];
+            }
             dpiold = dpi;
             spmv(A, p.data(), w.data());
             dpi = dotv(p, w);
@@ -396,7 +408,9 @@ struct Ksp {
             betaold = beta;
             if (dpi == 0.0 || (i > 0 && ((dpi > 0) - (dpi < 0)) * ((dpiold > 0) - (dpiold < 0)) < 0)) { reason = -10; break; }
             const double a = beta / dpi;
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) x[k] += a * p[k];
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) r[k] += -a * w[k];
             dp = nrm(r);
             if (hist) hist->push_back(dp);
@@ -437,12 +451,14 @@ struct Ksp {
         do {
             rho = dotv(r, rp);
             const double beta = (rho / rhoold) * (alpha / omegaold);
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) p[k] = r[k] + (-omegaold * beta) * v[k] + beta * p[k];
             applyPC(p, tmp); spmv(A, tmp.data(), v.data());
             d1 = dotv(v, rp);
             if (!std::isfinite(d1)) { reason = -9; break; }
             if (d1 == 0.0) { reason = -5; break; }
             alpha = rho / d1;
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) s[k] = -alpha * v[k] + r[k];
             applyPC(s, r); spmv(A, r.data(), t.data());
             d1 = dotv(s, t); d2 = dotv(t, t);
@@ -456,7 +472,9 @@ struct Ksp {
                 break;
             }
             omega = d1 / d2;
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) x[k] = alpha * p[k] + omega * s[k] + x[k];
+#pragma omp parallel for schedule(static)
             for (Index k = 0; k < n; ++k) r[k] = -omega * t[k] + s[k];
             dp = nrm(r);
             rhoold = rho; omegaold = omega;

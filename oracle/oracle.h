@@ -40,6 +40,8 @@ typedef struct {
 } orc_iter_report;
 
 const char* orc_last_error(void);
+/* OpenMP threads the oracle's parallel loops use. */
+int orc_threads(void);
 void* orc_mesh_new(int nv, const double* vertices, int nf, const int* fpo, const int* fpi, const int* owner,
                    const int* neighbour, const int* patch, int npatch);
 void orc_mesh_free(void* mesh);

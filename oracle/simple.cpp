@@ -125,10 +125,12 @@ IterReport Simple::iterate() {
     }
     std::vector<V3> gradP;
     gradient(p, gradP, c.pGrad);
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) mom.source[i] += (1.0 * m.V[i]) * (-1.0 * gradP[i]);
     // main.cpp:60-66
     rep.rU = relativeResidual(m, mom, U.v);
     const double alphaU = c.alphaU;
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) {  // relax (procedural_equation.cpp:298-305)
         const double extra = mom.diag[i] * (1 / alphaU - 1);
         mom.diag[i] += extra;
@@ -169,6 +171,7 @@ IterReport Simple::iterate() {
         VField X(m);
         X.calculated = true;
         X.trace.assign(m.nf, V3{});
+#pragma omp parallel for schedule(static)
         for (Index i = 0; i < m.nc; ++i) X.v[i] = rAU.v[i] * gradP[i];
         for (Index f = 0; f < m.nf; ++f) if (m.boundary(f)) X.trace[f] = rAU.trace[f] * gtr[f];
         std::vector<V3> Xf;
@@ -227,6 +230,7 @@ IterReport Simple::iterate() {
     }
     if (!pConverged && rep.pStatus != NumericalFailure) rep.pStatus = MaxIterations;
     // Corrections (main.cpp:95-98).
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) p.v[i] += c.alphaP * pPrime.v[i];
     std::vector<double> pressureFlux(m.nf, 0.0);
     {
@@ -240,8 +244,10 @@ IterReport Simple::iterate() {
     }
     std::vector<V3> gpp;
     gradient(pPrime, gpp, c.pGrad);
+#pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) U.v[i] += -1.0 * (rAU.v[i] * gpp[i]);
     phi.resize(m.nf);
+#pragma omp parallel for schedule(static)
     for (Index f = 0; f < m.nf; ++f) phi[f] = phiH[f] + pressureFlux[f];
     // Convergence (main.cpp:112-126; fvm_execution.cpp:91-177).
     {

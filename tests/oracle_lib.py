@@ -49,6 +49,8 @@ def oracle():
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
         L = C.CDLL(str(ORACLE))
         _sig(L, "orc_")
+        L.orc_threads.argtypes = []
+        L.orc_threads.restype = C.c_int
         L.orc_mesh_sizes.argtypes = [vp, ip, ip, ip, ip]
         L.orc_mesh_sizes.restype = None
         L.orc_mesh_rings.argtypes = [vp, ip, ip]
