@@ -24,6 +24,7 @@ namespace bsim {
 
 constexpr int kBlock = 256;          // 4 wavefronts of 64
 constexpr int kMaxReduceBlocks = 32768;  // one row per thread up to 8.4M rows
+constexpr int kTail = 1024;              // threads of a reduction tail
 
 inline int blocksFor(long long n, int block = kBlock) {
     long long b = (n + block - 1) / block;
@@ -131,16 +132,16 @@ __device__ inline void st3(double* p, long long i, D3 v) { p[3 * i] = v.x; p[3 *
 // the partials in block order with a fixed tree and hands them to a functor.
 // The kernel boundary orders the two stages, so no device-scope fence or
 // completion counter is needed and the result does not depend on scheduling.
-template <int K> __device__ inline void blockSum(double (&v)[K], double* smem /* K*kBlock */) {
+template <int K, int NT = kBlock> __device__ inline void blockSum(double (&v)[K], double* smem /* K*NT */) {
     const int t = threadIdx.x;
-    for (int k = 0; k < K; ++k) smem[k * kBlock + t] = v[k];
+    for (int k = 0; k < K; ++k) smem[k * NT + t] = v[k];
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (t < s)
-            for (int k = 0; k < K; ++k) smem[k * kBlock + t] += smem[k * kBlock + t + s];
+            for (int k = 0; k < K; ++k) smem[k * NT + t] += smem[k * NT + t + s];
         __syncthreads();
     }
-    for (int k = 0; k < K; ++k) v[k] = smem[k * kBlock];
+    for (int k = 0; k < K; ++k) v[k] = smem[k * NT];
     __syncthreads();
 }
 
@@ -154,14 +155,16 @@ template <int K> __device__ inline void blockPartial(double (&v)[K], double* par
     }
 }
 
-// F: device functor called by thread 0 with the K totals.
-template <int K, class F> __global__ void __launch_bounds__(kBlock) kReduceTail(int nb, const double* partials, F f) {
-    __shared__ double smem[K * kBlock];
+// Launched as one block of kTail threads: thread t sums partials t, t + kTail,
+// ... in order, then a fixed tree.  F: device functor called by thread 0
+// with the K totals.
+template <int K, class F> __global__ void __launch_bounds__(kTail) kReduceTail(int nb, const double* partials, F f) {
+    __shared__ double smem[K * kTail];
     double acc[K];
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock)
+    for (int b = threadIdx.x; b < nb; b += kTail)
         for (int k = 0; k < K; ++k) acc[k] += partials[k * kMaxReduceBlocks + b];
-    blockSum<K>(acc, smem);
+    blockSum<K, kTail>(acc, smem);
     if (threadIdx.x == 0) f(acc);
 }
 

@@ -1008,7 +1008,7 @@ bool DeviceSystem::update() {
     if (ready) {
         const int rb = reduceBlocks(static_cast<long long>(n) + nnz);
         kCompare<<<rb, kBlock, 0, stream>>>(n, diagBuf[0].p, diagBuf[1].p, nnz, valBuf[0].p, valBuf[1].p, partials.p);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, partials.p, StoreValue{sc.p, S_TMP});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, partials.p, StoreValue{sc.p, S_TMP});
         BSIM_LAUNCH_CHECK();
         changed = readScalar(S_TMP) != 0.0;
     }
@@ -1107,7 +1107,7 @@ double DeviceSystem::timeSpmv(int reps) {
 double DeviceSystem::trueResidual(const double* b, const double* x) {
     const int rb = reduceBlocks(n);
     kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, tmp.p, partials.p);
-    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, partials.p, StoreSqrt{sc.p, S_TR});
+    kReduceTail<1><<<1, kTail, 0, stream>>>(rb, partials.p, StoreSqrt{sc.p, S_TR});
     BSIM_LAUNCH_CHECK();
     ++counters.spmvs;
     ++counters.trueResidualChecks;
@@ -1124,7 +1124,7 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
     }
     if (!guessZero) { kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, r.p, part); ++counters.spmvs; }
     else kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, part);
-    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
+    kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
     BSIM_LAUNCH_CHECK();
     double dp = readScalar(S_DP);
     its = 0;
@@ -1147,7 +1147,7 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
             applyPC(r.p, z.p);
             kPlainDot<<<rb, kBlock, 0, stream>>>(n, z.p, r.p, flags.p, part);
         }
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgBeta{sc.p, flags.p, checkSign});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, CgBeta{sc.p, flags.p, checkSign});
         BSIM_LAUNCH_CHECK();
     };
     pcDot(0);
@@ -1157,9 +1157,9 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
         its = i + 1;
         kCgDirection<<<nb, kBlock, 0, stream>>>(n, z.p, p.p, sc.p, flags.p, i == 0);
         kCgSpmvDot<<<rb, kBlock, 0, stream>>>(view(), p.p, w.p, flags.p, part);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgAlpha{sc.p, flags.p, i});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, CgAlpha{sc.p, flags.p, i});
         kCgUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, r.p, w.p, sc.p, flags.p, part);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, CgNorm{sc.p, flags.p});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, CgNorm{sc.p, flags.p});
         BSIM_LAUNCH_CHECK();
         ++counters.spmvs;
         dp = readScalar(S_DP, &fl);
@@ -1193,7 +1193,7 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     } else {
         kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, r.p, part);
     }
-    kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
+    kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
     BSIM_LAUNCH_CHECK();
     double dp = readScalar(S_DP);
     its = 0;
@@ -1214,18 +1214,18 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     int i = 0, fl = 0;
     do {
         kBcgsRho<<<rb, kBlock, 0, stream>>>(n, r.p, rp.p, part);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsRho{sc.p});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, BcgsRho{sc.p});
         kBcgsP<<<nb, kBlock, 0, stream>>>(n, p.p, r.p, v.p, sc.p);
         applyPC(p.p, tmp.p);
         kBcgsSpmvD1<<<rb, kBlock, 0, stream>>>(view(), tmp.p, v.p, rp.p, part);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsAlpha{sc.p, flags.p});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, BcgsAlpha{sc.p, flags.p});
         kBcgsS<<<nb, kBlock, 0, stream>>>(n, s.p, v.p, r.p, sc.p, flags.p);
         BSIM_LAUNCH_CHECK();
         applyPC(s.p, r.p);
         kBcgsSpmvD2<<<rb, kBlock, 0, stream>>>(view(), r.p, t.p, s.p, flags.p, part);
-        kReduceTail<2><<<1, kBlock, 0, stream>>>(rb, part, BcgsOmega{sc.p, flags.p});
+        kReduceTail<2><<<1, kTail, 0, stream>>>(rb, part, BcgsOmega{sc.p, flags.p});
         kBcgsUpdate<<<rb, kBlock, 0, stream>>>(n, x, p.p, s.p, r.p, t.p, sc.p, flags.p, part);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, BcgsNorm{sc.p, flags.p});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, BcgsNorm{sc.p, flags.p});
         BSIM_LAUNCH_CHECK();
         counters.spmvs += 2;
         const double newDp = readScalar(S_DP, &fl);
@@ -1233,7 +1233,7 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
         if (fl & F_D1_ZERO) { reason = -5; break; }
         if (fl & F_D2_ZERO) {
             kDot<<<rb, kBlock, 0, stream>>>(n, s.p, s.p, part);
-            kReduceTail<1><<<1, kBlock, 0, stream>>>(rb, part, StoreValue{sc.p, S_TMP});
+            kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, StoreValue{sc.p, S_TMP});
             BSIM_LAUNCH_CHECK();
             if (readScalar(S_TMP) != 0.0) { reason = -5; break; }
             const double alpha = readScalar(S_ALPHA);
@@ -1300,7 +1300,7 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
         resetGuess();
         BSIM_LAUNCH_CHECK();
         kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, partials.p);
-        kReduceTail<1><<<1, kBlock, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
+        kReduceTail<1><<<1, kTail, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
         rhsNorm = readScalar(S_BN);
         initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
     }
@@ -1428,7 +1428,7 @@ void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T*
     } else {
         k3CopyNorm<T><<<rb, kBlock, 0, stream>>>(n, B, vec3(w.r), act, part);
     }
-    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_DP, act});
+    kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_DP, act});
     BSIM_LAUNCH_CHECK();
     syncScalars3();
     double dp[3] = {0.0, 0.0, 0.0};
@@ -1459,18 +1459,18 @@ void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T*
     int iter[3] = {0, 0, 0};
     while (active[0] || active[1] || active[2]) {
         k3Dot<T><<<rb, kBlock, 0, stream>>>(n, cvec3(w.r), cvec3(w.rp), act, part);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Rho{sc3.p, act});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Rho{sc3.p, act});
         k3BcgsP<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.p), cvec3(w.r), cvec3(w.v), sc3.p, act);
         k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.p), jv, vec3(w.tmp), act);
         k3SpmvD1<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.tmp), vec3(w.v), cvec3(w.rp), act, part);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Alpha{sc3.p, fl3.p, act});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Alpha{sc3.p, fl3.p, act});
         k3BcgsS<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.s), cvec3(w.v), cvec3(w.r), sc3.p, fl3.p, act);
         k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.s), jv, vec3(w.r), act);
         k3SpmvD2<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.r), vec3(w.t), cvec3(w.s), fl3.p, act, part);
-        kReduceTail<6><<<1, kBlock, 0, stream>>>(rb, part, Tail3Omega{sc3.p, fl3.p, act});
+        kReduceTail<6><<<1, kTail, 0, stream>>>(rb, part, Tail3Omega{sc3.p, fl3.p, act});
         k3BcgsUpdate<T><<<rb, kBlock, 0, stream>>>(n, X, cvec3(w.p), cvec3(w.s), vec3(w.r), cvec3(w.t), sc3.p, fl3.p,
                                                    act, part);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Norm{sc3.p, fl3.p, act});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Norm{sc3.p, fl3.p, act});
         BSIM_LAUNCH_CHECK();
         syncScalars3();
         bool changed = false;
@@ -1488,7 +1488,7 @@ void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T*
                 const bool only[3] = {c == 0, c == 1, c == 2};
                 upload(only);
                 k3Dot<T><<<rb, kBlock, 0, stream>>>(n, cvec3(w.s), cvec3(w.s), act, part);
-                kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_TMP, act});
+                kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Sqrt{sc3.p, S_TMP, act});
                 BSIM_LAUNCH_CHECK();
                 syncScalars3();
                 const double ss = hsc3[c * S_COUNT + S_TMP];
@@ -1547,10 +1547,10 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
     k3Copy<double><<<nb, kBlock, 0, stream>>>(n, cX, vec3(saved3), act3.p);
     if (!cfg.warmStart) k3Fill<double><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
     k3CopyNorm<double><<<rb, kBlock, 0, stream>>>(n, B, Vec3p<double>{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
-    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
+    kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
     if (cfg.warmStart) {
         k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.tmp), act3.p, partials.p);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
     }
     BSIM_LAUNCH_CHECK();
     syncScalars3();
@@ -1574,7 +1574,7 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
         bcgs3<double>(w64, view(), jv, b, x, run, !cfg.warmStart, cfg.maxIter, target, its, reason, h, dl);
         uploadAct(run);
         k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.tmp), act3.p, partials.p);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
         BSIM_LAUNCH_CHECK();
         syncScalars3();
     }
@@ -1644,7 +1644,7 @@ void DeviceSystem::solve3Mixed(const double* const b[3], double* const x[3], Sol
     uploadAct(use);
     if (!cfg.warmStart) k3Fill<double><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
     k3CopyNorm<double><<<rb, kBlock, 0, stream>>>(n, B, Vec3p<double>{{nullptr, nullptr, nullptr}}, act3.p, partials.p);
-    kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
+    kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_BN, act3.p});
     BSIM_LAUNCH_CHECK();
     syncScalars3();
     double rhsNorm[3], initial[3] = {0, 0, 0}, target[3], scale[3], dp[3] = {0, 0, 0}, prev[3] = {0, 0, 0};
@@ -1656,7 +1656,7 @@ void DeviceSystem::solve3Mixed(const double* const b[3], double* const x[3], Sol
         for (int c = 0; c < 3; ++c) live[c] = !done[c];
         uploadAct(live);
         k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.r), act3.p, partials.p);
-        kReduceTail<3><<<1, kBlock, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
+        kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
         BSIM_LAUNCH_CHECK();
         syncScalars3();
         bool run[3] = {false, false, false};

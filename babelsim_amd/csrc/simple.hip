@@ -322,7 +322,7 @@ IterRep SimpleSolver::iterate() {
     kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
     // rU before relaxation (main.cpp:60)
     kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, part);
-    kReduceTail<3><<<1, kBlock, 0, s>>>(reduceBlocks(nc), part, StoreSums<3>{red.p});
+    kReduceTail<3><<<1, kTail, 0, s>>>(reduceBlocks(nc), part, StoreSums<3>{red.p});
     kRelaxScale<<<nb, kBlock, 0, s>>>(mv, diag, off, src.p, Uprev.p, c.alphaU, rAU.p);
     if (nf > M.nfi) kOwnerTrace<<<blocksFor(nf - M.nfi), kBlock, 0, s>>>(mv, rAU.p, rAUtr.p);
     BSIM_LAUNCH_CHECK();
@@ -440,7 +440,7 @@ IterRep SimpleSolver::iterate() {
     t0 = Clock::now();
     // Convergence measures (main.cpp:112-126).
     kConvergence<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uprev.p, pPrime.p, p.p, phi.p, part);
-    kReduceTail<kRedK><<<1, kBlock, 0, s>>>(reduceBlocks(nc), part, StoreSums<kRedK>{red.p});
+    kReduceTail<kRedK><<<1, kTail, 0, s>>>(reduceBlocks(nc), part, StoreSums<kRedK>{red.p});
     BSIM_LAUNCH_CHECK();
     double h[kRedK];
     BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));

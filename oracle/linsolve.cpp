@@ -66,8 +66,8 @@ void spmv(const Csr& A, const double* x, double* y) {
 namespace {
 
 // Inner products are summed in a fixed blocked order: 256 strided partial sums
-// per block over at most 32768 blocks, pairwise trees inside each block, then
-// the same over the block sums.  Both sides of the parity check use this order
+// per block over at most 32768 blocks and a pairwise tree inside each block,
+// then 1024 strided sums of the block sums and a pairwise tree over them.  Both sides of the parity check use this order
 // (it is the deterministic reduction of the device library), so FP64 Krylov
 // histories can be compared bit for bit.
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
@@ -88,13 +88,14 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b) {
             for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
         partial[g] = acc[0];
     }
-    double acc[B];
-    for (long t = 0; t < B; ++t) {
+    constexpr long T = 1024;  // threads of the device's reduction tail
+    double acc[T];
+    for (long t = 0; t < T; ++t) {
         double s = 0.0;
-        for (long g = t; g < G; g += B) s += partial[g];
+        for (long g = t; g < G; g += T) s += partial[g];
         acc[t] = s;
     }
-    for (long s = B / 2; s > 0; s >>= 1)
+    for (long s = T / 2; s > 0; s >>= 1)
         for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
     return acc[0];
 }
