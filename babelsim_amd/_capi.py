@@ -73,6 +73,47 @@ def simple_config(**kw) -> SimpleConfig:
     return SimpleConfig(mom=mom, pre=pre, **base)
 
 
+class PisoConfig(C.Structure):
+    """Transient PISO controls (src/physics/piso/main.cpp:83-105) and per-equation methods."""
+
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double),
+                ("m_interp", C.c_int), ("m_grad", C.c_int), ("m_conv", C.c_int), ("m_diff", C.c_int),
+                ("m_coef_interp", C.c_int), ("m_coef_grad", C.c_int),
+                ("p_interp", C.c_int), ("p_grad", C.c_int), ("p_diff", C.c_int),
+                ("p_coef_interp", C.c_int), ("p_coef_grad", C.c_int),
+                ("time_method", C.c_int), ("max_iterations", C.c_int), ("n_correctors", C.c_int),
+                ("max_correctors", C.c_int), ("non_orth_corr", C.c_int),
+                ("alpha_u", C.c_double), ("coupling_tol", C.c_double), ("tol_mass", C.c_double),
+                ("tol_u", C.c_double), ("tol_mom", C.c_double), ("tol_p", C.c_double),
+                ("mom", SolverConfig), ("pre", SolverConfig)]
+
+
+class StepReport(C.Structure):
+    """One PISO time step: the values runPiso reports (main.cpp:282-289) and the outcome."""
+
+    _fields_ = [("dt", C.c_double), ("co_max", C.c_double), ("co_mean", C.c_double), ("mass", C.c_double),
+                ("dU", C.c_double), ("rU", C.c_double), ("dP", C.c_double), ("r_coupling_first", C.c_double),
+                ("r_coupling", C.c_double), ("lin_u", C.c_double), ("lin_p", C.c_double),
+                ("passes", C.c_int), ("correctors", C.c_int), ("mom_iters", C.c_int * 3), ("p_iters", C.c_int),
+                ("accepted", C.c_int), ("converged", C.c_int), ("linear_ok", C.c_int), ("status", C.c_int)]
+
+
+EULER, BDF2 = 1, 2
+STEP_COMPLETED, STEP_NOT_CONVERGED, STEP_NUMERICAL_FAILURE = 0, 1, 2
+
+
+def piso_config(**kw) -> PisoConfig:
+    """Defaults of runPiso's solution dictionary (main.cpp:83-105)."""
+    base = dict(rho=1.0, mu=0.01, m_interp=0, m_grad=0, m_conv=1, m_diff=0, m_coef_interp=0, m_coef_grad=0,
+                p_interp=0, p_grad=0, p_diff=0, p_coef_interp=0, p_coef_grad=0, time_method=BDF2,
+                max_iterations=1, n_correctors=2, max_correctors=0, non_orth_corr=1, alpha_u=1.0,
+                coupling_tol=np.finfo(np.float64).max, tol_mass=1e-8, tol_u=1e-7, tol_mom=1e-6, tol_p=1e-6)
+    mom = kw.pop("mom", None) or solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
+    pre = kw.pop("pre", None) or solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14)
+    base.update(kw)
+    return PisoConfig(mom=mom, pre=pre, **base)
+
+
 def ptr(a: np.ndarray | None, kind=dp):
     if a is None:
         return None
@@ -148,8 +189,19 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, dp, ip, ip],
     "bsim_simple_probe_start": [vp, C.c_int],
     "bsim_simple_probe_stop": [vp, ip, dp, dp],
+    "bsim_piso_create": [vp, C.POINTER(PisoConfig), ip, dp, ip, dp, C.POINTER(vp)],
+    "bsim_piso_set_fields": [vp, dp, dp],
+    "bsim_piso_start": [vp],
+    "bsim_piso_step": [vp, C.c_double, C.POINTER(StepReport)],
+    "bsim_piso_get_fields": [vp, dp, dp, dp],
+    "bsim_piso_temporal_flux": [vp, dp],
+    "bsim_piso_history": [vp, C.c_int, dp, C.c_int, ip],
+    "bsim_piso_timing": [vp, dp],
+    "bsim_piso_probe_start": [vp, C.c_int],
+    "bsim_piso_probe_stop": [vp, ip, dp, dp],
 }
-_VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp]}
+_VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp],
+         "bsim_piso_destroy": [vp]}
 
 
 def _declare_void(L):

@@ -2,6 +2,7 @@
 // systems and the SIMPLE driver.  Host arrays are in the caller's numbering.
 #include "../../include/bsim.h"
 #include "errors.hpp"
+#include "piso.hpp"
 #include "simple.hpp"
 
 #include <cstring>
@@ -21,6 +22,10 @@ struct bsim_system {
 struct bsim_simple {
     bsim_ctx* ctx;
     std::unique_ptr<bsim::SimpleSolver> s;
+};
+struct bsim_piso {
+    bsim_ctx* ctx;
+    std::unique_ptr<bsim::PisoSolver> s;
 };
 
 using namespace bsim;
@@ -506,5 +511,79 @@ int bsim_simple_probe_stop(bsim_simple* s, int* launches, double* seconds, doubl
     });
 }
 void bsim_simple_destroy(bsim_simple* s) { delete s; }
+
+int bsim_piso_create(bsim_ctx* ctx, const bsim_piso_config* cfg, const int* ubct, const double* ubcv, const int* pbct,
+                     const double* pbcv, bsim_piso** out) {
+    return guarded([&] {
+        if (!ctx || !cfg || !ubct || !ubcv || !pbct || !pbcv || !out) throw std::invalid_argument("null argument");
+        PisoCfg c;
+        c.rho = cfg->rho; c.mu = cfg->mu;
+        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
+        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
+        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
+        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
+        c.timeMethod = cfg->time_method;
+        c.maxIterations = cfg->max_iterations; c.nCorrectors = cfg->n_correctors;
+        c.maxCorrectors = cfg->max_correctors; c.nonOrthCorr = cfg->non_orth_corr;
+        c.alphaU = cfg->alpha_u; c.couplingTol = cfg->coupling_tol;
+        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
+        c.mom = toSolverCfg(&cfg->mom);
+        c.pre = toSolverCfg(&cfg->pre);
+        auto* s = new bsim_piso;
+        s->ctx = ctx;
+        try { s->s = std::make_unique<PisoSolver>(*ctx->c, c, ubct, ubcv, pbct, pbcv); }
+        catch (...) { delete s; throw; }
+        *out = s;
+    });
+}
+int bsim_piso_set_fields(bsim_piso* s, const double* U, const double* p) {
+    return guarded([&] { s->s->setFields(U, p); });
+}
+int bsim_piso_start(bsim_piso* s) { return guarded([&] { s->s->start(); }); }
+int bsim_piso_step(bsim_piso* s, double dt, bsim_step_report* o) {
+    return guarded([&] {
+        if (!o) throw std::invalid_argument("null report");
+        const StepRep r = s->s->step(dt);
+        o->dt = r.dt; o->co_max = r.coMax; o->co_mean = r.coMean; o->mass = r.mass; o->dU = r.dU; o->rU = r.rU;
+        o->dP = r.dP; o->r_coupling_first = r.rCouplingFirst; o->r_coupling = r.rCoupling; o->lin_u = r.linU;
+        o->lin_p = r.linP; o->passes = r.passes; o->correctors = r.correctors;
+        for (int c = 0; c < 3; ++c) o->mom_iters[c] = r.momIters[c];
+        o->p_iters = r.pIters; o->accepted = r.accepted; o->converged = r.converged; o->linear_ok = r.linearOk;
+        o->status = r.status;
+    });
+}
+int bsim_piso_get_fields(bsim_piso* s, double* U, double* p, double* phi) {
+    return guarded([&] { s->s->getFields(U, p, phi); });
+}
+int bsim_piso_temporal_flux(bsim_piso* s, double* out) {
+    return guarded([&] { s->s->temporalFlux(out); });
+}
+int bsim_piso_history(bsim_piso* s, int which, double* out, int cap, int* len) {
+    return guarded([&] {
+        if (which < 0 || which > 3) throw std::invalid_argument("history index is 0..3");
+        const std::vector<double>& h = which < 3 ? s->s->momHist[which] : s->s->pHist;
+        *len = static_cast<int>(h.size());
+        if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<std::size_t>(cap, h.size()));
+    });
+}
+int bsim_piso_timing(bsim_piso* s, double* o) {
+    return guarded([&] {
+        const PisoTiming& t = s->s->timing;
+        const double v[8] = {t.history, t.assembly, t.momentumSolve, t.corrector, t.pressureAssembly,
+                             t.pressureSolve, t.diagnostics, static_cast<double>(t.steps)};
+        std::memcpy(o, v, sizeof(v));
+    });
+}
+int bsim_piso_probe_start(bsim_piso* s, int capacity) {
+    return guarded([&] { s->s->pressureSystem().probe.start(capacity); });
+}
+int bsim_piso_probe_stop(bsim_piso* s, int* launches, double* seconds, double* bytes_per_launch) {
+    return guarded([&] {
+        KernelProbe& P = s->s->pressureSystem().probe;
+        *seconds = P.stop(*launches);
+        *bytes_per_launch = P.bytesPerLaunch;
+    });
+}
+void bsim_piso_destroy(bsim_piso* s) { delete s; }
 
 }  // extern "C"

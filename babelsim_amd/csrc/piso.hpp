@@ -1,0 +1,79 @@
+// Transient PISO time step resident in HBM: the order of
+// src/physics/piso/main.cpp:161-300 (laminar) with every field, history,
+// matrix and preconditioner kept on the device.  The host makes only the
+// decisions the reference makes on the host: linear convergence, the
+// corrector count and step acceptance.
+#pragma once
+
+#include "context.hpp"
+
+#include <limits>
+
+namespace bsim {
+
+enum : int { kEuler = 1, kBdf2 = 2 };
+enum : int { kStepCompleted = 0, kStepNotConverged = 1, kStepNumericalFailure = 2 };
+
+struct PisoCfg {
+    double rho = 1.0, mu = 0.01;
+    int mInterp = 0, mGrad = 0, mConv = 1, mDiff = 0, mCoefInterp = 0, mCoefGrad = 0;
+    int pInterp = 0, pGrad = 0, pDiff = 0, pCoefInterp = 0, pCoefGrad = 0;
+    int timeMethod = kBdf2;
+    int maxIterations = 1, nCorrectors = 2, maxCorrectors = 0, nonOrthCorr = 1;
+    double alphaU = 1.0, couplingTol = std::numeric_limits<double>::max();
+    double tolMass = 1e-8, tolU = 1e-7, tolMom = 1e-6, tolP = 1e-6;
+    SolverCfg mom, pre;
+};
+
+struct StepRep {
+    double dt = 0, coMax = 0, coMean = 0, mass = 0, dU = 0, rU = 0, dP = 0;
+    double rCouplingFirst = 0, rCoupling = 0, linU = 0, linP = 0;
+    int passes = 0, correctors = 0;
+    int momIters[3] = {0, 0, 0};
+    int pIters = 0, accepted = 0, converged = 0, linearOk = 0, status = kStepNumericalFailure;
+};
+
+// Per-phase wall time (seconds) accumulated over steps.
+struct PisoTiming {
+    double history = 0, assembly = 0, momentumSolve = 0, corrector = 0, pressureAssembly = 0, pressureSolve = 0,
+           diagnostics = 0;
+    int steps = 0;
+};
+
+class PisoSolver {
+public:
+    PisoSolver(Context& ctx, const PisoCfg& cfg, const int* ubct, const double* ubcv, const int* pbct,
+               const double* pbcv);
+    void setFields(const double* Ucaller, const double* pcaller);
+    void getFields(double* Ucaller, double* pcaller, double* phicaller);
+    void temporalFlux(double* caller);
+    void start();
+    StepRep step(double dt);
+    std::vector<double> momHist[3], pHist;
+    PisoTiming timing;
+    DeviceSystem& momentumSystem() { return *momSys; }
+    DeviceSystem& pressureSystem() { return *pSys; }
+
+private:
+    FieldArg fieldU(const double* ctx) const { return {U.p, uType.p, uVal.p, ctx, nullptr}; }
+    FieldArg fieldP() const { return {p.p, pType.p, pVal.p, nullptr, nullptr}; }
+    FieldArg fieldRAU() const { return {rAU.p, zgType.p, zgVal.p, nullptr, rAUtr.p}; }
+    void computeTemporalFlux();
+    Context& C;
+    PisoCfg c;
+    int refCell = 0;
+    bool inletOutlet = false;
+    DBuf<int> uType, pType, zgType;
+    DBuf<double> uVal, pVal, zgVal;
+    DBuf<double> U, p, phi, Uiter, HbyA, gradU, gradP, work9, rAU, rAUtr, invAP, rAUf, npr, kfMom, mu, imbalance,
+        pOld, corr, full, orthF, src, psrc, tf, fluxTmp, red;
+    // Time levels (history.h:6-27): slot `cur` is the previous level, 1-cur the older.
+    DBuf<double> Uh[2], phih[2], fluxh[2];  // fluxh: flux of the saved velocity
+    int cur = 0, levels = 0;
+    double dtCur = 0, dtPrev = 0;
+    DBuf<double> bcomp[3], xcomp[3];
+    std::unique_ptr<DeviceSystem> momSys, pSys;
+    bool started = false, stepped = false;
+};
+
+}  // namespace bsim

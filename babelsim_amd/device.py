@@ -1,5 +1,5 @@
 """Python handles over the device library: a case context (mesh in HBM), its
-face operators, persistent linear systems and the SIMPLE driver.
+face operators, persistent linear systems and the SIMPLE and PISO drivers.
 
 The method signatures mirror the operator set of BabelSim's
 ``include/babelsim/operators.h`` (one field at a time, boundary conditions per
@@ -11,7 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (IterReport, MeshData, SimpleConfig, SolveResult, SolverConfig, check, dp, ip, lib, ptr)
+from ._capi import (IterReport, MeshData, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, check, dp,
+                    ip, lib, ptr)
 
 
 def _f64(a):
@@ -266,3 +267,118 @@ class Simple:
         b = C.c_double()
         check(lib().bsim_simple_probe_stop(self.h, C.byref(n), C.byref(s), C.byref(b)))
         return n.value, s.value, b.value
+
+
+class Piso:
+    """Transient PISO resident on the GPU (src/physics/piso/main.cpp).
+
+    ``step(dt)`` is one ``time.advance()`` of length dt and the body of
+    runPiso's time loop; ``run`` adds the TimeStepper of src/io/case.cpp:391-403.
+    """
+
+    def __init__(self, ctx: Context, cfg: PisoConfig, u_bc, p_bc):
+        self.ctx = ctx
+        self.cfg = cfg
+        ut, uv = u_bc
+        pt, pv = p_bc
+        h = C.c_void_p()
+        check(lib().bsim_piso_create(ctx.h, C.byref(cfg), ptr(_i32(ut), ip), ptr(_f64(uv).reshape(-1)),
+                                     ptr(_i32(pt), ip), ptr(_f64(pv).reshape(-1)), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bsim_piso_destroy(self.h)
+            self.h = None
+
+    def set_fields(self, U=None, p=None):
+        check(lib().bsim_piso_set_fields(self.h, ptr(None if U is None else _f64(U)),
+                                         ptr(None if p is None else _f64(p))))
+
+    def start(self):
+        check(lib().bsim_piso_start(self.h))
+
+    def step(self, dt) -> StepReport:
+        rep = StepReport()
+        check(lib().bsim_piso_step(self.h, float(dt), C.byref(rep)))
+        return rep
+
+    def run(self, start_time, end_time, delta_t, max_steps=None):
+        """Steps from start_time to end_time with TimeStepper's rule (src/io/case.cpp:391-403);
+        stops at the first step that is not completed, like runPiso.  Returns the reports."""
+        stepper = TimeStepper(start_time, end_time, delta_t)
+        reports = []
+        while not stepper.finished() and (max_steps is None or len(reports) < max_steps):
+            dt = stepper.advance()
+            rep = self.step(dt)
+            reports.append(rep)
+            if rep.status != 0:
+                break
+        return reports
+
+    def fields(self):
+        U = np.zeros((self.ctx.nc, 3))
+        p = np.zeros(self.ctx.nc)
+        phi = np.zeros(self.ctx.nf)
+        check(lib().bsim_piso_get_fields(self.h, ptr(U), ptr(p), ptr(phi)))
+        return U, p, phi
+
+    def temporal_flux(self):
+        out = np.zeros(self.ctx.nf)
+        check(lib().bsim_piso_temporal_flux(self.h, ptr(out)))
+        return out
+
+    def history(self, which):
+        n = C.c_int()
+        check(lib().bsim_piso_history(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        check(lib().bsim_piso_history(self.h, which, ptr(out), n.value, C.byref(n)))
+        return out
+
+    def timing(self) -> dict:
+        out = np.zeros(8)
+        check(lib().bsim_piso_timing(self.h, ptr(out)))
+        keys = ["history", "assembly", "momentum_solve", "corrector", "pressure_assembly", "pressure_solve",
+                "diagnostics", "steps"]
+        return dict(zip(keys, out.tolist()))
+
+    def probe_start(self, capacity=8192):
+        check(lib().bsim_piso_probe_start(self.h, capacity))
+
+    def probe_stop(self):
+        n = C.c_int()
+        s = C.c_double()
+        b = C.c_double()
+        check(lib().bsim_piso_probe_stop(self.h, C.byref(n), C.byref(s), C.byref(b)))
+        return n.value, s.value, b.value
+
+
+class TimeStepper:
+    """src/io/case.cpp:388-403: t_{n+1} = start + (n+1) delta_t in long double,
+    snapped to end_time within 32 eps of the time scale; dt = t_{n+1} - t_n."""
+
+    def __init__(self, start_time, end_time, delta_t):
+        if not (np.isfinite(start_time) and np.isfinite(end_time) and np.isfinite(delta_t)) or \
+                end_time < start_time or delta_t <= 0:
+            raise ValueError("time control is invalid")
+        self.start, self.end, self.delta = float(start_time), float(end_time), float(delta_t)
+        self.value = self.start
+        self.step = 0
+
+    def finished(self) -> bool:
+        return self.value >= self.end
+
+    def advance(self) -> float:
+        if self.finished():
+            raise RuntimeError("cannot advance beyond endTime")
+        eps = np.finfo(np.float64).eps
+        tolerance = 32 * eps * max(abs(self.start), abs(self.end), self.delta)
+        nxt = float(np.longdouble(self.start) + np.longdouble(self.step + 1) * np.longdouble(self.delta))
+        if nxt >= self.end - tolerance:
+            nxt = self.end
+        dt = nxt - self.value
+        if not dt > 0:
+            raise RuntimeError("time step is below representable time precision")
+        self.value = nxt
+        self.step += 1
+        return dt

@@ -185,6 +185,62 @@ int bsim_simple_probe_start(bsim_simple* s, int capacity);
 int bsim_simple_probe_stop(bsim_simple* s, int* launches, double* seconds, double* bytes_per_launch);
 void bsim_simple_destroy(bsim_simple* s);
 
+/* ------------------------------------------------------------------- PISO
+ * Transient PISO (src/physics/piso/main.cpp:66-302, laminar) resident on the
+ * GPU.  The controls are runPiso's solution dictionary (main.cpp:83-105):
+ * maxIterations, nCorrectors, maxCorrectors (0 means nCorrectors),
+ * nonOrthogonalCorrections, velocityRelaxation, couplingTolerance and the
+ * continuity / velocity / momentum / pressure tolerances; time_method is the
+ * methods dictionary's time scheme.  The run loop and its TimeStepper
+ * (src/io/case.cpp:388-403) stay with the caller: each bsim_piso_step call is
+ * one `time.advance()` of length dt followed by the body of the while loop
+ * (main.cpp:161-299).  Field writing (main.cpp:298) stays with the caller too. */
+enum { BSIM_EULER = 1, BSIM_BDF2 = 2 };
+/* Step outcomes, SolverResult's completed / notConverged / numericalFailure
+ * (include/babelsim/solver.h).  A step that is not completed ends the run in
+ * the reference; the fields then hold the rejected step's values. */
+enum { BSIM_STEP_COMPLETED = 0, BSIM_STEP_NOT_CONVERGED = 1, BSIM_STEP_NUMERICAL_FAILURE = 2 };
+typedef struct {
+    double rho, mu;
+    int m_interp, m_grad, m_conv, m_diff, m_coef_interp, m_coef_grad;
+    int p_interp, p_grad, p_diff, p_coef_interp, p_coef_grad;
+    int time_method;
+    int max_iterations, n_correctors, max_correctors, non_orth_corr;
+    double alpha_u, coupling_tol, tol_mass, tol_u, tol_mom, tol_p;
+    bsim_solver_config mom, pre;
+} bsim_piso_config;
+
+/* The per-step record runPiso reports (main.cpp:282-289): CoMax, CoMean,
+ * mass, dU, rU, dP, rCouplingFirst, rCoupling, linU, linP, the outer passes
+ * and corrector passes used, Krylov iterations, accepted / converged /
+ * linear-ok flags and the step outcome. */
+typedef struct {
+    double dt, co_max, co_mean, mass, dU, rU, dP, r_coupling_first, r_coupling, lin_u, lin_p;
+    int passes, correctors, mom_iters[3], p_iters, accepted, converged, linear_ok, status;
+} bsim_step_report;
+
+typedef struct bsim_piso bsim_piso;
+int bsim_piso_create(bsim_ctx* ctx, const bsim_piso_config* cfg, const int* u_bc_type, const double* u_bc_value,
+                     const int* p_bc_type, const double* p_bc_value, bsim_piso** out);
+int bsim_piso_set_fields(bsim_piso* s, const double* U, const double* p);
+/* phi = flux(U) and the time histories from the current fields (main.cpp:123, 157-158). */
+int bsim_piso_start(bsim_piso* s);
+int bsim_piso_step(bsim_piso* s, double dt, bsim_step_report* report);
+int bsim_piso_get_fields(bsim_piso* s, double* U, double* p, double* phi);
+/* rho * temporalFluxCorrection of the last step (main.cpp:19-53, 166-167), one value per face. */
+int bsim_piso_temporal_flux(bsim_piso* s, double* out);
+/* which: 0..2 momentum components, 3 pressure (last solve). */
+int bsim_piso_history(bsim_piso* s, int which, double* out, int cap, int* len);
+/* Seconds per phase: history + ddtPhiCorr, momentum assembly, momentum solve,
+ * corrector (HbyA, fluxes, corrections, coupling), pressure assembly,
+ * pressure solve, diagnostics, steps. */
+int bsim_piso_timing(bsim_piso* s, double* out8);
+/* Diagnostics with no counterpart in the reference: HIP-event timing of the
+ * pressure solve's fine-level AMG smoother sweeps, as bsim_simple_probe_*. */
+int bsim_piso_probe_start(bsim_piso* s, int capacity);
+int bsim_piso_probe_stop(bsim_piso* s, int* launches, double* seconds, double* bytes_per_launch);
+void bsim_piso_destroy(bsim_piso* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,9 +3,10 @@
 
 #include <omp.h>
 
-#include "simple.hpp"
+#include "piso.hpp"
 
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <string>
 
@@ -316,6 +317,88 @@ int orc_simple_history(void* sp, int which, double* out, int cap, int* len) {
         const std::vector<double>& h = which < 3 ? s.momHist[which] : s.pHist;
         *len = static_cast<int>(h.size());
         std::memcpy(out, h.data(), sizeof(double) * std::min<int>(cap, static_cast<int>(h.size())));
+    });
+}
+
+void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, const double* ubcv, const int* pbct,
+                   const double* pbcv) {
+    Piso* s = nullptr;
+    guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        PisoConfig c;
+        c.rho = cfg->rho; c.mu = cfg->mu;
+        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
+        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
+        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
+        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
+        c.timeMethod = cfg->time_method;
+        c.maxIterations = cfg->max_iterations; c.nCorrectors = cfg->n_correctors;
+        c.maxCorrectors = cfg->max_correctors; c.nonOrthCorr = cfg->non_orth_corr;
+        c.alphaU = cfg->alpha_u; c.couplingTol = cfg->coupling_tol;
+        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
+        c.mom = toConfig(&cfg->mom); c.pre = toConfig(&cfg->pre);
+        std::unique_ptr<Piso> owned(new Piso(m, c));
+        std::vector<BC<V3>> ub(m.np);
+        std::vector<BC<double>> pb(m.np);
+        for (Index p = 0; p < m.np; ++p) {
+            ub[p].type = ubct[p]; ub[p].value = {ubcv[3 * p], ubcv[3 * p + 1], ubcv[3 * p + 2]};
+            pb[p].type = pbct[p]; pb[p].value = pbcv[p];
+        }
+        owned->setBoundaries(ub, pb);
+        s = owned.release();
+    });
+    return s;
+}
+void orc_piso_free(void* s) { delete static_cast<Piso*>(s); }
+int orc_piso_set_fields(void* sp, const double* U, const double* p) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        for (Index i = 0; i < s.m.nc; ++i) {
+            if (U) s.U.v[i] = {U[3 * i], U[3 * i + 1], U[3 * i + 2]};
+            if (p) s.p.v[i] = p[i];
+        }
+    });
+}
+int orc_piso_start(void* sp) { return guard([&] { static_cast<Piso*>(sp)->start(); }); }
+int orc_piso_step(void* sp, double dt, orc_step_report* o) {
+    return guard([&] {
+        const StepReport r = static_cast<Piso*>(sp)->step(dt);
+        o->dt = r.dt; o->co_max = r.coMax; o->co_mean = r.coMean; o->mass = r.mass; o->dU = r.dU; o->rU = r.rU;
+        o->dP = r.dP; o->r_coupling_first = r.rCouplingFirst; o->r_coupling = r.rCoupling; o->lin_u = r.linU;
+        o->lin_p = r.linP; o->passes = r.passes; o->correctors = r.correctors;
+        for (int c = 0; c < 3; ++c) o->mom_iters[c] = r.momIters[c];
+        o->p_iters = r.pIters; o->accepted = r.accepted; o->converged = r.converged; o->linear_ok = r.linearOk;
+        o->status = r.status;
+    });
+}
+int orc_piso_get_fields(void* sp, double* U, double* p, double* phi) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (U) store(s.U.v, U);
+        if (p) store(s.p.v, p);
+        if (phi) store(s.phi, phi);
+    });
+}
+int orc_piso_temporal_flux(void* sp, double* out) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (s.lastTemporalFlux.empty()) throw std::invalid_argument("no step taken");
+        store(s.lastTemporalFlux, out);
+    });
+}
+int orc_piso_history(void* sp, int which, double* out, int cap, int* len) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        const std::vector<double>& h = which < 3 ? s.momHist[which] : s.pHist;
+        *len = static_cast<int>(h.size());
+        std::memcpy(out, h.data(), sizeof(double) * std::min<int>(cap, static_cast<int>(h.size())));
+    });
+}
+int orc_time_weights(int method, int levels, double dt, double previous_dt, double* out5) {
+    return guard([&] {
+        const TimeWeights w = timeWeights(method, levels, dt, previous_dt);
+        out5[0] = w.a0; out5[1] = w.a1; out5[2] = w.a2;
+        temporalFluxWeights(method, levels, dt, previous_dt, out5[3], out5[4]);
     });
 }
 

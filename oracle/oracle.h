@@ -88,6 +88,35 @@ int orc_simple_iterate(void* s, int n, orc_iter_report* reports);
 int orc_simple_get_fields(void* s, double* U, double* p, double* phi);
 int orc_simple_history(void* s, int which, double* out, int cap, int* len);
 
+typedef struct {
+    double rho, mu;
+    int m_interp, m_grad, m_conv, m_diff, m_coef_interp, m_coef_grad;
+    int p_interp, p_grad, p_diff, p_coef_interp, p_coef_grad;
+    int time_method;                       /* 1 Euler, 2 BDF2 */
+    int max_iterations, n_correctors, max_correctors, non_orth_corr;
+    double alpha_u, coupling_tol, tol_mass, tol_u, tol_mom, tol_p;
+    orc_solver_config mom, pre;
+} orc_piso_config;
+
+typedef struct {
+    double dt, co_max, co_mean, mass, dU, rU, dP, r_coupling_first, r_coupling, lin_u, lin_p;
+    int passes, correctors, mom_iters[3], p_iters, accepted, converged, linear_ok, status;
+} orc_step_report;
+
+void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, const double* ubcv, const int* pbct,
+                   const double* pbcv);
+void orc_piso_free(void* s);
+int orc_piso_set_fields(void* s, const double* U, const double* p);
+int orc_piso_start(void* s);
+int orc_piso_step(void* s, double dt, orc_step_report* report);
+int orc_piso_get_fields(void* s, double* U, double* p, double* phi);
+/* rho * temporal flux correction of the last step (nf values). */
+int orc_piso_temporal_flux(void* s, double* out);
+/* which: 0..2 momentum components, 3 pressure (last solve). */
+int orc_piso_history(void* s, int which, double* out, int cap, int* len);
+/* Time-term weights a0, a1, a2 and ddtPhiCorr weights (previous, older). */
+int orc_time_weights(int method, int levels, double dt, double previous_dt, double* out5);
+
 #ifdef __cplusplus
 }
 #endif

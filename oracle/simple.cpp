@@ -8,12 +8,12 @@
 
 namespace orc {
 
-namespace {
-
 template <class T> void accumulate(Ldu<T>& a, const Ldu<T>& b, double f) {  // procedural_equation.cpp:62-70
     for (std::size_t i = 0; i < a.diag.size(); ++i) { a.diag[i] += f * b.diag[i]; a.source[i] += f * b.source[i]; }
     for (std::size_t i = 0; i < a.upper.size(); ++i) { a.upper[i] += f * b.upper[i]; a.lower[i] += f * b.lower[i]; }
 }
+template void accumulate(Ldu<double>&, const Ldu<double>&, double);
+template void accumulate(Ldu<V3>&, const Ldu<V3>&, double);
 
 template <class T> void applyLdu(const Mesh& m, const Ldu<T>& q, const std::vector<T>& x, std::vector<T>& y) {
     // procedural_equation.cpp:281-292
@@ -26,6 +26,21 @@ template <class T> void applyLdu(const Mesh& m, const Ldu<T>& q, const std::vect
         y[n] += q.lower[f] * x[o];
     }
 }
+template void applyLdu(const Mesh&, const Ldu<double>&, const std::vector<double>&, std::vector<double>&);
+template void applyLdu(const Mesh&, const Ldu<V3>&, const std::vector<V3>&, std::vector<V3>&);
+
+SField calculatedRatio(const Mesh& m, const std::vector<double>& num, const std::vector<double>& den) {
+    // geometry.cpp:9-22 + math.h fieldBinary: V/aP with traces V_o/aP_o
+    SField r(m);
+    r.calculated = true;
+    r.trace.assign(m.nf, 0.0);
+    for (Index i = 0; i < m.nc; ++i) r.v[i] = num[i] / den[i];
+    for (Index f = 0; f < m.nf; ++f)
+        if (m.boundary(f)) r.trace[f] = num[m.own[f]] / den[m.own[f]];
+    return r;
+}
+
+namespace {
 
 double sq(double v) { return v * v; }
 double sq(const V3& v) { return norm2(v); }
@@ -87,17 +102,6 @@ void Simple::start() {  // main.cpp:41: phi = math::flux(U, momentum options)
     flux(U, phi, c.mInterp, c.mGrad);
 }
 
-SField Simple::calculatedRatio(const std::vector<double>& num, const std::vector<double>& den) const {
-    // geometry.cpp:9-22 + math.h fieldBinary: V/aP with traces V_o/aP_o
-    SField r(m);
-    r.calculated = true;
-    r.trace.assign(m.nf, 0.0);
-    for (Index i = 0; i < m.nc; ++i) r.v[i] = num[i] / den[i];
-    for (Index f = 0; f < m.nf; ++f)
-        if (m.boundary(f)) r.trace[f] = num[m.own[f]] / den[m.own[f]];
-    return r;
-}
-
 void Simple::solveSystem(LinearSystem& sys, const Ldu<double>& q, const SolverConfig& sc, std::vector<double>& x,
                          SolveResult& res, std::vector<double>* hist) {
     Csr A;
@@ -141,7 +145,7 @@ IterReport Simple::iterate() {
     for (auto& v : mom.lower) v *= alphaU;
     for (auto& v : mom.source) v = alphaU * v;
     const std::vector<double> aP = mom.diag;
-    const SField rAU = calculatedRatio(m.V, aP);
+    const SField rAU = calculatedRatio(m, m.V, aP);
     // Momentum solve: one matrix, three right-hand sides (petsc_backend.cpp:156-198).
     {
         Csr A;

@@ -26,6 +26,9 @@ struct IterReport {
     int referenced = 0;
 };
 
+template <class T> void accumulate(Ldu<T>& a, const Ldu<T>& b, double f);
+template <class T> void applyLdu(const Mesh& m, const Ldu<T>& q, const std::vector<T>& x, std::vector<T>& y);
+SField calculatedRatio(const Mesh& m, const std::vector<double>& num, const std::vector<double>& den);
 template <class T> double relativeResidual(const Mesh& m, const Ldu<T>& q, const std::vector<T>& x);
 bool referenceIfUnanchored(const Mesh& m, Ldu<double>& q, double value);
 
@@ -41,7 +44,6 @@ struct Simple {
     void setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<double>>& pbc);
     void start();
     IterReport iterate();
-    SField calculatedRatio(const std::vector<double>& num, const std::vector<double>& den) const;
     void solveSystem(LinearSystem& sys, const Ldu<double>& q, const SolverConfig& sc, std::vector<double>& x,
                      SolveResult& res, std::vector<double>* hist);
 };

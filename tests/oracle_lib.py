@@ -7,7 +7,7 @@ from pathlib import Path
 
 import numpy as np
 
-from babelsim_amd._capi import IterReport, SimpleConfig, SolveResult, SolverConfig, ip, ptr
+from babelsim_amd._capi import IterReport, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, ip, ptr
 
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE = ROOT / "oracle" / "liboracle.so"
@@ -68,6 +68,17 @@ def oracle():
         L.orc_simple_iterate.argtypes = [vp, C.c_int, C.POINTER(IterReport)]
         L.orc_simple_get_fields.argtypes = [vp, dp, dp, dp]
         L.orc_simple_history.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+        L.orc_piso_new.restype = vp
+        L.orc_piso_new.argtypes = [vp, C.POINTER(PisoConfig), ip, dp, ip, dp]
+        L.orc_piso_free.argtypes = [vp]
+        L.orc_piso_free.restype = None
+        L.orc_piso_set_fields.argtypes = [vp, dp, dp]
+        L.orc_piso_start.argtypes = [vp]
+        L.orc_piso_step.argtypes = [vp, C.c_double, C.POINTER(StepReport)]
+        L.orc_piso_get_fields.argtypes = [vp, dp, dp, dp]
+        L.orc_piso_temporal_flux.argtypes = [vp, dp]
+        L.orc_piso_history.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+        L.orc_time_weights.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp]
         _orc = L
     return _orc
 
@@ -261,6 +272,71 @@ class OracleSimple:
         out = np.zeros(200000)
         self._call("orc_simple_history", which, ptr(out), len(out), C.byref(n))
         return out[:n.value].copy()
+
+
+class OraclePiso:
+    """The oracle's PISO driver with the same interface as babelsim_amd.device.Piso."""
+
+    def __init__(self, mesh: CpuMesh, cfg: PisoConfig, u_bc, p_bc):
+        L = oracle()
+        self.m = mesh
+        ut, uv = u_bc
+        pt, pv = p_bc
+        ut = np.ascontiguousarray(ut, np.int32)
+        pt = np.ascontiguousarray(pt, np.int32)
+        uv = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+        pv = np.ascontiguousarray(np.asarray(pv, np.float64).reshape(-1))
+        h = L.orc_piso_new(mesh.h, C.byref(cfg), ptr(ut, ip), ptr(uv), ptr(pt, ip), ptr(pv))
+        if not h:
+            raise ValueError(L.orc_last_error().decode())
+        self.h = vp(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle().orc_piso_free(self.h)
+            self.h = None
+
+    def _call(self, name, *args):
+        if getattr(oracle(), name)(self.h, *args):
+            raise ValueError(oracle().orc_last_error().decode())
+
+    def set_fields(self, U=None, p=None):
+        self._call("orc_piso_set_fields", ptr(None if U is None else np.ascontiguousarray(U, np.float64)),
+                   ptr(None if p is None else np.ascontiguousarray(p, np.float64)))
+
+    def start(self):
+        self._call("orc_piso_start")
+
+    def step(self, dt):
+        rep = StepReport()
+        self._call("orc_piso_step", float(dt), C.byref(rep))
+        return rep
+
+    def fields(self):
+        U = np.zeros((self.m.nc, 3))
+        p = np.zeros(self.m.nc)
+        phi = np.zeros(self.m.nf)
+        self._call("orc_piso_get_fields", ptr(U), ptr(p), ptr(phi))
+        return U, p, phi
+
+    def temporal_flux(self):
+        out = np.zeros(self.m.nf)
+        self._call("orc_piso_temporal_flux", ptr(out))
+        return out
+
+    def history(self, which):
+        n = C.c_int()
+        out = np.zeros(200000)
+        self._call("orc_piso_history", which, ptr(out), len(out), C.byref(n))
+        return out[:n.value].copy()
+
+
+def time_weights(method, levels, dt, previous_dt):
+    """(a0, a1, a2, previousWeight, olderWeight) of the oracle's time terms."""
+    out = np.zeros(5)
+    if oracle().orc_time_weights(method, levels, dt, previous_dt, ptr(out)):
+        raise ValueError(oracle().orc_last_error().decode())
+    return out
 
 
 def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, hist_cap=200000):

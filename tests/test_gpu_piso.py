@@ -1,0 +1,195 @@
+"""Device PISO (babelsim_amd/csrc/piso.hip) against the oracle's PISO step on
+the same renumbered meshes: per-step reports (iteration counts exactly,
+residual measures to 1e-8 relative), fields, the ddtPhiCorr flux and the
+Krylov histories.  Plus the textbook time-integration check and the
+reference's coupling-test behaviour run on the device."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as orc
+import piso_cases as pc
+from babelsim_amd import BDF2, EULER, STEP_COMPLETED, STEP_NOT_CONVERGED, solver_config
+from babelsim_amd.device import Context, Piso, TimeStepper
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def pair(mesh_data, cfg, bcs, U0=None):
+    arrays, _, _ = mesh_data.prepare()
+    arrays = arrays.arrays()
+    ub, pb = bcs
+    om = orc.oracle_mesh(arrays)
+    so = orc.OraclePiso(om, cfg, ub, pb)
+    ctx = Context(arrays, renumber=False)
+    sg = Piso(ctx, cfg, ub, pb)
+    if U0 is not None:
+        U0 = U0(om.geometry()["C"])
+        so.set_fields(U0, np.zeros(om.nc))
+        sg.set_fields(U0, np.zeros(om.nc))
+    so.start()
+    sg.start()
+    return so, sg
+
+
+def compare_step(a, b):
+    assert a.status == b.status and a.correctors == b.correctors and a.passes == b.passes
+    assert list(a.mom_iters) == list(b.mom_iters) and a.p_iters == b.p_iters
+    assert (a.accepted, a.converged, a.linear_ok) == (b.accepted, b.converged, b.linear_ok)
+    for key in ("rU", "dU", "dP", "r_coupling_first", "r_coupling", "co_max", "co_mean"):
+        assert abs(getattr(a, key) - getattr(b, key)) <= 1e-8 * abs(getattr(b, key)) + 1e-14, key
+    # mass is a ratio of round-off-sized imbalances to the flux scale
+    assert abs(a.mass - b.mass) <= 1e-3 * abs(b.mass) + 1e-13
+
+
+CASES = {
+    "cavity_bdf2": lambda: (cases.cavity2d_mesh(24), pc.cavity_piso_config(), cases.cavity_bcs(True), [0.01] * 5),
+    "cavity_euler_variable": lambda: (cases.cavity2d_mesh(24), pc.cavity_piso_config(time_method=EULER),
+                                      cases.cavity_bcs(True), [0.01, 0.02, 0.005, 0.01]),
+    "cavity_bdf2_variable_3corr": lambda: (cases.cavity2d_mesh(20), pc.cavity_piso_config(n_correctors=3),
+                                           cases.cavity_bcs(True), [0.02, 0.01, 0.015, 0.03, 0.01]),
+    "polyhedral_c2": lambda: (cases.cavity3d_mesh(7, split=0.15, seed=3), pc.c2_piso_config(),
+                              cases.cavity_bcs(False), [0.02, 0.01, 0.015, 0.015]),
+    "polyhedral_c2_two_nonorth": lambda: (cases.cavity3d_mesh(6, split=0.2, seed=11),
+                                          pc.c2_piso_config(non_orth_corr=2, n_correctors=3),
+                                          cases.cavity_bcs(False), [0.05, 0.05, 0.05]),
+    "channel_inlet_outlet": lambda: (pc.channel_mesh(24, 8), pc.cavity_piso_config(), pc.channel_bcs(),
+                                     [0.05] * 5),
+    "outer_passes_relaxed": lambda: (cases.cavity2d_mesh(16),
+                                     pc.cavity_piso_config(max_iterations=3, alpha_u=0.8, tol_u=1e-3,
+                                                           tol_mom=1e-3, tol_p=1e-2),
+                                     cases.cavity_bcs(True), [0.01] * 3),
+    "warm_start": lambda: (cases.cavity2d_mesh(20),
+                           pc.cavity_piso_config(mom=solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14,
+                                                                   warm_start=1),
+                                                 pre=solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14,
+                                                                   warm_start=1)),
+                           cases.cavity_bcs(True), [0.01] * 4),
+}
+
+
+@pytest.mark.parametrize("kind", list(CASES))
+def test_piso_parity(kind):
+    mesh, cfg, bcs, dts = CASES[kind]()
+    so, sg = pair(mesh, cfg, bcs)
+    for dt in dts:
+        ro, rg = so.step(dt), sg.step(dt)
+        compare_step(rg, ro)
+        np.testing.assert_allclose(sg.temporal_flux(), so.temporal_flux(), rtol=1e-7,
+                                   atol=1e-9 * max(np.abs(so.temporal_flux()).max(), 1e-300))
+        if ro.status != STEP_COMPLETED:
+            break
+    Ug, pg, phig = sg.fields()
+    Uo, po, phio = so.fields()
+    assert rel_l2(Ug, Uo) < 1e-9
+    assert rel_l2(pg, po) < 1e-9
+    assert rel_l2(phig, phio) < 1e-9
+    for which in range(4):
+        hg, ho = sg.history(which), so.history(which)
+        assert len(hg) == len(ho)
+        np.testing.assert_allclose(hg, ho, rtol=1e-8, atol=1e-300)
+
+
+def test_piso_adaptive_correctors_parity():
+    """couplingTolerance below the first corrector's residual: the device takes
+    the same number of extra correctors as the oracle."""
+    mesh = cases.cavity2d_mesh(24)
+    base = pc.cavity_piso_config(n_correctors=1)
+    so, _ = pair(mesh, base, cases.cavity_bcs(True))
+    tol = 0.3 * so.step(0.01).r_coupling
+    cfg = pc.cavity_piso_config(n_correctors=1, coupling_tol=tol, max_correctors=20)
+    so, sg = pair(mesh, cfg, cases.cavity_bcs(True))
+    for k in range(3):
+        ro, rg = so.step(0.01), sg.step(0.01)
+        compare_step(rg, ro)
+        assert rg.r_coupling <= tol
+        if k == 0:
+            assert rg.correctors > 1
+
+
+def test_piso_coupling_rejection_on_device():
+    """tests/piso_coupling_test.py:43-71 on the device: the cases/cavity 64 x 64
+    step is rejected with one corrector at half its coupling residual and
+    accepted with up to 30 correctors."""
+    mesh = cases.cavity2d_mesh(64)
+    arrays, _, _ = mesh.prepare()
+    arrays = arrays.arrays()
+    ub, pb = cases.cavity_bcs(True)
+    ctx = Context(arrays, renumber=False)
+    s = Piso(ctx, pc.cavity_piso_config(n_correctors=1), ub, pb)
+    s.start()
+    tol = 0.5 * s.step(0.01).r_coupling
+    assert tol > 1e-10
+    for adaptive in (False, True):
+        s = Piso(ctx, pc.cavity_piso_config(n_correctors=1, coupling_tol=tol, max_correctors=30 if adaptive else 1),
+                 ub, pb)
+        s.start()
+        r = s.step(0.01)
+        if adaptive:
+            assert r.status == STEP_COMPLETED and r.accepted and r.correctors > 1
+            assert r.r_coupling <= tol and r.co_max > 0
+        else:
+            assert r.status == STEP_NOT_CONVERGED and not r.accepted and r.mass < 1e-8 and r.linear_ok
+
+
+@pytest.mark.parametrize("method", [EULER, BDF2])
+def test_piso_shear_decay_textbook_on_device(method):
+    ny, nu, amp = 32, 0.1, 1e-6
+    dts = [0.05, 0.025, 0.05, 0.04, 0.06]
+    mesh = pc.shear_mesh(ny)
+    ctx = Context(mesh, renumber=True)
+    s = Piso(ctx, pc.shear_config(nu, method), *pc.shear_bcs())
+    y = ctx.geometry()["C"][:, 1]
+    U0 = np.zeros((ctx.nc, 3))
+    U0[:, 0] = amp * np.sin(np.pi * y)
+    s.set_fields(U0, np.zeros(ctx.nc))
+    s.start()
+    for dt in dts:
+        assert s.step(dt).status == STEP_COMPLETED
+    U, p, _ = s.fields()
+    order = np.argsort(y)
+    expect = pc.textbook_steps(pc.shear_operator(ny, nu), U0[order, 0], dts, method)
+    assert np.abs(U[order, 0] - expect).max() < 1e-6 * amp
+    assert np.abs(p).max() == 0.0
+
+
+def test_piso_run_follows_time_stepper():
+    """Piso.run takes the TimeStepper's steps (src/io/case.cpp:391-403): the
+    last step is shortened to land on endTime."""
+    mesh = cases.cavity2d_mesh(12)
+    ctx = Context(mesh)
+    s = Piso(ctx, pc.cavity_piso_config(), *cases.cavity_bcs(True))
+    s.start()
+    reps = s.run(0.0, 0.025, 0.01)
+    assert [r.dt for r in reps] == pytest.approx([0.01, 0.01, 0.005], rel=1e-12)
+    assert all(r.status == STEP_COMPLETED for r in reps)
+    t = TimeStepper(0.0, 0.03, 0.01)
+    while not t.finished():
+        t.advance()
+    assert t.value == 0.03 and t.step == 3
+
+
+def test_piso_mixed_precision_momentum():
+    """FP32 lockstep BiCGSTAB inside FP64 refinement for the momentum solves
+    and FP32 AMG for pressure: the fields after five BDF2 steps agree with FP64
+    to 1e-6 relative L2."""
+    mesh = cases.cavity3d_mesh(8, split=0.1, seed=7)
+    out = []
+    for prec in (0, 1):
+        cfg = pc.c2_piso_config(mom=solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14, precision=prec),
+                                pre=solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14, precision=prec))
+        ctx = Context(mesh)
+        s = Piso(ctx, cfg, *cases.cavity_bcs(False))
+        s.start()
+        for _ in range(5):
+            assert s.step(0.02).status == STEP_COMPLETED
+        out.append(s.fields())
+    for a, b in zip(out[1], out[0]):
+        assert rel_l2(a, b) < 1e-6

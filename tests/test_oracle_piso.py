@@ -1,0 +1,228 @@
+"""Known answers and invariants that pin the oracle's PISO step
+(oracle/piso.cpp) independently of its own code: the reference's time-term
+test values, the textbook backward-difference formulas, time-integration
+order against the exact semi-discrete solution, the reference's PISO
+coupling-test behaviour, and conservation."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as orc
+import piso_cases as pc
+from babelsim_amd import BDF2, EULER, STEP_COMPLETED, STEP_NOT_CONVERGED, MeshData
+
+AMPLITUDE = 1e-6  # Stokes regime: convection is ~1e-7 of the time term
+
+
+# ----------------------------------------------------------------- time weights
+def test_time_weights_known_answers():
+    """procedural_equation_test.cpp:43-46: variable-step BDF2 with old = 1,
+    older = 0, dt = 0.5, previous dt = 1 and a unit source 3 gives x = 2.25."""
+    a0, a1, a2, wp, wo = orc.time_weights(BDF2, 2, 0.5, 1.0)
+    assert (a1 * 1.0 + a2 * 0.0 + 3.0) / a0 == pytest.approx(2.25, rel=1e-14)
+    # Euler, and BDF2 with fewer than two saved levels, use 1/dt twice
+    # (procedural_equation.cpp:204, 234).
+    for method, levels in ((EULER, 1), (EULER, 5), (BDF2, 1)):
+        np.testing.assert_array_equal(orc.time_weights(method, levels, 0.25, 0.5), [4.0, 4.0, 0.0, 4.0, 0.0])
+    # The ddtPhiCorr weights (piso/main.cpp:28-33) are the old-level weights.
+    for dt, dtp in ((0.1, 0.1), (0.05, 0.1), (0.3, 0.1)):
+        a0, a1, a2, wp, wo = orc.time_weights(BDF2, 3, dt, dtp)
+        assert wp == pytest.approx(a1, rel=1e-15) and wo == pytest.approx(a2, rel=1e-15)
+
+
+@pytest.mark.parametrize("dt,dtp", [(0.1, 0.1), (0.05, 0.1), (0.3, 0.07)])
+def test_bdf2_differentiates_quadratics_exactly(dt, dtp):
+    """The variable-step backward difference is exact for quadratics in t and
+    annihilates constants."""
+    a0, a1, a2, _, _ = orc.time_weights(BDF2, 2, dt, dtp)
+    t1 = 0.7
+    t0, tm = t1 - dt, t1 - dt - dtp
+    for f, df in ((lambda t: 1.0, 0.0), (lambda t: t, 1.0), (lambda t: 3 * t * t - t, 6 * t1 - 1)):
+        assert a0 * f(t1) - a1 * f(t0) - a2 * f(tm) == pytest.approx(df, abs=1e-11)
+
+
+def test_previous_dt_zero_means_equal_steps():
+    """procedural_equation.cpp:209: a previous dt of 0 is read as dt."""
+    np.testing.assert_array_equal(orc.time_weights(BDF2, 2, 0.2, 0.0)[:3], orc.time_weights(BDF2, 2, 0.2, 0.2)[:3])
+
+
+# ---------------------------------------------------------------- shear decay
+def shear_run(ny, nu, method, dts):
+    arrays = pc.shear_mesh(ny).arrays()
+    om = orc.oracle_mesh(arrays)
+    y = om.geometry()["C"][:, 1]
+    order = np.argsort(y)
+    ub, pb = pc.shear_bcs()
+    U0 = np.zeros((om.nc, 3))
+    U0[:, 0] = AMPLITUDE * np.sin(np.pi * y)
+    s = orc.OraclePiso(om, pc.shear_config(nu, method), ub, pb)
+    s.set_fields(U0, np.zeros(om.nc))
+    s.start()
+    reps = [s.step(dt) for dt in dts]
+    U, p, phi = s.fields()
+    return U, p, phi, reps, U0[order, 0], order
+
+
+@pytest.mark.parametrize("method,dts", [(EULER, [0.1] * 6), (BDF2, [0.1] * 6),
+                                        (BDF2, [0.1, 0.05, 0.05, 0.12, 0.03, 0.08])])
+def test_shear_decay_is_textbook_time_integration(method, dts):
+    """With no convection and no pressure gradient, each PISO step is one
+    implicit Euler / variable-step BDF2 step of the diffusion operator."""
+    ny, nu = 16, 0.1
+    U, p, phi, reps, u0, order = shear_run(ny, nu, method, dts)
+    expect = pc.textbook_steps(pc.shear_operator(ny, nu), u0, dts, method)
+    assert np.abs(U[order, 0] - expect).max() < 1e-6 * AMPLITUDE
+    assert np.abs(U[:, 1:]).max() == 0.0 and np.abs(p).max() == 0.0
+    assert all(r.status == STEP_COMPLETED and r.mass < 1e-8 for r in reps)
+
+
+@pytest.mark.parametrize("method,order_expected", [(EULER, 1), (BDF2, 2)])
+def test_shear_decay_time_order(method, order_expected):
+    """Halving dt divides the error against the exact semi-discrete solution
+    exp(L t) u0 by 2 (Euler) or 4 (BDF2)."""
+    ny, nu, T = 16, 0.1, 1.0
+    L = pc.shear_operator(ny, nu)
+    errors = []
+    for n in (10, 20, 40):
+        U, _, _, _, u0, order = shear_run(ny, nu, method, [T / n] * n)
+        errors.append(np.abs(U[order, 0] - pc.semi_discrete_exact(L, u0, T)).max())
+    rates = np.log2(np.array(errors[:-1]) / np.array(errors[1:]))
+    assert np.all(np.abs(rates - order_expected) < 0.12), rates
+
+
+# ------------------------------------------------------------- coupling test
+def cavity64():
+    arrays = cases.cavity2d_mesh(64).arrays()
+    return orc.oracle_mesh(arrays)
+
+
+def test_coupling_rejection_and_adaptive_correction():
+    """tests/piso_coupling_test.py:43-71 on one rank: with couplingTolerance at
+    half of the one-corrector coupling residual, a single corrector is
+    conservative (mass < 1e-8, linear solves converged) but not accepted, and
+    maxCorrectors 30 accepts after more than one corrector."""
+    om = cavity64()
+    ub, pb = cases.cavity_bcs(True)
+    s = orc.OraclePiso(om, pc.cavity_piso_config(n_correctors=1), ub, pb)
+    s.start()
+    base = s.step(0.01)
+    assert base.status == STEP_COMPLETED and base.correctors == 1
+    tol = 0.5 * base.r_coupling
+    assert tol > 1e-10
+    for adaptive in (False, True):
+        cfg = pc.cavity_piso_config(n_correctors=1, coupling_tol=tol, max_correctors=30 if adaptive else 1)
+        s = orc.OraclePiso(om, cfg, ub, pb)
+        s.start()
+        r = s.step(0.01)
+        if not adaptive:
+            assert r.status == STEP_NOT_CONVERGED and not r.accepted and not r.converged
+            assert r.mass < 1e-8 and r.linear_ok
+        else:
+            assert r.status == STEP_COMPLETED and r.accepted and r.correctors > 1
+            assert r.r_coupling <= tol and r.co_max > 0
+
+
+def test_cavity_steps_conserve_mass_and_reduce_coupling():
+    """Each corrector rebuilds HbyA from the corrected velocity: the coupling
+    residual falls from the first to the last corrector, and every step's
+    flux is conservative."""
+    om = cavity64()
+    ub, pb = cases.cavity_bcs(True)
+    s = orc.OraclePiso(om, pc.cavity_piso_config(n_correctors=3), ub, pb)
+    s.start()
+    for _ in range(3):
+        r = s.step(0.01)
+        assert r.status == STEP_COMPLETED and r.correctors == 3
+        assert r.mass < 1e-8
+        assert r.r_coupling < r.r_coupling_first
+        assert 0 < r.co_mean <= r.co_max
+
+
+# ---------------------------------------------------------- ddtPhiCorr term
+def test_temporal_flux_bookkeeping():
+    """temporalFluxCorrection (piso/main.cpp:19-53): zero at the first step
+    (phi = flux(U) initially), zero on boundary faces, and at step 3 equal to
+    rho c (wp e_prev + wo e_older) with e = phi - flux(U) of the saved levels
+    and the coupling c = max(0, 1 - |e_prev| / |phi_prev|)."""
+    rho = 1.3
+    arrays = cases.cavity2d_mesh(12).arrays()
+    om = orc.oracle_mesh(arrays)
+    ub, pb = cases.cavity_bcs(True)
+    s = orc.OraclePiso(om, pc.cavity_piso_config(rho=rho, mu=0.013), ub, pb)
+    s.start()
+    boundary = np.asarray(arrays["neighbour"]) < 0
+    levels = [s.fields()]
+    dts = [0.02, 0.01, 0.015]
+    tfs = []
+    for dt in dts:
+        s.step(dt)
+        tfs.append(s.temporal_flux())
+        levels.append(s.fields())
+    assert np.all(tfs[0] == 0.0)
+    assert all(np.all(t[boundary] == 0.0) for t in tfs)
+
+    def mismatch(k):
+        U, _, phi = levels[k]
+        return phi - om.flux(U, *ub, 0, 0)
+
+    dt, dtp = dts[2], dts[1]
+    r = dt / dtp
+    wp, wo = (1 + r) / dt, -r * r / ((1 + r) * dt)
+    ep, eo = mismatch(2), mismatch(1)
+    phip = levels[2][2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c = np.where(ep == 0, 1.0, np.where(phip == 0, 0.0, np.maximum(0.0, 1 - np.abs(ep) / np.abs(phip))))
+    expect = np.where(boundary, 0.0, rho * c * (wp * ep + wo * eo))
+    assert np.abs(eo[~boundary]).max() > 0 and np.abs(ep[~boundary]).max() > 0
+    np.testing.assert_allclose(tfs[2], expect, rtol=1e-12, atol=1e-14 * np.abs(expect).max())
+
+
+# ------------------------------------------------------------------ others
+def test_quiescent_fluid_stays_at_rest():
+    arrays = cases.cavity3d_mesh(4, jitter=0.2, split=0.2, seed=5).arrays()
+    om = orc.oracle_mesh(arrays)
+    ut = np.array([pc.FV] * 6, np.int32)
+    pt = np.array([pc.ZG] * 6, np.int32)
+    s = orc.OraclePiso(om, pc.c2_piso_config(), (ut, np.zeros((6, 3))), (pt, np.zeros(6)))
+    s.start()
+    for _ in range(3):
+        r = s.step(0.1)
+        assert r.status == STEP_COMPLETED
+    U, p, phi = s.fields()
+    assert np.abs(U).max() == 0 and np.abs(p).max() == 0 and np.abs(phi).max() == 0
+
+
+def test_invalid_controls_raise():
+    om = orc.oracle_mesh(MeshData.hex((2, 2, 1)).arrays())
+    ub, pb = cases.cavity_bcs(True)
+    for bad in (dict(alpha_u=0.0), dict(alpha_u=1.5), dict(time_method=0), dict(n_correctors=0),
+                dict(max_correctors=1, n_correctors=2), dict(coupling_tol=0.0)):
+        with pytest.raises(ValueError):
+            orc.OraclePiso(om, pc.cavity_piso_config(**bad), ub, pb)
+    s = orc.OraclePiso(om, pc.cavity_piso_config(), ub, pb)
+    s.start()
+    with pytest.raises(ValueError):
+        s.step(0.0)
+
+
+def test_time_stepper_follows_case_cpp():
+    """src/io/case.cpp:391-403: t_n = start + n dt in extended precision,
+    snapped to endTime within 32 eps, last step shortened, no step past the
+    end."""
+    from babelsim_amd.device import TimeStepper
+    t = TimeStepper(0.0, 0.3, 0.1)
+    dts = []
+    while not t.finished():
+        dts.append(t.advance())
+    assert t.value == 0.3 and len(dts) == 3
+    t = TimeStepper(1.0, 1.25, 0.1)
+    dts = []
+    while not t.finished():
+        dts.append(t.advance())
+    assert dts == pytest.approx([0.1, 0.1, 0.05], rel=1e-12) and t.value == 1.25
+    with pytest.raises(RuntimeError):
+        t.advance()
+    with pytest.raises(ValueError):
+        TimeStepper(0.0, 1.0, 0.0)
