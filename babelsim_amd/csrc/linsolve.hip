@@ -26,39 +26,55 @@ __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 
 // Row i (held in storage slot q) of A x: the diagonal first, then the
 // off-diagonal entries in row order (the operation order of the oracle's
-// spmv); padding entries are skipped.
-// Row i (held in storage slot q) of A x: the diagonal first, then the
-// off-diagonal entries in row order (the operation order of the oracle's
 // spmv); padding entries are skipped.  The sum runs in the storage type T:
-// FP64 on the FP64 path, FP32 in the FP32 AMG levels.
-template <class T> __device__ inline T rowDot(const CsrView<T>& A, const T* x, int q, int i) {
-    T y = A.diag[i] * x[i];
+// FP64 on the FP64 path, FP32 in the FP32 AMG levels.  XF supplies the
+// vector entries (a plain array, or a prolongated one computed on the fly).
+// All lanes of a wavefront hold slots of one slice (see RowRange), so the
+// slice offsets are read with scalar loads.
+template <class T> struct PlainX {
+    const T* x;
+    __device__ T operator()(int j) const { return x[j]; }
+};
+// x + alpha xc[agg] (the kProlong update) evaluated where it is read.
+template <class T> struct ProlongedX {
+    const T* x;
+    const int* agg;
+    const T* xc;
+    T alpha;
+    __device__ T operator()(int j) const { return x[j] + alpha * xc[agg[j]]; }
+};
+template <class T, class XF> __device__ inline T rowDotG(const CsrView<T>& A, const XF& xf, int q, int i) {
+    T y = A.diag[i] * xf(i);
     if (A.sliceOff) {
         // Entries are fetched kC levels at a time so that their column, value
         // and x loads overlap; the sum still runs in entry order.
-        const RowRange R(A.sliceOff, A.K, q);
+        const int* so = A.sliceOff + static_cast<long long>(__builtin_amdgcn_readfirstlane(q >> 6)) * (A.K + 1);
+        const int lane = q & 63;
         constexpr int kC = 8;
         for (int k0 = 0; k0 < A.K; k0 += kC) {
-            if (R.so[k0 + 1] == R.so[k0]) break;  // no row of the slice has more entries
+            if (so[k0 + 1] == so[k0]) break;  // no row of the slice has more entries
             int e[kC], c[kC];
             T v[kC], xv[kC];
 #pragma unroll
-            for (int j = 0; j < kC; ++j) e[j] = R.valid(k0 + j) ? R.at(k0 + j) : -1;
+            for (int j = 0; j < kC; ++j) e[j] = (k0 + j < A.K && lane < so[k0 + j + 1] - so[k0 + j]) ? so[k0 + j] + lane : -1;
 #pragma unroll
             for (int j = 0; j < kC; ++j) {
                 c[j] = e[j] >= 0 ? A.col[e[j]] : -1;
                 v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
             }
 #pragma unroll
-            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
+            for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? xf(c[j]) : T(0);
 #pragma unroll
             for (int j = 0; j < kC; ++j)
                 if (c[j] >= 0) y += v[j] * xv[j];
         }
     } else {
-        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * x[A.col[k]];
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * xf(A.col[k]);
     }
     return y;
+}
+template <class T> __device__ inline T rowDot(const CsrView<T>& A, const T* x, int q, int i) {
+    return rowDotG<T>(A, PlainX<T>{x}, q, i);
 }
 // Krylov kernels run on matrices whose slot q holds row q.
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
@@ -637,6 +653,145 @@ template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, 
     const int i = globalThread();
     if (i < n) x[i] = x[i] + static_cast<T>(alpha) * xc[agg[i]];
 }
+
+// Fused V-cycle steps.  Each computes, element by element, exactly what the
+// unfused kernel sequence computes, so a cycle gives the same bits either
+// way; they only save launches on the levels where launches, not bytes,
+// set the time.
+//
+// kRestrict, then the coarse level's first damped-Jacobi step (kJacobiFirst)
+// into xc.
+template <class T>
+__global__ void kRestrictFirst(int nc, const int* mp, const int* mi, const T* r, T* bc, const T* dc, T* xc,
+                               double omega) {
+    const int I = globalThread();
+    if (I >= nc) return;
+    T s = 0;
+    for (int k = mp[I]; k < mp[I + 1]; ++k) s += r[mi[k]];
+    bc[I] = s;
+    xc[I] = static_cast<T>(omega) * s / dc[I];
+}
+// kProlong followed by one kJacobiSweep: the prolongated vector is formed
+// where the sweep reads it and is not stored.
+template <class T>
+__global__ void kProlongSweep(CsrView<T> A, const int* agg, const T* xc, double alpha, const T* b, const T* x, T* xo,
+                              double omega) {
+    const int q = globalThread();
+    if (q >= A.n) return;
+    const int i = A.perm ? A.perm[q] : q;
+    const ProlongedX<T> xf{x, agg, xc, static_cast<T>(alpha)};
+    const T xi = xf(i);
+    const T ax = rowDotG<T>(A, xf, q, i);
+    xo[i] = xi + static_cast<T>(omega) * (b[i] - ax) / A.diag[i];
+}
+
+// The levels at the bottom of the hierarchy, run by one workgroup: a barrier
+// between the steps replaces a kernel launch.  Level k of the tail has the
+// matrix `A[k]`, the aggregation of its rows into level k+1 (none on the
+// last level, which is solved by the dense inverse) and its x (result), b,
+// r and t (second sweep vector) arrays.
+constexpr int kTailThreads = 1024;
+constexpr int kMaxTailLevels = 12;
+// A level joins the single-workgroup tail when its rows and stored entries
+// fit one pass of the workgroup's threads (one row each, about a cache
+// line of entries): beyond that one compute unit's bandwidth is slower than
+// the launches it saves.
+constexpr int kTailRows = kTailThreads;
+constexpr int kTailEntries = 8 * kTailThreads;
+// Prolongation and the first post-sweep run as one kernel on coarse levels
+// with thin rows (the fused sweep gathers agg and xc for every entry).
+constexpr int kFuseMaxRowLength = 16;
+template <class T> struct TailLevels {
+    int count;
+    CsrView<T> A[kMaxTailLevels];
+    const int* agg[kMaxTailLevels];
+    const int* memPtr[kMaxTailLevels];
+    const int* memIdx[kMaxTailLevels];
+    T* x[kMaxTailLevels];
+    T* b[kMaxTailLevels];
+    T* r[kMaxTailLevels];
+    T* t[kMaxTailLevels];
+    const double* inv;  // dense inverse of the last level, column-major
+    double omega, alpha;
+    int pre, post;
+};
+template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(TailLevels<T> L) {
+    const int tid = threadIdx.x;
+    const int later = L.pre - 1 + L.post;
+    T* cur[kMaxTailLevels];
+    T* nxt[kMaxTailLevels];
+    const int last = L.count - 1;
+    for (int k = 0; k < last; ++k) {  // descend: pre-smoothing, residual, restriction
+        const CsrView<T>& A = L.A[k];
+        const T* b = L.b[k];
+        T* c = later % 2 == 0 ? L.x[k] : L.t[k];
+        T* o = later % 2 == 0 ? L.t[k] : L.x[k];
+        for (int i = tid; i < A.n; i += kTailThreads) c[i] = static_cast<T>(L.omega) * b[i] / A.diag[i];
+        __syncthreads();
+        for (int s = 1; s < L.pre; ++s) {
+            for (int q = tid; q < A.n; q += kTailThreads) {
+                const int i = A.perm ? A.perm[q] : q;
+                const T ax = rowDot(A, c, q, i);
+                o[i] = c[i] + static_cast<T>(L.omega) * (b[i] - ax) / A.diag[i];
+            }
+            __syncthreads();
+            T* w = c; c = o; o = w;
+        }
+        for (int q = tid; q < A.n; q += kTailThreads) {
+            const int i = A.perm ? A.perm[q] : q;
+            L.r[k][i] = b[i] - rowDot(A, c, q, i);
+        }
+        __syncthreads();
+        const int nc = L.A[k + 1].n;
+        for (int I = tid; I < nc; I += kTailThreads) {
+            T s = 0;
+            for (int m = L.memPtr[k][I]; m < L.memPtr[k][I + 1]; ++m) s += L.r[k][L.memIdx[k][m]];
+            L.b[k + 1][I] = s;
+        }
+        __syncthreads();
+        cur[k] = c;
+        nxt[k] = o;
+    }
+    {  // the last level: x = inv b (the kDenseApply sum)
+        const int n = L.A[last].n;
+        const T* b = L.b[last];
+        for (int i = tid; i < n; i += kTailThreads) {
+            double s = 0.0;
+            int j = 0;
+            for (; j + 8 <= n; j += 8) {
+                double v[8], w[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    v[u] = L.inv[static_cast<long long>(j + u) * n + i];
+                    w[u] = static_cast<double>(b[j + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += v[u] * w[u];
+            }
+            for (; j < n; ++j) s += L.inv[static_cast<long long>(j) * n + i] * static_cast<double>(b[j]);
+            L.x[last][i] = static_cast<T>(s);
+        }
+        __syncthreads();
+    }
+    for (int k = last - 1; k >= 0; --k) {  // ascend: prolongation, post-smoothing
+        const CsrView<T>& A = L.A[k];
+        const T* b = L.b[k];
+        T* c = cur[k];
+        T* o = nxt[k];
+        const T* xc = L.x[k + 1];
+        for (int i = tid; i < A.n; i += kTailThreads) c[i] = c[i] + static_cast<T>(L.alpha) * xc[L.agg[k][i]];
+        __syncthreads();
+        for (int s = 0; s < L.post; ++s) {
+            for (int q = tid; q < A.n; q += kTailThreads) {
+                const int i = A.perm ? A.perm[q] : q;
+                const T ax = rowDot(A, c, q, i);
+                o[i] = c[i] + static_cast<T>(L.omega) * (b[i] - ax) / A.diag[i];
+            }
+            __syncthreads();
+            T* w = c; c = o; o = w;
+        }
+    }
+}
 __global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
                           double* cd, double* cv) {
     const int e = globalThread();
@@ -864,6 +1019,14 @@ void AmgDevice::setup(const CsrView<double>& A, const Pattern& P, const SolverCf
             L.push_back(std::move(lv));
         }
         hostAgg0 = H[0].agg;
+        // The tail: the coarsest level and the levels above it that are within
+        // kTailRows rows and kTailEntries stored entries (never level 0,
+        // whose cycle also forms z . r).
+        const int nl = static_cast<int>(L.size());
+        tailStart = nl - 1;
+        while (tailStart > 1 && L[tailStart - 1]->n <= kTailRows && L[tailStart - 1]->nnz <= kTailEntries) --tailStart;
+        if (nl - tailStart > kMaxTailLevels) tailStart = nl - kMaxTailLevels;
+        if (tailStart < 1) tailStart = nl - 1;
         const int nco = L.back()->n;
         lu.alloc(static_cast<std::size_t>(nco) * nco);
         piv.alloc(nco);
@@ -911,16 +1074,50 @@ std::vector<int> AmgDevice::sizes() const {
 
 std::vector<double> AmgDevice::coarseDiag(int level) const { return L.at(level)->diag.toHost(stream); }
 
+template <class T>
+void AmgDevice::levelArrays(const Level& lv, const T*& dg, const T*& vl, T*& x, T*& b, T*& r, T*& t) const {
+    if constexpr (std::is_same<T, float>::value) {
+        dg = lv.diag32.p; vl = lv.val32.p; x = lv.x32.p; b = lv.b32.p; r = lv.r32.p; t = lv.t32.p;
+    } else {
+        dg = lv.diag.p; vl = lv.val.p; x = lv.x.p; b = lv.b.p; r = lv.r.p; t = lv.t.p;
+    }
+}
+
+// The levels from tailStart to the coarsest in one launch of kCycleTail: the
+// right-hand side is in level tailStart's b array and the result goes to its
+// x array, as cycle(tailStart, b, x) would leave them.
+template <class T> void AmgDevice::tail() {
+    TailLevels<T> a{};
+    a.count = static_cast<int>(L.size()) - tailStart;
+    for (int k = 0; k < a.count; ++k) {
+        const Level& lv = *L[tailStart + k];
+        const T *dg, *vl;
+        levelArrays<T>(lv, dg, vl, a.x[k], a.b[k], a.r[k], a.t[k]);
+        a.A[k] = CsrView<T>{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K, lv.perm};
+        a.agg[k] = lv.agg.p;
+        a.memPtr[k] = lv.memPtr.p;
+        a.memIdx[k] = lv.memIdx.p;
+    }
+    a.inv = inv.p;
+    a.omega = cfg.amgOmega;
+    a.alpha = cfg.amgCorrection;
+    a.pre = cfg.amgPre;
+    a.post = cfg.amgPost;
+    kCycleTail<T><<<1, kTailThreads, 0, stream>>>(a);
+}
+
 // One V-cycle on level l: b is the right-hand side, and the result is left
 // in xOut.  Sweeps alternate between xOut and the level's spare vector; the
 // first write goes to whichever of the two makes the last one land in xOut.
-template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut) {
+// firstDone: the first damped-Jacobi step is already in that vector (the
+// parent's restriction wrote it).  On coarse levels with thin rows the
+// prolongation and the first post-sweep run as one kernel.
+template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut, bool firstDone) {
     Level& lv = *L[l];
     hipStream_t s = stream;
-    T* r = mixed ? reinterpret_cast<T*>(lv.r32.p) : reinterpret_cast<T*>(lv.r.p);
-    T* spare = mixed ? reinterpret_cast<T*>(lv.t32.p) : reinterpret_cast<T*>(lv.t.p);
-    const T* dg = mixed ? reinterpret_cast<const T*>(lv.diag32.p) : reinterpret_cast<const T*>(lv.diag.p);
-    const T* vl = mixed ? reinterpret_cast<const T*>(lv.val32.p) : reinterpret_cast<const T*>(lv.val.p);
+    const T *dg, *vl;
+    T *xl, *bl, *r, *spare;
+    levelArrays<T>(lv, dg, vl, xl, bl, r, spare);
     const CsrView<T> A{lv.n, lv.ptr, lv.col, dg, vl, lv.sliceOff, lv.K, lv.perm};
     const int nb = blocksFor(lv.n);
     if (l == static_cast<int>(L.size()) - 1) {
@@ -929,7 +1126,6 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut) {
     }
     const double w = cfg.amgOmega;
     const int pre = cfg.amgPre, post = cfg.amgPost;
-    // Every sweep after the first damped-Jacobi step writes the other vector.
     const int later = pre - 1 + post;
     T* cur = later % 2 == 0 ? xOut : spare;
     T* nxt = later % 2 == 0 ? spare : xOut;
@@ -940,16 +1136,31 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut) {
         if (pr) pr->end(s);
         std::swap(cur, nxt);
     };
-    kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, cur, w);
+    if (!firstDone) kJacobiFirst<T><<<nb, kBlock, 0, s>>>(lv.n, dg, b, cur, w);
     for (int k = 1; k < pre; ++k) sweep();
-    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, cur, r);
     Level& cl = *L[l + 1];
-    T* bc = mixed ? reinterpret_cast<T*>(cl.b32.p) : reinterpret_cast<T*>(cl.b.p);
-    T* xc = mixed ? reinterpret_cast<T*>(cl.x32.p) : reinterpret_cast<T*>(cl.x.p);
-    kRestrict<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, bc);
-    cycle<T>(l + 1, bc, xc);
-    kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, xc, cur, cfg.amgCorrection);
-    for (int k = 0; k < post; ++k) {
+    const T *cdg, *cvl;
+    T *cx, *cb, *cr, *ct;
+    levelArrays<T>(cl, cdg, cvl, cx, cb, cr, ct);
+    const bool childTail = l + 1 >= tailStart;
+    T* childCur = childTail ? nullptr : (later % 2 == 0 ? cx : ct);
+    const bool fuse = l >= 1 && lv.sliceOff && lv.K <= kFuseMaxRowLength;
+    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, cur, r);
+    if (childCur)
+        kRestrictFirst<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, cb, cdg, childCur, w);
+    else
+        kRestrict<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, cb);
+    if (childTail) tail<T>();
+    else cycle<T>(l + 1, cb, cx, true);
+    int k0 = 0;
+    if (fuse && post > 0) {
+        kProlongSweep<T><<<nb, kBlock, 0, s>>>(A, lv.agg.p, cx, cfg.amgCorrection, b, cur, nxt, w);
+        std::swap(cur, nxt);
+        k0 = 1;
+    } else {
+        kProlong<T><<<nb, kBlock, 0, s>>>(lv.n, lv.agg.p, cx, cur, cfg.amgCorrection);
+    }
+    for (int k = k0; k < post; ++k) {
         if constexpr (std::is_same<T, double>::value) {
             if (l == 0 && k == post - 1 && dotPartials) {  // last sweep also forms z . r
                 kJacobiSweepDot<<<reduceBlocks(lv.n), kBlock, 0, s>>>(A, b, cur, nxt, w, dotPartials);
@@ -969,12 +1180,12 @@ bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
     bool fused = false;
     if (mixed) {
         kToFloat<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b32.p);
-        cycle<float>(0, l0.b32.p, l0.x32.p);
+        cycle<float>(0, l0.b32.p, l0.x32.p, false);
         kToDouble<<<nb, kBlock, 0, s>>>(l0.n, l0.x32.p, z);
     } else {
         fused = zrPartials && L.size() > 1 && cfg.amgPost > 0;
         dotPartials = fused ? zrPartials : nullptr;
-        cycle<double>(0, r, z);
+        cycle<double>(0, r, z, false);
         dotPartials = nullptr;
     }
     BSIM_LAUNCH_CHECK();

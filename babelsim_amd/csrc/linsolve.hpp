@@ -195,7 +195,13 @@ private:
         DBuf<double> x, b, r, t;
         DBuf<float> x32, b32, r32, t32;
     };
-    template <class T> void cycle(int l, const T* b, T* xOut);
+    // Levels from tailStart down run in one single-workgroup kernel (they are
+    // small enough that launches, not bytes, would set their time).
+    template <class T> void cycle(int l, const T* b, T* xOut, bool firstDone);
+    template <class T> void tail();
+    template <class T> void levelArrays(const Level& lv, const T*& diag, const T*& val, T*& x, T*& b, T*& r,
+                                        T*& t) const;
+    int tailStart = 0;
     hipStream_t stream;
     std::vector<std::unique_ptr<Level>> L;
     DBuf<double> lu, inv;
