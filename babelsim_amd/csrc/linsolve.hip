@@ -955,6 +955,7 @@ double KernelProbe::stop(int& launches) {
 // ------------------------------------------------------------------ AMG
 void AmgDevice::setup(const CsrView<double>& A, const Pattern& P, const SolverCfg& c) {
     const auto t0 = Clock::now();
+    dropGraphs();  // the cycle's settings and arrays may change below
     cfg = c;
     mixed = c.precision == 1;
     hipStream_t s = stream;
@@ -1173,24 +1174,60 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut, bool firstD
     if (cur != xOut) throw std::logic_error("V-cycle result left outside its output vector");
 }
 
-bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
+void AmgDevice::run(const double* r, double* z, double* zrPartials) {
     Level& l0 = *L[0];
     hipStream_t s = stream;
     const int nb = blocksFor(l0.n);
-    bool fused = false;
     if (mixed) {
         kToFloat<<<nb, kBlock, 0, s>>>(l0.n, r, l0.b32.p);
         cycle<float>(0, l0.b32.p, l0.x32.p, false);
         kToDouble<<<nb, kBlock, 0, s>>>(l0.n, l0.x32.p, z);
     } else {
-        fused = zrPartials && L.size() > 1 && cfg.amgPost > 0;
-        dotPartials = fused ? zrPartials : nullptr;
+        dotPartials = zrPartials;
         cycle<double>(0, r, z, false);
         dotPartials = nullptr;
     }
     BSIM_LAUNCH_CHECK();
+}
+
+bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
+    const bool fused = !mixed && zrPartials && L.size() > 1 && cfg.amgPost > 0;
+    double* partials = fused ? zrPartials : nullptr;
+    if (probe && probe->on) {  // events bracket single kernels only in launches from the stream
+        run(r, z, partials);
+        return fused;
+    }
+    const GraphKey key{r, z, partials};
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+        // The cycle is launches only (no copies, fills or host waits), all
+        // with arguments fixed by the hierarchy and the key.
+        BSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            run(r, z, partials);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BSIM_HIP(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t ge = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BSIM_HIP(e);
+        it = graphs.emplace(key, ge).first;
+    }
+    BSIM_HIP(hipGraphLaunch(it->second, stream));
     return fused;
 }
+
+void AmgDevice::dropGraphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+}
+
+AmgDevice::~AmgDevice() { dropGraphs(); }
 
 // ------------------------------------------------------------- systems
 DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)

@@ -8,6 +8,7 @@
 #include "amg_host.hpp"
 #include "device_common.hpp"
 
+#include <map>
 #include <memory>
 
 namespace bsim {
@@ -168,6 +169,9 @@ private:
 class AmgDevice {
 public:
     AmgDevice(hipStream_t s) : stream(s) {}
+    ~AmgDevice();
+    AmgDevice(const AmgDevice&) = delete;
+    AmgDevice& operator=(const AmgDevice&) = delete;
     void setup(const CsrView<double>& A, const Pattern& pattern, const SolverCfg& cfg);
     // z = M^{-1} r (FP64 in/out).  With zrPartials, the FP64 cycle's last
     // fine-level sweep also stores the per-block partials of z . r there and
@@ -202,6 +206,19 @@ private:
     template <class T> void levelArrays(const Level& lv, const T*& diag, const T*& val, T*& x, T*& b, T*& r,
                                         T*& t) const;
     int tailStart = 0;
+    // One V-cycle's launches run from the stream directly or, when no probe
+    // is timing kernels, as a captured graph replayed per application.
+    void run(const double* r, double* z, double* zrPartials);
+    struct GraphKey {
+        const double* r;
+        const double* z;
+        const double* partials;
+        bool operator<(const GraphKey& o) const {
+            return r != o.r ? r < o.r : z != o.z ? z < o.z : partials < o.partials;
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    void dropGraphs();
     hipStream_t stream;
     std::vector<std::unique_ptr<Level>> L;
     DBuf<double> lu, inv;
