@@ -114,6 +114,11 @@ def piso_config(**kw) -> PisoConfig:
     return PisoConfig(mom=mom, pre=pre, **base)
 
 
+def set_graph_replay(enable: bool) -> None:
+    """AMG V-cycles as captured-graph replays (default) or stream launches."""
+    check(lib().bsim_set_graph_replay(int(bool(enable))))
+
+
 def ptr(a: np.ndarray | None, kind=dp):
     if a is None:
         return None
@@ -158,6 +163,7 @@ def _declare(L: C.CDLL) -> None:
 vp = C.c_void_p
 u64p = C.POINTER(C.c_uint64)
 _DEVICE_SIGNATURES: dict = {
+    "bsim_set_graph_replay": [C.c_int],
     "bsim_ctx_create": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "bsim_ctx_sizes": [vp, ip, ip, ip, ip, ip],
     "bsim_ctx_geometry": [vp] + [dp] * 10,

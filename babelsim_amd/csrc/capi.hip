@@ -76,6 +76,10 @@ void lduOut(Context& C, const std::vector<double>& dg, const std::vector<double>
 
 extern "C" {
 
+int bsim_set_graph_replay(int enable) {
+    return guarded([&] { setGraphReplay(enable != 0); });
+}
+
 int bsim_ctx_create(const bsim_mesh* mesh, int renumber, int device, bsim_ctx** out) {
     return guarded([&] {
         if (!mesh) throw std::invalid_argument("mesh is null");

@@ -7,6 +7,7 @@
 #include "linsolve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <type_traits>
@@ -1190,10 +1191,17 @@ void AmgDevice::run(const double* r, double* z, double* zrPartials) {
     BSIM_LAUNCH_CHECK();
 }
 
+namespace {
+std::atomic<bool> g_graphReplay{true};
+}
+void setGraphReplay(bool on) { g_graphReplay = on; }
+bool graphReplay() { return g_graphReplay; }
+
 bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
     const bool fused = !mixed && zrPartials && L.size() > 1 && cfg.amgPost > 0;
     double* partials = fused ? zrPartials : nullptr;
-    if (probe && probe->on) {  // events bracket single kernels only in launches from the stream
+    // Events bracket single kernels only in launches from the stream.
+    if ((probe && probe->on) || !graphReplay()) {
         run(r, z, partials);
         return fused;
     }
@@ -1214,16 +1222,19 @@ bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
         BSIM_HIP(hipStreamEndCapture(stream, &g));
         hipGraphExec_t ge = nullptr;
         const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) (void)hipGraphDestroy(g);
         BSIM_HIP(e);
-        it = graphs.emplace(key, ge).first;
+        it = graphs.emplace(key, Captured{g, ge}).first;  // the graph lives as long as its executable
     }
-    BSIM_HIP(hipGraphLaunch(it->second, stream));
+    BSIM_HIP(hipGraphLaunch(it->second.exec, stream));
     return fused;
 }
 
 void AmgDevice::dropGraphs() {
-    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : graphs) {
+        (void)hipGraphExecDestroy(kv.second.exec);
+        (void)hipGraphDestroy(kv.second.graph);
+    }
     graphs.clear();
 }
 

@@ -81,6 +81,10 @@ struct SystemCounters {
     double setupSeconds = 0, solveSeconds = 0;
 };
 
+// Process-wide switch between graph replay and stream launches of AMG cycles.
+void setGraphReplay(bool on);
+bool graphReplay();
+
 class DeviceSystem {
 public:
     DeviceSystem(hipStream_t stream, Pattern pattern);
@@ -217,7 +221,11 @@ private:
             return r != o.r ? r < o.r : z != o.z ? z < o.z : partials < o.partials;
         }
     };
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    struct Captured {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::map<GraphKey, Captured> graphs;
     void dropGraphs();
     hipStream_t stream;
     std::vector<std::unique_ptr<Level>> L;

@@ -81,6 +81,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=3, help="timed oracle iterations for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rebuild-interval", type=int, default=8)
+    ap.add_argument("--stream-launches", action="store_true",
+                    help="launch the AMG cycles kernel by kernel instead of replaying captured graphs "
+                         "(same results; for tracing tools)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device-to-device copies (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -107,6 +110,9 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.stream_launches:
+        from babelsim_amd import set_graph_replay
+        set_graph_replay(False)
     if args.workload == "cylinder":
         return bench_cylinder(args)
     import cases

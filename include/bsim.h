@@ -20,6 +20,11 @@ extern "C" {
 /* ---------------------------------------------------------------- errors */
 const char* bsim_last_error(void);
 int bsim_version(void);
+/* No counterpart in the reference: whether AMG V-cycles replay a captured HIP
+ * graph (1, the default) or launch their kernels one by one from the stream
+ * (0).  Both give the same bits; stream launches are for tools that trace
+ * single kernels.  Applies to every context of the process. */
+int bsim_set_graph_replay(int enable);
 
 /* ------------------------------------------------------- host mesh lists
  * A polyhedral face list exactly as Mesh::polyhedral consumes it

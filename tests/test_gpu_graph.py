@@ -57,3 +57,24 @@ def test_piso_graph_replay_matches_stream_launches():
     np.testing.assert_array_equal(ha, hb)
     for a, b in zip(fa, fb):
         np.testing.assert_array_equal(a, b)
+
+
+def test_stream_launch_switch_gives_same_bits():
+    """bsim_set_graph_replay(0) launches the cycles from the stream."""
+    from babelsim_amd import set_graph_replay
+    mesh = cases.cavity2d_mesh(24)
+    out = []
+    try:
+        for replay in (True, False):
+            set_graph_replay(replay)
+            ctx = Context(mesh)
+            s = Simple(ctx, cases.c1_config(), *cases.cavity_bcs(True))
+            s.start()
+            s.iterate(5)
+            out.append((s.fields(), s.history(3)))
+    finally:
+        set_graph_replay(True)
+    (fa, ha), (fb, hb) = out
+    np.testing.assert_array_equal(ha, hb)
+    for a, b in zip(fa, fb):
+        np.testing.assert_array_equal(a, b)
