@@ -153,6 +153,7 @@ def _declare(L: C.CDLL) -> None:
     L.bsim_mesh_arrays.restype = None
     L.bsim_mesh_free.argtypes = [C.c_void_p]
     L.bsim_mesh_free.restype = None
+
     for name, args in _DEVICE_SIGNATURES.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -164,6 +165,13 @@ vp = C.c_void_p
 u64p = C.POINTER(C.c_uint64)
 _DEVICE_SIGNATURES: dict = {
     "bsim_set_graph_replay": [C.c_int],
+    "bsim_mesh_read": [C.c_char_p, C.POINTER(vp)],
+    "bsim_mesh_write": [vp, C.c_char_p],
+    "bsim_mesh_patch": [vp, C.c_int, C.c_char_p, C.c_int, ip],
+    "bsim_write_field_csv": [C.c_char_p, C.c_int, C.c_char_p, C.c_int, ip, dp, C.c_int, dp],
+    "bsim_write_result_metadata": [C.c_char_p, C.c_int, C.c_int, vp, ip, C.c_longlong, C.c_char_p, C.c_int,
+                                   C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), ip],
+    "bsim_read_field_values": [C.c_char_p, C.c_char_p, C.c_int, C.c_int, dp],
     "bsim_ctx_create": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "bsim_ctx_sizes": [vp, ip, ip, ip, ip, ip],
     "bsim_ctx_geometry": [vp] + [dp] * 10,
@@ -273,3 +281,59 @@ class MeshData:
         h = C.c_void_p()
         check(lib().bsim_mesh_prepare(self.handle, int(renumber), C.byref(h), ptr(cperm, ip), ptr(fperm, ip)))
         return MeshData(h.value), cperm, fperm
+
+    @classmethod
+    def read(cls, path) -> "MeshData":
+        """A BABELSIM_MESH 3 file (src/io/mesh_reader.cpp:44-111)."""
+        h = C.c_void_p()
+        check(lib().bsim_mesh_read(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    def write(self, path) -> None:
+        check(lib().bsim_mesh_write(self.handle, str(path).encode()))
+
+    def patches(self) -> list:
+        """[(name, kind)] with kind one of PATCH_KINDS."""
+        out = []
+        buf = C.create_string_buffer(256)
+        for k in range(self.sizes()[3]):
+            kind = C.c_int()
+            check(lib().bsim_mesh_patch(self.handle, k, buf, len(buf), C.byref(kind)))
+            out.append((buf.value.decode(), PATCH_KINDS[kind.value]))
+        return out
+
+
+PATCH_KINDS = ("generic", "wall", "inlet", "outlet", "symmetry", "processor")
+
+
+def write_field_csv(time_dir, name, values, centres, global_ids=None, rank=0) -> None:
+    """<time_dir>/rank-NNNN/<name>.csv (src/parallel/parallel_writer.cpp:42-75)."""
+    v = np.ascontiguousarray(values, np.float64)
+    n = v.shape[0]
+    ncomp = 1 if v.ndim == 1 else int(np.prod(v.shape[1:]))
+    c = np.ascontiguousarray(centres, np.float64).reshape(n, 3)
+    g = None if global_ids is None else np.ascontiguousarray(global_ids, np.int32)
+    check(lib().bsim_write_field_csv(str(time_dir).encode(), int(rank), name.encode(), n, ptr(g, ip), ptr(c), ncomp,
+                                     ptr(v)))
+
+
+def write_result_metadata(time_dir, mesh: "MeshData", time_name, fields, global_ids=None, rank=0, ranks=1,
+                          global_cells=None) -> None:
+    """metadata.bs and mesh.geometry (parallel_writer.cpp:116-169); fields are
+    (name, type, location) with location "cell", "face" or "vertex"."""
+    locs = {"cell": 0, "face": 1, "vertex": 2}
+    names = (C.c_char_p * len(fields))(*[f[0].encode() for f in fields])
+    types = (C.c_char_p * len(fields))(*[f[1].encode() for f in fields])
+    where = np.array([locs[f[2]] for f in fields], np.int32)
+    g = None if global_ids is None else np.ascontiguousarray(global_ids, np.int32)
+    nc = mesh.sizes()[4]
+    check(lib().bsim_write_result_metadata(str(time_dir).encode(), int(rank), int(ranks), mesh.handle, ptr(g, ip),
+                                           int(nc if global_cells is None else global_cells),
+                                           str(time_name).encode(), len(fields), names, types, ptr(where, ip)))
+
+
+def read_field_values(path, ncells, ncomp=1, name="") -> np.ndarray:
+    """Nonuniform cell values by global id (src/io/field_reader.cpp:143-241)."""
+    out = np.zeros(ncells * ncomp)
+    check(lib().bsim_read_field_values(str(path).encode(), name.encode(), int(ncells), int(ncomp), ptr(out)))
+    return out if ncomp == 1 else out.reshape(ncells, ncomp)

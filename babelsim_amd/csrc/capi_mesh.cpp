@@ -1,8 +1,10 @@
 // C ABI: host mesh lists (generation, wrapping, renumbering).
 #include "../../include/bsim.h"
 #include "errors.hpp"
+#include "io_host.hpp"
 #include "mesh_host.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 struct bsim_mesh {
@@ -77,5 +79,69 @@ void bsim_mesh_arrays(const bsim_mesh* m, double* vertices, int* fpo, int* fpi, 
 }
 
 void bsim_mesh_free(bsim_mesh* m) { delete m; }
+
+int bsim_mesh_read(const char* path, bsim_mesh** out) {
+    return bsim::guarded([&] {
+        if (!path || !out) throw std::invalid_argument("null argument");
+        auto* m = new bsim_mesh;
+        try { m->d = bsim::readMeshFile(path); }
+        catch (...) { delete m; throw; }
+        *out = m;
+    });
+}
+
+int bsim_mesh_write(const bsim_mesh* m, const char* path) {
+    return bsim::guarded([&] {
+        if (!m || !path) throw std::invalid_argument("null argument");
+        bsim::writeMeshFile(m->d, path);
+    });
+}
+
+int bsim_mesh_patch(const bsim_mesh* m, int patch, char* name, int cap, int* kind) {
+    return bsim::guarded([&] {
+        if (!m || patch < 0 || patch >= m->d.npatch) throw std::invalid_argument("patch index out of range");
+        const bool named = patch < static_cast<int>(m->d.patchName.size());
+        const std::string n = named ? m->d.patchName[patch] : "patch" + std::to_string(patch);
+        if (name && cap > 0) {
+            const std::size_t k = std::min<std::size_t>(n.size(), static_cast<std::size_t>(cap - 1));
+            std::memcpy(name, n.data(), k);
+            name[k] = '\0';
+        }
+        if (kind) *kind = named ? m->d.patchKind[patch] : bsim::kPatchGeneric;
+    });
+}
+
+int bsim_write_field_csv(const char* time_dir, int rank, const char* name, int ncells, const int* global_ids,
+                         const double* centres, int ncomp, const double* values) {
+    return bsim::guarded([&] {
+        if (!time_dir || !name || !centres || !values || ncells < 0 || rank < 0)
+            throw std::invalid_argument("invalid field output arguments");
+        bsim::writeFieldCsv(time_dir, rank, name, ncells, global_ids, centres, ncomp, values);
+    });
+}
+
+int bsim_write_result_metadata(const char* time_dir, int rank, int ranks, const bsim_mesh* m, const int* global_ids,
+                               long long global_cells, const char* time_name, int nfields, const char* const* names,
+                               const char* const* types, const int* locations) {
+    return bsim::guarded([&] {
+        if (!time_dir || !m || !time_name || nfields <= 0 || !names || !types || !locations || rank < 0 ||
+            ranks <= rank)
+            throw std::invalid_argument("invalid result metadata arguments");
+        std::vector<std::string> n, t;
+        std::vector<int> l(locations, locations + nfields);
+        for (int k = 0; k < nfields; ++k) {
+            n.emplace_back(names[k] ? names[k] : "");
+            t.emplace_back(types[k] ? types[k] : "");
+        }
+        bsim::writeResultMetadata(time_dir, rank, ranks, m->d, global_ids, global_cells, time_name, n, t, l);
+    });
+}
+
+int bsim_read_field_values(const char* path, const char* name, int ncells, int ncomp, double* out) {
+    return bsim::guarded([&] {
+        if (!path || !out) throw std::invalid_argument("null argument");
+        bsim::readFieldValues(path, name ? name : "", ncells, ncomp, out);
+    });
+}
 
 }  // extern "C"

@@ -341,6 +341,8 @@ MeshDesc prepare(const MeshDesc& in, bool renumber, std::vector<int>& cellNewToO
     MeshDesc out;
     out.vert = in.vert;
     out.npatch = in.npatch;
+    out.patchName = in.patchName;
+    out.patchKind = in.patchKind;
     out.fpo.push_back(0);
     faceNewToOld = idx;
     for (int nfc = 0; nfc < nf; ++nfc) {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace bsim {
@@ -10,11 +11,16 @@ namespace bsim {
 // A polyhedral mesh as BabelSim's Mesh::polyhedral consumes it
 // (include/babelsim/mesh.h:116-121, PolyhedralFaceSpec): vertex rings, owner,
 // neighbour (-1 on boundaries) and patch (-1 on interior faces).
+// Patch roles of the mesh file (include/babelsim/mesh.h:79-86).
+enum PatchKindId : int { kPatchGeneric = 0, kPatchWall, kPatchInlet, kPatchOutlet, kPatchSymmetry, kPatchProcessor };
+
 struct MeshDesc {
     std::vector<double> vert;     // 3 * nv
     std::vector<int> fpo, fpi;    // face ring CSR
     std::vector<int> own, nei, patch;
     int npatch = 0;
+    std::vector<std::string> patchName;  // empty unless read from a file
+    std::vector<int> patchKind;
     int ncells() const;
     int nfaces() const { return static_cast<int>(own.size()); }
 };

@@ -48,6 +48,35 @@ void bsim_mesh_arrays(const bsim_mesh* m, double* vertices, int* face_point_offs
                       int* owner, int* neighbour, int* patch);
 void bsim_mesh_free(bsim_mesh* m);
 
+/* ------------------------------------------------------------ file formats
+ * BABELSIM_MESH 3 text: readMeshFile (src/io/mesh_reader.cpp:44-111), with its
+ * topology checks; the writer emits the same format.  Patch roles are the
+ * file's kinds (include/babelsim/mesh.h:79-86; "mirror" reads as symmetry). */
+enum { BSIM_PATCH_GENERIC = 0, BSIM_PATCH_WALL = 1, BSIM_PATCH_INLET = 2, BSIM_PATCH_OUTLET = 3,
+       BSIM_PATCH_SYMMETRY = 4, BSIM_PATCH_PROCESSOR = 5 };
+int bsim_mesh_read(const char* path, bsim_mesh** out);
+int bsim_mesh_write(const bsim_mesh* m, const char* path);
+/* Name (NUL-terminated, truncated to cap) and role of a patch; meshes not read
+ * from a file report "patch<k>" and BSIM_PATCH_GENERIC. */
+int bsim_mesh_patch(const bsim_mesh* m, int patch, char* name, int cap, int* kind);
+/* Result output of one rank (src/parallel/parallel_writer.cpp:14-75):
+ * <time_dir>/rank-NNNN/<name>.csv with the header global_id,x,y,z,value0..
+ * and 17 significant digits.  global_ids may be NULL (ids 0..n-1, the
+ * caller's cell order); centres are 3 per cell; ncomp is 1, 3 or 9. */
+int bsim_write_field_csv(const char* time_dir, int rank, const char* name, int ncells, const int* global_ids,
+                         const double* centres, int ncomp, const double* values);
+/* metadata.bs and mesh.geometry of one rank (parallel_writer.cpp:116-169);
+ * field types are the reference's words (scalar, vector, tensor) and
+ * locations 0 cell, 1 face, 2 vertex. */
+int bsim_write_result_metadata(const char* time_dir, int rank, int ranks, const bsim_mesh* m, const int* global_ids,
+                               long long global_cells, const char* time_name, int nfields, const char* const* names,
+                               const char* const* types, const int* locations);
+/* Nonuniform cell values for a restart (src/io/field_reader.cpp:143-241):
+ * a directory of rank-NNNN subdirectories holding <name>.csv, one .csv file, or a text file of
+ * "id value..." rows with # comments.  Every id in [0, ncells) exactly once;
+ * out holds ncomp values per cell in id order. */
+int bsim_read_field_values(const char* path, const char* name, int ncells, int ncomp, double* out);
+
 /* ------------------------------------------------------ configuration */
 enum { BSIM_FIXED_VALUE = 0, BSIM_FIXED_GRADIENT = 1, BSIM_ZERO_GRADIENT = 2, BSIM_INLET_OUTLET = 3,
        BSIM_SYMMETRY = 4 };                                   /* field.h:24-33 BoundaryType */
