@@ -3,11 +3,11 @@
 The Python layer only loads ``libbsim.so`` and forwards plain arrays to it.
 """
 from ._capi import (BDF2, EULER, STEP_COMPLETED, STEP_NOT_CONVERGED, STEP_NUMERICAL_FAILURE,  # noqa: F401
-                    IterReport, MeshData, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, check,
-                    lib, piso_config, read_field_values, set_graph_replay, simple_config, solver_config,
+                    IterReport, KOmegaConfig, MeshData, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, check,
+                    komega_config, lib, piso_config, read_field_values, set_graph_replay, simple_config, solver_config,
                     write_field_csv, write_result_metadata)
 
 __all__ = ["MeshData", "SolverConfig", "SolveResult", "SimpleConfig", "IterReport", "PisoConfig", "StepReport",
            "lib", "check", "simple_config", "solver_config", "piso_config", "EULER", "BDF2", "STEP_COMPLETED",
            "STEP_NOT_CONVERGED", "STEP_NUMERICAL_FAILURE", "set_graph_replay", "read_field_values",
-           "write_field_csv", "write_result_metadata"]
+           "write_field_csv", "write_result_metadata", "KOmegaConfig", "komega_config"]

@@ -60,7 +60,35 @@ class SimpleConfig(C.Structure):
 class IterReport(C.Structure):
     _fields_ = [("rU", C.c_double), ("dU", C.c_double), ("dP", C.c_double), ("mass", C.c_double),
                 ("linP", C.c_double), ("mom_iters", C.c_int * 3), ("p_iters", C.c_int),
-                ("mom_status", C.c_int), ("p_status", C.c_int), ("converged", C.c_int), ("referenced", C.c_int)]
+                ("mom_status", C.c_int), ("p_status", C.c_int), ("converged", C.c_int), ("referenced", C.c_int),
+                ("d_turb", C.c_double), ("r_turb", C.c_double), ("bounded_omega", C.c_double),
+                ("k_iters", C.c_int), ("w_iters", C.c_int), ("turb_linear", C.c_int), ("turb_converged", C.c_int)]
+
+
+class KOmegaConfig(C.Structure):
+    """Wilcox k-omega constants, bounds and controls (src/physics/RANS/k_omega.cpp:26-48) and the
+    kTransport / omegaTransport equation methods and solvers."""
+
+    _fields_ = [("beta_star", C.c_double), ("beta", C.c_double), ("gamma", C.c_double), ("sigma_k", C.c_double),
+                ("sigma_omega", C.c_double), ("k_min", C.c_double), ("omega_min", C.c_double),
+                ("viscosity_ratio_max", C.c_double), ("relaxation", C.c_double), ("tolerance", C.c_double),
+                ("k_interp", C.c_int), ("k_grad", C.c_int), ("k_conv", C.c_int), ("k_diff", C.c_int),
+                ("k_coef_interp", C.c_int), ("k_coef_grad", C.c_int),
+                ("w_interp", C.c_int), ("w_grad", C.c_int), ("w_conv", C.c_int), ("w_diff", C.c_int),
+                ("w_coef_interp", C.c_int), ("w_coef_grad", C.c_int),
+                ("k_solver", SolverConfig), ("w_solver", SolverConfig)]
+
+
+def komega_config(**kw) -> KOmegaConfig:
+    """Defaults of the reference's KOmega constructor (k_omega.cpp:33-48)."""
+    base = dict(beta_star=0.09, beta=3.0 / 40.0, gamma=5.0 / 9.0, sigma_k=0.5, sigma_omega=0.5, k_min=1e-12,
+                omega_min=1e-12, viscosity_ratio_max=1e5, relaxation=0.7, tolerance=1e-6,
+                k_interp=0, k_grad=0, k_conv=1, k_diff=0, k_coef_interp=0, k_coef_grad=0,
+                w_interp=0, w_grad=0, w_conv=1, w_diff=0, w_coef_interp=0, w_coef_grad=0)
+    ks = kw.pop("k_solver", None) or solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
+    ws = kw.pop("w_solver", None) or solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
+    base.update(kw)
+    return KOmegaConfig(k_solver=ks, w_solver=ws, **base)
 
 
 def simple_config(**kw) -> SimpleConfig:

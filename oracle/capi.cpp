@@ -257,6 +257,126 @@ int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag,
     });
 }
 
+namespace {
+SimpleConfig simpleConfig(const orc_simple_config* cfg) {
+    SimpleConfig c;
+    c.rho = cfg->rho; c.mu = cfg->mu;
+    c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
+    c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
+    c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
+    c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
+    c.alphaU = cfg->alpha_u; c.alphaP = cfg->alpha_p; c.nonOrthCorr = cfg->non_orth_corr;
+    c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
+    c.mom = toConfig(&cfg->mom); c.pre = toConfig(&cfg->pre);
+    return c;
+}
+KOmegaConfig komegaConfig(const orc_komega_config* k) {
+    KOmegaConfig c;
+    c.betaStar = k->beta_star; c.beta = k->beta; c.gamma = k->gamma; c.sigmaK = k->sigma_k;
+    c.sigmaOmega = k->sigma_omega; c.kMin = k->k_min; c.omegaMin = k->omega_min;
+    c.viscosityRatioMax = k->viscosity_ratio_max; c.relaxation = k->relaxation; c.tolerance = k->tolerance;
+    c.kOpts = {k->k_interp, k->k_grad, k->k_conv, k->k_diff, k->k_coef_interp, k->k_coef_grad};
+    c.wOpts = {k->w_interp, k->w_grad, k->w_conv, k->w_diff, k->w_coef_interp, k->w_coef_grad};
+    c.kSolver = toConfig(&k->k_solver); c.wSolver = toConfig(&k->w_solver);
+    return c;
+}
+std::vector<BC<double>> scalarBcs(const Mesh& m, const int* t, const double* v) {
+    std::vector<BC<double>> b(m.np);
+    for (Index p = 0; p < m.np; ++p) { b[p].type = t[p]; b[p].value = v[p]; }
+    return b;
+}
+}  // namespace
+
+void* orc_simple_new_komega(void* mesh, const orc_simple_config* cfg, const orc_komega_config* kw, const int* ubct,
+                            const double* ubcv, const int* pbct, const double* pbcv, const int* kbct,
+                            const double* kbcv, const int* wbct, const double* wbcv) {
+    Simple* s = nullptr;
+    guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        SimpleConfig c = simpleConfig(cfg);
+        c.turbulence = 1;
+        c.kw = komegaConfig(kw);
+        std::unique_ptr<Simple> owned(new Simple(m, c));
+        std::vector<BC<V3>> ub(m.np);
+        for (Index p = 0; p < m.np; ++p) { ub[p].type = ubct[p]; ub[p].value = {ubcv[3 * p], ubcv[3 * p + 1], ubcv[3 * p + 2]}; }
+        owned->setBoundaries(ub, scalarBcs(m, pbct, pbcv));
+        owned->turb->setBoundaries(scalarBcs(m, kbct, kbcv), scalarBcs(m, wbct, wbcv));
+        s = owned.release();
+    });
+    return s;
+}
+int orc_simple_set_turbulence(void* sp, const double* k, const double* omega) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        if (!s.turb) throw std::invalid_argument("laminar SIMPLE has no turbulence fields");
+        for (Index i = 0; i < s.m.nc; ++i) {
+            if (k) s.turb->k.v[i] = k[i];
+            if (omega) s.turb->omega.v[i] = omega[i];
+        }
+    });
+}
+int orc_simple_get_turbulence(void* sp, double* k, double* omega, double* mut) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        if (!s.turb) throw std::invalid_argument("laminar SIMPLE has no turbulence fields");
+        if (k) store(s.turb->k.v, k);
+        if (omega) store(s.turb->omega.v, omega);
+        if (mut) store(s.turb->mut, mut);
+    });
+}
+int orc_komega_transport(void* mesh, const orc_komega_config* kw, double rho, double mu, int m_grad,
+                         const double* U, const int* ubct, const double* ubcv, const double* phi,
+                         const double* k0, const int* kbct, const double* kbcv, const double* w0, const int* wbct,
+                         const double* wbcv, double* k, double* omega, double* mut, double* out4) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        KOmega model(m, komegaConfig(kw), rho, mu);
+        model.setBoundaries(scalarBcs(m, kbct, kbcv), scalarBcs(m, wbct, wbcv));
+        model.k.v.assign(k0, k0 + m.nc);
+        model.omega.v.assign(w0, w0 + m.nc);
+        const std::vector<double> ph(phi, phi + m.nf);
+        model.start(ph);
+        VField Uf = makeField<V3>(m, U, ubct, ubcv, phi);
+        const TransportReport r = model.solveTransport(Uf, ph, m_grad);
+        if (!r.healthy) throw std::runtime_error("transport solve failed");
+        store(model.k.v, k);
+        store(model.omega.v, omega);
+        store(model.mut, mut);
+        out4[0] = r.rOmega; out4[1] = r.rK; out4[2] = r.dOmega; out4[3] = r.dK;
+    });
+}
+int orc_strain_squared(const double* g9, double* out) {
+    return guard([&] {
+        T3 g;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) g.r[i][j] = g9[3 * i + j];
+        *out = strainSquared(g);
+    });
+}
+int orc_stress_divergence(void* mesh, const double* U, const int* bct, const double* bcv, const double* ctx,
+                          const double* coef, const double* coef_trace, int interp, int gmethod, double* out) {
+    return guard([&] {
+        // coef (cell values and boundary traces) * (transpose(grad U) - 2/3 tr(grad U) I), then div.
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        VField Uf = makeField<V3>(m, U, bct, bcv, ctx);
+        std::vector<T3> g, gt;
+        gradient(Uf, g, gmethod);
+        gradientTracesVec(Uf, g, gt);
+        auto stress = [](double mu_, const T3& G) {
+            const double tr = G.r[0][0] + G.r[1][1] + G.r[2][2];
+            T3 s;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) s.r[i][j] = mu_ * (G.r[j][i] - (2.0 / 3.0) * (i == j ? tr : 0.0));
+            return s;
+        };
+        std::vector<T3> T(m.nc), Tt(m.nf);
+        for (Index i = 0; i < m.nc; ++i) T[i] = stress(coef[i], g[i]);
+        for (Index f = 0; f < m.nf; ++f) if (m.boundary(f)) Tt[f] = stress(coef_trace[f], gt[f]);
+        std::vector<V3> d;
+        tensorDivergence(m, T, Tt, interp, gmethod, d);
+        store(d, out);
+    });
+}
+
 void* orc_simple_new(void* mesh, const orc_simple_config* cfg, const int* ubct, const double* ubcv,
                      const int* pbct, const double* pbcv) {
     Simple* s = nullptr;
@@ -300,6 +420,9 @@ int orc_simple_iterate(void* sp, int n, orc_iter_report* reports) {
             for (int c = 0; c < 3; ++c) o.mom_iters[c] = r.momIters[c];
             o.p_iters = r.pIters; o.mom_status = r.momStatus; o.p_status = r.pStatus;
             o.converged = r.converged; o.referenced = r.referenced;
+            o.d_turb = r.dTurb; o.r_turb = r.rTurb; o.bounded_omega = r.boundedOmega;
+            o.k_iters = r.kIters; o.w_iters = r.wIters; o.turb_linear = r.turbLinear;
+            o.turb_converged = r.turbConverged;
         }
     });
 }
@@ -314,7 +437,9 @@ int orc_simple_get_fields(void* sp, double* U, double* p, double* phi) {
 int orc_simple_history(void* sp, int which, double* out, int cap, int* len) {
     return guard([&] {
         Simple& s = *static_cast<Simple*>(sp);
-        const std::vector<double>& h = which < 3 ? s.momHist[which] : s.pHist;
+        if (which > 3 && !s.turb) throw std::invalid_argument("laminar SIMPLE has no turbulence histories");
+        const std::vector<double>& h = which < 3 ? s.momHist[which] : which == 3 ? s.pHist
+                                     : which == 4 ? s.turb->kHist : s.turb->wHist;
         *len = static_cast<int>(h.size());
         std::memcpy(out, h.data(), sizeof(double) * std::min<int>(cap, static_cast<int>(h.size())));
     });

@@ -37,7 +37,17 @@ typedef struct {
 typedef struct {
     double rU, dU, dP, mass, linP;
     int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
+    double d_turb, r_turb, bounded_omega;
+    int k_iters, w_iters, turb_linear, turb_converged;
 } orc_iter_report;
+
+typedef struct {
+    double beta_star, beta, gamma, sigma_k, sigma_omega, k_min, omega_min, viscosity_ratio_max, relaxation,
+        tolerance;
+    int k_interp, k_grad, k_conv, k_diff, k_coef_interp, k_coef_grad;
+    int w_interp, w_grad, w_conv, w_diff, w_coef_interp, w_coef_grad;
+    orc_solver_config k_solver, w_solver;
+} orc_komega_config;
 
 const char* orc_last_error(void);
 /* OpenMP threads the oracle's parallel loops use. */
@@ -87,6 +97,23 @@ int orc_simple_start(void* s);
 int orc_simple_iterate(void* s, int n, orc_iter_report* reports);
 int orc_simple_get_fields(void* s, double* U, double* p, double* phi);
 int orc_simple_history(void* s, int which, double* out, int cap, int* len);
+/* Wilcox k-omega for the SIMPLE driver: k and omega boundary conditions and
+ * initial values; before start(). */
+void* orc_simple_new_komega(void* mesh, const orc_simple_config* cfg, const orc_komega_config* kw, const int* ubct,
+                            const double* ubcv, const int* pbct, const double* pbcv, const int* kbct,
+                            const double* kbcv, const int* wbct, const double* wbcv);
+int orc_simple_set_turbulence(void* s, const double* k, const double* omega);
+int orc_simple_get_turbulence(void* s, double* k, double* omega, double* mut);
+/* One solveTransport of a fresh model on given U, phi, k, omega (the set-up
+ * of tests/rans_equations_test.cpp:17-48); out4 = rOmega, rK, dOmega, dK. */
+int orc_komega_transport(void* mesh, const orc_komega_config* kw, double rho, double mu, int m_grad,
+                         const double* U, const int* ubct, const double* ubcv, const double* phi,
+                         const double* k0, const int* kbct, const double* kbcv, const double* w0, const int* wbct,
+                         const double* wbcv, double* k, double* omega, double* mut, double* out4);
+/* Operator-level pieces of the model, for known answers. */
+int orc_strain_squared(const double* g9, double* out);
+int orc_stress_divergence(void* mesh, const double* U, const int* bct, const double* bcv, const double* ctx,
+                          const double* coef, const double* coef_trace, int interp, int gmethod, double* out);
 
 typedef struct {
     double rho, mu;

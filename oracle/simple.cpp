@@ -86,6 +86,8 @@ bool referenceIfUnanchored(const Mesh& m, Ldu<double>& q, double value) {
 Simple::Simple(const Mesh& mesh, const SimpleConfig& cfg) : m(mesh), c(cfg) {
     U = VField(m); p = SField(m); muEff = SField(m); pPrime = SField(m);
     muEff.v.assign(m.nc, c.mu);  // model.cpp:58-63: constant cell field, default (zeroGradient) patches
+    if (c.turbulence == 1) turb = std::make_unique<KOmega>(m, c.kw, c.rho, c.mu);
+    else if (c.turbulence != 0) throw std::invalid_argument("unsupported turbulence model");
 }
 
 void Simple::setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<double>>& pbc) {
@@ -98,8 +100,11 @@ void Simple::setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<
     }
 }
 
-void Simple::start() {  // main.cpp:41: phi = math::flux(U, momentum options)
-    flux(U, phi, c.mInterp, c.mGrad);
+void Simple::start() {
+    // The model is built before phi exists (main.cpp:33, 41): its first
+    // bounding sees a zero face flux.
+    if (turb) turb->start(std::vector<double>(m.nf, 0.0));
+    flux(U, phi, c.mInterp, c.mGrad);  // main.cpp:41: phi = math::flux(U, momentum options)
 }
 
 void Simple::solveSystem(LinearSystem& sys, const Ldu<double>& q, const SolverConfig& sc, std::vector<double>& x,
@@ -122,7 +127,8 @@ IterReport Simple::iterate() {
     addConvection(mom, phi, U, c.mConv, c.mInterp, c.mGrad, c.rho);
     {
         std::vector<double> kf;
-        interpolate(muEff, kf, c.mCoefInterp, c.mCoefGrad);
+        if (turb) interpolate(turb->effectiveViscosity(), kf, c.mCoefInterp, c.mCoefGrad);
+        else interpolate(muEff, kf, c.mCoefInterp, c.mCoefGrad);
         Ldu<V3> contrib; contrib.init(m);
         addDiffusion<V3>(contrib, kf, U, c.mGrad, c.mDiff, nullptr);
         accumulate(mom, contrib, -(-1.0));
@@ -131,6 +137,11 @@ IterReport Simple::iterate() {
     gradient(p, gradP, c.pGrad);
 #pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) mom.source[i] += (1.0 * m.V[i]) * (-1.0 * gradP[i]);
+    if (turb) {  // main.cpp:57-59: the explicit remainder of the eddy-viscosity stress
+        std::vector<V3> divT;
+        turb->stressDivergence(U, c.mInterp, c.mGrad, divT);
+        for (Index i = 0; i < m.nc; ++i) mom.source[i] += (1.0 * m.V[i]) * divT[i];
+    }
     // main.cpp:60-66
     rep.rU = relativeResidual(m, mom, U.v);
     const double alphaU = c.alphaU;
@@ -253,6 +264,20 @@ IterReport Simple::iterate() {
     phi.resize(m.nf);
 #pragma omp parallel for schedule(static)
     for (Index f = 0; f < m.nf; ++f) phi[f] = phiH[f] + pressureFlux[f];
+    // Turbulence transport (main.cpp:100-110).
+    bool turbulenceConverged = true;
+    if (turb) {
+        const TransportReport t = turb->solveTransport(U, phi, c.mGrad);
+        rep.kIters = t.itK;
+        rep.wIters = t.itOmega;
+        rep.boundedOmega = t.boundedOmegaCells;
+        if (!t.healthy) throw std::runtime_error("turbulence transport failed");
+        rep.dTurb = std::max(t.dOmega, t.dK);
+        rep.rTurb = std::max(t.rOmega, t.rK);
+        rep.turbLinear = t.statusK == Converged && t.statusOmega == Converged;
+        turbulenceConverged = rep.turbLinear && rep.dTurb <= c.kw.tolerance && rep.rTurb <= c.kw.tolerance;
+        rep.turbConverged = turbulenceConverged;
+    }
     // Convergence (main.cpp:112-126; fvm_execution.cpp:91-177).
     {
         double d2 = 0, c2 = 0;
@@ -275,7 +300,8 @@ IterReport Simple::iterate() {
         rep.mass = l1 / std::max(scale, 1e-30);
     }
     const bool linearConverged = rep.momStatus == Converged && pConverged;
-    rep.converged = linearConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP && rep.mass <= c.tolMass;
+    rep.converged = linearConverged && turbulenceConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU &&
+                    rep.dP <= c.tolP && rep.mass <= c.tolMass;
     return rep;
 }
 

@@ -2,6 +2,9 @@
 #pragma once
 
 #include "internal.hpp"
+#include "rans.hpp"
+
+#include <memory>
 
 namespace orc {
 
@@ -15,6 +18,8 @@ struct SimpleConfig {
     int nonOrthCorr = 1;
     double tolMass = 1e-8, tolU = 1e-7, tolMom = 1e-6, tolP = 1e-6;
     SolverConfig mom, pre;
+    int turbulence = 0;  // 0 laminar, 1 Wilcox k-omega
+    KOmegaConfig kw;
 };
 
 struct IterReport {
@@ -24,6 +29,8 @@ struct IterReport {
     int momStatus = 0, pStatus = 0;
     int converged = 0;
     int referenced = 0;
+    double dTurb = 0, rTurb = 0, boundedOmega = 0;
+    int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1;
 };
 
 template <class T> void accumulate(Ldu<T>& a, const Ldu<T>& b, double f);
@@ -40,6 +47,7 @@ struct Simple {
     std::vector<double> phi;
     LinearSystem momSys, pSys;
     std::vector<double> momHist[3], pHist;
+    std::unique_ptr<KOmega> turb;
     Simple(const Mesh& mesh, const SimpleConfig& cfg);
     void setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<double>>& pbc);
     void start();

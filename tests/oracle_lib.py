@@ -7,7 +7,8 @@ from pathlib import Path
 
 import numpy as np
 
-from babelsim_amd._capi import IterReport, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, ip, ptr
+from babelsim_amd._capi import (IterReport, KOmegaConfig, PisoConfig, SimpleConfig, SolveResult, SolverConfig,
+                                StepReport, ip, ptr)
 
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE = ROOT / "oracle" / "liboracle.so"
@@ -68,6 +69,15 @@ def oracle():
         L.orc_simple_iterate.argtypes = [vp, C.c_int, C.POINTER(IterReport)]
         L.orc_simple_get_fields.argtypes = [vp, dp, dp, dp]
         L.orc_simple_history.argtypes = [vp, C.c_int, dp, C.c_int, ip]
+        L.orc_simple_new_komega.restype = vp
+        L.orc_simple_new_komega.argtypes = [vp, C.POINTER(SimpleConfig), C.POINTER(KOmegaConfig), ip, dp, ip, dp,
+                                            ip, dp, ip, dp]
+        L.orc_simple_set_turbulence.argtypes = [vp, dp, dp]
+        L.orc_simple_get_turbulence.argtypes = [vp, dp, dp, dp]
+        L.orc_strain_squared.argtypes = [dp, dp]
+        L.orc_stress_divergence.argtypes = [vp, dp, ip, dp, dp, dp, dp, C.c_int, C.c_int, dp]
+        L.orc_komega_transport.argtypes = [vp, C.POINTER(KOmegaConfig), C.c_double, C.c_double, C.c_int, dp, ip, dp,
+                                           dp, dp, ip, dp, dp, ip, dp, dp, dp, dp, dp]
         L.orc_piso_new.restype = vp
         L.orc_piso_new.argtypes = [vp, C.POINTER(PisoConfig), ip, dp, ip, dp]
         L.orc_piso_free.argtypes = [vp]
@@ -272,6 +282,70 @@ class OracleSimple:
         out = np.zeros(200000)
         self._call("orc_simple_history", which, ptr(out), len(out), C.byref(n))
         return out[:n.value].copy()
+
+
+def _bc_arrays(bc, ncomp):
+    t, v = bc
+    return np.ascontiguousarray(t, np.int32), np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1))
+
+
+class OracleSimpleKOmega(OracleSimple):
+    """The oracle's SIMPLE driver with the Wilcox k-omega model."""
+
+    def __init__(self, mesh: CpuMesh, cfg: SimpleConfig, kw: KOmegaConfig, u_bc, p_bc, k_bc, w_bc):
+        L = oracle()
+        self.m = mesh
+        ut, uv = _bc_arrays(u_bc, 3)
+        pt, pv = _bc_arrays(p_bc, 1)
+        kt, kv = _bc_arrays(k_bc, 1)
+        wt, wv = _bc_arrays(w_bc, 1)
+        h = L.orc_simple_new_komega(mesh.h, C.byref(cfg), C.byref(kw), ptr(ut, ip), ptr(uv), ptr(pt, ip), ptr(pv),
+                                    ptr(kt, ip), ptr(kv), ptr(wt, ip), ptr(wv))
+        if not h:
+            raise ValueError(L.orc_last_error().decode())
+        self.h = vp(h)
+
+    def set_turbulence(self, k=None, omega=None):
+        self._call("orc_simple_set_turbulence", ptr(None if k is None else np.ascontiguousarray(k, np.float64)),
+                   ptr(None if omega is None else np.ascontiguousarray(omega, np.float64)))
+
+    def turbulence(self):
+        k, w, mut = np.zeros(self.m.nc), np.zeros(self.m.nc), np.zeros(self.m.nc)
+        self._call("orc_simple_get_turbulence", ptr(k), ptr(w), ptr(mut))
+        return k, w, mut
+
+
+def strain_squared(g):
+    out = C.c_double()
+    g9 = np.ascontiguousarray(g, np.float64).reshape(9)
+    if oracle().orc_strain_squared(ptr(g9), C.byref(out)):
+        raise ValueError(oracle().orc_last_error().decode())
+    return out.value
+
+
+def stress_divergence(mesh: CpuMesh, U, u_bc, coef, coef_trace, interp=0, gmethod=0, ctx=None):
+    ut, uv = _bc_arrays(u_bc, 3)
+    out = np.zeros((mesh.nc, 3))
+    c = None if ctx is None else np.ascontiguousarray(ctx, np.float64)
+    if oracle().orc_stress_divergence(mesh.h, ptr(np.ascontiguousarray(U, np.float64)), ptr(ut, ip), ptr(uv), ptr(c),
+                                      ptr(np.ascontiguousarray(coef, np.float64)),
+                                      ptr(np.ascontiguousarray(coef_trace, np.float64)), interp, gmethod, ptr(out)):
+        raise ValueError(oracle().orc_last_error().decode())
+    return out
+
+
+def komega_transport(mesh: CpuMesh, kw: KOmegaConfig, rho, mu, m_grad, U, u_bc, phi, k0, k_bc, w0, w_bc):
+    ut, uv = _bc_arrays(u_bc, 3)
+    kt, kv = _bc_arrays(k_bc, 1)
+    wt, wv = _bc_arrays(w_bc, 1)
+    k, w, mut, rep = np.zeros(mesh.nc), np.zeros(mesh.nc), np.zeros(mesh.nc), np.zeros(4)
+    if oracle().orc_komega_transport(mesh.h, C.byref(kw), rho, mu, m_grad, ptr(np.ascontiguousarray(U, np.float64)),
+                                     ptr(ut, ip), ptr(uv), ptr(np.ascontiguousarray(phi, np.float64)),
+                                     ptr(np.ascontiguousarray(k0, np.float64)), ptr(kt, ip), ptr(kv),
+                                     ptr(np.ascontiguousarray(w0, np.float64)), ptr(wt, ip), ptr(wv), ptr(k), ptr(w),
+                                     ptr(mut), ptr(rep)):
+        raise ValueError(oracle().orc_last_error().decode())
+    return k, w, mut, rep
 
 
 class OraclePiso:
