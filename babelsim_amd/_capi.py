@@ -223,6 +223,9 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_amg_hierarchy": [C.c_int, C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp],
     "bsim_simple_create": [vp, C.POINTER(SimpleConfig), ip, dp, ip, dp, C.POINTER(vp)],
     "bsim_simple_set_fields": [vp, dp, dp],
+    "bsim_simple_enable_komega": [vp, C.POINTER(KOmegaConfig), ip, dp, ip, dp],
+    "bsim_simple_set_turbulence": [vp, dp, dp],
+    "bsim_simple_get_turbulence": [vp, dp, dp, dp],
     "bsim_simple_start": [vp],
     "bsim_simple_iterate": [vp, C.c_int, C.POINTER(IterReport)],
     "bsim_simple_get_fields": [vp, dp, dp, dp],

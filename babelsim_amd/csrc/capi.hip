@@ -464,6 +464,33 @@ int bsim_simple_create(bsim_ctx* ctx, const bsim_simple_config* cfg, const int* 
 int bsim_simple_set_fields(bsim_simple* s, const double* U, const double* p) {
     return guarded([&] { s->s->setFields(U, p); });
 }
+int bsim_simple_enable_komega(bsim_simple* s, const bsim_komega_config* k, const int* kbct, const double* kbcv,
+                              const int* wbct, const double* wbcv) {
+    return guarded([&] {
+        if (!k || !kbct || !kbcv || !wbct || !wbcv) throw std::invalid_argument("null argument");
+        KOmegaCfg c;
+        c.betaStar = k->beta_star; c.beta = k->beta; c.gamma = k->gamma; c.sigmaK = k->sigma_k;
+        c.sigmaOmega = k->sigma_omega; c.kMin = k->k_min; c.omegaMin = k->omega_min;
+        c.viscosityRatioMax = k->viscosity_ratio_max; c.relaxation = k->relaxation; c.tolerance = k->tolerance;
+        c.kOpts = {k->k_interp, k->k_grad, k->k_conv, k->k_diff, k->k_coef_interp, k->k_coef_grad};
+        c.wOpts = {k->w_interp, k->w_grad, k->w_conv, k->w_diff, k->w_coef_interp, k->w_coef_grad};
+        c.kSolver = toSolverCfg(&k->k_solver);
+        c.wSolver = toSolverCfg(&k->w_solver);
+        s->s->enableKOmega(c, kbct, kbcv, wbct, wbcv);
+    });
+}
+int bsim_simple_set_turbulence(bsim_simple* s, const double* k, const double* omega) {
+    return guarded([&] {
+        if (!s->s->turbulence()) throw std::invalid_argument("laminar SIMPLE has no turbulence fields");
+        s->s->turbulence()->setFields(k, omega);
+    });
+}
+int bsim_simple_get_turbulence(bsim_simple* s, double* k, double* omega, double* mut) {
+    return guarded([&] {
+        if (!s->s->turbulence()) throw std::invalid_argument("laminar SIMPLE has no turbulence fields");
+        s->s->turbulence()->getFields(k, omega, mut);
+    });
+}
 int bsim_simple_start(bsim_simple* s) { return guarded([&] { s->s->start(); }); }
 int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports) {
     return guarded([&] {
@@ -474,6 +501,9 @@ int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports) {
             for (int c = 0; c < 3; ++c) o.mom_iters[c] = r.momIters[c];
             o.p_iters = r.pIters; o.mom_status = r.momStatus; o.p_status = r.pStatus;
             o.converged = r.converged; o.referenced = r.referenced;
+            o.d_turb = r.dTurb; o.r_turb = r.rTurb; o.bounded_omega = r.boundedOmega;
+            o.k_iters = r.kIters; o.w_iters = r.wIters; o.turb_linear = r.turbLinear;
+            o.turb_converged = r.turbConverged;
         }
     });
 }
@@ -482,8 +512,10 @@ int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi) {
 }
 int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* len) {
     return guarded([&] {
-        if (which < 0 || which > 3) throw std::invalid_argument("history index is 0..3");
-        const std::vector<double>& h = which < 3 ? s->s->momHist[which] : s->s->pHist;
+        if (which < 0 || which > 5) throw std::invalid_argument("history index is 0..5");
+        if (which > 3 && !s->s->turbulence()) throw std::invalid_argument("laminar SIMPLE has no turbulence histories");
+        const std::vector<double>& h = which < 3 ? s->s->momHist[which] : which == 3 ? s->s->pHist
+                                     : which == 4 ? s->s->turbulence()->kHist : s->s->turbulence()->wHist;
         *len = static_cast<int>(h.size());
         if (out) std::memcpy(out, h.data(), sizeof(double) * std::min<std::size_t>(cap, h.size()));
     });

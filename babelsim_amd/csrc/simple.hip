@@ -148,6 +148,12 @@ SimpleSolver::SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, 
     C.checkErrors();
 }
 
+void SimpleSolver::enableKOmega(const KOmegaCfg& kw, const int* kbct, const double* kbcv, const int* wbct,
+                                const double* wbcv) {
+    turb = std::make_unique<KOmegaDevice>(C, kw, c.rho, c.mu, kbct, kbcv, wbct, wbcv);
+    started = false;
+}
+
 FieldArg SimpleSolver::fieldU(const double* ctx) const { return {U.p, uType.p, uVal.p, ctx, nullptr}; }
 FieldArg SimpleSolver::fieldP() const { return {p.p, pType.p, pVal.p, nullptr, nullptr}; }
 FieldArg SimpleSolver::fieldPP() const { return {pPrime.p, ppType.p, ppVal.p, nullptr, nullptr}; }
@@ -179,6 +185,7 @@ void SimpleSolver::start() {
     const DeviceMesh& M = *C.mesh;
     const MeshView mv = M.view();
     hipStream_t s = C.stream;
+    if (turb) turb->start();  // the model exists before phi (main.cpp:33, 41)
     ops::seedFluxContext(mv, U.p, ctxSeed.p, s);
     const double* g = nullptr;
     if (c.mInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, fieldU(ctxSeed.p), c.mGrad, gradU.p, work9.p, C.err.p, s); g = gradU.p; }
@@ -207,12 +214,20 @@ IterRep SimpleSolver::iterate() {
     kFillD<<<blocksFor(3LL * nc), kBlock, 0, s>>>(3LL * nc, src.p, 0.0);
     const FieldArg Uf = fieldU(phi.p);
     const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
-                           c.mDiff != kOrthogonal;
+                           c.mDiff != kOrthogonal || turb;
     if (needGradU) ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s);
     ops::assembleConvection(mv, 3, Uf, phi.p, c.mConv, c.mInterp, gradU.p, c.rho, diag, off, src.p, C.err.p, s);
+    if (turb) {
+        // laplacian(muEff) with the current eddy viscosity (main.cpp:53)
+        const FieldArg mf{turb->muEff(), zgType.p, zgVal.p, nullptr, turb->muEffTrace()};
+        const double* gm = nullptr;
+        if (c.mCoefInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 1, mf, c.mCoefGrad, gradPP.p, work9.p, C.err.p, s); gm = gradPP.p; }
+        ops::interpolate(mv, 1, mf, c.mCoefInterp, gm, kfMom.p, C.err.p, s);
+    }
     ops::assembleDiffusion(mv, 3, Uf, kfMom.p, gradU.p, c.mDiff, -(-1.0), diag, off, src.p, C.err.p, s);
     ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
     kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
+    if (turb) turb->addStressSource(Uf, gradU.p, c.mInterp, c.mGrad, src.p);  // main.cpp:57-59
     // rU before relaxation (main.cpp:60)
     kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, part);
     kReduceTail<3><<<1, kTail, 0, s>>>(reduceBlocks(nc), part, StoreSums<3>{red.p});
@@ -330,6 +345,20 @@ IterRep SimpleSolver::iterate() {
     kCorrectUP<<<nb, kBlock, 0, s>>>(mv, p.p, pPrime.p, c.alphaP, U.p, rAU.p, gradPP.p);
     BSIM_LAUNCH_CHECK();
     timing.correction += since(t0);
+    // Turbulence transport (main.cpp:100-110).
+    bool turbulenceConverged = true;
+    if (turb) {
+        const TransportRep t = turb->solveTransport(fieldU(phi.p), phi.p, c.mGrad);
+        rep.kIters = t.itK;
+        rep.wIters = t.itOmega;
+        rep.boundedOmega = t.boundedOmega;
+        if (!t.healthy) throw std::runtime_error("turbulence transport failed");
+        rep.dTurb = std::max(t.dOmega, t.dK);
+        rep.rTurb = std::max(t.rOmega, t.rK);
+        rep.turbLinear = t.statusK == kConverged && t.statusOmega == kConverged;
+        turbulenceConverged = rep.turbLinear && rep.dTurb <= turb->tolerance() && rep.rTurb <= turb->tolerance();
+        rep.turbConverged = turbulenceConverged;
+    }
     t0 = Clock::now();
     // Convergence measures (main.cpp:112-126).
     kConvergence<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uprev.p, pPrime.p, p.p, phi.p, part);
@@ -342,7 +371,8 @@ IterRep SimpleSolver::iterate() {
     rep.dP = std::sqrt(h[2]) / std::max(std::sqrt(h[3]), 1e-30);
     rep.mass = h[4] / std::max(h[5], 1e-30);
     const bool linearConverged = rep.momStatus == kConverged && pConverged;
-    rep.converged = linearConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP && rep.mass <= c.tolMass;
+    rep.converged = linearConverged && turbulenceConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU &&
+                    rep.dP <= c.tolP && rep.mass <= c.tolMass;
     timing.convergence += since(t0);
     ++timing.iterations;
     return rep;

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "context.hpp"
+#include "komega.hpp"
 
 namespace bsim {
 
@@ -22,6 +23,8 @@ struct IterRep {
     double rU = 0, dU = 0, dP = 0, mass = 0, linP = 0;
     int momIters[3] = {0, 0, 0};
     int pIters = 0, momStatus = 0, pStatus = 0, converged = 0, referenced = 0;
+    double dTurb = 0, rTurb = 0, boundedOmega = 0;
+    int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1;
 };
 
 // Per-phase wall time (seconds) accumulated over iterations.
@@ -35,6 +38,9 @@ class SimpleSolver {
 public:
     SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, const double* ubcv, const int* pbct,
                  const double* pbcv);
+    // Wilcox k-omega (src/physics/RANS/model.cpp:35-40); before start().
+    void enableKOmega(const KOmegaCfg& kw, const int* kbct, const double* kbcv, const int* wbct, const double* wbcv);
+    KOmegaDevice* turbulence() { return turb.get(); }
     void setFields(const double* Ucaller, const double* pcaller);
     void getFields(double* Ucaller, double* pcaller, double* phicaller);
     void start();
@@ -59,6 +65,7 @@ private:
         kfMom, mu, kfP, divPhiH, pPrime, corr, full, src, psrc, ctxSeed, red;
     DBuf<double> bcomp[3], xcomp[3];  // momentum components, one array each
     std::unique_ptr<DeviceSystem> momSys, pSys;
+    std::unique_ptr<KOmegaDevice> turb;
     bool started = false;
 };
 

@@ -220,6 +220,22 @@ class Simple:
         check(lib().bsim_simple_set_fields(self.h, ptr(None if U is None else _f64(U)),
                                            ptr(None if p is None else _f64(p))))
 
+    def enable_komega(self, kw, k_bc, w_bc):
+        """turbulenceModel kOmega (src/physics/RANS/k_omega.cpp); before start()."""
+        kt, kv = k_bc
+        wt, wv = w_bc
+        check(lib().bsim_simple_enable_komega(self.h, C.byref(kw), ptr(_i32(kt), ip), ptr(_f64(kv).reshape(-1)),
+                                              ptr(_i32(wt), ip), ptr(_f64(wv).reshape(-1))))
+
+    def set_turbulence(self, k=None, omega=None):
+        check(lib().bsim_simple_set_turbulence(self.h, ptr(None if k is None else _f64(k)),
+                                               ptr(None if omega is None else _f64(omega))))
+
+    def turbulence(self):
+        k, w, mut = np.zeros(self.ctx.nc), np.zeros(self.ctx.nc), np.zeros(self.ctx.nc)
+        check(lib().bsim_simple_get_turbulence(self.h, ptr(k), ptr(w), ptr(mut)))
+        return k, w, mut
+
     def start(self):
         check(lib().bsim_simple_start(self.h))
 

@@ -188,19 +188,46 @@ typedef struct {
     bsim_solver_config mom, pre;
 } bsim_simple_config;
 
+/* One SIMPLE outer iteration (main.cpp:114-125 report): the turbulence
+ * entries are dTurb, rTurb (max over omega and k), the omega cells the last
+ * bounding raised, Krylov iterations of the k and omega solves, and whether
+ * they converged and the model is settled (zero / one when laminar). */
 typedef struct {
     double rU, dU, dP, mass, linP;
     int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
+    double d_turb, r_turb, bounded_omega;
+    int k_iters, w_iters, turb_linear, turb_converged;
 } bsim_iter_report;
+
+/* Wilcox (1988) k-omega (src/physics/RANS/k_omega.cpp:25-157): constants,
+ * bounds, relaxation and tolerance as the constructor reads them (33-48), and
+ * the kTransport / omegaTransport equations' methods and linear solvers. */
+typedef struct {
+    double beta_star, beta, gamma, sigma_k, sigma_omega, k_min, omega_min, viscosity_ratio_max, relaxation,
+        tolerance;
+    int k_interp, k_grad, k_conv, k_diff, k_coef_interp, k_coef_grad;
+    int w_interp, w_grad, w_conv, w_diff, w_coef_interp, w_coef_grad;
+    bsim_solver_config k_solver, w_solver;
+} bsim_komega_config;
 
 typedef struct bsim_simple bsim_simple;
 int bsim_simple_create(bsim_ctx* ctx, const bsim_simple_config* cfg, const int* u_bc_type, const double* u_bc_value,
                        const int* p_bc_type, const double* p_bc_value, bsim_simple** out);
 int bsim_simple_set_fields(bsim_simple* s, const double* U, const double* p);
+/* turbulenceModel kOmega for this SIMPLE run (model.cpp:35-40), with the k and
+ * omega boundary conditions; call before bsim_simple_start.  The momentum
+ * diffusion then uses muEff = mu + mut and gains the explicit stress
+ * remainder (main.cpp:53-59); each iteration ends with the omega and k
+ * solves (main.cpp:100-110). */
+int bsim_simple_enable_komega(bsim_simple* s, const bsim_komega_config* cfg, const int* k_bc_type,
+                              const double* k_bc_value, const int* w_bc_type, const double* w_bc_value);
+int bsim_simple_set_turbulence(bsim_simple* s, const double* k, const double* omega);
+int bsim_simple_get_turbulence(bsim_simple* s, double* k, double* omega, double* mut);
 int bsim_simple_start(bsim_simple* s);
 int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports);
 int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi);
-/* which: 0..2 momentum components, 3 pressure correction (last solve). */
+/* which: 0..2 momentum components, 3 pressure correction (last solve), 4 k and
+ * 5 omega (last transport solves, k-omega only). */
 int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* len);
 /* Seconds per phase: assembly, momentum solve, Rhie-Chow, pressure assembly,
  * pressure solve, correction, convergence, iterations. */

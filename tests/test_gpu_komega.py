@@ -1,0 +1,151 @@
+"""Device Wilcox k-omega inside SIMPLE (babelsim_amd/csrc/komega.hip) against
+the oracle's SIMPLE with the same model (oracle/rans.cpp) on the same meshes:
+per-iteration reports (Krylov iteration counts and bounded-cell counts
+exactly, residual measures to 1e-8 relative), U, p, phi, k, omega, mut and the
+six Krylov histories.  The cases cover a walled channel (production, wall
+omega, the stress remainder with muEff varying), the jittered polyhedral
+cavity with least-squares gradients and corrected diffusion, and the slip
+channel whose symmetry patches use the reflected vector-gradient trace."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as orc
+from babelsim_amd import MeshData, komega_config, simple_config, solver_config
+from babelsim_amd.device import Context, Simple
+
+pytestmark = pytest.mark.gpu
+
+FV, FG, ZG, IO, SYM = range(5)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def channel_case():
+    """Walled channel at Re 1000 based on the height: uniform inlet with k, omega,
+    fixed-pressure outlet, no-slip walls with a small k and a large omega,
+    front/back symmetry."""
+    mesh = MeshData.hex((16, 10, 1), (0, 0, 0), (2.0, 1.0, 0.1))
+    ub = (np.array([FV, ZG, FV, FV, SYM, SYM], np.int32), np.array([[1.0, 0, 0]] + [[0, 0, 0]] * 5))
+    pb = (np.array([ZG, FV, ZG, ZG, SYM, SYM], np.int32), np.zeros(6))
+    kb = (np.array([FV, ZG, FV, FV, ZG, ZG], np.int32), np.array([1e-3, 0, 1e-10, 1e-10, 0, 0]))
+    wb = (np.array([FV, ZG, FV, FV, ZG, ZG], np.int32), np.array([1.0, 0, 200.0, 200.0, 0, 0]))
+    cfg = cases.c1_config(mu=1e-3, tol_mass=1e-12, tol_u=1e-12, tol_mom=1e-12, tol_p=1e-12)
+    kw = komega_config()
+    return mesh, cfg, kw, ub, pb, kb, wb, 10, None
+
+
+def cavity3d_case():
+    """Jittered, split 3-D cavity with least-squares gradients, corrected
+    diffusion for U, p, k and omega, linearUpwind transport."""
+    mesh = cases.cavity3d_mesh(6, split=0.15, seed=3)
+    ub, pb = cases.cavity_bcs(False)
+    kb = (np.array([FV] * 6, np.int32), np.array([1e-8, 1e-8, 1e-8, 1e-4, 1e-8, 1e-8]))
+    wb = (np.array([FV] * 6, np.int32), np.array([50.0, 50.0, 50.0, 5.0, 50.0, 50.0]))
+    cfg = cases.c2_config(mu=1.0 / 400.0, tol_mass=1e-12, tol_u=1e-12, tol_mom=1e-12, tol_p=1e-12)
+    s = solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
+    kw = komega_config(k_grad=1, w_grad=1, k_diff=1, w_diff=1, k_coef_grad=1, w_coef_grad=1, k_solver=s, w_solver=s)
+    return mesh, cfg, kw, ub, pb, kb, wb, 6, None
+
+
+def slip_case():
+    """Uniform stream along a slip channel: symmetry patches on four sides,
+    omega and k decaying downstream; AMG-preconditioned transport solves."""
+    mesh = MeshData.hex((24, 2, 1), (0, 0, 0), (2.0, 0.2, 0.1))
+    ub = (np.array([FV, ZG, SYM, SYM, SYM, SYM], np.int32), np.array([[1.0, 0, 0]] + [[0, 0, 0]] * 5))
+    pb = (np.array([ZG, FV, SYM, SYM, SYM, SYM], np.int32), np.zeros(6))
+    kb = (np.array([FV, ZG, ZG, ZG, ZG, ZG], np.int32), np.array([1e-4, 0, 0, 0, 0, 0]))
+    wb = (np.array([FV, ZG, ZG, ZG, ZG, ZG], np.int32), np.array([4.0, 0, 0, 0, 0, 0]))
+    tight = solver_config(solver=1, pc=1, rtol=1e-12, atol=1e-30)
+    cfg = simple_config(mu=1e-6, m_conv=1, alpha_u=0.7, alpha_p=0.3, mom=tight,
+                        pre=solver_config(solver=0, pc=2, rtol=1e-12, atol=1e-30))
+    amg = solver_config(solver=1, pc=2, rtol=1e-12, atol=1e-30)
+    kw = komega_config(k_conv=1, w_conv=1, k_solver=amg, w_solver=amg, relaxation=0.9)
+
+    def init(nc):
+        U0 = np.zeros((nc, 3))
+        U0[:, 0] = 1.0
+        return U0, np.full(nc, 1e-4), np.full(nc, 4.0)
+
+    return mesh, cfg, kw, ub, pb, kb, wb, 8, init
+
+
+CASES = {"channel": channel_case, "cavity3d": cavity3d_case, "slip": slip_case}
+
+
+def run_pair(case):
+    mesh, cfg, kw, ub, pb, kb, wb, iters, init = CASES[case]()
+    arrays, _, _ = mesh.prepare()
+    arrays = arrays.arrays()
+    om = orc.oracle_mesh(arrays)
+    so = orc.OracleSimpleKOmega(om, cfg, kw, ub, pb, kb, wb)
+    ctx = Context(arrays, renumber=False)
+    sg = Simple(ctx, cfg, ub, pb)
+    sg.enable_komega(kw, kb, wb)
+    if init is not None:
+        U0, k0, w0 = init(om.nc)
+        for s in (so, sg):
+            s.set_fields(U0, np.zeros(om.nc))
+            s.set_turbulence(k0, w0)
+    so.start()
+    sg.start()
+    return so, sg, so.iterate(iters), sg.iterate(iters)
+
+
+def close(a, b, rtol=1e-8, atol=1e-14):
+    return abs(a - b) <= rtol * abs(b) + atol
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_komega_simple_parity(case):
+    print(f"komega parity case={case}", flush=True)
+    so, sg, ro, rg = run_pair(case)
+    for i, (a, b) in enumerate(zip(rg, ro)):
+        assert list(a.mom_iters) == list(b.mom_iters) and a.p_iters == b.p_iters, i
+        assert (a.k_iters, a.w_iters) == (b.k_iters, b.w_iters), i
+        assert (a.turb_linear, a.turb_converged, a.converged) == (b.turb_linear, b.turb_converged, b.converged), i
+        assert a.bounded_omega == b.bounded_omega and a.referenced == b.referenced, i
+        for key in ("rU", "dU", "dP", "d_turb", "r_turb"):
+            assert close(getattr(a, key), getattr(b, key)), (i, key, getattr(a, key), getattr(b, key))
+    Ug, pg, phig = sg.fields()
+    Uo, po, phio = so.fields()
+    assert rel_l2(Ug, Uo) < 1e-9
+    assert rel_l2(pg, po) < 1e-9
+    assert rel_l2(phig, phio) < 1e-9
+    kg, wg, mg = sg.turbulence()
+    ko, wo, mo = so.turbulence()
+    assert rel_l2(kg, ko) < 1e-9 and rel_l2(wg, wo) < 1e-9 and rel_l2(mg, mo) < 1e-9
+    for which in range(6):
+        hg, ho = sg.history(which), so.history(which)
+        assert len(hg) == len(ho), which
+        np.testing.assert_allclose(hg, ho, rtol=1e-8, atol=1e-300)
+
+
+def test_turbulence_fields_round_trip_and_laminar_errors():
+    """set/get of k and omega before start; the closure mut = rho k / omega
+    holds after start; a laminar run rejects the turbulence calls."""
+    mesh, cfg, kw, ub, pb, kb, wb, _, _ = channel_case()
+    arrays, _, _ = mesh.prepare()
+    arrays = arrays.arrays()
+    ctx = Context(arrays, renumber=False)
+    lam = Simple(ctx, cfg, ub, pb)
+    with pytest.raises(ValueError):
+        lam.turbulence()
+    with pytest.raises(ValueError):
+        lam.history(4)
+    sg = Simple(ctx, cfg, ub, pb)
+    sg.enable_komega(kw, kb, wb)
+    rng = np.random.default_rng(2)
+    k0 = 1e-3 * (1 + rng.random(ctx.nc))
+    w0 = 2.0 * (1 + rng.random(ctx.nc))
+    sg.set_turbulence(k0, w0)
+    sg.start()
+    k, w, mut = sg.turbulence()
+    np.testing.assert_array_equal(k, k0)
+    np.testing.assert_array_equal(w, w0)
+    np.testing.assert_allclose(mut, cfg.rho * k0 / w0, rtol=1e-15)
