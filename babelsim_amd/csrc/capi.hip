@@ -523,8 +523,8 @@ int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* le
 int bsim_simple_timing(bsim_simple* s, double* o) {
     return guarded([&] {
         const SimpleTiming& t = s->s->timing;
-        const double v[8] = {t.assembly, t.momentumSolve, t.rhieChow, t.pressureAssembly, t.pressureSolve,
-                             t.correction, t.convergence, static_cast<double>(t.iterations)};
+        const double v[9] = {t.assembly, t.momentumSolve, t.rhieChow, t.pressureAssembly, t.pressureSolve,
+                             t.correction, t.convergence, static_cast<double>(t.iterations), t.turbulence};
         std::memcpy(o, v, sizeof(v));
     });
 }

@@ -348,6 +348,7 @@ IterRep SimpleSolver::iterate() {
     // Turbulence transport (main.cpp:100-110).
     bool turbulenceConverged = true;
     if (turb) {
+        t0 = Clock::now();
         const TransportRep t = turb->solveTransport(fieldU(phi.p), phi.p, c.mGrad);
         rep.kIters = t.itK;
         rep.wIters = t.itOmega;
@@ -358,6 +359,7 @@ IterRep SimpleSolver::iterate() {
         rep.turbLinear = t.statusK == kConverged && t.statusOmega == kConverged;
         turbulenceConverged = rep.turbLinear && rep.dTurb <= turb->tolerance() && rep.rTurb <= turb->tolerance();
         rep.turbConverged = turbulenceConverged;
+        timing.turbulence += since(t0);
     }
     t0 = Clock::now();
     // Convergence measures (main.cpp:112-126).

@@ -30,7 +30,7 @@ struct IterRep {
 // Per-phase wall time (seconds) accumulated over iterations.
 struct SimpleTiming {
     double assembly = 0, momentumSolve = 0, rhieChow = 0, pressureAssembly = 0, pressureSolve = 0, correction = 0,
-           convergence = 0;
+           convergence = 0, turbulence = 0;
     int iterations = 0;
 };
 

@@ -259,10 +259,10 @@ class Simple:
         return out
 
     def timing(self) -> dict:
-        out = np.zeros(8)
+        out = np.zeros(9)
         check(lib().bsim_simple_timing(self.h, ptr(out)))
         keys = ["assembly", "momentum_solve", "rhie_chow", "pressure_assembly", "pressure_solve", "correction",
-                "convergence", "iterations"]
+                "convergence", "iterations", "turbulence"]
         return dict(zip(keys, out.tolist()))
 
     def time_spmv(self, which=1, reps=20):

@@ -15,6 +15,14 @@ about 400k cells extruded one cell; PISO with 2 correctors, BDF2, dt at a
 Courant number of 0.5; momentum by FP32 BiCGSTAB inside FP64 refinement,
 pressure by FP64 CG + aggregation AMG.  One step is one PISO time step.
 
+--workload channel: steady RANS plane channel with the Wilcox k-omega model
+on a 512 x 96 x 100 hex lattice (4.9M cells) graded towards both walls to a
+first-cell y+ of about 1, bulk inflow U = 1, Re_bulk = 40,000, inlet k and
+omega from 5 % intensity and a 0.1 h length scale; SIMPLE with segregated
+omega and k BiCGSTAB in FP32 inside FP64 refinement and the pressure
+correction by FP32-AMG CG inside FP64 refinement.  One step is one SIMPLE
+outer iteration including both transport solves.
+
 Prints one JSON line (rank 0, one GPU).
 """
 from __future__ import annotations
@@ -73,12 +81,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["cavity", "cylinder"], default="cavity")
+    ap.add_argument("--workload", choices=["cavity", "cylinder", "channel"], default="cavity")
     ap.add_argument("--n", type=int, default=126, help="hex lattice cells per side")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--probe-steps", type=int, default=2,
                     help="steps after the timed region in which the dominant kernel is timed with HIP events")
-    ap.add_argument("--cpu-iters", type=int, default=3, help="timed oracle iterations for the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=None,
+                    help="timed oracle iterations for the CPU baseline (3; 1 for the channel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rebuild-interval", type=int, default=8)
     ap.add_argument("--stream-launches", action="store_true",
@@ -115,6 +124,8 @@ def main():
         set_graph_replay(False)
     if args.workload == "cylinder":
         return bench_cylinder(args)
+    if args.workload == "channel":
+        return bench_channel(args)
     import cases
     from babelsim_amd import solver_config
     from babelsim_amd.device import Context, Simple
@@ -175,7 +186,8 @@ def main():
     log(f"spmv: {spmv_s * 1e6:.1f} us per launch, {spmv_bytes / spmv_s / 1e9:.0f} GB/s")
     traffic = measured_traffic(ctx.nc)
     cpu = None
-    if not args.no_cpu and args.cpu_iters > 0:
+    cpu_iters = 3 if args.cpu_iters is None else args.cpu_iters
+    if not args.no_cpu and cpu_iters > 0:
         import oracle_lib as orc
         arrays = mesh.arrays()
         t3 = time.time()
@@ -184,11 +196,11 @@ def main():
         so.start()
         so.iterate(1)  # the first outer iteration is outside the timing window (runtime.cpp:117-130)
         t4 = time.time()
-        so.iterate(args.cpu_iters)
+        so.iterate(cpu_iters)
         t5 = time.time()
         threads = int(orc.oracle().orc_threads())
-        cpu = {"value": args.cpu_iters / (t5 - t4), "unit": "outer_iterations/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_iters} timed FP64 oracle SIMPLE outer iteration(s) on the same "
+        cpu = {"value": cpu_iters / (t5 - t4), "unit": "outer_iterations/s", "cores": threads, "kind": "port",
+               "sample": f"{cpu_iters} timed FP64 oracle SIMPLE outer iteration(s) on the same "
                          f"{nc_in}-cell mesh after one untimed iteration (setup {t4 - t3:.1f}s excluded); "
                          f"OpenMP over {threads} threads, the assembly face loops serial"}
         log(f"cpu baseline: {cpu['value']:.4f} it/s")
@@ -287,7 +299,8 @@ def bench_cylinder(args):
     sweep_avg = sweep_s / max(launches, 1)
     achieved = sweep_bytes / sweep_avg / 1e9 if launches else 0.0
     cpu = None
-    if not args.no_cpu and args.cpu_iters > 0:
+    cpu_iters = 3 if args.cpu_iters is None else args.cpu_iters
+    if not args.no_cpu and cpu_iters > 0:
         import oracle_lib as orc
         t3 = time.time()
         om = orc.oracle_mesh(arrays)
@@ -296,12 +309,12 @@ def bench_cylinder(args):
         so.start()
         so.step(dt)  # the first step outside the timing window
         t4 = time.time()
-        for _ in range(args.cpu_iters):
+        for _ in range(cpu_iters):
             so.step(dt)
         t5 = time.time()
         threads = int(orc.oracle().orc_threads())
-        cpu = {"value": args.cpu_iters / (t5 - t4), "unit": "time_steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_iters} timed FP64 oracle PISO steps on the same {ctx.nc}-cell mesh after one "
+        cpu = {"value": cpu_iters / (t5 - t4), "unit": "time_steps/s", "cores": threads, "kind": "port",
+               "sample": f"{cpu_iters} timed FP64 oracle PISO steps on the same {ctx.nc}-cell mesh after one "
                          f"untimed step (setup {t4 - t3:.1f}s excluded); OpenMP over {threads} threads"}
         log(f"cpu baseline: {cpu['value']:.4f} steps/s")
     line = {
@@ -331,6 +344,104 @@ def bench_cylinder(args):
                      "launches": launches, "seconds_per_launch": sweep_avg,
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "timing": f"HIP events around every sweep launch over {args.probe_steps} steps run "
+                               "right after the timed region with stream launches"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_channel(args):
+    import channel_mesh as ch
+    from babelsim_amd.device import Context, Simple
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    mesh = ch.graded_channel_mesh()
+    ctx = Context(mesh, renumber=True)
+    log(f"graded channel: N={ctx.nc} F={ctx.nf} nnz_off={ctx.nnz_off} ({time.time() - t0:.1f}s)")
+    cfg, kw = ch.channel_config(precision=1, rebuild_interval=args.rebuild_interval)
+    ub, pb, kb, wb = ch.channel_bcs()
+    sim = Simple(ctx, cfg, ub, pb)
+    sim.enable_komega(kw, kb, wb)
+    U0, k0, w0 = ch.initial_fields(ctx.nc)
+    sim.set_fields(U0, np.zeros(ctx.nc))
+    sim.set_turbulence(k0, w0)
+    sim.start()
+    for k in range(args.warmup):
+        r = sim.iterate(1)[0]
+        log(f"warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters} k_its={r.k_iters} "
+            f"w_its={r.w_iters} dTurb={r.d_turb:.2e}")
+    before = sim.timing()
+    t1 = time.time()
+    timed = sim.iterate(args.steps)
+    t2 = time.time()
+    after = sim.timing()
+    sim.probe_start(capacity=512 * args.probe_steps)
+    sim.iterate(args.probe_steps)
+    launches, sweep_s, sweep_bytes = sim.probe_stop()
+    for k, r in enumerate(timed):
+        if not r.turb_linear or r.mom_status != 0 or r.p_status != 0:
+            raise RuntimeError(f"timed iteration {k} has an unconverged linear solve: mom_status={r.mom_status} "
+                               f"p_status={r.p_status} turb_linear={r.turb_linear}")
+    sec = t2 - t1
+    last = timed[-1]
+    log(f"timed {args.steps} its in {sec:.3f}s -> {args.steps / sec:.3f} it/s; rU={last.rU:.3e} "
+        f"mom_its={list(last.mom_iters)} p_its={last.p_iters} k_its={last.k_iters} w_its={last.w_iters}")
+    timing = {k: (after[k] - before[k]) / args.steps for k in after if k != "iterations"}
+    sweep_avg = sweep_s / max(launches, 1)
+    achieved = sweep_bytes / sweep_avg / 1e9 if launches else 0.0
+    log(f"fine-level sweep: {launches} launches, {sweep_avg * 1e6:.1f} us each, {achieved:.0f} GB/s")
+    cpu = None
+    cpu_iters = 1 if args.cpu_iters is None else args.cpu_iters
+    if not args.no_cpu and cpu_iters > 0:
+        import oracle_lib as orc
+        t3 = time.time()
+        om = orc.oracle_mesh(mesh.arrays())
+        so = orc.OracleSimpleKOmega(om, cfg, kw, ub, pb, kb, wb)
+        U0, k0, w0 = ch.initial_fields(om.nc)
+        so.set_fields(U0, np.zeros(om.nc))
+        so.set_turbulence(k0, w0)
+        so.start()
+        t4 = time.time()
+        so.iterate(cpu_iters)
+        t5 = time.time()
+        threads = int(orc.oracle().orc_threads())
+        cpu = {"value": cpu_iters / (t5 - t4), "unit": "outer_iterations/s", "cores": threads, "kind": "port",
+               "sample": f"the first {cpu_iters} FP64 oracle SIMPLE k-omega outer iteration(s) on the same "
+                         f"{om.nc}-cell mesh (setup {t4 - t3:.1f}s excluded); OpenMP over {threads} threads, "
+                         "the assembly face loops serial"}
+        log(f"cpu baseline: {cpu['value']:.4f} it/s")
+    line = {
+        "metric": "SIMPLE k-omega outer iterations/s, graded plane channel at 4.9M hex cells",
+        "value": args.steps / sec,
+        "unit": "outer_iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "plane channel 512x96x100 hex, wall grading to y+ ~1, Re_bulk=40000, Wilcox k-omega",
+                   "cells": ctx.nc, "faces": ctx.nf, "nnz": ctx.nc + ctx.nnz_off,
+                   "momentum": "FP32 BiCGSTAB + Jacobi in FP64 refinement",
+                   "turbulence": "omega then k, FP32 BiCGSTAB + Jacobi in FP64 refinement",
+                   "pressure": "FP32-AMG CG in FP64 refinement",
+                   "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters),
+                                      "k_its": last.k_iters, "w_its": last.w_iters, "rU": last.rU,
+                                      "d_turb": last.d_turb},
+                   "phase_seconds_per_iteration": timing, "device_memory_used_GB": device_memory_used_gb()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "kJacobiSweep (pressure AMG fine-level smoother sweep, sliced rows)",
+                     "launches": launches, "seconds_per_launch": sweep_avg,
+                     "algorithmic_bytes_per_launch": sweep_bytes,
+                     "timing": f"HIP events around every sweep launch over {args.probe_steps} iterations run "
                                "right after the timed region with stream launches"},
         "cpu_baseline": cpu,
     }

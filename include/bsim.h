@@ -230,8 +230,8 @@ int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi);
  * 5 omega (last transport solves, k-omega only). */
 int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* len);
 /* Seconds per phase: assembly, momentum solve, Rhie-Chow, pressure assembly,
- * pressure solve, correction, convergence, iterations. */
-int bsim_simple_timing(bsim_simple* s, double* out8);
+ * pressure solve, correction, convergence, iterations, turbulence transport. */
+int bsim_simple_timing(bsim_simple* s, double* out9);
 /* Times `reps` launches of the solver's fused SpMV on the current pressure
  * matrix with HIP events on the context stream; returns mean seconds and the
  * matrix size. */

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import cases
+import channel_mesh as ch
 import oracle_lib as orc
 from babelsim_amd import MeshData, komega_config, simple_config, solver_config
 from babelsim_amd.device import Context, Simple
@@ -75,7 +76,16 @@ def slip_case():
     return mesh, cfg, kw, ub, pb, kb, wb, 8, init
 
 
-CASES = {"channel": channel_case, "cavity3d": cavity3d_case, "slip": slip_case}
+def graded_case():
+    """The wall-graded RANS channel (first cell at y+ ~ 1, Re_bulk 40,000) on a
+    short 24 x 16 x 2 section, FP64 solves."""
+    mesh = ch.graded_channel_mesh(24, 16, 2, length=0.75, width=0.064)
+    cfg, kw = ch.channel_config(precision=0)
+    ub, pb, kb, wb = ch.channel_bcs()
+    return mesh, cfg, kw, ub, pb, kb, wb, 8, ch.initial_fields
+
+
+CASES = {"channel": channel_case, "cavity3d": cavity3d_case, "slip": slip_case, "graded": graded_case}
 
 
 def run_pair(case):
