@@ -142,6 +142,28 @@ def piso_config(**kw) -> PisoConfig:
     return PisoConfig(mom=mom, pre=pre, **base)
 
 
+REQ_STEADY, REQ_TRANSIENT = 0, 1
+(SERVE_CONVERGED, SERVE_NOT_CONVERGED, SERVE_NUMERICAL_FAILURE, SERVE_REJECTED, SERVE_ERROR,
+ SERVE_CANCELLED) = range(6)
+
+
+class Request(C.Structure):
+    """One served solve: a steady SIMPLE run or a transient PISO run on a registered case."""
+
+    _fields_ = [("kind", C.c_int), ("case_id", C.c_int), ("simple", C.POINTER(SimpleConfig)),
+                ("piso", C.POINTER(PisoConfig)), ("u_bc_type", ip), ("u_bc_value", dp), ("p_bc_type", ip),
+                ("p_bc_value", dp), ("U0", dp), ("p0", dp), ("max_iterations", C.c_int), ("dt", C.c_double),
+                ("arrival", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("slot", C.c_int), ("reused_context", C.c_int),
+                ("released", C.c_double), ("admitted", C.c_double), ("started", C.c_double),
+                ("finished", C.c_double), ("rU", C.c_double), ("dU", C.c_double), ("dP", C.c_double),
+                ("mass", C.c_double), ("reserved_bytes", C.c_double), ("measured_bytes", C.c_double),
+                ("message", C.c_char * 256)]
+
+
 def set_graph_replay(enable: bool) -> None:
     """AMG V-cycles as captured-graph replays (default) or stream launches."""
     check(lib().bsim_set_graph_replay(int(bool(enable))))
@@ -244,9 +266,18 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_piso_timing": [vp, dp],
     "bsim_piso_probe_start": [vp, C.c_int],
     "bsim_piso_probe_stop": [vp, ip, dp, dp],
+    "bsim_server_create": [C.c_int, C.c_int, C.c_double, C.POINTER(vp)],
+    "bsim_server_add_case": [vp, vp, ip],
+    "bsim_server_submit": [vp, C.POINTER(Request), ip],
+    "bsim_server_wait": [vp, C.c_int, C.POINTER(RequestResult)],
+    "bsim_server_done": [vp, C.c_int, ip],
+    "bsim_server_fields": [vp, C.c_int, dp, dp, dp],
+    "bsim_server_release": [vp, C.c_int],
+    "bsim_server_stats": [vp, dp],
+    "bsim_server_estimate": [C.c_int, C.c_int, C.c_int, dp, dp],
 }
 _VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp],
-         "bsim_piso_destroy": [vp]}
+         "bsim_piso_destroy": [vp], "bsim_server_destroy": [vp]}
 
 
 def _declare_void(L):
@@ -254,6 +285,8 @@ def _declare_void(L):
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = None
+    L.bsim_server_clock.argtypes = [vp]
+    L.bsim_server_clock.restype = C.c_double
 
 
 class MeshData:

@@ -1,18 +1,10 @@
 // C ABI of the device library (include/bsim.h): contexts, operators, linear
 // systems and the SIMPLE driver.  Host arrays are in the caller's numbering.
-#include "../../include/bsim.h"
+#include "capi_types.hpp"
 #include "errors.hpp"
-#include "piso.hpp"
-#include "simple.hpp"
 
 #include <cstring>
 
-struct bsim_mesh {
-    bsim::MeshDesc d;
-};
-struct bsim_ctx {
-    std::unique_ptr<bsim::Context> c;
-};
 struct bsim_system {
     bsim_ctx* ctx;
     std::unique_ptr<bsim::DeviceSystem> sys;
@@ -73,6 +65,42 @@ void lduOut(Context& C, const std::vector<double>& dg, const std::vector<double>
 }
 
 }  // namespace
+
+namespace bsim {
+
+SimpleCfg toSimpleCfg(const bsim_simple_config& cfg) {
+    SimpleCfg c;
+    c.rho = cfg.rho; c.mu = cfg.mu;
+    c.mInterp = cfg.m_interp; c.mGrad = cfg.m_grad; c.mConv = cfg.m_conv; c.mDiff = cfg.m_diff;
+    c.mCoefInterp = cfg.m_coef_interp; c.mCoefGrad = cfg.m_coef_grad;
+    c.pInterp = cfg.p_interp; c.pGrad = cfg.p_grad; c.pDiff = cfg.p_diff;
+    c.pCoefInterp = cfg.p_coef_interp; c.pCoefGrad = cfg.p_coef_grad;
+    c.alphaU = cfg.alpha_u; c.alphaP = cfg.alpha_p; c.nonOrthCorr = cfg.non_orth_corr;
+    c.tolMass = cfg.tol_mass; c.tolU = cfg.tol_u; c.tolMom = cfg.tol_mom; c.tolP = cfg.tol_p;
+    c.mom = toSolverCfg(&cfg.mom);
+    c.pre = toSolverCfg(&cfg.pre);
+    if (c.nonOrthCorr < 0 || c.nonOrthCorr > 20) throw std::invalid_argument("nonOrthogonalCorrections must lie in [0, 20]");
+    return c;
+}
+
+PisoCfg toPisoCfg(const bsim_piso_config& cfg) {
+    PisoCfg c;
+    c.rho = cfg.rho; c.mu = cfg.mu;
+    c.mInterp = cfg.m_interp; c.mGrad = cfg.m_grad; c.mConv = cfg.m_conv; c.mDiff = cfg.m_diff;
+    c.mCoefInterp = cfg.m_coef_interp; c.mCoefGrad = cfg.m_coef_grad;
+    c.pInterp = cfg.p_interp; c.pGrad = cfg.p_grad; c.pDiff = cfg.p_diff;
+    c.pCoefInterp = cfg.p_coef_interp; c.pCoefGrad = cfg.p_coef_grad;
+    c.timeMethod = cfg.time_method;
+    c.maxIterations = cfg.max_iterations; c.nCorrectors = cfg.n_correctors;
+    c.maxCorrectors = cfg.max_correctors; c.nonOrthCorr = cfg.non_orth_corr;
+    c.alphaU = cfg.alpha_u; c.couplingTol = cfg.coupling_tol;
+    c.tolMass = cfg.tol_mass; c.tolU = cfg.tol_u; c.tolMom = cfg.tol_mom; c.tolP = cfg.tol_p;
+    c.mom = toSolverCfg(&cfg.mom);
+    c.pre = toSolverCfg(&cfg.pre);
+    return c;
+}
+
+}  // namespace bsim
 
 extern "C" {
 
@@ -442,17 +470,8 @@ int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const 
 int bsim_simple_create(bsim_ctx* ctx, const bsim_simple_config* cfg, const int* ubct, const double* ubcv,
                        const int* pbct, const double* pbcv, bsim_simple** out) {
     return guarded([&] {
-        SimpleCfg c;
-        c.rho = cfg->rho; c.mu = cfg->mu;
-        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
-        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
-        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
-        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
-        c.alphaU = cfg->alpha_u; c.alphaP = cfg->alpha_p; c.nonOrthCorr = cfg->non_orth_corr;
-        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
-        c.mom = toSolverCfg(&cfg->mom);
-        c.pre = toSolverCfg(&cfg->pre);
-        if (c.nonOrthCorr < 0 || c.nonOrthCorr > 20) throw std::invalid_argument("nonOrthogonalCorrections must lie in [0, 20]");
+        if (!ctx || !cfg || !out) throw std::invalid_argument("null argument");
+        const SimpleCfg c = toSimpleCfg(*cfg);
         auto* s = new bsim_simple;
         s->ctx = ctx;
         try { s->s = std::make_unique<SimpleSolver>(*ctx->c, c, ubct, ubcv, pbct, pbcv); }
@@ -552,19 +571,7 @@ int bsim_piso_create(bsim_ctx* ctx, const bsim_piso_config* cfg, const int* ubct
                      const double* pbcv, bsim_piso** out) {
     return guarded([&] {
         if (!ctx || !cfg || !ubct || !ubcv || !pbct || !pbcv || !out) throw std::invalid_argument("null argument");
-        PisoCfg c;
-        c.rho = cfg->rho; c.mu = cfg->mu;
-        c.mInterp = cfg->m_interp; c.mGrad = cfg->m_grad; c.mConv = cfg->m_conv; c.mDiff = cfg->m_diff;
-        c.mCoefInterp = cfg->m_coef_interp; c.mCoefGrad = cfg->m_coef_grad;
-        c.pInterp = cfg->p_interp; c.pGrad = cfg->p_grad; c.pDiff = cfg->p_diff;
-        c.pCoefInterp = cfg->p_coef_interp; c.pCoefGrad = cfg->p_coef_grad;
-        c.timeMethod = cfg->time_method;
-        c.maxIterations = cfg->max_iterations; c.nCorrectors = cfg->n_correctors;
-        c.maxCorrectors = cfg->max_correctors; c.nonOrthCorr = cfg->non_orth_corr;
-        c.alphaU = cfg->alpha_u; c.couplingTol = cfg->coupling_tol;
-        c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
-        c.mom = toSolverCfg(&cfg->mom);
-        c.pre = toSolverCfg(&cfg->pre);
+        const PisoCfg c = toPisoCfg(*cfg);
         auto* s = new bsim_piso;
         s->ctx = ctx;
         try { s->s = std::make_unique<PisoSolver>(*ctx->c, c, ubct, ubcv, pbct, pbcv); }

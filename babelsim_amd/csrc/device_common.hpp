@@ -31,6 +31,10 @@ inline int blocksFor(long long n, int block = kBlock) {
     return static_cast<int>(b < 1 ? 1 : b);
 }
 
+// Device bytes held by DBufs that the calling thread allocated minus those it
+// released; the serving runtime reads its change across one request.
+inline thread_local long long t_deviceBytes = 0;
+
 // Owning device allocation of T[n].
 template <class T> struct DBuf {
     T* p = nullptr;
@@ -47,12 +51,18 @@ template <class T> struct DBuf {
     ~DBuf() { release(); }
     void alloc(std::size_t count) {
         release();
+        if (count) {
+            BSIM_HIP(hipMalloc(&p, sizeof(T) * count));
+            t_deviceBytes += static_cast<long long>(sizeof(T) * count);
+        }
         n = count;
-        if (count) BSIM_HIP(hipMalloc(&p, sizeof(T) * count));
     }
     void ensure(std::size_t count) { if (count > n) alloc(count); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            t_deviceBytes -= static_cast<long long>(sizeof(T) * n);
+        }
         p = nullptr;
         n = 0;
     }

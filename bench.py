@@ -23,6 +23,15 @@ omega and k BiCGSTAB in FP32 inside FP64 refinement and the pressure
 correction by FP32-AMG CG inside FP64 refinement.  One step is one SIMPLE
 outer iteration including both transport solves.
 
+--workload serve: the serving stream of steady laminar lid-driven cavity and
+channel requests (256 from a seed, jittered hex meshes log-uniform between
+1e4 and 1e6 cells on a ladder of size buckets, Re uniform in [50, 1000],
+Poisson arrivals at a fixed rate) through the serving runtime: HBM-budget
+admission, concurrent requests on their own streams, resident per-mesh
+contexts.  The stream runs with mixed precision (FP32 AMG and inner Krylov
+in FP64 refinement) and again in plain FP64; value is the mixed path's
+converged solves/s, with p50/p99 time-to-converged for both.
+
 Prints one JSON line (rank 0, one GPU).
 """
 from __future__ import annotations
@@ -81,7 +90,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["cavity", "cylinder", "channel"], default="cavity")
+    ap.add_argument("--workload", choices=["cavity", "cylinder", "channel", "serve"], default="cavity")
+    ap.add_argument("--requests", type=int, default=256, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests (worker threads)")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="serve: device memory budget")
+    ap.add_argument("--cells-min", type=float, default=1e4)
+    ap.add_argument("--cells-max", type=float, default=1e6)
+    ap.add_argument("--serve-precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--max-outer", type=int, default=3000, help="serve: outer-iteration cap per request")
+    ap.add_argument("--serve-trace", default=None,
+                    help="serve: write one CSV row per request (size, iterations, server-clock times) to this path")
     ap.add_argument("--n", type=int, default=126, help="hex lattice cells per side")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--probe-steps", type=int, default=2,
@@ -126,6 +145,8 @@ def main():
         return bench_cylinder(args)
     if args.workload == "channel":
         return bench_channel(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     import cases
     from babelsim_amd import solver_config
     from babelsim_amd.device import Context, Simple
@@ -443,6 +464,135 @@ def bench_channel(args):
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "timing": f"HIP events around every sweep launch over {args.probe_steps} iterations run "
                                "right after the timed region with stream launches"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_stream(args, specs, meshes, precision, log):
+    """One pass of the stream through a fresh server; returns its summary."""
+    import serve_cases as sc
+    from babelsim_amd import SERVE_CONVERGED
+    from babelsim_amd.serve import Server
+
+    srv = Server(slots=args.slots, memory_budget=args.budget_gb * 1e9)
+    case_ids = {key: srv.add_case(m) for key, m in meshes.items()}
+    lead = 0.2
+    t0 = srv.clock() + lead
+    ids = []
+    for sp in specs:
+        cfg = sc.serve_config(sp.re, precision=precision)
+        ub, pb = sc.bcs_for(sp)
+        ids.append(srv.submit_steady(case_ids[(sp.kind, sp.side)], cfg, ub, pb, max_iterations=args.max_outer,
+                                     arrival=t0 + sp.arrival))
+    results = []
+    for k, rid in enumerate(ids):
+        r = srv.wait(rid)
+        results.append(r)
+        if (k + 1) % 32 == 0 or k + 1 == len(ids):
+            log(f"{'mixed' if precision else 'fp64'}: {k + 1}/{len(ids)} done at t={r.finished - t0:.1f}s")
+    stats = srv.stats()
+    srv.close()
+    if args.serve_trace:
+        with open(f"{args.serve_trace}.{'mixed' if precision else 'fp64'}.csv", "w") as f:
+            f.write("index,kind,cells,re,status,iterations,slot,reused,released,admitted,started,finished\n")
+            for sp, r in zip(specs, results):
+                f.write(f"{sp.index},{sp.kind},{sp.cells},{sp.re:.1f},{r.status},{r.iterations},{r.slot},"
+                        f"{int(r.reused_context)},{r.released - t0:.4f},{r.admitted - t0:.4f},"
+                        f"{r.started - t0:.4f},{r.finished - t0:.4f}\n")
+    ok = [r for r in results if r.status == SERVE_CONVERGED]
+    end = max(r.finished for r in results)
+    makespan = end - t0
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    queue = np.array([r.admitted - r.released for r in results])
+    its = np.array([r.iterations for r in results])
+    return {
+        "solves_per_s": len(ok) / makespan,
+        "converged": len(ok), "requests": len(results),
+        "statuses": {str(s): int(sum(r.status == s for r in results)) for s in sorted({r.status for r in results})},
+        "makespan_s": makespan,
+        "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+        "mean_queue_s": float(queue.mean()),
+        "outer_iterations": {"total": int(its.sum()), "mean": float(its.mean()), "max": int(its.max())},
+        "cell_iterations_per_s": float(sum(sp.cells * r.iterations for sp, r in zip(specs, results)) / makespan),
+        "server": stats,
+        "max_measured_solver_bytes": max(r.measured_bytes for r in results),
+    }
+
+
+def bench_serve(args):
+    import serve_cases as sc
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    specs = sc.stream(count=args.requests, rate=args.rate, cells_min=args.cells_min, cells_max=args.cells_max)
+    meshes = {}
+    for sp in specs:
+        key = (sp.kind, sp.side)
+        if key not in meshes:
+            meshes[key] = sc.mesh_for(sp)
+    log(f"stream: {len(specs)} requests over {specs[-1].arrival:.1f}s of arrivals, {len(meshes)} distinct meshes, "
+        f"{sum(sp.cells for sp in specs)} cells in total ({time.time() - t0:.1f}s)")
+    runs = {}
+    order = {"both": (1, 0), "mixed": (1,), "fp64": (0,)}[args.serve_precision]
+    for prec in order:
+        name = "mixed" if prec else "fp64"
+        t1 = time.time()
+        runs[name] = run_stream(args, specs, meshes, prec, log)
+        r = runs[name]
+        log(f"{name}: {r['converged']}/{r['requests']} converged in {r['makespan_s']:.1f}s -> "
+            f"{r['solves_per_s']:.3f} solves/s, p50 {r['p50_s']:.2f}s p99 {r['p99_s']:.2f}s "
+            f"(wall {time.time() - t1:.1f}s)")
+    head = runs.get("mixed") or runs["fp64"]
+    cpu = None
+    cpu_iters = 1 if args.cpu_iters is None else args.cpu_iters
+    if not args.no_cpu and cpu_iters > 0:
+        # The oracle solves the stream's first request of the smallest
+        # bucket to convergence, FP64, on the host cores.
+        import oracle_lib as orc
+        sp = min(specs, key=lambda q: (q.cells, q.index))
+        ub, pb = sc.bcs_for(sp)
+        om = orc.oracle_mesh(meshes[(sp.kind, sp.side)].arrays())
+        so = orc.OracleSimple(om, sc.serve_config(sp.re), ub, pb)
+        so.start()
+        t4 = time.time()
+        n = 0
+        for n in range(1, args.max_outer + 1):
+            if so.iterate(1)[-1].converged:
+                break
+        t5 = time.time()
+        threads = int(orc.oracle().orc_threads())
+        cpu = {"value": 1.0 / (t5 - t4), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"one request of the stream ({sp.kind}, {sp.cells} cells, Re {sp.re:.0f}) solved to "
+                         f"convergence ({n} outer iterations) by the FP64 oracle SIMPLE, OpenMP over {threads} "
+                         "threads; the smallest request of the stream"}
+        log(f"cpu baseline: {cpu['value']:.4f} solves/s on the smallest request")
+    line = {
+        "metric": "served solves/s with p50/p99 time-to-converged (steady laminar stream, 1e4-1e6 cells)",
+        "value": head["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(specs),
+        "warmup": 0,
+        "ms_per_step": 1e3 / head["solves_per_s"] if head["solves_per_s"] > 0 else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if "mixed" in runs else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving stream: {len(specs)} steady laminar cavity/channel requests, jittered hex "
+                               f"meshes log-uniform in [{args.cells_min:.0e}, {args.cells_max:.0e}] cells on "
+                               f"{len(meshes)} size buckets, Re ~ U[50, 1000], Poisson arrivals at "
+                               f"{args.rate}/s",
+                   "slots": args.slots, "budget_GB": args.budget_gb, "outer_tolerance": 1e-5,
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                   "runs": runs,
+                   "mixed_over_fp64": (runs["mixed"]["solves_per_s"] / runs["fp64"]["solves_per_s"]
+                                       if "mixed" in runs and "fp64" in runs else None)},
+        "roofline": None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -302,6 +302,75 @@ int bsim_piso_probe_start(bsim_piso* s, int capacity);
 int bsim_piso_probe_stop(bsim_piso* s, int* launches, double* seconds, double* bytes_per_launch);
 void bsim_piso_destroy(bsim_piso* s);
 
+
+/* ---------------------------------------------------------------- serving
+ * No counterpart in the reference, where one RunTime is active per thread
+ * (src/runtime/runtime.cpp:15, 48-51) and a Case owns one mesh
+ * (src/io/case.cpp:280-298).  A server owns `slots` worker threads on one
+ * device.  Meshes are registered once as cases; a request names a case and
+ * is a steady SIMPLE run to convergence (bsim_simple_* semantics, capped at
+ * max_iterations outer iterations) or a transient PISO run of max_iterations
+ * steps of length dt (bsim_piso_* semantics).  Requests become eligible at
+ * their arrival time on the server clock (seconds since create; earlier
+ * times mean at once) and are admitted in arrival order while their device
+ * memory estimate fits the budget next to what is running; each runs on its
+ * own context and stream, so admitted requests overlap on the device.  A
+ * case's contexts (mesh and geometry in HBM) stay resident after a request
+ * and are reused by the next request on that case, until their memory is
+ * needed (least recently used first).  Results equal those of a solo run of
+ * the same request. */
+enum { BSIM_REQ_STEADY = 0, BSIM_REQ_TRANSIENT = 1 };
+/* Request outcomes: converged (steady) or every step completed (transient);
+ * not converged within max_iterations, or a step not converged; numerical
+ * failure; rejected (larger than the whole budget); error (message in the
+ * result); cancelled (still queued when the server was destroyed). */
+enum { BSIM_SERVE_CONVERGED = 0, BSIM_SERVE_NOT_CONVERGED = 1, BSIM_SERVE_NUMERICAL_FAILURE = 2,
+       BSIM_SERVE_REJECTED = 3, BSIM_SERVE_ERROR = 4, BSIM_SERVE_CANCELLED = 5 };
+typedef struct {
+    int kind, case_id;
+    const bsim_simple_config* simple;  /* BSIM_REQ_STEADY */
+    const bsim_piso_config* piso;      /* BSIM_REQ_TRANSIENT */
+    const int* u_bc_type;
+    const double* u_bc_value;
+    const int* p_bc_type;
+    const double* p_bc_value;
+    const double* U0;  /* optional initial fields, caller numbering (NULL: zero) */
+    const double* p0;
+    int max_iterations;
+    double dt, arrival;
+} bsim_request;
+typedef struct {
+    int status, iterations, slot, reused_context;
+    /* server-clock seconds: eligible, admitted, context ready, finished */
+    double released, admitted, started, finished;
+    double rU, dU, dP, mass;              /* last iteration or step */
+    double reserved_bytes, measured_bytes; /* admission reservation; solver allocations measured */
+    char message[256];
+} bsim_request_result;
+typedef struct bsim_server bsim_server;
+int bsim_server_create(int device, int slots, double memory_budget_bytes, bsim_server** out);
+/* Registers a mesh (copied); case ids count from 0. */
+int bsim_server_add_case(bsim_server* s, const bsim_mesh* mesh, int* case_id);
+/* Copies the request's configuration, conditions and fields; returns its id. */
+int bsim_server_submit(bsim_server* s, const bsim_request* req, int* id);
+/* Blocks until the request is done. */
+int bsim_server_wait(bsim_server* s, int id, bsim_request_result* out);
+int bsim_server_done(bsim_server* s, int id, int* done);
+/* U (3 per cell), p (per cell) and phi (per face) of a finished request, caller numbering. */
+int bsim_server_fields(bsim_server* s, int id, double* U, double* p, double* phi);
+/* Drops a finished request's record and fields. */
+int bsim_server_release(bsim_server* s, int id);
+/* budget, peak reserved, reserved, peak running, running, completed,
+ * rejected, contexts created, context reuses, evictions, resident idle contexts */
+int bsim_server_stats(bsim_server* s, double* out11);
+double bsim_server_clock(bsim_server* s);
+/* The admission estimates: device bytes of one context (mesh and geometry)
+ * and of one request's solver on a mesh of the given sizes.  A case's context
+ * is reserved at its measured size once one has been built. */
+int bsim_server_estimate(int kind, int ncells, int nfaces, double* context_bytes, double* solver_bytes);
+/* Cancels queued requests, waits for running ones, frees every context. */
+void bsim_server_destroy(bsim_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
 def test_every_symbol_has_a_python_signature():
     bound = set(_capi._DEVICE_SIGNATURES) | set(_capi._VOID) | {
         "bsim_last_error", "bsim_version", "bsim_mesh_generate_hex", "bsim_mesh_from_arrays", "bsim_mesh_prepare",
-        "bsim_mesh_sizes", "bsim_mesh_arrays", "bsim_mesh_free"}
+        "bsim_mesh_sizes", "bsim_mesh_arrays", "bsim_mesh_free", "bsim_server_clock"}
     assert set(declared_symbols()) <= bound
 
 
