@@ -20,14 +20,17 @@ constexpr long long kMiB = 1LL << 20;
 // orth: ~156 B per face), sliced half-face maps (~16 B per interior face) and
 // cell arrays.  Solvers: the SIMPLE/PISO fields, two double-buffered systems
 // with their Krylov vectors and the AMG hierarchy.  The figures are upper
-// bounds of what the measured allocations of cavity and channel meshes gave;
+// bounds of what the measured allocations of cavity and channel meshes gave
+// (at most 1.94 GB for a 1M-cell steady request, against 2.49 GB reserved);
 // a case's context reservation is replaced by its measured size once built.
 long long Server::contextEstimate(int nc, int nf) {
     return 200LL * nf + 160LL * nc + 2 * kMiB;
 }
 long long Server::solverEstimate(int kind, int nc, int nf) {
-    const long long perCell = kind == kRequestTransient ? 1900 : 1700;
-    return perCell * nc + 120LL * nf + 4 * kMiB;
+    const long long perCell = kind == kRequestTransient ? 2200 : 2000;
+    // The fixed part: two systems' reduction partials (6 x 32768 doubles
+    // each), Krylov scalars and the solver's own reduction buffers.
+    return perCell * nc + 160LL * nf + 10 * kMiB;
 }
 
 Server::Server(int dev, int slots, long long budgetBytes) : device(dev), budget(budgetBytes) {

@@ -91,6 +91,8 @@ def check_equal(server, ids, solo):
         r = server.wait(rid)
         assert r.status == status, r.message
         assert r.iterations == its
+        # The admission reservation covers what the solver really allocated.
+        assert 0 < r.measured_bytes <= r.reserved_bytes
         Us, ps, phis = server.fields(rid, cid)
         np.testing.assert_array_equal(Us, U)
         np.testing.assert_array_equal(ps, p)
