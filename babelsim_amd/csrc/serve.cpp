@@ -194,6 +194,10 @@ void Server::worker(int slot) {
             r->status = kServeError;
             r->error = e.what();
             ctx.reset();  // a context that saw a failure is not kept
+        } catch (...) {
+            r->status = kServeError;
+            r->error = "unknown error";
+            ctx.reset();
         }
         const double tf = now();
         lk.lock();

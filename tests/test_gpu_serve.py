@@ -213,3 +213,23 @@ def test_served_request_matches_the_oracle():
     Uo, po, phio = so.fields()
     for a, b in ((U, Uo), (p, po), (phi, phio)):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
+
+
+def test_failed_request_reports_an_error_and_the_server_goes_on(solo):
+    """A request whose solver rejects its configuration inside the worker
+    (negative viscosity: SimpleSolver's own check) ends with SERVE_ERROR and
+    the message; its context is dropped and the next request is served as a
+    solo run would be."""
+    from babelsim_amd import SERVE_ERROR
+    w = workload()[1]
+    with Server(slots=1, memory_budget=8e9) as srv:
+        cid = srv.add_case(w["mesh"])
+        bad = sc.serve_config(400.0)
+        bad.mu = -1.0
+        r = srv.wait(srv.submit_steady(cid, bad, *w["bcs"], max_iterations=5))
+        assert r.status == SERVE_ERROR
+        assert "viscosity" in r.message
+        st = srv.stats()
+        assert st["resident_idle"] == 0 and st["reserved"] == 0
+        ids = [(srv.submit_steady(cid, w["cfg"], *w["bcs"], max_iterations=CAP), cid)]
+        check_equal(srv, ids, [solo[1]])
