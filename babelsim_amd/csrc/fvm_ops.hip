@@ -59,13 +59,33 @@ template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, i
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) A[i][j] += wv[i] * dv[j];
         for (int i = 0; i < 3; ++i) for (int k = 0; k < NC; ++k) rhs[i * NC + k] += wv[i] * diff[k];
     };
-    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
-        if (m.mface[e] < 0) continue;
-        const int other = m.mcol[e];
-        double diff[NC];
-        cellValue<NC>(x, other, val);
-        for (int k = 0; k < NC; ++k) diff[k] = val[k] - xc[k];
-        addRow(ld3(m.C, other) - cc, diff);
+    // Interior neighbours kC at a time: their columns, then their centres and
+    // values, are loaded together; the rows are added in entry order.
+    constexpr int kC = NC == 1 ? 8 : 4;
+    const RowRange R = m.row(c);
+    for (int k0 = 0; k0 < R.K && R.valid(k0); k0 += kC) {
+        int other[kC];
+#pragma unroll
+        for (int j = 0; j < kC; ++j) {
+            const int k = k0 + j;
+            const int e = k < R.K && R.valid(k) ? R.at(k) : -1;
+            other[j] = e >= 0 && m.mface[e] >= 0 ? m.mcol[e] : -1;
+        }
+        double nv[kC][NC];
+        D3 nc_[kC];
+#pragma unroll
+        for (int j = 0; j < kC; ++j)
+            if (other[j] >= 0) {
+                cellValue<NC>(x, other[j], nv[j]);
+                nc_[j] = ld3(m.C, other[j]);
+            }
+#pragma unroll
+        for (int j = 0; j < kC; ++j)
+            if (other[j] >= 0) {
+                double diff[NC];
+                for (int k = 0; k < NC; ++k) diff[k] = nv[j][k] - xc[k];
+                addRow(nc_[j] - cc, diff);
+            }
     }
     for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
         const int f = m.bface[e];
