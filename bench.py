@@ -29,8 +29,9 @@ channel requests (256 from a seed, jittered hex meshes log-uniform between
 Poisson arrivals at a fixed rate) through the serving runtime: HBM-budget
 admission, concurrent requests on their own streams, resident per-mesh
 contexts.  The stream runs with mixed precision (FP32 AMG and inner Krylov
-in FP64 refinement) and again in plain FP64; value is the mixed path's
-converged solves/s, with p50/p99 time-to-converged for both.
+in FP64 refinement) (`--serve-precision both` runs it again in plain FP64;
+about 12 minutes per pass); value is the mixed path's converged solves/s,
+with p50/p99 time-to-converged.
 
 Prints one JSON line (rank 0, one GPU).
 """
@@ -97,7 +98,8 @@ def parse():
     ap.add_argument("--budget-gb", type=float, default=64.0, help="serve: device memory budget")
     ap.add_argument("--cells-min", type=float, default=1e4)
     ap.add_argument("--cells-max", type=float, default=1e6)
-    ap.add_argument("--serve-precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--serve-precision", choices=["both", "mixed", "fp64"], default="mixed",
+                    help="serve: the full stream takes about 12 minutes per precision on one MI355X")
     ap.add_argument("--max-outer", type=int, default=3000, help="serve: outer-iteration cap per request")
     ap.add_argument("--serve-trace", default=None,
                     help="serve: write one CSV row per request (size, iterations, server-clock times) to this path")
