@@ -111,8 +111,11 @@ def parse():
                     help="timed oracle iterations for the CPU baseline (3; 1 for the channel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rebuild-interval", type=int, default=8)
-    ap.add_argument("--amg-strength", type=float, default=0.7,
+    ap.add_argument("--amg-strength", type=float, default=0.65,
                     help="cavity: strength threshold of the pressure AMG aggregation")
+    ap.add_argument("--amg-correction", type=float, default=2.0,
+                    help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
+    ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
     ap.add_argument("--stream-launches", action="store_true",
                     help="launch the AMG cycles kernel by kernel instead of replaying captured graphs "
                          "(same results; for tracing tools)")
@@ -168,7 +171,8 @@ def main():
         mom=solver_config(solver=1, pc=1, rtol=1e-6, atol=1e-14, precision=prec,
                           rebuild_interval=args.rebuild_interval),
         pre=solver_config(solver=0, pc=2, rtol=1e-6, atol=1e-14, precision=prec,
-                          rebuild_interval=args.rebuild_interval, amg_strength=args.amg_strength))
+                          rebuild_interval=args.rebuild_interval, amg_strength=args.amg_strength,
+                          amg_correction=args.amg_correction, amg_omega=args.amg_omega))
     ub, pb = cases.cavity_bcs(two_d=False)
     ctx = Context(mesh, renumber=True)
     log(f"context ready: N={ctx.nc} F={ctx.nf} nnz_off={ctx.nnz_off} orthogonal={ctx.orthogonal} "
@@ -244,7 +248,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3-D lid-driven cavity Re=400, {n}^3 hex lattice, +-20% jitter, ~10% prism splits",
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
-                   "pressure": f"CG + aggregation AMG (strength {args.amg_strength})",
+                   "pressure": f"CG + aggregation AMG (strength {args.amg_strength}, correction "
+                               f"{args.amg_correction}, Jacobi weight {args.amg_omega})",
                    "momentum": "BiCGSTAB + Jacobi",
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
                    "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb},
