@@ -116,6 +116,10 @@ def parse():
     ap.add_argument("--amg-correction", type=float, default=2.0,
                     help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
     ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="cavity, cylinder: start the momentum and pressure solves from the current unknowns "
+                         "(the equation.<name>.warmStart case setting; off, its default, in the cavity and "
+                         "cylinder_re100 cases)")
     ap.add_argument("--stream-launches", action="store_true",
                     help="launch the AMG cycles kernel by kernel instead of replaying captured graphs "
                          "(same results; for tracing tools)")
@@ -169,9 +173,10 @@ def main():
     prec = 1 if args.precision == "mixed" else 0
     cfg = cases.c2_config(
         mom=solver_config(solver=1, pc=1, rtol=1e-6, atol=1e-14, precision=prec,
-                          rebuild_interval=args.rebuild_interval),
+                          rebuild_interval=args.rebuild_interval, warm_start=int(args.warm_start)),
         pre=solver_config(solver=0, pc=2, rtol=1e-6, atol=1e-14, precision=prec,
-                          rebuild_interval=args.rebuild_interval, amg_strength=args.amg_strength,
+                          rebuild_interval=args.rebuild_interval, warm_start=int(args.warm_start),
+                          amg_strength=args.amg_strength,
                           amg_correction=args.amg_correction, amg_omega=args.amg_omega))
     ub, pb = cases.cavity_bcs(two_d=False)
     ctx = Context(mesh, renumber=True)
@@ -250,7 +255,7 @@ def main():
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
                    "pressure": f"CG + aggregation AMG (strength {args.amg_strength}, correction "
                                f"{args.amg_correction}, Jacobi weight {args.amg_omega})",
-                   "momentum": "BiCGSTAB + Jacobi",
+                   "momentum": "BiCGSTAB + Jacobi", "warm_start": args.warm_start,
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
                    "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -291,8 +296,10 @@ def bench_cylinder(args):
     # count as strong.
     cfg = piso_config(rho=1.0, mu=0.01, m_grad=1, m_diff=1, m_coef_grad=1, p_grad=1, p_diff=2, p_coef_grad=1,
                       time_method=BDF2, n_correctors=2, non_orth_corr=2, tol_mass=1e-7,
-                      mom=solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-12, precision=1),
-                      pre=solver_config(solver=0, pc=2, rtol=1e-9, atol=1e-13, amg_strength=0.85))
+                      mom=solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-12, precision=1,
+                                        warm_start=int(args.warm_start)),
+                      pre=solver_config(solver=0, pc=2, rtol=1e-9, atol=1e-13, amg_strength=0.85,
+                                        warm_start=int(args.warm_start)))
     bcs = cm.cylinder_bcs()
     U0 = cm.potential_flow(ctx.geometry()["C"])
     sim = Piso(ctx, cfg, *bcs)
@@ -368,7 +375,7 @@ def bench_cylinder(args):
                    "cells": ctx.nc, "faces": ctx.nf, "dt": dt, "correctors": 2, "non_orthogonal_corrections": 2,
                    "pressure_diffusion": "limitedCorrected",
                    "time": "BDF2", "momentum": "FP32 BiCGSTAB + Jacobi in FP64 refinement",
-                   "pressure": "FP64 CG + aggregation AMG (strength 0.85)",
+                   "pressure": "FP64 CG + aggregation AMG (strength 0.85)", "warm_start": args.warm_start,
                    "last_step": {"CoMax": last.co_max, "p_its": last.p_iters, "mom_its": list(last.mom_iters),
                                  "coupling": last.r_coupling, "mass": last.mass},
                    "phase_seconds_per_step": timing, "device_memory_used_GB": device_memory_used_gb()},
