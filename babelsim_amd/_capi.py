@@ -253,7 +253,7 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_simple_get_fields": [vp, dp, dp, dp],
     "bsim_simple_history": [vp, C.c_int, dp, C.c_int, ip],
     "bsim_simple_timing": [vp, dp],
-    "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, dp, ip, ip],
+    "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, C.c_int, dp, ip, ip],
     "bsim_simple_probe_start": [vp, C.c_int],
     "bsim_simple_probe_stop": [vp, ip, dp, dp],
     "bsim_piso_create": [vp, C.POINTER(PisoConfig), ip, dp, ip, dp, C.POINTER(vp)],

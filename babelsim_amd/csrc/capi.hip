@@ -547,10 +547,10 @@ int bsim_simple_timing(bsim_simple* s, double* o) {
         std::memcpy(o, v, sizeof(v));
     });
 }
-int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, int* n, int* nnz) {
+int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, int cold, double* seconds, int* n, int* nnz) {
     return guarded([&] {
         DeviceSystem& D = which == 0 ? s->s->momentumSystem() : s->s->pressureSystem();
-        *seconds = D.timeSpmv(reps);
+        *seconds = D.timeSpmv(reps, cold != 0);
         *n = D.n;
         *nnz = D.nnz;
     });

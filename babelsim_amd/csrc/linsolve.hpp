@@ -104,7 +104,11 @@ public:
     // order, with BiCGSTAB (Jacobi or no preconditioner, FP64) run in lockstep.
     void solve3(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3]);
     void spmv(const double* x, double* y);   // y = A x with the effective matrix
-    double timeSpmv(int reps);               // mean seconds of the solver's SpMV kernel (HIP events)
+    // Mean seconds of the solver's SpMV kernel (HIP events).  cold: each launch
+    // follows a read-only pass over a buffer twice the size of the 256 MB
+    // Infinity Cache, so the matrix and vectors come from HBM rather than from
+    // that cache.
+    double timeSpmv(int reps, bool cold = false);
     std::vector<double> hist;               // unpreconditioned residual norm per Krylov iteration
     SystemCounters counters;
     SolverCfg cfg;

@@ -265,11 +265,13 @@ class Simple:
                 "convergence", "iterations", "turbulence"]
         return dict(zip(keys, out.tolist()))
 
-    def time_spmv(self, which=1, reps=20):
+    def time_spmv(self, which=1, reps=20, cold=False):
+        """Mean seconds of one SpMV on the momentum (0) or pressure (1) matrix;
+        cold: each launch follows a 512 MiB pass that evicts the Infinity Cache."""
         s = C.c_double()
         n = C.c_int()
         nnz = C.c_int()
-        check(lib().bsim_simple_time_spmv(self.h, which, reps, C.byref(s), C.byref(n), C.byref(nnz)))
+        check(lib().bsim_simple_time_spmv(self.h, which, reps, int(cold), C.byref(s), C.byref(n), C.byref(nnz)))
         return s.value, n.value, nnz.value
 
     def probe_start(self, capacity=8192):

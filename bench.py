@@ -126,6 +126,10 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device-to-device copies (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--read-ceiling", action="store_true",
+                    help="only time torch's read-only sum over 256 MiB and 2 GiB of HBM (each after a 1 GiB "
+                         "read pass that evicts the Infinity Cache) and exit: the practical read bandwidth "
+                         "that the SpMV and smoother figures compare with")
     return ap.parse_args()
 
 
@@ -142,10 +146,36 @@ def copy_calibration():
     return 0
 
 
+def read_ceiling():
+    import torch
+    flush = torch.zeros(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    out = torch.empty((), dtype=torch.float64, device="cuda")
+    res = {}
+    for mib in (256, 2048):
+        a = torch.ones((mib << 20) // 8, dtype=torch.float64, device="cuda")
+        times = []
+        for _ in range(12):
+            flush.sum()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            torch.sum(a, dim=0, out=out)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) * 1e-3)
+        t = sorted(times[2:])[len(times[2:]) // 2]
+        res[f"{mib}MiB"] = {"seconds": t, "GB/s": a.numel() * 8 / t / 1e9}
+        del a
+    print(json.dumps({"read_only_sum_after_eviction": res}), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         return copy_calibration()
+    if args.read_ceiling:
+        return read_ceiling()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
@@ -213,11 +243,15 @@ def main():
     sweep_avg = sweep_s / max(launches, 1)
     achieved = sweep_bytes / sweep_avg / 1e9 if launches else 0.0
     log(f"fine-level sweep: {launches} launches, {sweep_avg * 1e6:.1f} us each, {achieved:.0f} GB/s")
-    # Auxiliary: the pressure matrix SpMV alone (after the timed region).
-    spmv_s, N, nnz_off = sim.time_spmv(1, reps=50)
+    # Auxiliary: the pressure matrix SpMV alone (after the timed region),
+    # from HBM (each launch after a pass that evicts the 256 MB Infinity
+    # Cache) and back to back (where the cache may hold part of the matrix).
+    spmv_s, N, nnz_off = sim.time_spmv(1, reps=20, cold=True)
+    spmv_b2b_s, _, _ = sim.time_spmv(1, reps=50)
     nnz = N + nnz_off
     spmv_bytes = 12 * nnz_off + 24 * N
-    log(f"spmv: {spmv_s * 1e6:.1f} us per launch, {spmv_bytes / spmv_s / 1e9:.0f} GB/s")
+    log(f"spmv: {spmv_s * 1e6:.1f} us per launch from HBM, {spmv_bytes / spmv_s / 1e9:.0f} GB/s; "
+        f"back to back {spmv_b2b_s * 1e6:.1f} us, {spmv_bytes / spmv_b2b_s / 1e9:.0f} GB/s")
     traffic = measured_traffic(ctx.nc)
     cpu = None
     cpu_iters = 3 if args.cpu_iters is None else args.cpu_iters
@@ -266,7 +300,9 @@ def main():
                      "timing": f"HIP events around every sweep launch over {args.probe_steps} iterations run "
                                "right after the timed region with stream launches",
                      "spmv": {"seconds_per_launch": spmv_s, "algorithmic_bytes_per_launch": spmv_bytes,
-                              "GB/s": spmv_bytes / spmv_s / 1e9}},
+                              "GB/s": spmv_bytes / spmv_s / 1e9, "frac": spmv_bytes / spmv_s / 1e9 / PEAK_HBM_GBS,
+                              "cache": "each launch after a 512 MiB pass that evicts the Infinity Cache",
+                              "back_to_back_GB/s": spmv_bytes / spmv_b2b_s / 1e9}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

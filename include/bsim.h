@@ -232,10 +232,13 @@ int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* le
 /* Seconds per phase: assembly, momentum solve, Rhie-Chow, pressure assembly,
  * pressure solve, correction, convergence, iterations, turbulence transport. */
 int bsim_simple_timing(bsim_simple* s, double* out9);
-/* Times `reps` launches of the solver's fused SpMV on the current pressure
- * matrix with HIP events on the context stream; returns mean seconds and the
- * matrix size. */
-int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, double* seconds, int* n, int* nnz_off);
+/* Times `reps` launches of the solver's SpMV on the current momentum (which
+ * 0) or pressure (1) matrix with HIP events on the context stream; returns
+ * mean seconds and the matrix size.  cold != 0: every launch follows a
+ * read-only pass over a 512 MiB buffer (twice the 256 MB Infinity Cache), so its operands
+ * are read from HBM; cold == 0: back-to-back launches, which may find part of
+ * a matrix up to that size in the cache. */
+int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, int cold, double* seconds, int* n, int* nnz_off);
 /* Diagnostics with no counterpart in the reference: HIP-event timing of the
  * fine-level AMG smoother sweeps of the pressure solve.  start() creates a
  * pool of `capacity` event pairs and times every later sweep launch (up to

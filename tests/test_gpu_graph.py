@@ -78,3 +78,27 @@ def test_stream_launch_switch_gives_same_bits():
     np.testing.assert_array_equal(ha, hb)
     for a, b in zip(fa, fb):
         np.testing.assert_array_equal(a, b)
+
+
+def test_spmv_timing_modes_leave_the_solve_unchanged():
+    """bsim_simple_time_spmv from HBM (Infinity Cache evicted before every
+    launch) and back to back: both return a positive mean time and the matrix
+    size, and the iterations after them give the same bits as a run without."""
+    mesh = cases.cavity3d_mesh(9, split=0.1, seed=5)
+    cfg = cases.c2_config()
+    out = []
+    for timed in (False, True):
+        ctx = Context(mesh)
+        s = Simple(ctx, cfg, *cases.cavity_bcs(False))
+        s.start()
+        s.iterate(2)
+        if timed:
+            for cold in (True, False):
+                sec, n, nnz = s.time_spmv(1, reps=3, cold=cold)
+                assert n == ctx.nc and nnz > 0 and 0.0 < sec < 1.0
+        s.iterate(2)
+        out.append((s.fields(), s.history(3)))
+    (fa, ha), (fb, hb) = out
+    np.testing.assert_array_equal(ha, hb)
+    for a, b in zip(fa, fb):
+        np.testing.assert_array_equal(a, b)
