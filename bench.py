@@ -117,7 +117,7 @@ def parse():
                     help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
     ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
     ap.add_argument("--warm-start", action="store_true",
-                    help="cavity, cylinder: start the momentum and pressure solves from the current unknowns "
+                    help="cavity, cylinder, channel: start the linear solves from the current unknowns "
                          "(the equation.<name>.warmStart case setting; off, its default, in the cavity and "
                          "cylinder_re100 cases)")
     ap.add_argument("--stream-launches", action="store_true",
@@ -439,7 +439,7 @@ def bench_channel(args):
     mesh = ch.graded_channel_mesh()
     ctx = Context(mesh, renumber=True)
     log(f"graded channel: N={ctx.nc} F={ctx.nf} nnz_off={ctx.nnz_off} ({time.time() - t0:.1f}s)")
-    cfg, kw = ch.channel_config(precision=1, rebuild_interval=args.rebuild_interval)
+    cfg, kw = ch.channel_config(precision=1, rebuild_interval=args.rebuild_interval, warm_start=int(args.warm_start))
     ub, pb, kb, wb = ch.channel_bcs()
     sim = Simple(ctx, cfg, ub, pb)
     sim.enable_komega(kw, kb, wb)
@@ -508,7 +508,7 @@ def bench_channel(args):
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": ctx.nc + ctx.nnz_off,
                    "momentum": "FP32 BiCGSTAB + Jacobi in FP64 refinement",
                    "turbulence": "omega then k, FP32 BiCGSTAB + Jacobi in FP64 refinement",
-                   "pressure": "FP32-AMG CG in FP64 refinement (strength 0.85)",
+                   "pressure": "FP32-AMG CG in FP64 refinement (strength 0.85)", "warm_start": args.warm_start,
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters),
                                       "k_its": last.k_iters, "w_its": last.w_iters, "rU": last.rU,
                                       "d_turb": last.d_turb},
