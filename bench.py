@@ -117,7 +117,7 @@ def parse():
                     help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
     ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
     ap.add_argument("--warm-start", action="store_true",
-                    help="cavity, cylinder, channel: start the linear solves from the current unknowns "
+                    help="all workloads: start the linear solves from the current unknowns "
                          "(the equation.<name>.warmStart case setting; off, its default, in the cavity and "
                          "cylinder_re100 cases)")
     ap.add_argument("--stream-launches", action="store_true",
@@ -538,7 +538,7 @@ def run_stream(args, specs, meshes, precision, log):
     t0 = srv.clock() + lead
     ids = []
     for sp in specs:
-        cfg = sc.serve_config(sp.re, precision=precision)
+        cfg = sc.serve_config(sp.re, precision=precision, warm_start=int(args.warm_start))
         ub, pb = sc.bcs_for(sp)
         ids.append(srv.submit_steady(case_ids[(sp.kind, sp.side)], cfg, ub, pb, max_iterations=args.max_outer,
                                      arrival=t0 + sp.arrival))
@@ -616,7 +616,7 @@ def bench_serve(args):
         sp = min(specs, key=lambda q: (q.cells, q.index))
         ub, pb = sc.bcs_for(sp)
         om = orc.oracle_mesh(meshes[(sp.kind, sp.side)].arrays())
-        so = orc.OracleSimple(om, sc.serve_config(sp.re), ub, pb)
+        so = orc.OracleSimple(om, sc.serve_config(sp.re, warm_start=int(args.warm_start)), ub, pb)
         so.start()
         t4 = time.time()
         n = 0

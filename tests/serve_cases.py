@@ -59,19 +59,22 @@ def channel_bcs():
     return (ut, uv), (pt, np.zeros(6))
 
 
-def serve_config(re: float, precision: int = 0, tol: float = 1e-5, rtol: float = 1e-6):
+def serve_config(re: float, precision: int = 0, tol: float = 1e-5, rtol: float = 1e-6, warm_start: int = 0):
     """Laminar SIMPLE on a jittered mesh: least-squares gradients, corrected
     diffusion, linearUpwind, alpha_U 0.7 / alpha_p 0.3; BiCGSTAB+Jacobi
     momentum and CG+AMG pressure (strength threshold 0.65, over-correction
     2.0, Jacobi weight 0.9) at rtol (1e-6); the four outer tolerances
     `tol`.  precision 1 runs the AMG and the momentum Krylov in FP32 inside
-    FP64 refinement."""
+    FP64 refinement.  warm_start starts both linear solves from the current
+    unknowns."""
     return simple_config(rho=1.0, mu=1.0 / re, m_grad=1, m_diff=1, m_coef_grad=1, p_grad=1, p_diff=1,
                          p_coef_grad=1, alpha_u=0.7, alpha_p=0.3, non_orth_corr=1,
                          tol_mass=tol, tol_u=tol, tol_mom=tol, tol_p=tol,
-                         mom=solver_config(solver=1, pc=1, rtol=rtol, atol=1e-14, precision=precision),
+                         mom=solver_config(solver=1, pc=1, rtol=rtol, atol=1e-14, precision=precision,
+                                           warm_start=warm_start),
                          pre=solver_config(solver=0, pc=2, rtol=rtol, atol=1e-14, precision=precision,
-                                           amg_strength=0.65, amg_correction=2.0, amg_omega=0.9))
+                                           amg_strength=0.65, amg_correction=2.0, amg_omega=0.9,
+                                           warm_start=warm_start))
 
 
 def stream(count: int = 256, rate: float = 4.0, seed: int = 20261207, cells_min: float = 1e4,
