@@ -12,7 +12,7 @@ import meshes
 from babelsim_amd import solver_config
 from babelsim_amd.device import Context, Simple, System, amg_hierarchy, csr_solve
 import oracle_lib as orc
-from test_oracle_reference import fields, scalar_bcs, vector_bcs
+from op_fields import fields, scalar_bcs, vector_bcs
 
 pytestmark = pytest.mark.gpu
 

@@ -6,6 +6,7 @@ import pytest
 
 import meshes
 from oracle_lib import oracle_mesh, reference, reference_mesh
+from op_fields import fields, scalar_bcs, vector_bcs
 
 pytestmark = pytest.mark.skipif(reference() is None, reason="reference discretisation layer not built here")
 
@@ -17,29 +18,6 @@ def close(a, b, rtol=RTOL):
     scale = max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
     err = float(np.max(np.abs(a - b))) if a.size else 0.0
     assert err <= rtol * scale, f"max abs err {err:.3e} (scale {scale:.3e})"
-
-
-def scalar_bcs(npatch):
-    types = [0, 1, 2, 4, 3, 0][:npatch] if npatch > 1 else [0]
-    vals = [1.5, 0.7, 0.0, 0.0, 2.0, -0.5][:npatch]
-    return np.array(types, np.int32), np.array(vals)
-
-
-def vector_bcs(npatch):
-    types = [0, 1, 2, 4, 3, 0][:npatch] if npatch > 1 else [0]
-    vals = np.array([[1, 0.5, -0.2], [0.3, -0.1, 0.2], [0, 0, 0], [0, 0, 0], [0.4, 0.2, 0.1], [-0.5, 0.25, 1.0]])
-    return np.array(types, np.int32), vals[:npatch]
-
-
-def fields(om, seed=3):
-    g = om.geometry()
-    rng = np.random.default_rng(seed)
-    C = g["C"]
-    s = np.sin(C[:, 0]) + 0.3 * C[:, 1] ** 2 - 0.2 * C[:, 2] + 0.01 * rng.standard_normal(om.nc)
-    U = np.stack([np.cos(C[:, 1]) + C[:, 2], 0.5 * C[:, 0] * C[:, 2], -0.3 * C[:, 1] + 0.1 * C[:, 0]], 1)
-    U += 0.01 * rng.standard_normal(U.shape)
-    phi = rng.standard_normal(om.nf)
-    return s, U, phi
 
 
 @pytest.fixture(params=list(meshes.ALL))
