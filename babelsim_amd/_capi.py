@@ -242,6 +242,8 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_csr_solve": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult),
                        dp, C.c_int, ip],
     "bsim_csr_spmv": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp],
+    "bsim_csr_solve_noalloc": [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, C.POINTER(SolverConfig),
+                               C.POINTER(SolveResult), dp, C.c_int, ip],
     "bsim_amg_hierarchy": [C.c_int, C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp],
     "bsim_simple_create": [vp, C.POINTER(SimpleConfig), ip, dp, ip, dp, C.POINTER(vp)],
     "bsim_simple_set_fields": [vp, dp, dp],

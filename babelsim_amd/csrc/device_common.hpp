@@ -35,21 +35,29 @@ inline int blocksFor(long long n, int block = kBlock) {
 // released; the serving runtime reads its change across one request.
 inline thread_local long long t_deviceBytes = 0;
 
-// Owning device allocation of T[n].
+// Device allocation of T[n]: owning (alloc) or a view of memory that someone
+// else owns (borrow).  A borrowed buffer never allocates or frees: growing it
+// past its size throws instead.
 template <class T> struct DBuf {
     T* p = nullptr;
     std::size_t n = 0;
+    bool owned = true;
     DBuf() = default;
     explicit DBuf(std::size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) { o.p = nullptr; o.n = 0; o.owned = true; }
     DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; owned = o.owned;
+            o.p = nullptr; o.n = 0; o.owned = true;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(std::size_t count) {
+        if (!owned) throw std::logic_error("a borrowed device buffer cannot be reallocated");
         release();
         if (count) {
             BSIM_HIP(hipMalloc(&p, sizeof(T) * count));
@@ -57,14 +65,21 @@ template <class T> struct DBuf {
         }
         n = count;
     }
+    void borrow(T* ptr, std::size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
+    }
     void ensure(std::size_t count) { if (count > n) alloc(count); }
     void release() {
-        if (p) {
+        if (p && owned) {
             (void)hipFree(p);
             t_deviceBytes -= static_cast<long long>(sizeof(T) * n);
         }
         p = nullptr;
         n = 0;
+        owned = true;
     }
     void upload(const T* src, std::size_t count, hipStream_t s) {
         ensure(count);

@@ -1263,7 +1263,46 @@ DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
     BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hact3), sizeof(int) * 4 * kActSlots, hipHostMallocDefault));
 }
 
+namespace {
+constexpr std::size_t kWsAlign = 32;  // doubles: 256 bytes
+std::size_t wsRound(std::size_t v) { return (v + kWsAlign - 1) / kWsAlign * kWsAlign; }
+constexpr int kWsVectors = 12;        // r z p w rp v t s tmp guess saved jinv
+}  // namespace
+
+std::size_t DeviceSystem::workspaceDoubles(int n) {
+    return 2 * kWsAlign + 6 * static_cast<std::size_t>(kMaxReduceBlocks) +
+           kWsVectors * wsRound(static_cast<std::size_t>(n));
+}
+
+DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern, const SystemWorkspace& ws)
+    : n(pattern.n), nnz(pattern.nnzStore), stream(st), pat(std::move(pattern)) {
+    static_assert(S_COUNT + 2 <= 32, "host scalar mirror");
+    if (!ws.diag || (!ws.val && nnz > 0) || !ws.scratch) throw std::invalid_argument("null workspace");
+    if (ws.doubles < workspaceDoubles(n)) throw std::invalid_argument("workspace is too small for the system");
+    borrowed = true;
+    // Both matrix slots view the caller's values: the first update() makes
+    // them effective without a comparison, and nothing here writes them.
+    for (int k = 0; k < 2; ++k) {
+        diagBuf[k].borrow(const_cast<double*>(ws.diag), n);
+        valBuf[k].borrow(const_cast<double*>(ws.val), nnz);
+    }
+    double* q = ws.scratch;
+    sc.borrow(q, S_COUNT);
+    q += kWsAlign;
+    flags.borrow(reinterpret_cast<int*>(q), 1);
+    q += kWsAlign;
+    partials.borrow(q, 6 * static_cast<std::size_t>(kMaxReduceBlocks));
+    q += 6 * static_cast<std::size_t>(kMaxReduceBlocks);
+    const std::size_t step = wsRound(static_cast<std::size_t>(n));
+    for (DBuf<double>* b : {&r, &z, &p, &w, &rp, &v, &t, &s, &tmp, &guess, &saved, &jinv}) {
+        b->borrow(q, static_cast<std::size_t>(n));
+        q += step;
+    }
+    hostScalars = hostLocal;
+}
+
 DeviceSystem::~DeviceSystem() {
+    if (borrowed) return;
     if (hostScalars) (void)hipHostFree(hostScalars);
     if (hsc3) (void)hipHostFree(hsc3);
     if (hact3) (void)hipHostFree(hact3);
@@ -1290,6 +1329,9 @@ bool DeviceSystem::update() {
 }
 
 void DeviceSystem::prepare(const SolverCfg& c) {
+    if (borrowed && (c.pc == 2 || c.precision != 0 || c.solver < 0 || c.solver > 1))
+        throw std::invalid_argument("a system in a fixed workspace runs FP64 CG or BiCGSTAB with no "
+                                    "preconditioner or Jacobi only");
     if (!configured || !(c == cfg)) {
         cfg = c;
         configured = true;
@@ -1354,6 +1396,7 @@ void DeviceSystem::spmv(const double* x, double* y) {
 
 double DeviceSystem::timeSpmv(int reps, bool cold) {
     if (reps <= 0) throw std::invalid_argument("reps must be positive");
+    if (borrowed) throw std::logic_error("SpMV timing allocates; not available in a fixed workspace");
     kFill<<<blocksFor(n), kBlock, 0, stream>>>(n, p.p, 1.0);
     kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);  // warm
     if (!cold) {
@@ -1812,6 +1855,7 @@ void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T*
 }
 
 void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes res[3], std::vector<double> hists[3]) {
+    if (borrowed) throw std::logic_error("the lockstep vector solve allocates; not available in a fixed workspace");
     if (!ready || !configured || setupPending)
         throw std::logic_error("linear system must update and prepare before solving a right-hand side");
     if (cfg.solver == 1 && cfg.pc != 2 && cfg.precision == 1) {

@@ -85,10 +85,26 @@ struct SystemCounters {
 void setGraphReplay(bool on);
 bool graphReplay();
 
+// Device memory a system can run in without allocating: the caller's matrix
+// (diagonal and off-diagonal values in the pattern's storage order) and a
+// scratch region of at least workspaceDoubles(n) doubles, 256-byte aligned,
+// for the Krylov vectors, reduction partials and device scalars.  A system
+// built on it runs FP64 CG or BiCGSTAB with no preconditioner or Jacobi;
+// anything that would allocate (AMG, mixed precision, the lockstep vector
+// solve, SpMV timing) throws instead.
+struct SystemWorkspace {
+    const double* diag;
+    const double* val;
+    double* scratch;
+    std::size_t doubles;
+};
+
 class DeviceSystem {
 public:
     DeviceSystem(hipStream_t stream, Pattern pattern);
+    DeviceSystem(hipStream_t stream, Pattern pattern, const SystemWorkspace& ws);
     ~DeviceSystem();
+    static std::size_t workspaceDoubles(int n);
     int n, nnz;  // rows, stored off-diagonal entries
     // Assembly writes the candidate matrix here; update() compares it with the
     // effective matrix and swaps it in when any value differs.
@@ -170,7 +186,9 @@ private:
     DBuf<double> sc;            // device scalars
     DBuf<int> flags;
     DBuf<double> partials;      // per-block reduction partials
-    double* hostScalars = nullptr;  // pinned mirror
+    double* hostScalars = nullptr;  // pinned mirror (or hostLocal on a workspace)
+    double hostLocal[32] = {};
+    bool borrowed = false;      // runs in a SystemWorkspace
 };
 
 // Aggregation AMG V-cycle in FP64 or FP32 with an FP64 dense coarsest solve.

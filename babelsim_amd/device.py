@@ -179,6 +179,20 @@ def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, device=0, hist_cap
     return x, res, hist[:ln.value].copy()
 
 
+def csr_solve_noalloc(ptr_, col, diag, val, b, x, cfg: SolverConfig, device=0, stream=0, hist_cap=100000):
+    """bsim_csr_solve_noalloc on device arrays: ptr_ and col int32, diag, val, b
+    and x float64, each an object with data_ptr() (a torch tensor on the
+    device); x holds the initial guess and is overwritten.  stream is a
+    hipStream_t handle (0: the null stream)."""
+    res = SolveResult()
+    hist = np.zeros(hist_cap)
+    ln = C.c_int()
+    check(lib().bsim_csr_solve_noalloc(device, stream or None, len(diag), ptr_.data_ptr(), col.data_ptr(),
+                                       diag.data_ptr(), val.data_ptr(), b.data_ptr(), x.data_ptr(), C.byref(cfg),
+                                       C.byref(res), ptr(hist), hist_cap, C.byref(ln)))
+    return res, hist[:ln.value].copy()
+
+
 def csr_spmv(ptr_, col, diag, val, x, device=0):
     y = np.zeros(len(diag))
     check(lib().bsim_csr_spmv(device, len(diag), ptr(_i32(ptr_), ip), ptr(_i32(col), ip), ptr(_f64(diag)),

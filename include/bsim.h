@@ -173,6 +173,22 @@ int bsim_csr_solve(int device, int n, const int* ptr, const int* col, const doub
                    int cap, int* len);
 int bsim_csr_spmv(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
                   const double* x, double* y);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code between HIP graph captures, tight serving loops, harnesses that
+ * forbid hipMalloc): bsim_csr_solve on caller-owned DEVICE arrays (ptr n+1,
+ * col and val nnz, diag, b and x n), enqueued on `stream` (a hipStream_t; NULL
+ * is the null stream), with the same results, history and status.  It
+ * allocates nothing: its Krylov vectors, reduction partials and scalars live in
+ * a fixed device array of the library (49.5 MiB, held on each device it runs
+ * on until the process ends).  Limits: n <= BSIM_NOALLOC_MAX_ROWS, FP64 CG or
+ * BiCGSTAB with BSIM_PC_NONE or BSIM_PC_JACOBI (larger systems, AMG and mixed
+ * precision return 2).  Calls on one device share that array; the library
+ * serialises them and each returns after its stream has drained, so calls
+ * that overlap on one device wait for each other rather than race. */
+#define BSIM_NOALLOC_MAX_ROWS 524288
+int bsim_csr_solve_noalloc(int device, void* stream, int n, const int* ptr, const int* col, const double* diag,
+                           const double* val, const double* b, double* x, const bsim_solver_config* cfg,
+                           bsim_solve_result* res, double* hist, int cap, int* len);
 int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
                        const bsim_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
 
