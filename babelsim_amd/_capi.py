@@ -62,7 +62,8 @@ class IterReport(C.Structure):
                 ("linP", C.c_double), ("mom_iters", C.c_int * 3), ("p_iters", C.c_int),
                 ("mom_status", C.c_int), ("p_status", C.c_int), ("converged", C.c_int), ("referenced", C.c_int),
                 ("d_turb", C.c_double), ("r_turb", C.c_double), ("bounded_omega", C.c_double),
-                ("k_iters", C.c_int), ("w_iters", C.c_int), ("turb_linear", C.c_int), ("turb_converged", C.c_int)]
+                ("k_iters", C.c_int), ("w_iters", C.c_int), ("turb_linear", C.c_int), ("turb_converged", C.c_int),
+                ("numerical_failure", C.c_int)]
 
 
 class KOmegaConfig(C.Structure):

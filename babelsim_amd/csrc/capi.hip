@@ -516,6 +516,7 @@ int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports) {
         for (int k = 0; k < n; ++k) {
             const IterRep r = s->s->iterate();
             bsim_iter_report& o = reports[k];
+            o.numerical_failure = r.failed;
             o.rU = r.rU; o.dU = r.dU; o.dP = r.dP; o.mass = r.mass; o.linP = r.linP;
             for (int c = 0; c < 3; ++c) o.mom_iters[c] = r.momIters[c];
             o.p_iters = r.pIters; o.mom_status = r.momStatus; o.p_status = r.pStatus;
@@ -523,6 +524,13 @@ int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports) {
             o.d_turb = r.dTurb; o.r_turb = r.rTurb; o.bounded_omega = r.boundedOmega;
             o.k_iters = r.kIters; o.w_iters = r.wIters; o.turb_linear = r.turbLinear;
             o.turb_converged = r.turbConverged;
+            if (r.failed) {
+                for (int j = k + 1; j < n; ++j) {
+                    reports[j] = bsim_iter_report{};
+                    reports[j].numerical_failure = 1;
+                }
+                break;
+            }
         }
     });
 }

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 
 struct bsim_mesh {
     bsim::MeshDesc d;
@@ -35,7 +36,13 @@ int bsim_mesh_from_arrays(int nv, const double* vertices, int nf, const int* fpo
                           const int* neighbour, const int* patch, int npatch, bsim_mesh** out) {
     return bsim::guarded([&] {
         if (nv <= 0 || nf <= 0 || npatch <= 0) throw std::invalid_argument("mesh arrays are empty");
-        auto* m = new bsim_mesh;
+        if (!vertices || !fpo || !fpi || !owner || !neighbour || !patch || !out)
+            throw std::invalid_argument("null argument");
+        // The ring offsets decide how much of face_point_ids is read.
+        if (fpo[0] != 0) throw std::invalid_argument("face point offsets must start at 0");
+        for (int f = 0; f < nf; ++f)
+            if (fpo[f + 1] < fpo[f]) throw std::invalid_argument("face point offsets must be non-decreasing");
+        auto m = std::make_unique<bsim_mesh>();
         m->d.vert.assign(vertices, vertices + 3 * static_cast<long long>(nv));
         m->d.fpo.assign(fpo, fpo + nf + 1);
         m->d.fpi.assign(fpi, fpi + fpo[nf]);
@@ -43,7 +50,9 @@ int bsim_mesh_from_arrays(int nv, const double* vertices, int nf, const int* fpo
         m->d.nei.assign(neighbour, neighbour + nf);
         m->d.patch.assign(patch, patch + nf);
         m->d.npatch = npatch;
-        *out = m;
+        const std::string bad = m->d.topologyError();
+        if (!bad.empty()) throw std::invalid_argument("invalid polyhedral mesh: " + bad);
+        *out = m.release();
     });
 }
 

@@ -109,27 +109,9 @@ MeshDesc readMeshFile(const std::string& path) {
     if (take<std::string>(in, path, "end marker") != "end") invalidMesh(path, "expected end");
     std::string trailing;
     if (in >> trailing) invalidMesh(path, "unexpected trailing token " + trailing);
-    // Topology checks of Mesh::polyhedral (src/core/mesh.cpp:321-360).
-    const int nc = d.ncells();
-    for (int f = 0; f < nf; ++f) {
-        if (d.own[f] < 0 || d.own[f] >= nc) invalidMesh(path, "face owner out of range");
-        if (d.nei[f] >= 0) {
-            if (d.nei[f] >= nc || d.nei[f] == d.own[f]) invalidMesh(path, "face neighbour out of range");
-            if (d.patch[f] != -1) invalidMesh(path, "interior face with a patch");
-        } else if (d.nei[f] != -1 || d.patch[f] < 0 || d.patch[f] >= np) {
-            invalidMesh(path, "boundary face without a valid patch");
-        }
-        if (d.fpo[f + 1] - d.fpo[f] < 3) invalidMesh(path, "face with fewer than three vertices");
-        std::vector<int> ring(d.fpi.begin() + d.fpo[f], d.fpi.begin() + d.fpo[f + 1]);
-        std::sort(ring.begin(), ring.end());
-        if (std::adjacent_find(ring.begin(), ring.end()) != ring.end()) invalidMesh(path, "face repeats a vertex");
-    }
-    std::vector<char> hasFace(nc, 0);
-    for (int f = 0; f < nf; ++f) {
-        hasFace[d.own[f]] = 1;
-        if (d.nei[f] >= 0) hasFace[d.nei[f]] = 1;
-    }
-    if (std::find(hasFace.begin(), hasFace.end(), 0) != hasFace.end()) invalidMesh(path, "cell with no faces");
+    // Topology checks of Mesh::polyhedral (src/core/mesh.cpp:321-367).
+    const std::string bad = d.topologyError();
+    if (!bad.empty()) invalidMesh(path, bad);
     return d;
 }
 

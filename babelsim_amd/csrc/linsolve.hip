@@ -1394,45 +1394,63 @@ void DeviceSystem::spmv(const double* x, double* y) {
     BSIM_LAUNCH_CHECK();
 }
 
+namespace {
+// HIP events destroyed on every path out of a timing routine.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    explicit EventSet(int count) {
+        ev.reserve(count);
+        for (int k = 0; k < count; ++k) {
+            hipEvent_t e;
+            BSIM_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
+    ~EventSet() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+};
+}  // namespace
+
+// A diagnostic: the cold mode allocates a 512 MiB buffer outside any serving
+// reservation, so it is not for contexts under admission control.
 double DeviceSystem::timeSpmv(int reps, bool cold) {
     if (reps <= 0) throw std::invalid_argument("reps must be positive");
     if (borrowed) throw std::logic_error("SpMV timing allocates; not available in a fixed workspace");
     kFill<<<blocksFor(n), kBlock, 0, stream>>>(n, p.p, 1.0);
     kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);  // warm
     if (!cold) {
-        hipEvent_t e0, e1;
-        BSIM_HIP(hipEventCreate(&e0));
-        BSIM_HIP(hipEventCreate(&e1));
-        BSIM_HIP(hipEventRecord(e0, stream));
+        EventSet E(2);
+        BSIM_HIP(hipEventRecord(E.ev[0], stream));
         for (int k = 0; k < reps; ++k) kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);
-        BSIM_HIP(hipEventRecord(e1, stream));
-        BSIM_HIP(hipEventSynchronize(e1));
+        BSIM_HIP(hipEventRecord(E.ev[1], stream));
+        BSIM_HIP(hipEventSynchronize(E.ev[1]));
         float ms = 0.0f;
-        BSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        BSIM_HIP(hipEventDestroy(e0));
-        BSIM_HIP(hipEventDestroy(e1));
+        BSIM_HIP(hipEventElapsedTime(&ms, E.ev[0], E.ev[1]));
         return ms * 1e-3 / reps;
     }
     constexpr int kFlush = 1 << 26;  // 2^26 doubles = 512 MiB
     DBuf<double> flush;
     flush.alloc(kFlush);
     kFill<<<blocksFor(kFlush), kBlock, 0, stream>>>(kFlush, flush.p, 0.0);
-    std::vector<hipEvent_t> ev(2 * reps);
-    for (auto& e : ev) BSIM_HIP(hipEventCreate(&e));
+    EventSet E(2 * reps);
     for (int k = 0; k < reps; ++k) {
         kTouch<<<4096, kBlock, 0, stream>>>(kFlush, flush.p, w.p);
-        BSIM_HIP(hipEventRecord(ev[2 * k], stream));
+        BSIM_HIP(hipEventRecord(E.ev[2 * k], stream));
         kSpmvPlain<<<blocksFor(n), kBlock, 0, stream>>>(view(), p.p, w.p);
-        BSIM_HIP(hipEventRecord(ev[2 * k + 1], stream));
+        BSIM_HIP(hipEventRecord(E.ev[2 * k + 1], stream));
     }
-    BSIM_HIP(hipEventSynchronize(ev.back()));
+    BSIM_HIP(hipEventSynchronize(E.ev.back()));
     double sec = 0.0;
     for (int k = 0; k < reps; ++k) {
         float ms = 0.0f;
-        BSIM_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        BSIM_HIP(hipEventElapsedTime(&ms, E.ev[2 * k], E.ev[2 * k + 1]));
         sec += ms * 1e-3;
     }
-    for (auto& e : ev) BSIM_HIP(hipEventDestroy(e));
+    // The buffer is freed only after the last launch that reads it.
+    BSIM_HIP(hipStreamSynchronize(stream));
     flush.release();
     return sec / reps;
 }

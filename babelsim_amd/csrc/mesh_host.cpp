@@ -11,6 +11,44 @@
 
 namespace bsim {
 
+std::string MeshDesc::topologyError() const {
+    const long long nv = static_cast<long long>(vert.size() / 3);
+    const int nf = nfaces();
+    if (nv <= 0 || nf <= 0 || npatch <= 0) return "mesh arrays are empty";
+    if (vert.size() % 3 != 0) return "vertex array is not a list of points";
+    for (double v : vert)
+        if (!std::isfinite(v)) return "mesh has a non-finite vertex";
+    if (static_cast<int>(fpo.size()) != nf + 1 || nei.size() != own.size() || patch.size() != own.size())
+        return "face arrays differ in length";
+    if (fpo[0] != 0) return "face point offsets must start at 0";
+    for (int f = 0; f < nf; ++f)
+        if (fpo[f + 1] < fpo[f]) return "face point offsets must be non-decreasing";
+    if (static_cast<std::size_t>(fpo[nf]) != fpi.size()) return "face point offsets do not end at the point count";
+    for (int f = 0; f < nf; ++f) {
+        if (own[f] < 0) return "face owner out of range";
+        if (nei[f] >= 0) {
+            if (nei[f] == own[f]) return "face neighbour out of range";
+            if (patch[f] != -1) return "interior face with a patch";
+        } else if (nei[f] != -1 || patch[f] < 0 || patch[f] >= npatch) {
+            return "boundary face without a valid patch";
+        }
+        if (fpo[f + 1] - fpo[f] < 3) return "face with fewer than three vertices";
+        std::vector<int> ring(fpi.begin() + fpo[f], fpi.begin() + fpo[f + 1]);
+        for (int v : ring)
+            if (v < 0 || v >= nv) return "face vertex index out of range";
+        std::sort(ring.begin(), ring.end());
+        if (std::adjacent_find(ring.begin(), ring.end()) != ring.end()) return "face repeats a vertex";
+    }
+    const int nc = ncells();
+    std::vector<char> hasFace(nc, 0);
+    for (int f = 0; f < nf; ++f) {
+        hasFace[own[f]] = 1;
+        if (nei[f] >= 0) hasFace[nei[f]] = 1;
+    }
+    if (std::find(hasFace.begin(), hasFace.end(), 0) != hasFace.end()) return "cell with no faces";
+    return {};
+}
+
 int MeshDesc::ncells() const {
     int n = 0;
     for (std::size_t f = 0; f < own.size(); ++f) {
@@ -253,6 +291,8 @@ MeshDesc generateHex(int nx, int ny, int nz, const double lo[3], const double hi
 }
 
 MeshDesc prepare(const MeshDesc& in, bool renumber, std::vector<int>& cellNewToOld, std::vector<int>& faceNewToOld) {
+    const std::string bad = in.topologyError();
+    if (!bad.empty()) throw std::invalid_argument("invalid polyhedral mesh: " + bad);
     const int nc = in.ncells();
     const int nf = in.nfaces();
     std::vector<int> oldToNew(nc);

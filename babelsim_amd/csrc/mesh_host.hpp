@@ -23,6 +23,13 @@ struct MeshDesc {
     std::vector<int> patchKind;
     int ncells() const;
     int nfaces() const { return static_cast<int>(own.size()); }
+    // The topology checks of Mesh::polyhedral (src/core/mesh.cpp:321-367):
+    // finite vertices, face rings inside [0, nv) with at least three distinct
+    // vertices, owners in range, neighbours in range and not the owner,
+    // interior faces without a patch and boundary faces with one in
+    // [0, npatch), every cell with a face.  Returns the first failure, or an
+    // empty string.
+    std::string topologyError() const;
 };
 
 // Hex lattice nx*ny*nz over [lo,hi] with six patches (-x,+x,-y,+y,-z,+z).

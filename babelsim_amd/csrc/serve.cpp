@@ -247,7 +247,7 @@ void Server::run(ServeRequest& r, std::unique_ptr<Context>& ctx) {
             peak = std::max(peak, t_deviceBytes - base);
             r.iterations = it;
             r.rU = rep.rU; r.dU = rep.dU; r.dP = rep.dP; r.mass = rep.mass;
-            if (!std::isfinite(rep.rU) || !std::isfinite(rep.dP)) { r.status = kServeNumericalFailure; break; }
+            if (rep.failed) { r.status = kServeNumericalFailure; break; }  // runSimple's numericalFailure
             if (rep.converged) { r.status = kServeConverged; break; }
         }
         s.getFields(r.U.data(), r.p.data(), r.phi.data());

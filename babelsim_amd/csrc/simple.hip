@@ -239,6 +239,10 @@ IterRep SimpleSolver::iterate() {
     C.checkErrors();
     rep.rU = std::sqrt(hr[0]) / std::max(std::sqrt(hr[1]) + std::sqrt(hr[2]), 1e-30);
     timing.assembly += since(t0);
+    if (!std::isfinite(rep.rU)) {  // main.cpp:61
+        rep.failed = 1;
+        return rep;
+    }
     t0 = Clock::now();
     // Momentum solve: one matrix, three right-hand sides (petsc_backend.cpp:156-198).
     momSys->update();
@@ -261,6 +265,10 @@ IterRep SimpleSolver::iterate() {
     kMerge<<<nb, kBlock, 0, s>>>(nc, xcomp[0].p, xcomp[1].p, xcomp[2].p, U.p);
     BSIM_LAUNCH_CHECK();
     timing.momentumSolve += since(t0);
+    if (rep.momStatus == kNumericalFailure) {  // main.cpp:67
+        rep.failed = 1;
+        return rep;
+    }
     t0 = Clock::now();
     // Rhie-Chow (main.cpp:71-79).  grad(p) is unchanged since assembly.
     const double* gU = nullptr;
@@ -332,10 +340,14 @@ IterRep SimpleSolver::iterate() {
         const SolveRes r = pSys->solve(psrc.p, pPrime.p, 0);
         pHist = pSys->hist;
         rep.pIters += r.iterations;
-        if (r.status == kNumericalFailure) rep.pStatus = kNumericalFailure;
         pConverged = pConverged && r.status == kConverged;
         rep.linP = r.relative;
         timing.pressureSolve += since(t0);
+        if (r.status == kNumericalFailure) {  // main.cpp:88
+            rep.pStatus = kNumericalFailure;
+            rep.failed = 1;
+            return rep;
+        }
     }
     if (!pConverged && rep.pStatus != kNumericalFailure) rep.pStatus = kMaxIterations;
     t0 = Clock::now();
@@ -353,9 +365,13 @@ IterRep SimpleSolver::iterate() {
         rep.kIters = t.itK;
         rep.wIters = t.itOmega;
         rep.boundedOmega = t.boundedOmega;
-        if (!t.healthy) throw std::runtime_error("turbulence transport failed");
         rep.dTurb = std::max(t.dOmega, t.dK);
         rep.rTurb = std::max(t.rOmega, t.rK);
+        if (!t.healthy || !std::isfinite(rep.dTurb) || !std::isfinite(rep.rTurb)) {  // main.cpp:106
+            timing.turbulence += since(t0);
+            rep.failed = 1;
+            return rep;
+        }
         rep.turbLinear = t.statusK == kConverged && t.statusOmega == kConverged;
         turbulenceConverged = rep.turbLinear && rep.dTurb <= turb->tolerance() && rep.rTurb <= turb->tolerance();
         rep.turbConverged = turbulenceConverged;
@@ -375,6 +391,8 @@ IterRep SimpleSolver::iterate() {
     const bool linearConverged = rep.momStatus == kConverged && pConverged;
     rep.converged = linearConverged && turbulenceConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU &&
                     rep.dP <= c.tolP && rep.mass <= c.tolMass;
+    // main.cpp:115-116
+    if (!std::isfinite(rep.dU) || !std::isfinite(rep.dP) || !std::isfinite(rep.mass)) rep.failed = 1;
     timing.convergence += since(t0);
     ++timing.iterations;
     return rep;

@@ -25,6 +25,7 @@ struct IterRep {
     int pIters = 0, momStatus = 0, pStatus = 0, converged = 0, referenced = 0;
     double dTurb = 0, rTurb = 0, boundedOmega = 0;
     int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1;
+    int failed = 0;  // runSimple's numericalFailure: the iteration stopped where it was detected
 };
 
 // Per-phase wall time (seconds) accumulated over iterations.

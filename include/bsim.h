@@ -213,6 +213,10 @@ typedef struct {
     int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
     double d_turb, r_turb, bounded_omega;
     int k_iters, w_iters, turb_linear, turb_converged;
+    /* runSimple's numericalFailure (main.cpp:61-62, 67, 88, 106, 115-116): a
+     * non-finite measure or an unhealthy velocity, pressure or turbulence
+     * solve ends the iteration at that point and the run. */
+    int numerical_failure;
 } bsim_iter_report;
 
 /* Wilcox (1988) k-omega (src/physics/RANS/k_omega.cpp:25-157): constants,
@@ -240,6 +244,8 @@ int bsim_simple_enable_komega(bsim_simple* s, const bsim_komega_config* cfg, con
 int bsim_simple_set_turbulence(bsim_simple* s, const double* k, const double* omega);
 int bsim_simple_get_turbulence(bsim_simple* s, double* k, double* omega, double* mut);
 int bsim_simple_start(bsim_simple* s);
+/* Runs up to n iterations; it stops after one that ends in numerical failure,
+ * and the reports after it are zero with numerical_failure = 1. */
 int bsim_simple_iterate(bsim_simple* s, int n, bsim_iter_report* reports);
 int bsim_simple_get_fields(bsim_simple* s, double* U, double* p, double* phi);
 /* which: 0..2 momentum components, 3 pressure correction (last solve), 4 k and

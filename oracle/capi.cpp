@@ -423,6 +423,14 @@ int orc_simple_iterate(void* sp, int n, orc_iter_report* reports) {
             o.d_turb = r.dTurb; o.r_turb = r.rTurb; o.bounded_omega = r.boundedOmega;
             o.k_iters = r.kIters; o.w_iters = r.wIters; o.turb_linear = r.turbLinear;
             o.turb_converged = r.turbConverged;
+            o.numerical_failure = r.failed;
+            if (r.failed) {
+                for (int j = k + 1; j < n; ++j) {
+                    reports[j] = orc_iter_report{};
+                    reports[j].numerical_failure = 1;
+                }
+                break;
+            }
         }
     });
 }

@@ -39,6 +39,7 @@ typedef struct {
     int mom_iters[3], p_iters, mom_status, p_status, converged, referenced;
     double d_turb, r_turb, bounded_omega;
     int k_iters, w_iters, turb_linear, turb_converged;
+    int numerical_failure;
 } orc_iter_report;
 
 typedef struct {

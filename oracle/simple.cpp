@@ -144,6 +144,10 @@ IterReport Simple::iterate() {
     }
     // main.cpp:60-66
     rep.rU = relativeResidual(m, mom, U.v);
+    if (!std::isfinite(rep.rU)) {  // main.cpp:61
+        rep.failed = 1;
+        return rep;
+    }
     const double alphaU = c.alphaU;
 #pragma omp parallel for schedule(static)
     for (Index i = 0; i < m.nc; ++i) {  // relax (procedural_equation.cpp:298-305)
@@ -175,6 +179,10 @@ IterReport Simple::iterate() {
             if (r.status == NumericalFailure) rep.momStatus = NumericalFailure;
             else if (r.status != Converged && rep.momStatus != NumericalFailure) rep.momStatus = MaxIterations;
         }
+    }
+    if (rep.momStatus == NumericalFailure) {  // main.cpp:67
+        rep.failed = 1;
+        return rep;
     }
     // Rhie-Chow (main.cpp:71-79).
     gradient(p, gradP, c.pGrad);
@@ -239,9 +247,13 @@ IterReport Simple::iterate() {
         SolveResult r;
         solveSystem(pSys, peq, c.pre, pPrime.v, r, &pHist);
         rep.pIters += r.iterations;
-        if (r.status == NumericalFailure) rep.pStatus = NumericalFailure;
         pConverged = pConverged && r.status == Converged;
         rep.linP = r.relative;
+        if (r.status == NumericalFailure) {  // main.cpp:88
+            rep.pStatus = NumericalFailure;
+            rep.failed = 1;
+            return rep;
+        }
     }
     if (!pConverged && rep.pStatus != NumericalFailure) rep.pStatus = MaxIterations;
     // Corrections (main.cpp:95-98).
@@ -271,9 +283,12 @@ IterReport Simple::iterate() {
         rep.kIters = t.itK;
         rep.wIters = t.itOmega;
         rep.boundedOmega = t.boundedOmegaCells;
-        if (!t.healthy) throw std::runtime_error("turbulence transport failed");
         rep.dTurb = std::max(t.dOmega, t.dK);
         rep.rTurb = std::max(t.rOmega, t.rK);
+        if (!t.healthy || !std::isfinite(rep.dTurb) || !std::isfinite(rep.rTurb)) {  // main.cpp:106
+            rep.failed = 1;
+            return rep;
+        }
         rep.turbLinear = t.statusK == Converged && t.statusOmega == Converged;
         turbulenceConverged = rep.turbLinear && rep.dTurb <= c.kw.tolerance && rep.rTurb <= c.kw.tolerance;
         rep.turbConverged = turbulenceConverged;
@@ -302,6 +317,8 @@ IterReport Simple::iterate() {
     const bool linearConverged = rep.momStatus == Converged && pConverged;
     rep.converged = linearConverged && turbulenceConverged && rep.rU <= c.tolMom && rep.dU <= c.tolU &&
                     rep.dP <= c.tolP && rep.mass <= c.tolMass;
+    // main.cpp:115-116
+    if (!std::isfinite(rep.dU) || !std::isfinite(rep.dP) || !std::isfinite(rep.mass)) rep.failed = 1;
     return rep;
 }
 

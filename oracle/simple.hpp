@@ -31,6 +31,7 @@ struct IterReport {
     int referenced = 0;
     double dTurb = 0, rTurb = 0, boundedOmega = 0;
     int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1;
+    int failed = 0;  // runSimple's numericalFailure (main.cpp:61-62, 67, 88, 106, 115-116)
 };
 
 template <class T> void accumulate(Ldu<T>& a, const Ldu<T>& b, double f);

@@ -233,3 +233,29 @@ def test_failed_request_reports_an_error_and_the_server_goes_on(solo):
         assert st["resident_idle"] == 0 and st["reserved"] == 0
         ids = [(srv.submit_steady(cid, w["cfg"], *w["bcs"], max_iterations=CAP), cid)]
         check_equal(srv, ids, [solo[1]])
+
+
+def test_numerical_failure_ends_a_served_request(solo):
+    """runSimple returns numericalFailure at the first non-finite measure or
+    unhealthy solve (simple/main.cpp:61-67, 88, 115-116); a served request
+    ends there with SERVE_NUMERICAL_FAILURE after one iteration instead of
+    running to its cap, and the device and the oracle both stop the run."""
+    from babelsim_amd import SERVE_NUMERICAL_FAILURE
+    w = workload()[1]
+    mesh = w["mesh"]
+    nc = mesh.sizes()[4]
+    U0 = np.zeros((nc, 3))
+    U0[nc // 2, 0] = np.nan
+    with Server(slots=1, memory_budget=8e9) as srv:
+        cid = srv.add_case(mesh)
+        r = srv.wait(srv.submit_steady(cid, w["cfg"], *w["bcs"], max_iterations=50, U0=U0))
+        assert r.status == SERVE_NUMERICAL_FAILURE and r.iterations == 1
+        ids = [(srv.submit_steady(cid, w["cfg"], *w["bcs"], max_iterations=CAP), cid)]
+        check_equal(srv, ids, [solo[1]])
+    ub, pb = w["bcs"]
+    s = Simple(Context(mesh, renumber=True), w["cfg"], ub, pb)
+    s.set_fields(U=U0)
+    s.start()
+    reps = s.iterate(3)
+    assert [r.numerical_failure for r in reps] == [1, 1, 1]
+    assert reps[1].p_iters == 0 and reps[2].mom_iters[0] == 0

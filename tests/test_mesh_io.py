@@ -138,3 +138,28 @@ def test_renumbered_output_keeps_original_ids(tmp_path):
     back = read_field_values(tmp_path / "r", n, 1, name="f")
     c0 = geometry(mesh)["C"]
     np.testing.assert_allclose(back, c0[:, 0] * 10 + c0[:, 1], rtol=1e-12)
+
+
+@pytest.mark.parametrize("field,index,value,message", [
+    ("fpi", 5, 10_000, "face vertex index out of range"),
+    ("fpi", 0, -1, "face vertex index out of range"),
+    ("owner", 3, -1, "face owner out of range"),
+    ("patch", -1, 99, "boundary face without a valid patch"),
+    ("neighbour", 0, -5, "boundary face without a valid patch"),
+    ("fpo", 2, 0, "face point offsets must be non-decreasing"),
+    ("vertices", 4, float("nan"), "non-finite vertex"),
+])
+def test_arrays_are_checked_before_use(field, index, value, message):
+    """bsim_mesh_from_arrays rejects what Mesh::polyhedral rejects
+    (src/core/mesh.cpp:321-367) with rc 2, before any array is indexed."""
+    a = {k: np.array(v, copy=True) for k, v in MeshData.hex((3, 2, 2)).arrays().items()}
+    a[field][index] = value
+    with pytest.raises(ValueError, match=message):
+        MeshData.from_arrays(a)
+
+
+def test_valid_arrays_round_trip():
+    a = MeshData.hex((3, 2, 2)).arrays()
+    b = MeshData.from_arrays(a).arrays()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])

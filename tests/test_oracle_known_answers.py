@@ -413,3 +413,24 @@ def test_reference_only_when_unanchored():
     s2 = orc.OracleSimple(om, cases.c1_config(), ub, (pt, pv))
     s2.start()
     assert s2.iterate(1)[0].referenced == 0
+
+
+def test_simple_stops_at_a_numerical_failure():
+    """runSimple returns numericalFailure at a non-finite momentum residual
+    before relaxing or solving (simple/main.cpp:60-62): the oracle's iteration
+    ends there and later iterations of the same call are not run."""
+    arrays = cases.cavity2d_mesh(8).arrays()
+    ub, pb = cases.cavity_bcs(two_d=True)
+    s = orc.OracleSimple(orc.oracle_mesh(arrays), cases.c1_config(), ub, pb)
+    nc = len(arrays["owner"]) and orc.oracle_mesh(arrays).nc
+    U = np.zeros((nc, 3))
+    U[3, 1] = np.nan
+    s.set_fields(U, np.zeros(nc))
+    s.start()
+    reps = s.iterate(3)
+    assert [r.numerical_failure for r in reps] == [1, 1, 1]
+    assert reps[0].mom_iters[0] == 0 and reps[0].p_iters == 0
+    # A finite start runs normally.
+    s2 = orc.OracleSimple(orc.oracle_mesh(arrays), cases.c1_config(), ub, pb)
+    s2.start()
+    assert [r.numerical_failure for r in s2.iterate(2)] == [0, 0]
