@@ -127,6 +127,49 @@ class StepReport(C.Structure):
                 ("accepted", C.c_int), ("converged", C.c_int), ("linear_ok", C.c_int), ("status", C.c_int)]
 
 
+_U64, _F64 = C.c_uint64, C.c_double
+
+
+class EquationCounters(C.Structure):
+    """EquationPerformanceCounters (solver_control.h:36-60) of one persistent system."""
+
+    _fields_ = ([("identity", C.c_char * 64), ("ksp_type", C.c_char * 16), ("pc_type", C.c_char * 16),
+                 ("last_status", C.c_char * 24)] +
+                [(n, _U64) for n in ("matrix_pattern_builds", "matrix_value_updates", "rhs_only_solves",
+                                     "linear_solves", "krylov_iterations", "preconditioner_setups",
+                                     "preconditioner_reuses", "preconditioner_retries", "true_residual_checks",
+                                     "sparse_matvecs", "preconditioner_applications")] +
+                [(n, _F64) for n in ("last_initial_residual", "last_final_residual", "last_relative_residual",
+                                     "pattern_build_seconds", "matrix_update_seconds", "preconditioner_seconds",
+                                     "rhs_seconds", "solve_seconds", "residual_check_seconds")])
+
+    def as_dict(self) -> dict:
+        return {n: (getattr(self, n).decode() if isinstance(getattr(self, n), bytes) else getattr(self, n))
+                for n, _ in self._fields_}
+
+
+class Performance(C.Structure):
+    """PerformanceCounters: the 31 values of Case::performance (src/io/case.cpp:72-104), in order."""
+
+    _fields_ = ([(n, _U64) for n in ("linear_solves", "krylov_iterations", "sparse_matvecs", "halo_exchanges",
+                                     "halo_bytes", "global_reductions", "equation_assemblies",
+                                     "preconditioner_setups", "preconditioner_applications", "output_writes")] +
+                [(n, _F64) for n in ("elapsed_seconds", "assembly_seconds", "preconditioner_seconds",
+                                     "preconditioner_apply_seconds", "linear_solve_seconds",
+                                     "sparse_matvec_seconds", "halo_seconds", "global_reduction_seconds",
+                                     "output_seconds")] +
+                [(n, _U64) for n in ("matrix_pattern_builds", "matrix_value_updates", "rhs_only_solves",
+                                     "true_residual_checks")] +
+                [(n, _F64) for n in ("matrix_update_seconds", "rhs_seconds", "residual_check_seconds",
+                                     "pattern_build_seconds")] +
+                [(n, _U64) for n in ("preconditioner_reuses", "preconditioner_retries",
+                                     "reported_iteration_intervals")] +
+                [("iteration_compute_seconds", _F64)])
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 EULER, BDF2 = 1, 2
 STEP_COMPLETED, STEP_NOT_CONVERGED, STEP_NUMERICAL_FAILURE = 0, 1, 2
 
@@ -240,6 +283,7 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_system_solve_ldu": [vp, dp, dp, dp, C.c_int, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult)],
     "bsim_system_history": [vp, dp, C.c_int, ip],
     "bsim_system_counters": [vp, u64p],
+    "bsim_system_performance": [vp, C.POINTER(EquationCounters)],
     "bsim_csr_solve": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult),
                        dp, C.c_int, ip],
     "bsim_csr_spmv": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp],
@@ -256,6 +300,7 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_simple_get_fields": [vp, dp, dp, dp],
     "bsim_simple_history": [vp, C.c_int, dp, C.c_int, ip],
     "bsim_simple_timing": [vp, dp],
+    "bsim_simple_performance": [vp, C.POINTER(Performance), C.POINTER(EquationCounters), C.c_int, ip],
     "bsim_simple_time_spmv": [vp, C.c_int, C.c_int, C.c_int, dp, ip, ip],
     "bsim_simple_probe_start": [vp, C.c_int],
     "bsim_simple_probe_stop": [vp, ip, dp, dp],
@@ -267,6 +312,7 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_piso_temporal_flux": [vp, dp],
     "bsim_piso_history": [vp, C.c_int, dp, C.c_int, ip],
     "bsim_piso_timing": [vp, dp],
+    "bsim_piso_performance": [vp, C.POINTER(Performance), C.POINTER(EquationCounters), C.c_int, ip],
     "bsim_piso_probe_start": [vp, C.c_int],
     "bsim_piso_probe_stop": [vp, ip, dp, dp],
     "bsim_server_create": [C.c_int, C.c_int, C.c_double, C.POINTER(vp)],

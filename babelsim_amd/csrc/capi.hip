@@ -375,6 +375,96 @@ int bsim_system_counters(bsim_system* sys, uint64_t* o) {
     });
 }
 
+namespace {
+// Names of petsc_backend.cpp:30-60 (kspName, pcName, solveStatusName).
+const char* kspName(int solver) { return solver == 0 ? "cg" : solver == 1 ? "bcgs" : "unknown"; }
+const char* pcName(int pc) { return pc == 0 ? "none" : pc == 1 ? "jacobi" : pc == 2 ? "amg" : "unknown"; }
+const char* statusName(int st) {
+    return st == kConverged ? "converged" : st == kMaxIterations ? "maxIterations" : st == kNumericalFailure ? "numericalFailure" : "";
+}
+void copyName(char* dst, std::size_t cap, const char* src) {
+    std::strncpy(dst, src, cap - 1);
+    dst[cap - 1] = '\0';
+}
+bsim_equation_counters equationCounters(const DeviceSystem& D, const char* identity) {
+    bsim_equation_counters o{};
+    const SystemCounters& c = D.counters;
+    copyName(o.identity, sizeof(o.identity), identity);
+    copyName(o.ksp_type, sizeof(o.ksp_type), kspName(D.cfg.solver));
+    copyName(o.pc_type, sizeof(o.pc_type), pcName(D.cfg.pc));
+    copyName(o.last_status, sizeof(o.last_status), statusName(c.lastStatus));
+    o.matrix_pattern_builds = 1;
+    o.matrix_value_updates = c.matrixUpdates;
+    o.rhs_only_solves = c.rhsOnly;
+    o.linear_solves = c.solves;
+    o.krylov_iterations = c.krylovIterations;
+    o.preconditioner_setups = c.pcSetups;
+    o.preconditioner_reuses = c.pcReuses;
+    o.preconditioner_retries = c.retries;
+    o.true_residual_checks = c.trueResidualChecks;
+    o.sparse_matvecs = c.spmvs;
+    o.preconditioner_applications = c.pcApplications;
+    o.last_initial_residual = c.lastInitial;
+    o.last_final_residual = c.lastFinal;
+    o.last_relative_residual = c.lastRelative;
+    o.pattern_build_seconds = c.patternSeconds;
+    o.matrix_update_seconds = c.matrixUpdateSeconds;
+    o.preconditioner_seconds = c.setupSeconds;
+    o.rhs_seconds = c.rhsSeconds;
+    o.solve_seconds = c.solveSeconds;
+    o.residual_check_seconds = c.residualSeconds;
+    return o;
+}
+// Sums the systems into the run record (petsc_backend.cpp:236-286 keeps the
+// totals and the per-equation snapshots side by side).
+void addSystem(bsim_performance& t, const bsim_equation_counters& e) {
+    t.linear_solves += e.linear_solves;
+    t.krylov_iterations += e.krylov_iterations;
+    t.sparse_matvecs += e.sparse_matvecs;
+    t.equation_assemblies += e.matrix_value_updates + e.rhs_only_solves;
+    t.preconditioner_setups += e.preconditioner_setups;
+    t.preconditioner_applications += e.preconditioner_applications;
+    t.preconditioner_seconds += e.preconditioner_seconds;
+    t.linear_solve_seconds += e.solve_seconds;
+    t.matrix_pattern_builds += e.matrix_pattern_builds;
+    t.matrix_value_updates += e.matrix_value_updates;
+    t.rhs_only_solves += e.rhs_only_solves;
+    t.true_residual_checks += e.true_residual_checks;
+    t.matrix_update_seconds += e.matrix_update_seconds;
+    t.rhs_seconds += e.rhs_seconds;
+    t.residual_check_seconds += e.residual_check_seconds;
+    t.pattern_build_seconds += e.pattern_build_seconds;
+    t.preconditioner_reuses += e.preconditioner_reuses;
+    t.preconditioner_retries += e.preconditioner_retries;
+}
+void runPerformance(const std::vector<std::pair<const DeviceSystem*, const char*>>& systems, const RunClock& run,
+                    std::uint64_t reductions, double assemblySeconds, bsim_performance* total,
+                    bsim_equation_counters* out, int cap, int* count) {
+    bsim_performance t{};
+    int k = 0;
+    for (const auto& s : systems) {
+        const bsim_equation_counters e = equationCounters(*s.first, s.second);
+        addSystem(t, e);
+        if (out && k < cap) out[k] = e;
+        ++k;
+    }
+    t.global_reductions = reductions;
+    t.elapsed_seconds = run.elapsed();
+    t.assembly_seconds = assemblySeconds;
+    t.reported_iteration_intervals = run.intervals;
+    t.iteration_compute_seconds = run.window;
+    if (total) *total = t;
+    if (count) *count = k;
+}
+}  // namespace
+
+int bsim_system_performance(bsim_system* sys, bsim_equation_counters* out) {
+    return guarded([&] {
+        if (!sys || !out) throw std::invalid_argument("null argument");
+        *out = equationCounters(*sys->sys, "");
+    });
+}
+
 void bsim_system_destroy(bsim_system* sys) { delete sys; }
 
 namespace {
@@ -555,6 +645,22 @@ int bsim_simple_timing(bsim_simple* s, double* o) {
         std::memcpy(o, v, sizeof(v));
     });
 }
+int bsim_simple_performance(bsim_simple* s, bsim_performance* total, bsim_equation_counters* systems, int cap,
+                            int* count) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null argument");
+        SimpleSolver& S = *s->s;
+        std::vector<std::pair<const DeviceSystem*, const char*>> sys = {
+            {&S.momentumSystem(), "vector:momentum:U"}, {&S.pressureSystem(), "scalar:pressureCorrection:pPrime"}};
+        std::uint64_t reads = S.run.reductions;
+        if (const KOmegaDevice* t = S.turbulence()) {
+            if (t->kSystem()) sys.push_back({t->kSystem(), "scalar:kTransport:k"});
+            if (t->omegaSystem()) sys.push_back({t->omegaSystem(), "scalar:omegaTransport:omega"});
+            reads += t->reads;
+        }
+        runPerformance(sys, S.run, reads, S.timing.assembly + S.timing.pressureAssembly, total, systems, cap, count);
+    });
+}
 int bsim_simple_time_spmv(bsim_simple* s, int which, int reps, int cold, double* seconds, int* n, int* nnz) {
     return guarded([&] {
         DeviceSystem& D = which == 0 ? s->s->momentumSystem() : s->s->pressureSystem();
@@ -623,6 +729,17 @@ int bsim_piso_timing(bsim_piso* s, double* o) {
         const double v[8] = {t.history, t.assembly, t.momentumSolve, t.corrector, t.pressureAssembly,
                              t.pressureSolve, t.diagnostics, static_cast<double>(t.steps)};
         std::memcpy(o, v, sizeof(v));
+    });
+}
+int bsim_piso_performance(bsim_piso* s, bsim_performance* total, bsim_equation_counters* systems, int cap,
+                          int* count) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null argument");
+        PisoSolver& S = *s->s;
+        const std::vector<std::pair<const DeviceSystem*, const char*>> sys = {
+            {&S.momentumSystem(), "vector:momentum:U"}, {&S.pressureSystem(), "scalar:pressure:p"}};
+        runPerformance(sys, S.run, S.run.reductions, S.timing.assembly + S.timing.pressureAssembly, total, systems,
+                       cap, count);
     });
 }
 int bsim_piso_probe_start(bsim_piso* s, int capacity) {

@@ -297,6 +297,7 @@ double KOmegaDevice::boundOmega() {
     BSIM_LAUNCH_CHECK();
     double v = 0.0;
     BSIM_HIP(hipMemcpyAsync(&v, red.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    ++reads;
     BSIM_HIP(hipStreamSynchronize(s));
     return v;
 }
@@ -371,6 +372,7 @@ double KOmegaDevice::residual(DeviceSystem& sys, const double* x) {
     BSIM_LAUNCH_CHECK();
     double h[3];
     BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    ++reads;
     BSIM_HIP(hipStreamSynchronize(s));
     return std::sqrt(h[0]) / std::max(std::sqrt(h[1]) + std::sqrt(h[2]), 1e-30);
 }
@@ -431,6 +433,7 @@ TransportRep KOmegaDevice::solveTransport(FieldArg U, const double* phi, int mGr
     BSIM_LAUNCH_CHECK();
     double h[4];
     BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    ++reads;
     C.checkErrors();
     rep.dK = std::sqrt(h[0]) / std::max(std::sqrt(h[1]), 1e-30);
     rep.dOmega = std::sqrt(h[2]) / std::max(std::sqrt(h[3]), 1e-30);

@@ -61,6 +61,11 @@ private:
     DBuf<double> mutT, muEffT, kdT, wdT, ctxK, ctxW, kf, qf;
     DBuf<double> rows[3], rowsT[3], gtr;
     std::unique_ptr<DeviceSystem> kSys, wSys;
+
+public:
+    const DeviceSystem* kSystem() const { return kSys.get(); }
+    const DeviceSystem* omegaSystem() const { return wSys.get(); }
+    std::uint64_t reads = 0;  // device reductions read by the host
 };
 
 }  // namespace bsim

@@ -1250,6 +1250,7 @@ AmgDevice::~AmgDevice() { dropGraphs(); }
 // ------------------------------------------------------------- systems
 DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
     : n(pattern.n), nnz(pattern.nnzStore), stream(st), pat(std::move(pattern)) {
+    const auto t0 = Clock::now();
     for (int k = 0; k < 2; ++k) { diagBuf[k].alloc(n); valBuf[k].alloc(std::max(nnz, 1)); }
     sc.alloc(S_COUNT);
     flags.alloc(1);
@@ -1261,6 +1262,7 @@ DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
     act3.alloc(3);
     BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc3), sizeof(double) * (3 * S_COUNT + 2), hipHostMallocDefault));
     BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hact3), sizeof(int) * 4 * kActSlots, hipHostMallocDefault));
+    counters.patternSeconds = since(t0);
 }
 
 namespace {
@@ -1309,6 +1311,7 @@ DeviceSystem::~DeviceSystem() {
 }
 
 bool DeviceSystem::update() {
+    const auto t0 = Clock::now();
     bool changed = true;
     if (ready) {
         const int rb = reduceBlocks(static_cast<long long>(n) + nnz);
@@ -1325,6 +1328,7 @@ bool DeviceSystem::update() {
     } else {
         ++counters.rhsOnly;
     }
+    counters.matrixUpdateSeconds += since(t0);
     return changed;
 }
 
@@ -1365,8 +1369,6 @@ void DeviceSystem::setup(bool reuse) {
         pcMatrixUpdate = counters.matrixUpdates;
         pcRhsSolves = 0;
         freshIterations[0] = freshIterations[1] = freshIterations[2] = 0;
-    } else {
-        ++counters.pcReuses;
     }
     setupPending = false;
     forceRebuild = false;
@@ -1455,14 +1457,17 @@ double DeviceSystem::timeSpmv(int reps, bool cold) {
     return sec / reps;
 }
 
-double DeviceSystem::trueResidual(const double* b, const double* x) {
+double DeviceSystem::trueResidual(const double* b, const double* x, bool check) {
+    const auto t0 = Clock::now();
     const int rb = reduceBlocks(n);
     kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, tmp.p, partials.p);
     kReduceTail<1><<<1, kTail, 0, stream>>>(rb, partials.p, StoreSqrt{sc.p, S_TR});
     BSIM_LAUNCH_CHECK();
     ++counters.spmvs;
-    ++counters.trueResidualChecks;
-    return readScalar(S_TR);
+    if (check) ++counters.trueResidualChecks;
+    const double v = readScalar(S_TR);
+    counters.residualSeconds += since(t0);
+    return v;
 }
 
 void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason) {
@@ -1629,7 +1634,7 @@ SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
 SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const FirstRun* first) {
     if (!ready || !configured || setupPending)
         throw std::logic_error("linear system must update and prepare before solving a right-hand side");
-    const auto t0 = Clock::now();
+    const auto tRhs = Clock::now();
     const int nb = blocksFor(n);
     hist.clear();
     ++counters.solves;
@@ -1653,17 +1658,19 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
         kCopyNorm<<<reduceBlocks(n), kBlock, 0, stream>>>(n, b, nullptr, partials.p);
         kReduceTail<1><<<1, kTail, 0, stream>>>(reduceBlocks(n), partials.p, StoreSqrt{sc.p, S_BN});
         rhsNorm = readScalar(S_BN);
-        initial = cfg.warmStart ? trueResidual(b, x) : rhsNorm;
+        counters.rhsSeconds += since(tRhs);
+        initial = cfg.warmStart ? trueResidual(b, x, false) : rhsNorm;  // petsc_linear_system.cpp:410
     }
     const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;
     const double target = std::max(cfg.atol, cfg.rtol * scale);
     const bool reused = cfg.pc != 0 && pcMatrixUpdate < counters.matrixUpdates;
+    if (reused) ++counters.pcReuses;  // per right-hand side (petsc_backend.cpp:262-266)
     SolveRes res;
     if (std::isfinite(initial) && initial <= target) {
         res.status = kConverged; res.iterations = 0; res.initial = initial; res.final_ = initial;
         res.relative = initial / scale;
         ++counters.trueResidualChecks;
-        counters.solveSeconds += since(t0);
+        counters.last(res.status, res.initial, res.final_, res.relative);
         return res;
     }
     int total = 0;
@@ -1675,9 +1682,11 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
         for (int restart = 0; restart < 3; ++restart) {
             int its = 0;
             const bool given = first && attempt == 0 && restart == 0;
+            const auto tKsp = Clock::now();
             if (given) { its = first->its; reason = first->reason; }
             else if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
             else bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            if (!given) counters.solveSeconds += since(tKsp);
             attemptIts += its;
             total += its;
             fin = given ? first->fin : trueResidual(b, x);
@@ -1706,7 +1715,7 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
     }
     if (total > 0) ++pcRhsSolves;
     counters.krylovIterations += total;
-    counters.solveSeconds += since(t0);
+    counters.last(res.status, res.initial, res.final_, res.relative);
     return res;
 }
 
@@ -1912,7 +1921,7 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
     for (int c = 0; c < 3; ++c) {
         fr[c].rhsNorm = hsc3[c * S_COUNT + S_BN];
         fr[c].initial = cfg.warmStart ? hsc3[c * S_COUNT + S_TR] : fr[c].rhsNorm;
-        if (cfg.warmStart) { ++fr[c].delta.spmvs; ++fr[c].delta.trueResidualChecks; }
+        if (cfg.warmStart) ++fr[c].delta.spmvs;  // the initial residual is not a check
         const double scale = std::max(fr[c].initial, fr[c].rhsNorm) > 0.0 ? std::max(fr[c].initial, fr[c].rhsNorm) : 1.0;
         target[c] = std::max(cfg.atol, cfg.rtol * scale);
         run[c] = !(std::isfinite(fr[c].initial) && fr[c].initial <= target[c]);
@@ -1923,7 +1932,9 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
     SystemCounters dl[3];
     if (run[0] || run[1] || run[2]) {
         const double* jv = cfg.pc == 1 ? jinv.p : nullptr;
+        const auto tKsp = Clock::now();
         bcgs3<double>(w64, view(), jv, b, x, run, !cfg.warmStart, cfg.maxIter, target, its, reason, h, dl);
+        counters.solveSeconds += since(tKsp);
         uploadAct(run);
         k3ResidualNorm<double><<<rb, kBlock, 0, stream>>>(view(), B, cX, vec3(w64.tmp), act3.p, partials.p);
         kReduceTail<3><<<1, kTail, 0, stream>>>(rb, partials.p, Tail3Sqrt{sc3.p, S_TR, act3.p});
@@ -1992,6 +2003,7 @@ void DeviceSystem::solve3Mixed(const double* const b[3], double* const x[3], Sol
     const Vec3p<double> X{{x[0], x[1], x[2]}};
     const CVec3p<double> cX{{x[0], x[1], x[2]}};
     counters.solves += ncomp;
+    if (cfg.pc != 0 && pcMatrixUpdate < counters.matrixUpdates) counters.pcReuses += ncomp;
     const bool use[3] = {true, ncomp > 1, ncomp > 2};
     uploadAct(use);
     if (!cfg.warmStart) k3Fill<double><<<nb, kBlock, 0, stream>>>(n, X, 0.0, act3.p);
@@ -2069,6 +2081,7 @@ void DeviceSystem::solve3Mixed(const double* const b[3], double* const x[3], Sol
         else r.status = kNumericalFailure;
         if (total[c] > 0) ++pcRhsSolves;
         counters.krylovIterations += total[c];
+        counters.last(r.status, r.initial, r.final_, r.relative);
     }
     counters.solveSeconds += since(t0);
 }

@@ -75,10 +75,23 @@ private:
     int used = 0, pending = 0;
 };
 
+// EquationPerformanceCounters (include/babelsim/solver_control.h:36-60) of one
+// persistent system, counted as src/backend/petsc_backend.cpp:236-286 does:
+// a preconditioner reuse per right-hand side solved with a lagged PC, a true
+// residual check per Krylov restart or accepted initial guess (the warm-start
+// initial residual itself is not one), solve seconds inside the Krylov calls
+// only.  Device time is measured on the host around work that ends in a
+// stream synchronisation.
 struct SystemCounters {
     std::uint64_t matrixUpdates = 0, rhsOnly = 0, pcSetups = 0, pcReuses = 0, retries = 0, solves = 0,
                   krylovIterations = 0, trueResidualChecks = 0, spmvs = 0, pcApplications = 0;
-    double setupSeconds = 0, solveSeconds = 0;
+    double setupSeconds = 0, solveSeconds = 0, matrixUpdateSeconds = 0, rhsSeconds = 0, residualSeconds = 0,
+           patternSeconds = 0;
+    int lastStatus = -1;  // -1 before the first solve
+    double lastInitial = 0, lastFinal = 0, lastRelative = 0;
+    void last(int status, double initial, double fin, double relative) {
+        lastStatus = status; lastInitial = initial; lastFinal = fin; lastRelative = relative;
+    }
 };
 
 // Process-wide switch between graph replay and stream launches of AMG cycles.
@@ -135,7 +148,8 @@ public:
 private:
     void setup(bool reuse);
     void applyPC(const double* r, double* z);
-    double trueResidual(const double* b, const double* x);
+    // ||b - A x||; counted as a true-residual check unless `check` is false.
+    double trueResidual(const double* b, const double* x, bool check = true);
     void cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     void bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     double readScalar(int idx, int* flags = nullptr);

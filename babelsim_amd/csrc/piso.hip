@@ -362,6 +362,10 @@ void PisoSolver::computeTemporalFlux() {
 }
 
 StepRep PisoSolver::step(double dt) {
+    struct Record {  // RunTime::recordIteration at every way out of the step
+        RunClock& r;
+        ~Record() { r.record(); }
+    } record{run};
     if (!(dt > 0) || !std::isfinite(dt)) throw std::invalid_argument("history requires positive finite dt");
     if (!started) start();
     const DeviceMesh& M = *C.mesh;
@@ -432,6 +436,7 @@ StepRep PisoSolver::step(double dt) {
         BSIM_LAUNCH_CHECK();
         double hr[3];
         BSIM_HIP(hipMemcpyAsync(hr, red.p + kSlotRU, sizeof(hr), hipMemcpyDeviceToHost, s));
+        ++run.reductions;
         C.checkErrors();
         rep.rU = std::sqrt(hr[0]) / std::max(std::sqrt(hr[1]) + std::sqrt(hr[2]), 1e-30);
         timing.assembly += since(t0);
@@ -541,6 +546,7 @@ StepRep PisoSolver::step(double dt) {
             if (adaptive) {
                 double h[6];
                 BSIM_HIP(hipMemcpyAsync(h, slot, sizeof(h), hipMemcpyDeviceToHost, s));
+                ++run.reductions;
                 BSIM_HIP(hipStreamSynchronize(s));
                 done = used >= c.nCorrectors && couplingOf(h) <= c.couplingTol;
             }
@@ -556,6 +562,7 @@ StepRep PisoSolver::step(double dt) {
         BSIM_HIP(hipMemcpyAsync(hs.data(), red.p + kSlotDiag, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
         BSIM_HIP(hipMemcpyAsync(hs.data() + 8, red.p + kSlotCoupling, sizeof(double) * 8 * used,
                                 hipMemcpyDeviceToHost, s));
+        ++run.reductions;
         C.checkErrors();
         for (int k = 0; k < used; ++k) {
             const double* h = hs.data() + 8 + 8 * k;

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "context.hpp"
+#include "perf.hpp"
 
 #include <limits>
 
@@ -51,6 +52,7 @@ public:
     StepRep step(double dt);
     std::vector<double> momHist[3], pHist;
     PisoTiming timing;
+    RunClock run;
     DeviceSystem& momentumSystem() { return *momSys; }
     DeviceSystem& pressureSystem() { return *pSys; }
 

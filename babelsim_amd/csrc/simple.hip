@@ -196,6 +196,10 @@ void SimpleSolver::start() {
 
 IterRep SimpleSolver::iterate() {
     if (!started) start();
+    struct Record {  // RunTime::recordIteration at every way out of the iteration
+        RunClock& r;
+        ~Record() { r.record(); }
+    } record{run};
     const DeviceMesh& M = *C.mesh;
     const MeshView mv = M.view();
     hipStream_t s = C.stream;
@@ -236,6 +240,7 @@ IterRep SimpleSolver::iterate() {
     BSIM_LAUNCH_CHECK();
     double hr[3];
     BSIM_HIP(hipMemcpyAsync(hr, red.p, sizeof(hr), hipMemcpyDeviceToHost, s));
+    ++run.reductions;
     C.checkErrors();
     rep.rU = std::sqrt(hr[0]) / std::max(std::sqrt(hr[1]) + std::sqrt(hr[2]), 1e-30);
     timing.assembly += since(t0);
@@ -331,6 +336,7 @@ IterRep SimpleSolver::iterate() {
         BSIM_LAUNCH_CHECK();
         int nconst = 0;
         BSIM_HIP(hipMemcpyAsync(&nconst, notConst, sizeof(int), hipMemcpyDeviceToHost, s));
+        ++run.reductions;
         C.checkErrors();
         rep.referenced = nconst == 0;
         timing.pressureAssembly += since(t0);
@@ -384,6 +390,7 @@ IterRep SimpleSolver::iterate() {
     BSIM_LAUNCH_CHECK();
     double h[kRedK];
     BSIM_HIP(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    ++run.reductions;
     C.checkErrors();
     rep.dU = std::sqrt(h[0]) / std::max(std::sqrt(h[1]), 1e-30);
     rep.dP = std::sqrt(h[2]) / std::max(std::sqrt(h[3]), 1e-30);

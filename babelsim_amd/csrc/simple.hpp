@@ -6,6 +6,7 @@
 
 #include "context.hpp"
 #include "komega.hpp"
+#include "perf.hpp"
 
 namespace bsim {
 
@@ -48,6 +49,7 @@ public:
     IterRep iterate();
     std::vector<double> momHist[3], pHist;
     SimpleTiming timing;
+    RunClock run;
     DeviceSystem& momentumSystem() { return *momSys; }
     DeviceSystem& pressureSystem() { return *pSys; }
 

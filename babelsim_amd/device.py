@@ -11,8 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (IterReport, MeshData, PisoConfig, SimpleConfig, SolveResult, SolverConfig, StepReport, check, dp,
-                    ip, lib, ptr)
+from ._capi import (EquationCounters, IterReport, MeshData, Performance, PisoConfig, SimpleConfig, SolveResult,
+                    SolverConfig, StepReport, check, dp, ip, lib, ptr)
 
 
 def _f64(a):
@@ -21,6 +21,15 @@ def _f64(a):
 
 def _i32(a):
     return np.ascontiguousarray(a, np.int32)
+
+
+def _run_performance(fn, handle) -> tuple[dict, list[dict]]:
+    """The run's PerformanceCounters and its systems' counters."""
+    total = Performance()
+    systems = (EquationCounters * 8)()
+    count = C.c_int()
+    check(fn(handle, C.byref(total), systems, 8, C.byref(count)))
+    return total.as_dict(), [systems[k].as_dict() for k in range(count.value)]
 
 
 class Context:
@@ -167,6 +176,12 @@ class System:
                 "true_residual_checks", "spmvs", "pc_applications"]
         return dict(zip(keys, list(out)))
 
+    def performance(self) -> dict:
+        """EquationPerformanceCounters of this system (bsim_system_performance)."""
+        out = EquationCounters()
+        check(lib().bsim_system_performance(self.h, C.byref(out)))
+        return out.as_dict()
+
 
 def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, device=0, hist_cap=100000):
     n = len(diag)
@@ -279,6 +294,10 @@ class Simple:
                 "convergence", "iterations", "turbulence"]
         return dict(zip(keys, out.tolist()))
 
+    def performance(self) -> tuple[dict, list[dict]]:
+        """PerformanceCounters of the run (Case::performance's 31 values) and per system."""
+        return _run_performance(lib().bsim_simple_performance, self.h)
+
     def time_spmv(self, which=1, reps=20, cold=False):
         """Mean seconds of one SpMV on the momentum (0) or pressure (1) matrix;
         cold: each launch follows a 512 MiB pass that evicts the Infinity Cache."""
@@ -373,6 +392,10 @@ class Piso:
         keys = ["history", "assembly", "momentum_solve", "corrector", "pressure_assembly", "pressure_solve",
                 "diagnostics", "steps"]
         return dict(zip(keys, out.tolist()))
+
+    def performance(self) -> tuple[dict, list[dict]]:
+        """PerformanceCounters of the run and per system (momentum, pressure)."""
+        return _run_performance(lib().bsim_piso_performance, self.h)
 
     def probe_start(self, capacity=8192):
         check(lib().bsim_piso_probe_start(self.h, capacity))

@@ -164,6 +164,50 @@ int bsim_system_history(bsim_system* sys, double* out, int cap, int* len);
  * rhs-only solves, PC setups, PC reuses, retries, solves, Krylov iterations,
  * true-residual checks, SpMVs, PC applications. */
 int bsim_system_counters(bsim_system* sys, uint64_t* out10);
+/* EquationPerformanceCounters (include/babelsim/solver_control.h:36-60) of one
+ * persistent system, counted as src/backend/petsc_backend.cpp:236-286 counts
+ * them (a preconditioner reuse per right-hand side solved with a lagged
+ * preconditioner; a true-residual check per Krylov restart or accepted initial
+ * guess; solve seconds inside the Krylov calls only), plus the system's SpMVs
+ * and preconditioner applications.  ksp_type, pc_type and last_status use the
+ * reference's words ("cg", "bcgs"; "none", "jacobi", "amg" for the aggregation
+ * AMG; "converged", "maxIterations", "numericalFailure", empty before the first
+ * solve).  Seconds are host wall time around device work that ends in a
+ * stream synchronisation.  The identity is set by the solver drivers below
+ * ("vector:momentum:U", "scalar:pressureCorrection:pPrime", ...) and empty
+ * for a stand-alone system, whose identity the caller holds. */
+typedef struct {
+    char identity[64], ksp_type[16], pc_type[16], last_status[24];
+    uint64_t matrix_pattern_builds, matrix_value_updates, rhs_only_solves, linear_solves, krylov_iterations,
+        preconditioner_setups, preconditioner_reuses, preconditioner_retries, true_residual_checks, sparse_matvecs,
+        preconditioner_applications;
+    double last_initial_residual, last_final_residual, last_relative_residual;
+    double pattern_build_seconds, matrix_update_seconds, preconditioner_seconds, rhs_seconds, solve_seconds,
+        residual_check_seconds;
+} bsim_equation_counters;
+int bsim_system_performance(bsim_system* sys, bsim_equation_counters* out);
+
+/* PerformanceCounters (include/babelsim/solver_control.h:62-107) of a solver
+ * run: the 31 values of Case::performance (src/io/case.cpp:72-104) in that
+ * order, summed over the run's systems.  On one device there are no halo
+ * exchanges, output writes or MPI reductions: global_reductions counts the
+ * device reductions the host reads (the convergence measures and residual
+ * norms the reference reduces over ranks), assembly_seconds the face-loop
+ * assembly phases, and the SpMV and preconditioner-application seconds are not
+ * measured apart from the solves (0).  reported_iteration_intervals and
+ * iteration_compute_seconds follow RunTime::recordIteration
+ * (src/runtime/runtime.cpp:117-130): the window opens at the end of the first
+ * outer iteration or time step. */
+typedef struct {
+    uint64_t linear_solves, krylov_iterations, sparse_matvecs, halo_exchanges, halo_bytes, global_reductions,
+        equation_assemblies, preconditioner_setups, preconditioner_applications, output_writes;
+    double elapsed_seconds, assembly_seconds, preconditioner_seconds, preconditioner_apply_seconds,
+        linear_solve_seconds, sparse_matvec_seconds, halo_seconds, global_reduction_seconds, output_seconds;
+    uint64_t matrix_pattern_builds, matrix_value_updates, rhs_only_solves, true_residual_checks;
+    double matrix_update_seconds, rhs_seconds, residual_check_seconds, pattern_build_seconds;
+    uint64_t preconditioner_reuses, preconditioner_retries, reported_iteration_intervals;
+    double iteration_compute_seconds;
+} bsim_performance;
 void bsim_system_destroy(bsim_system* sys);
 
 /* Stand-alone CSR solve / SpMV / AMG hierarchy (diagonal stored apart from the
@@ -254,6 +298,11 @@ int bsim_simple_history(bsim_simple* s, int which, double* out, int cap, int* le
 /* Seconds per phase: assembly, momentum solve, Rhie-Chow, pressure assembly,
  * pressure solve, correction, convergence, iterations, turbulence transport. */
 int bsim_simple_timing(bsim_simple* s, double* out9);
+/* The run's PerformanceCounters and its systems' counters (momentum,
+ * pressure correction, then k and omega with k-omega); count receives the
+ * number of systems, at most cap are written. */
+int bsim_simple_performance(bsim_simple* s, bsim_performance* total, bsim_equation_counters* systems, int cap,
+                            int* count);
 /* Times `reps` launches of the solver's SpMV on the current momentum (which
  * 0) or pressure (1) matrix with HIP events on the context stream; returns
  * mean seconds and the matrix size.  cold != 0: every launch follows a
@@ -321,6 +370,9 @@ int bsim_piso_history(bsim_piso* s, int which, double* out, int cap, int* len);
  * corrector (HbyA, fluxes, corrections, coupling), pressure assembly,
  * pressure solve, diagnostics, steps. */
 int bsim_piso_timing(bsim_piso* s, double* out8);
+/* As bsim_simple_performance: momentum and pressure systems. */
+int bsim_piso_performance(bsim_piso* s, bsim_performance* total, bsim_equation_counters* systems, int cap,
+                          int* count);
 /* Diagnostics with no counterpart in the reference: HIP-event timing of the
  * pressure solve's fine-level AMG smoother sweeps, as bsim_simple_probe_*. */
 int bsim_piso_probe_start(bsim_piso* s, int capacity);

@@ -244,6 +244,49 @@ int orc_csr_solve(int n, const int* ptr, const int* col, const double* diag, con
     });
 }
 
+struct OrcSystem {
+    const Mesh* mesh;
+    LinearSystem sys;
+};
+void* orc_system_new(void* mesh) {
+    OrcSystem* out = nullptr;
+    guard([&] { out = new OrcSystem{static_cast<const Mesh*>(mesh), {}}; });
+    return out;
+}
+void orc_system_free(void* s) { delete static_cast<OrcSystem*>(s); }
+int orc_system_solve_ldu(void* sp, const double* diag, const double* upper, const double* lower, int nrhs,
+                         const double* source, double* x, const orc_solver_config* cfg, orc_solve_result* results) {
+    return guard([&] {
+        if (nrhs != 1 && nrhs != 3) throw std::invalid_argument("linear systems have one or three right-hand sides");
+        OrcSystem& S = *static_cast<OrcSystem*>(sp);
+        const Mesh& m = *S.mesh;
+        std::vector<double> d(diag, diag + m.nc), up(upper, upper + m.nf), lo(lower, lower + m.nf);
+        Csr A;
+        lduToCsr(m, d, up, lo, A);
+        if (!S.sys.ready) S.sys.A = A;
+        S.sys.update(A.diag, A.val);
+        S.sys.prepare(toConfig(cfg));
+        for (int comp = 0; comp < nrhs; ++comp) {
+            std::vector<double> b(m.nc), xv(m.nc);
+            for (Index i = 0; i < m.nc; ++i) { b[i] = source[i * nrhs + comp]; xv[i] = x[i * nrhs + comp]; }
+            const SolveResult r = S.sys.solve(b, xv, comp);
+            for (Index i = 0; i < m.nc; ++i) x[i * nrhs + comp] = xv[i];
+            results[comp].status = r.status; results[comp].iterations = r.iterations; results[comp].initial = r.initial;
+            results[comp].final_residual = r.final_; results[comp].relative = r.relative;
+        }
+    });
+}
+int orc_system_counters(void* sp, uint64_t* o, int* last_status, double* last3) {
+    return guard([&] {
+        const LinearSystem& s = static_cast<OrcSystem*>(sp)->sys;
+        const uint64_t v[8] = {s.matrixUpdates, s.rhsOnly, s.pcSetups, s.pcReuses, s.retries, s.solves,
+                               s.krylovIterations, s.trueResidualChecks};
+        std::memcpy(o, v, sizeof(v));
+        if (last_status) *last_status = s.lastStatus;
+        if (last3) { last3[0] = s.lastInitial; last3[1] = s.lastFinal; last3[2] = s.lastRelative; }
+    });
+}
+
 int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,
                       const orc_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1) {
     return guard([&] {
@@ -440,6 +483,27 @@ int orc_simple_get_fields(void* sp, double* U, double* p, double* phi) {
         if (U) store(s.U.v, U);
         if (p) store(s.p.v, p);
         if (phi) store(s.phi, phi);
+    });
+}
+namespace {
+void systemCounters(const LinearSystem& s, uint64_t* o) {
+    const uint64_t v[8] = {s.matrixUpdates, s.rhsOnly, s.pcSetups, s.pcReuses, s.retries, s.solves,
+                           s.krylovIterations, s.trueResidualChecks};
+    std::memcpy(o, v, sizeof(v));
+}
+}  // namespace
+int orc_simple_counters(void* sp, int which, uint64_t* out8) {
+    return guard([&] {
+        Simple& s = *static_cast<Simple*>(sp);
+        if (which != 0 && which != 1) throw std::invalid_argument("system index is 0 (momentum) or 1 (pressure)");
+        systemCounters(which == 0 ? s.momSys : s.pSys, out8);
+    });
+}
+int orc_piso_counters(void* sp, int which, uint64_t* out8) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (which != 0 && which != 1) throw std::invalid_argument("system index is 0 (momentum) or 1 (pressure)");
+        systemCounters(which == 0 ? s.momSys : s.pSys, out8);
     });
 }
 int orc_simple_history(void* sp, int which, double* out, int cap, int* len) {

@@ -214,6 +214,12 @@ struct LinearSystem {  // one persistent system per equation identity (petsc_lin
     bool ready = false, configured = false, setupPending = false, forceRebuild = false;
     SolverConfig cfg;
     std::uint64_t matrixUpdates = 0, pcMatrixUpdate = 0, pcRhsSolves = 0, pcSetups = 0, rhsOnly = 0, retries = 0;
+    // Accounting of src/backend/petsc_backend.cpp:236-286: a reuse per
+    // right-hand side solved with a lagged PC, a true-residual check per Krylov
+    // restart or accepted initial guess.
+    std::uint64_t solves = 0, krylovIterations = 0, pcReuses = 0, trueResidualChecks = 0;
+    int lastStatus = -1;
+    double lastInitial = 0, lastFinal = 0, lastRelative = 0;
     int freshIterations[3] = {0, 0, 0};
     Amg* amg = nullptr;
     std::vector<double> hist;  // residual history of the last solve (every KSP iteration, all restarts)

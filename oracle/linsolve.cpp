@@ -511,10 +511,17 @@ SolveResult LinearSystem::solve(const std::vector<double>& b, std::vector<double
     const double scale = std::max(initial, rhsNorm) > 0.0 ? std::max(initial, rhsNorm) : 1.0;
     const double target = std::max(cfg.atol, cfg.rtol * scale);
     const bool reused = cfg.pc != 0 && pcMatrixUpdate < matrixUpdates;
+    ++solves;
+    if (reused) ++pcReuses;
     SolveResult res;
+    auto record = [&]() {
+        lastStatus = res.status; lastInitial = res.initial; lastFinal = res.final_; lastRelative = res.relative;
+    };
     if (std::isfinite(initial) && initial <= target) {
         res.status = Converged; res.iterations = 0; res.initial = initial; res.final_ = initial;
         res.relative = initial / scale;
+        ++trueResidualChecks;
+        record();
         return res;
     }
     int total = 0;
@@ -529,6 +536,7 @@ SolveResult LinearSystem::solve(const std::vector<double>& b, std::vector<double
             else k.bcgs(b, x, guessZero, cfg.maxIter - attemptIts);
             attemptIts += k.its; total += k.its; reason = k.reason;
             fin = trueResidual();
+            ++trueResidualChecks;
             if ((std::isfinite(fin) && fin <= target) || reason <= 0 || !std::isfinite(fin) || attemptIts >= cfg.maxIter) break;
             internalTarget *= 0.5;
             guessZero = false;
@@ -552,6 +560,8 @@ SolveResult LinearSystem::solve(const std::vector<double>& b, std::vector<double
         break;
     }
     if (total > 0) ++pcRhsSolves;
+    krylovIterations += total;
+    record();
     return res;
 }
 

@@ -87,6 +87,17 @@ int orc_assemble_diffusion(void* mesh, int ncomp, const double* x, const int* bc
 int orc_csr_solve(int n, const int* ptr, const int* col, const double* diag, const double* val, const double* b,
                   double* x, const orc_solver_config* cfg, orc_solve_result* res, double* hist, int hist_cap,
                   int* hist_len);
+/* One persistent system on a mesh, fed LDU arrays (ComputeBackend::solve,
+ * petsc_backend.cpp:123-198): nrhs 1 or 3 (sources and unknowns interleaved
+ * per cell); x holds the guess (warm start) and receives the solution. */
+void* orc_system_new(void* mesh);
+void orc_system_free(void* sys);
+int orc_system_solve_ldu(void* sys, const double* diag, const double* upper, const double* lower, int nrhs,
+                         const double* source, double* x, const orc_solver_config* cfg, orc_solve_result* results);
+/* matrix updates, rhs-only solves, PC setups, PC reuses, retries, solves,
+ * Krylov iterations, true-residual checks; last status, initial, final and
+ * relative residual. */
+int orc_system_counters(void* sys, uint64_t* out8, int* last_status, double* last3);
 int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,
                       const orc_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
 
@@ -97,6 +108,9 @@ int orc_simple_set_fields(void* s, const double* U, const double* p);
 int orc_simple_start(void* s);
 int orc_simple_iterate(void* s, int n, orc_iter_report* reports);
 int orc_simple_get_fields(void* s, double* U, double* p, double* phi);
+/* Counters of the momentum (0) or pressure (1) system, as orc_system_counters. */
+int orc_simple_counters(void* s, int which, uint64_t* out8);
+int orc_piso_counters(void* s, int which, uint64_t* out8);
 int orc_simple_history(void* s, int which, double* out, int cap, int* len);
 /* Wilcox k-omega for the SIMPLE driver: k and omega boundary conditions and
  * initial values; before start(). */

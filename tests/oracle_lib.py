@@ -60,6 +60,15 @@ def oracle():
         L.orc_csr_solve.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig),
                                     C.POINTER(SolveResult), dp, C.c_int, ip]
         L.orc_amg_hierarchy.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp]
+        L.orc_system_new.restype = vp
+        L.orc_system_new.argtypes = [vp]
+        L.orc_system_free.restype = None
+        L.orc_system_free.argtypes = [vp]
+        L.orc_system_solve_ldu.argtypes = [vp, dp, dp, dp, C.c_int, dp, dp, C.POINTER(SolverConfig),
+                                           C.POINTER(SolveResult)]
+        L.orc_system_counters.argtypes = [vp, C.POINTER(C.c_uint64), ip, dp]
+        L.orc_simple_counters.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_piso_counters.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_simple_new.restype = vp
         L.orc_simple_new.argtypes = [vp, C.POINTER(SimpleConfig), ip, dp, ip, dp]
         L.orc_simple_free.argtypes = [vp]
@@ -277,6 +286,9 @@ class OracleSimple:
         self._call("orc_simple_get_fields", ptr(U), ptr(p), ptr(phi))
         return U, p, phi
 
+    def counters(self, which):
+        return _solver_counters(oracle().orc_simple_counters, self.h, which)
+
     def history(self, which):
         n = C.c_int()
         out = np.zeros(200000)
@@ -398,6 +410,9 @@ class OraclePiso:
         self._call("orc_piso_temporal_flux", ptr(out))
         return out
 
+    def counters(self, which):
+        return _solver_counters(oracle().orc_piso_counters, self.h, which)
+
     def history(self, which):
         n = C.c_int()
         out = np.zeros(200000)
@@ -411,6 +426,57 @@ def time_weights(method, levels, dt, previous_dt):
     if oracle().orc_time_weights(method, levels, dt, previous_dt, ptr(out)):
         raise ValueError(oracle().orc_last_error().decode())
     return out
+
+
+COUNTER_KEYS = ["matrix_value_updates", "rhs_only_solves", "preconditioner_setups", "preconditioner_reuses",
+                "preconditioner_retries", "linear_solves", "krylov_iterations", "true_residual_checks"]
+
+
+def _solver_counters(fn, h, which) -> dict:
+    """Counters of a solver's momentum (0) or pressure (1) system, under the
+    names of EquationPerformanceCounters."""
+    out = (C.c_uint64 * 8)()
+    if fn(h, which, out):
+        raise RuntimeError(oracle().orc_last_error().decode())
+    return dict(zip(COUNTER_KEYS, list(out)))
+
+
+class OracleSystem:
+    """A persistent oracle system fed LDU arrays (ComputeBackend::solve); the
+    same calls and counter names as babelsim_amd.device.System."""
+
+    def __init__(self, mesh: CpuMesh):
+        self.mesh = mesh
+        self.h = oracle().orc_system_new(mesh.h)
+        if not self.h:
+            raise RuntimeError(oracle().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle().orc_system_free(self.h)
+            self.h = None
+
+    def solve(self, diag, upper, lower, source, x, cfg: SolverConfig):
+        src = np.ascontiguousarray(source, np.float64)
+        nrhs = 1 if src.ndim == 1 else src.shape[1]
+        xo = np.ascontiguousarray(x, np.float64).copy()
+        res = (SolveResult * nrhs)()
+        rc = oracle().orc_system_solve_ldu(self.h, ptr(np.ascontiguousarray(diag, np.float64)),
+                                           ptr(np.ascontiguousarray(upper, np.float64)),
+                                           ptr(np.ascontiguousarray(lower, np.float64)), nrhs, ptr(src), ptr(xo),
+                                           C.byref(cfg), res)
+        if rc:
+            raise RuntimeError(oracle().orc_last_error().decode())
+        return xo, list(res)
+
+    def counters(self) -> dict:
+        out = (C.c_uint64 * 8)()
+        st = C.c_int()
+        last = np.zeros(3)
+        oracle().orc_system_counters(self.h, out, C.byref(st), ptr(last))
+        keys = ["matrix_updates", "rhs_only", "pc_setups", "pc_reuses", "retries", "solves", "krylov_iterations",
+                "true_residual_checks"]
+        return dict(zip(keys, list(out)))
 
 
 def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, hist_cap=200000):

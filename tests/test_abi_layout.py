@@ -23,6 +23,8 @@ STRUCTS = {
     "bsim_step_report": _capi.StepReport,
     "bsim_request": _capi.Request,
     "bsim_request_result": _capi.RequestResult,
+    "bsim_equation_counters": _capi.EquationCounters,
+    "bsim_performance": _capi.Performance,
 }
 
 

@@ -1,0 +1,69 @@
+"""The oracle's persistent linear system against the reference's own solve
+contract expectations (tests/petsc_backend_test.cpp, tests/residual_restart_test.cpp):
+preconditioner reuse and rebuild intervals, retry of a stale preconditioner,
+rebuild after iteration growth, right-hand-side-only solves, true-residual
+restarts with honest rejection, and the warm-start shortcut.  The same
+scenarios run on the HIP library in tests/test_gpu_contract.py."""
+import pytest
+
+import contract_cases as cc
+import meshes
+import oracle_lib as orc
+
+
+def new_system(arrays):
+    return orc.OracleSystem(orc.oracle_mesh(arrays))
+
+
+@pytest.mark.parametrize("pc", [cc.PC_JACOBI, cc.PC_AMG])
+def test_preconditioner_reuse(pc):
+    cc.preconditioner_reuse(new_system, pc)
+
+
+def test_retry_after_stale_preconditioner():
+    cc.retry_after_stale_preconditioner(new_system)
+
+
+def test_rebuild_after_iteration_growth():
+    cc.rebuild_after_iteration_growth(new_system)
+
+
+@pytest.mark.parametrize("pc", [cc.PC_JACOBI, cc.PC_AMG])
+def test_scalar_and_vector_solves(pc):
+    cc.scalar_and_vector_solves(new_system, pc)
+
+
+@pytest.mark.parametrize("pc", [cc.PC_JACOBI, cc.PC_AMG])
+def test_repeated_face_pairs(pc):
+    cc.repeated_face_pairs(new_system, meshes.split_interface(), pc)
+
+
+def test_residual_restarts():
+    repaired, rejected = cc.residual_restarts(new_system)
+    # The strictest targets lie below what FP64 CG reaches on this system.
+    assert rejected >= 1
+
+
+@pytest.mark.parametrize("pc", [cc.PC_JACOBI, cc.PC_AMG])
+def test_reuses_count_per_right_hand_side(pc):
+    cc.reuses_count_per_right_hand_side(new_system, pc)
+
+
+def test_simple_system_counters():
+    """The oracle's SIMPLE solves one three-component momentum system and
+    (non-orthogonal corrections + 1) pressure systems per outer iteration
+    (petsc_backend.cpp:123-198; simple/main.cpp:63-93), each updating its
+    matrix once per assembly."""
+    import cases
+    arrays = cases.cavity3d_mesh(6).arrays()
+    ub, pb = cases.cavity_bcs(two_d=False)
+    cfg = cases.c2_config()
+    s = orc.OracleSimple(orc.oracle_mesh(arrays), cfg, ub, pb)
+    s.start()
+    n = 4
+    s.iterate(n)
+    mom, pre = s.counters(0), s.counters(1)
+    solves = cfg.non_orth_corr + 1
+    assert mom["linear_solves"] == 3 * n and mom["matrix_value_updates"] + mom["rhs_only_solves"] == n
+    assert pre["linear_solves"] == solves * n and pre["matrix_value_updates"] + pre["rhs_only_solves"] == solves * n
+    assert mom["true_residual_checks"] >= 3 * n and pre["krylov_iterations"] > 0
