@@ -148,6 +148,13 @@ class EquationCounters(C.Structure):
                 for n, _ in self._fields_}
 
 
+class EquMethods(C.Structure):
+    """Operator options of an equation (include/babelsim/methods.h)."""
+
+    _fields_ = [("interp", C.c_int), ("grad", C.c_int), ("conv", C.c_int), ("diff", C.c_int),
+                ("coef_interp", C.c_int), ("coef_grad", C.c_int)]
+
+
 class Performance(C.Structure):
     """PerformanceCounters: the 31 values of Case::performance (src/io/case.cpp:72-104), in order."""
 
@@ -284,6 +291,38 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_system_history": [vp, dp, C.c_int, ip],
     "bsim_system_counters": [vp, u64p],
     "bsim_system_performance": [vp, C.POINTER(EquationCounters)],
+    "bsim_field_create": [vp, C.c_int, C.c_int, dp, C.POINTER(vp)],
+    "bsim_field_set": [vp, dp],
+    "bsim_field_get": [vp, dp],
+    "bsim_field_set_boundary": [vp, ip, dp],
+    "bsim_field_set_flux": [vp, vp],
+    "bsim_equ_create": [vp, C.POINTER(EquMethods), C.POINTER(SolverConfig), C.POINTER(vp)],
+    "bsim_equ_reset": [vp],
+    "bsim_equ_copy": [vp, C.POINTER(vp)],
+    "bsim_equ_add": [vp, vp, C.c_double],
+    "bsim_equ_scale": [vp, C.c_double],
+    "bsim_equ_div": [vp, vp, C.c_double],
+    "bsim_equ_laplacian": [vp, vp, C.c_double, C.c_double],
+    "bsim_equ_reaction": [vp, vp, C.c_double, C.c_double],
+    "bsim_equ_source": [vp, vp, dp, C.c_double],
+    "bsim_equ_ddt": [vp, vp, C.c_double, vp, C.c_double, C.c_int, vp, C.c_double],
+    "bsim_equ_relax": [vp, vp, C.c_double],
+    "bsim_equ_reference": [vp, C.c_int, C.c_double],
+    "bsim_equ_reference_if_unanchored": [vp, C.c_double, ip],
+    "bsim_equ_diagonal": [vp, vp],
+    "bsim_equ_rhs": [vp, vp],
+    "bsim_equ_apply": [vp, vp, vp],
+    "bsim_equ_residual": [vp, vp, vp],
+    "bsim_equ_face_flux": [vp, vp, vp],
+    "bsim_equ_relative_residual": [vp, vp, dp],
+    "bsim_equ_solve": [vp, C.POINTER(SolverConfig), C.POINTER(SolveResult)],
+    "bsim_equ_performance": [vp, C.POINTER(EquationCounters)],
+    "bsim_math_norms": [C.POINTER(vp), C.c_int, dp],
+    "bsim_math_relative_change": [vp, vp, dp],
+    "bsim_math_relative_magnitude": [vp, vp, dp],
+    "bsim_math_flux_balance": [vp, dp],
+    "bsim_math_courant": [vp, C.c_double, dp],
+    "bsim_math_face_add": [vp, vp, C.c_double, C.c_int],
     "bsim_csr_solve": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig), C.POINTER(SolveResult),
                        dp, C.c_int, ip],
     "bsim_csr_spmv": [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp],
@@ -326,7 +365,8 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_server_estimate": [C.c_int, C.c_int, C.c_int, dp, dp],
 }
 _VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp],
-         "bsim_piso_destroy": [vp], "bsim_server_destroy": [vp]}
+         "bsim_piso_destroy": [vp], "bsim_server_destroy": [vp], "bsim_field_destroy": [vp],
+         "bsim_equ_destroy": [vp]}
 
 
 def _declare_void(L):

@@ -375,6 +375,8 @@ int bsim_system_counters(bsim_system* sys, uint64_t* o) {
     });
 }
 
+}  // extern "C"
+
 namespace {
 // Names of petsc_backend.cpp:30-60 (kspName, pcName, solveStatusName).
 const char* kspName(int solver) { return solver == 0 ? "cg" : solver == 1 ? "bcgs" : "unknown"; }
@@ -386,7 +388,9 @@ void copyName(char* dst, std::size_t cap, const char* src) {
     std::strncpy(dst, src, cap - 1);
     dst[cap - 1] = '\0';
 }
-bsim_equation_counters equationCounters(const DeviceSystem& D, const char* identity) {
+}  // namespace
+
+bsim_equation_counters bsim::equationCounters(const DeviceSystem& D, const char* identity) {
     bsim_equation_counters o{};
     const SystemCounters& c = D.counters;
     copyName(o.identity, sizeof(o.identity), identity);
@@ -415,6 +419,8 @@ bsim_equation_counters equationCounters(const DeviceSystem& D, const char* ident
     o.residual_check_seconds = c.residualSeconds;
     return o;
 }
+
+namespace {
 // Sums the systems into the run record (petsc_backend.cpp:236-286 keeps the
 // totals and the per-equation snapshots side by side).
 void addSystem(bsim_performance& t, const bsim_equation_counters& e) {
@@ -457,6 +463,7 @@ void runPerformance(const std::vector<std::pair<const DeviceSystem*, const char*
     if (count) *count = k;
 }
 }  // namespace
+extern "C" {
 
 int bsim_system_performance(bsim_system* sys, bsim_equation_counters* out) {
     return guarded([&] {

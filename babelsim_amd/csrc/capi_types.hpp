@@ -20,4 +20,6 @@ namespace bsim {
 SimpleCfg toSimpleCfg(const bsim_simple_config& cfg);
 // bsim_piso_config -> PisoCfg (src/physics/piso/main.cpp:83-105).
 PisoCfg toPisoCfg(const bsim_piso_config& cfg);
+// EquationPerformanceCounters of one persistent system (petsc_backend.cpp:236-286 accounting).
+bsim_equation_counters equationCounters(const DeviceSystem& system, const char* identity);
 }  // namespace bsim

@@ -236,6 +236,80 @@ int bsim_csr_solve_noalloc(int device, void* stream, int n, const int* ptr, cons
 int bsim_amg_hierarchy(int device, int n, const int* ptr, const int* col, const double* diag, const double* val,
                        const bsim_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
 
+/* ------------------------------------------------ fields and equations
+ * The procedural equation seam resident on a context, so that a solver
+ * written against equ:: and math:: (include/babelsim/equ.h:16-91,
+ * include/babelsim/math.h:259-267, src/internal/fvm_execution.h:38-49) keeps
+ * its fields and matrices in HBM: assembly writes the equation's own matrix
+ * and a solve hands it to the equation's persistent linear system.
+ *
+ * bsim_field: Field<T> (include/babelsim/field.h:57-374).  location 0 is a
+ * cell field of ncomp 1 or 3 values per cell with a boundary condition per
+ * patch (zero gradient until set), location 1 a face field of one value per
+ * face.  Values cross the ABI in the caller's numbering. */
+typedef struct bsim_field bsim_field;
+enum { BSIM_CELL = 0, BSIM_FACE = 1 };
+enum { BSIM_STEADY = 0 }; /* with BSIM_EULER and BSIM_BDF2 below: TimeMethod */
+int bsim_field_create(bsim_ctx* ctx, int location, int ncomp, const double* values, bsim_field** out);
+void bsim_field_destroy(bsim_field* f);
+int bsim_field_set(bsim_field* f, const double* values);
+int bsim_field_get(const bsim_field* f, double* values);
+/* Field::setBoundary per patch: npatch types and ncomp values each. */
+int bsim_field_set_boundary(bsim_field* f, const int* bc_type, const double* bc_value);
+/* Field::setBoundaryFlux (field.h:277-282): the inletOutlet direction context. */
+int bsim_field_set_flux(bsim_field* f, const bsim_field* face_flux);
+
+/* bsim_equ: equ::Equation<T> bound to a cell field, with its operator options
+ * (methods.h) and linear solver configuration (EquationControl). */
+typedef struct bsim_equ bsim_equ;
+typedef struct {
+    int interp, grad, conv, diff, coef_interp, coef_grad;
+} bsim_equ_methods;
+int bsim_equ_create(bsim_field* unknown, const bsim_equ_methods* methods, const bsim_solver_config* linear,
+                    bsim_equ** out);
+void bsim_equ_destroy(bsim_equ* a);
+int bsim_equ_reset(bsim_equ* a);                                         /* Equation::reset */
+int bsim_equ_copy(const bsim_equ* a, bsim_equ** out);                    /* Equation::copy */
+int bsim_equ_add(bsim_equ* a, const bsim_equ* b, double factor);         /* equ::add */
+int bsim_equ_scale(bsim_equ* a, double factor);                          /* equ::scale */
+int bsim_equ_div(bsim_equ* a, const bsim_field* flux, double factor);    /* equ::div */
+/* equ::laplacian: coefficient NULL for the constant k, else a cell or face field. */
+int bsim_equ_laplacian(bsim_equ* a, const bsim_field* coefficient, double k, double multiplier);
+/* equ::reaction: coefficient NULL for the constant value, else factor * field. */
+int bsim_equ_reaction(bsim_equ* a, const bsim_field* coefficient, double value, double factor);
+/* equ::source: value NULL for the constant (ncomp values), else factor * field. */
+int bsim_equ_source(bsim_equ* a, const bsim_field* value, const double* constant, double factor);
+/* equ::ddt: capacity NULL for the constant; method BSIM_STEADY, BSIM_EULER or
+ * BSIM_BDF2 (older required; previous_dt 0 means dt). */
+int bsim_equ_ddt(bsim_equ* a, const bsim_field* capacity, double cap, const bsim_field* previous, double dt, int method,
+                 const bsim_field* older, double previous_dt);
+int bsim_equ_relax(bsim_equ* a, const bsim_field* previous, double alpha);        /* equ::relax */
+int bsim_equ_reference(bsim_equ* a, int cell, double value);                       /* Equation::reference */
+int bsim_equ_reference_if_unanchored(bsim_equ* a, double value, int* applied);     /* referenceIfUnanchored */
+int bsim_equ_diagonal(const bsim_equ* a, bsim_field* out);                         /* Equation::diagonal */
+int bsim_equ_rhs(const bsim_equ* a, bsim_field* out);                              /* Equation::rhs */
+int bsim_equ_apply(const bsim_equ* a, const bsim_field* x, bsim_field* out);       /* equ::apply */
+int bsim_equ_residual(const bsim_equ* a, const bsim_field* x, bsim_field* out);    /* equ::residual, b - A x */
+int bsim_equ_face_flux(const bsim_equ* a, const bsim_field* solution, bsim_field* out);  /* equ::faceFlux */
+int bsim_equ_relative_residual(const bsim_equ* a, const bsim_field* x, double* out);    /* diagnostics:: */
+/* equ::solve: the unknown is solved in place; cfg NULL uses the equation's
+ * configuration; results receives ncomp entries. */
+int bsim_equ_solve(bsim_equ* a, const bsim_solver_config* cfg, bsim_solve_result* results);
+int bsim_equ_performance(const bsim_equ* a, bsim_equation_counters* out);
+
+/* math::normsL2 (field_math.cpp:30-62) of n fields. */
+int bsim_math_norms(const bsim_field* const* fields, int n, double* out);
+/* FvmExecution::relativeChange / relativeMagnitude (fvm_execution.cpp:91-146). */
+int bsim_math_relative_change(const bsim_field* current, const bsim_field* previous, double* out);
+int bsim_math_relative_magnitude(const bsim_field* value, const bsim_field* reference, double* out);
+/* fluxBalance (fvm_execution.cpp:148-175): l1, l2, maximum, relative. */
+int bsim_math_flux_balance(const bsim_field* flux, double* out4);
+/* courantNumber (fvm_execution.cpp:177-199): maximum, mean. */
+int bsim_math_courant(const bsim_field* flux, double dt, double* out2);
+/* math::add / subtract (field_math.cpp:65-82): target += factor * increment on
+ * all faces (region 0) or interior faces (region 1). */
+int bsim_math_face_add(const bsim_field* increment, bsim_field* target, double factor, int region);
+
 /* ----------------------------------------------------------------- SIMPLE
  * Steady SIMPLE (src/physics/simple/main.cpp:11-131) resident on the GPU. */
 typedef struct {

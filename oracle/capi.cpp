@@ -166,6 +166,18 @@ int orc_normal_gradient(void* mesh, const double* x, const int* bct, const doubl
         store(o, out);
     });
 }
+int orc_diffusion_flux(void* mesh, const double* x, const int* bct, const double* bcv, const double* ctx,
+                       const double* kf, int gmethod, int dmethod, double* out) {
+    return guard([&] {
+        const Mesh& m = *static_cast<Mesh*>(mesh);
+        auto f = makeField<double>(m, x, bct, bcv, ctx);
+        std::vector<V3> g;
+        gradient(f, g, gmethod);
+        std::vector<double> o;
+        diffusionFlux(std::vector<double>(kf, kf + m.nf), f, g, dmethod, o);
+        store(o, out);
+    });
+}
 int orc_divergence_face(void* mesh, const double* phi, double* out) {
     return guard([&] {
         const Mesh& m = *static_cast<Mesh*>(mesh);

@@ -72,6 +72,9 @@ int orc_flux(void* mesh, const double* U, const int* bct, const double* bcv, con
              int gmethod, double* out);
 int orc_normal_gradient(void* mesh, const double* x, const int* bct, const double* bcv, int gmethod, int dmethod,
                         double* out);
+/* k_f times integratedNormalGradient with the field's gradient (operators.cpp:1004-1024). */
+int orc_diffusion_flux(void* mesh, const double* x, const int* bct, const double* bcv, const double* flux_ctx,
+                       const double* kf, int gmethod, int dmethod, double* out);
 int orc_divergence_face(void* mesh, const double* phi, double* out);
 int orc_explicit_laplacian(void* mesh, const double* x, const int* bct, const double* bcv, const double* kf,
                            double k, int gmethod, int dmethod, double* out);

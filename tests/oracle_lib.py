@@ -36,6 +36,7 @@ def _sig(L, prefix):
     f("flux").argtypes = [vp, dp, ip, dp, dp, C.c_int, C.c_int, dp]
     f("normal_gradient").argtypes = [vp, dp, ip, dp, C.c_int, C.c_int, dp]
     f("divergence_face").argtypes = [vp, dp, dp]
+    f("diffusion_flux").argtypes = [vp, dp, ip, dp, dp, dp, C.c_int, C.c_int, dp]
     f("explicit_laplacian").argtypes = [vp, dp, ip, dp, dp, C.c_double, C.c_int, C.c_int, dp]
     f("explicit_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, dp]
     f("assemble_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -192,6 +193,13 @@ class CpuMesh:
         out = np.zeros(self.nf)
         self.call("normal_gradient", ptr(np.ascontiguousarray(x, np.float64)), ptr(t, ip), ptr(v), gmethod,
                   dmethod, ptr(out))
+        return out
+
+    def diffusion_flux(self, x, bct, bcv, kf, gmethod, dmethod, ctx=None):
+        t, v = self._bc(bct, bcv, 1)
+        out = np.zeros(self.nf)
+        self.call("diffusion_flux", ptr(np.ascontiguousarray(x, np.float64)), ptr(t, ip), ptr(v), ptr(ctx),
+                  ptr(np.ascontiguousarray(kf, np.float64)), gmethod, dmethod, ptr(out))
         return out
 
     def divergence_face(self, phi):

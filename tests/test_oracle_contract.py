@@ -40,7 +40,7 @@ def test_repeated_face_pairs(pc):
 
 def test_residual_restarts():
     repaired, rejected = cc.residual_restarts(new_system)
-    # The strictest targets lie below what FP64 CG reaches on this system.
+    # The strictest targets lie below what FP64 CG reaches in this row order.
     assert rejected >= 1
 
 
