@@ -36,7 +36,8 @@ def _sig(L, prefix):
     f("flux").argtypes = [vp, dp, ip, dp, dp, C.c_int, C.c_int, dp]
     f("normal_gradient").argtypes = [vp, dp, ip, dp, C.c_int, C.c_int, dp]
     f("divergence_face").argtypes = [vp, dp, dp]
-    f("diffusion_flux").argtypes = [vp, dp, ip, dp, dp, dp, C.c_int, C.c_int, dp]
+    if hasattr(L, prefix + "diffusion_flux"):  # the oracle's; the reference shim has no such export
+        f("diffusion_flux").argtypes = [vp, dp, ip, dp, dp, dp, C.c_int, C.c_int, dp]
     f("explicit_laplacian").argtypes = [vp, dp, ip, dp, dp, C.c_double, C.c_int, C.c_int, dp]
     f("explicit_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, dp]
     f("assemble_convection").argtypes = [vp, C.c_int, dp, ip, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
