@@ -158,32 +158,48 @@ template <class T> T correctedFaceValue(const Field<T>& f, const typename GradOf
     return f.m->boundary(face) ? reconstructedBoundary(f, g, face, flux) : correctedInternal(f, g, face);
 }
 
+// Face loops that scatter into both cells of a face run as gathers: every
+// cell, on its own thread, visits its faces in ascending face order and
+// applies the face's update to its own entries.  Each entry then receives the
+// same updates in the same order as in a serial loop over faces, so results
+// are bit-identical to that loop and independent of the thread count.  The
+// owner's visit also writes the face's own entries (LDU upper/lower).
+template <class F> void cellFaces(const Mesh& m, F&& face) {
+    parallelFor(m.nc, [&](Index c) {
+        for (Index k = m.cfo[c]; k < m.cfo[c + 1]; ++k) {
+            const Index f = m.cfi[k];
+            face(f, c, m.own[f] == c);
+        }
+    });
+}
+
 // Green-Gauss, two passes on non-orthogonal meshes (operators.cpp:275-316, 365-405).
 template <class T> void greenGauss(const Field<T>& x, std::vector<typename GradOf<T>::type>& out) {
     using G = typename GradOf<T>::type;
     const Mesh& m = *x.m;
     out.assign(m.nc, G{});
-    for (Index f = 0; f < m.nf; ++f) {
-        const Index o = m.own[f], n = m.nei[f];
-        const T value = interpolatedFaceValue(x, f);
+    std::vector<T> fv(m.nf);
+    parallelFor(m.nf, [&](Index f) { fv[f] = interpolatedFaceValue(x, f); });
+    cellFaces(m, [&](Index f, Index cell, bool own) {
+        const T value = fv[f];
         for (int c = 0; c < ncomp<T>(); ++c) {
             const V3 contrib = comp(value, c) * m.Sf[f];
-            gradRow(out[o], c) += contrib * m.invV[o];
-            if (n != kNone) gradRow(out[n], c) -= contrib * m.invV[n];
+            if (own) gradRow(out[cell], c) += contrib * m.invV[cell];
+            else gradRow(out[cell], c) -= contrib * m.invV[cell];
         }
-    }
+    });
     if (m.orthogonal) return;
     const std::vector<G> initial = out;
     out.assign(m.nc, G{});
-    for (Index f = 0; f < m.nf; ++f) {
-        const Index o = m.own[f], n = m.nei[f];
-        const T value = correctedFaceValue(x, initial.data(), f);
+    parallelFor(m.nf, [&](Index f) { fv[f] = correctedFaceValue(x, initial.data(), f); });
+    cellFaces(m, [&](Index f, Index cell, bool own) {
+        const T value = fv[f];
         for (int c = 0; c < ncomp<T>(); ++c) {
             const V3 contrib = comp(value, c) * m.Sf[f];
-            gradRow(out[o], c) += contrib * m.invV[o];
-            if (n != kNone) gradRow(out[n], c) -= contrib * m.invV[n];
+            if (own) gradRow(out[cell], c) += contrib * m.invV[cell];
+            else gradRow(out[cell], c) -= contrib * m.invV[cell];
         }
-    }
+    });
 }
 
 // Least squares with boundary mirror rows (operators.cpp:318-363, 407-458).
@@ -305,10 +321,10 @@ void flux(VField& U, std::vector<double>& out, int interp, int gmethod) {  // op
 void divergenceFace(const Mesh& m, const std::vector<double>& phi, std::vector<double>& out) {
     // operators.cpp:1055-1063 via addFaceDivergence (460-475)
     out.assign(m.nc, 0.0);
-    for (Index f = 0; f < m.nf; ++f) {
-        out[m.own[f]] += phi[f] * m.invV[m.own[f]];
-        if (m.nei[f] != kNone) out[m.nei[f]] -= phi[f] * m.invV[m.nei[f]];
-    }
+    cellFaces(m, [&](Index f, Index c, bool own) {
+        if (own) out[c] += phi[f] * m.invV[c];
+        else out[c] -= phi[f] * m.invV[c];
+    });
 }
 
 void divergenceVector(const VField& U, std::vector<double>& out, int interp, int gmethod) {
@@ -317,12 +333,16 @@ void divergenceVector(const VField& U, std::vector<double>& out, int interp, int
     std::vector<T3> g;
     if (interp == Corrected) gradient(U, g, gmethod);
     out.assign(m.nc, 0.0);
-    for (Index f = 0; f < m.nf; ++f) {
+    std::vector<double> qf(m.nf);
+    parallelFor(m.nf, [&](Index f) {
         const V3 fv = interp == Corrected ? correctedFaceValue(U, g.data(), f) : interpolatedFaceValue(U, f);
-        const double q = dot(fv, m.Sf[f]);
-        out[m.own[f]] += q * m.invV[m.own[f]];
-        if (m.nei[f] != kNone) out[m.nei[f]] -= q * m.invV[m.nei[f]];
-    }
+        qf[f] = dot(fv, m.Sf[f]);
+    });
+    cellFaces(m, [&](Index f, Index c, bool own) {
+        const double q = qf[f];
+        if (own) out[c] += q * m.invV[c];
+        else out[c] -= q * m.invV[c];
+    });
 }
 
 void normalGradient(const SField& p, const std::vector<V3>* g, int gmethod, int dm, std::vector<double>& out) {
@@ -355,36 +375,52 @@ template <class T> void addConvection(Ldu<T>& eq, const std::vector<double>& phi
     std::vector<typename GradOf<T>::type> g;
     const bool haveG = conv == LinearUpwind || (conv == Central && interp == Corrected);
     if (haveG) gradient(x, g, gmethod);
-    for (Index f = 0; f < m.nf; ++f) {
-        const Index o = m.own[f], n = m.nei[f];
+    // F times the deferred correction of each interior face, computed once.
+    std::vector<T> fcorr;
+    if (haveG) {
+        fcorr.assign(m.nf, T{});
+        parallelFor(m.nf, [&](Index f) {
+            const Index n = m.nei[f];
+            if (n == kNone) return;
+            const double F = scale * phi[f];
+            if (conv == LinearUpwind) fcorr[f] = F * luCorrection(x, g.data(), F >= 0.0 ? m.own[f] : n, f);
+            else if (conv == Central) fcorr[f] = F * (correctedInternal(x, g.data(), f) - internalFaceValue(x, f));
+        });
+    }
+    cellFaces(m, [&](Index f, Index o, bool own) {
+        const Index n = m.nei[f];
         const double F = scale * phi[f];
         if (n != kNone) {
+            // The owner's and the neighbour's halves of the face's update.
             if (conv == Upwind || conv == LinearUpwind) {
-                eq.diag[o] += std::max(F, 0.0);
-                eq.upper[f] += std::min(F, 0.0);
-                eq.diag[n] += std::max(-F, 0.0);
-                eq.lower[f] += std::min(-F, 0.0);
+                if (own) {
+                    eq.diag[o] += std::max(F, 0.0);
+                    eq.upper[f] += std::min(F, 0.0);
+                    eq.lower[f] += std::min(-F, 0.0);
+                } else {
+                    eq.diag[o] += std::max(-F, 0.0);
+                }
                 if (conv == LinearUpwind) {
-                    const Index up = F >= 0.0 ? o : n;
-                    const T corr = luCorrection(x, g.data(), up, f);
-                    eq.source[o] -= F * corr;
-                    eq.source[n] += F * corr;
+                    if (own) eq.source[o] -= fcorr[f];
+                    else eq.source[o] += fcorr[f];
                 }
             } else if (conv == Central) {
                 const double w = m.w[f];
-                eq.diag[o] += F * w;
-                eq.upper[f] += F * (1.0 - w);
-                eq.lower[f] -= F * w;
-                eq.diag[n] -= F * (1.0 - w);
+                if (own) {
+                    eq.diag[o] += F * w;
+                    eq.upper[f] += F * (1.0 - w);
+                    eq.lower[f] -= F * w;
+                } else {
+                    eq.diag[o] -= F * (1.0 - w);
+                }
                 if (haveG) {
-                    const T corr = correctedInternal(x, g.data(), f) - internalFaceValue(x, f);
-                    eq.source[o] -= F * corr;
-                    eq.source[n] += F * corr;
+                    if (own) eq.source[o] -= fcorr[f];
+                    else eq.source[o] += fcorr[f];
                 }
             } else {
                 throw std::invalid_argument("unsupported convection method");
             }
-            continue;
+            return;
         }
         const BC<T> cond = faceCondition(x, f, F);
         if (conv == Upwind || conv == LinearUpwind) {
@@ -394,7 +430,7 @@ template <class T> void addConvection(Ldu<T>& eq, const std::vector<double>& phi
             } else {
                 eq.source[o] -= F * boundaryTrace(x, f, F);
             }
-            continue;
+            return;
         }
         if (conv != Central) throw std::invalid_argument("unsupported convection method");
         switch (cond.type) {
@@ -415,7 +451,7 @@ template <class T> void addConvection(Ldu<T>& eq, const std::vector<double>& phi
             const T corr = reconstructedBoundary(x, g.data(), f, F) - boundaryTrace(x, f, F);
             eq.source[o] -= F * corr;
         }
-    }
+    });
 }
 
 template <class T> void addDiffusion(Ldu<T>& eq, const std::vector<double>& kf, const Field<T>& x, int gmethod,
@@ -430,24 +466,35 @@ template <class T> void addDiffusion(Ldu<T>& eq, const std::vector<double>& kf, 
     } else if (dm != Orthogonal) {
         throw std::invalid_argument("unsupported diffusion method");
     }
-    for (Index f = 0; f < m.nf; ++f) {
+    // The deferred correction of each interior face, computed once.
+    std::vector<T> fcorr;
+    if (correctedDiffusion(dm)) {
+        fcorr.assign(m.nf, T{});
+        parallelFor(m.nf, [&](Index f) {
+            const Index o = m.own[f], n = m.nei[f];
+            const double gamma = kf[f];
+            if (n == kNone || !(gamma > 0.0) || !std::isfinite(gamma)) return;
+            const T orthogonal = m.orth[f] * (x.v[n] - x.v[o]);
+            fcorr[f] = gamma * (integratedNormalGradient(x, *g, f, dm) - orthogonal);
+        });
+    }
+    cellFaces(m, [&](Index f, Index c, bool own) {
         const Index o = m.own[f], n = m.nei[f];
         const double gamma = kf[f];
         if (gamma < 0.0 || !std::isfinite(gamma)) throw std::invalid_argument("face diffusivity must be non-negative and finite");
-        if (gamma == 0.0) continue;
+        if (gamma == 0.0) return;
         const double coef = gamma * m.orth[f];
         if (n != kNone) {
-            eq.diag[o] += coef;
-            eq.diag[n] += coef;
-            eq.upper[f] -= coef;
-            eq.lower[f] -= coef;
-            if (correctedDiffusion(dm)) {
-                const T orthogonal = m.orth[f] * (x.v[n] - x.v[o]);
-                const T corr = gamma * (integratedNormalGradient(x, *g, f, dm) - orthogonal);
-                eq.source[o] += corr;
-                eq.source[n] -= corr;
+            eq.diag[c] += coef;
+            if (own) {
+                eq.upper[f] -= coef;
+                eq.lower[f] -= coef;
             }
-            continue;
+            if (correctedDiffusion(dm)) {
+                if (own) eq.source[c] += fcorr[f];
+                else eq.source[c] -= fcorr[f];
+            }
+            return;
         }
         const BC<T> cond = faceCondition(x, f);
         if (cond.type == FixedValue || (isVector && cond.type == Symmetry)) {
@@ -462,7 +509,7 @@ template <class T> void addDiffusion(Ldu<T>& eq, const std::vector<double>& kf, 
             if constexpr (isVector) eq.source[o] += (gamma * m.Af[f]) * cond.value;
             else eq.source[o] += (gamma * cond.value) * m.Af[f];
         }
-    }
+    });
 }
 
 template <class T> void explicitConvection(const std::vector<double>& phi, const Field<T>& x, std::vector<T>& out,
@@ -473,7 +520,7 @@ template <class T> void explicitConvection(const std::vector<double>& phi, const
     const bool haveG = conv == LinearUpwind || (conv == Central && interp == Corrected);
     if (haveG) gradient(x, g, gmethod);
     out.assign(m.nc, T{});
-    for (Index f = 0; f < m.nf; ++f) {
+    cellFaces(m, [&](Index f, Index c, bool own) {
         const Index o = m.own[f], n = m.nei[f];
         const double q = phi[f];
         T fv{};
@@ -494,9 +541,9 @@ template <class T> void explicitConvection(const std::vector<double>& phi, const
             fv = interpolatedFaceValue(x, f, q);
         }
         const T integ = q * fv;
-        out[o] += integ * m.invV[o];
-        if (n != kNone) out[n] -= integ * m.invV[n];
-    }
+        if (own) out[c] += integ * m.invV[c];
+        else out[c] -= integ * m.invV[c];
+    });
 }
 
 void explicitLaplacian(const SField& x, const std::vector<double>* kf, double k, int gmethod, int dm,
@@ -507,7 +554,7 @@ void explicitLaplacian(const SField& x, const std::vector<double>* kf, double k,
     if (correctedDiffusion(dm)) gradient(x, g, gmethod);
     else if (dm != Orthogonal) throw std::invalid_argument("unsupported diffusion method");
     out.assign(m.nc, 0.0);
-    for (Index f = 0; f < m.nf; ++f) {
+    cellFaces(m, [&](Index f, Index c, bool own) {
         const Index o = m.own[f], n = m.nei[f];
         double q = 0.0;
         if (n != kNone) {
@@ -524,9 +571,9 @@ void explicitLaplacian(const SField& x, const std::vector<double>* kf, double k,
         }
         const double d = kf ? (*kf)[f] : k;
         const double integ = d * q;
-        out[o] += integ * m.invV[o];
-        if (n != kNone) out[n] -= integ * m.invV[n];
-    }
+        if (own) out[c] += integ * m.invV[c];
+        else out[c] -= integ * m.invV[c];
+    });
 }
 
 void gradientTraces(const SField& p, const std::vector<V3>& g, std::vector<V3>& traces) {

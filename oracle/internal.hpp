@@ -104,6 +104,10 @@ struct Mesh {
     std::vector<double> V, invV;
     std::vector<Index> cfo, cfi, cfs;     // cell -> faces CSR (input face order) and signs
     std::vector<std::vector<Index>> patchFaces;
+    // Off-diagonal pattern of a cell operator: rows sorted by (column, face),
+    // each entry's face and whether the row is the face's owner.
+    std::vector<Index> csrPtr, csrCol, csrFace;
+    std::vector<char> csrOwn;
     bool orthogonal = true;
     bool boundary(Index f) const { return nei[f] == kNone; }
 };
@@ -240,6 +244,7 @@ struct AmgLevel {
     Csr A;
     std::vector<Index> agg;          // fine -> coarse aggregate (size n), empty on coarsest
     Index nc = 0;
+    std::vector<Index> memPtr, memIdx;  // coarse -> fine members, ascending (the restriction's gather)
 };
 struct Amg {
     std::vector<AmgLevel> levels;

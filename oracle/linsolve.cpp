@@ -17,18 +17,8 @@ namespace orc {
 Csr csrPattern(const Mesh& m) {
     Csr A;
     A.n = m.nc;
-    A.ptr.assign(m.nc + 1, 0);
-    std::vector<std::vector<std::pair<Index, Index>>> rows(m.nc);
-    for (Index f = 0; f < m.nf; ++f) {
-        if (m.nei[f] == kNone) continue;
-        rows[m.own[f]].push_back({m.nei[f], f});
-        rows[m.nei[f]].push_back({m.own[f], f});
-    }
-    for (Index i = 0; i < m.nc; ++i) {
-        std::sort(rows[i].begin(), rows[i].end());
-        A.ptr[i + 1] = A.ptr[i] + static_cast<Index>(rows[i].size());
-        for (auto& e : rows[i]) A.col.push_back(e.first);
-    }
+    A.ptr = m.csrPtr;
+    A.col = m.csrCol;
     A.diag.assign(m.nc, 0.0);
     A.val.assign(A.col.size(), 0.0);
     return A;
@@ -37,21 +27,16 @@ Csr csrPattern(const Mesh& m) {
 void lduToCsr(const Mesh& m, const std::vector<double>& diag, const std::vector<double>& upper,
               const std::vector<double>& lower, Csr& A) {
     // Row owner holds upper[f] (column neighbour); row neighbour holds lower[f].
-    A = csrPattern(m);
-    std::vector<std::vector<std::pair<Index, Index>>> rows(m.nc);
-    for (Index f = 0; f < m.nf; ++f) {
-        if (m.nei[f] == kNone) continue;
-        rows[m.own[f]].push_back({m.nei[f], f});
-        rows[m.nei[f]].push_back({m.own[f], f});
-    }
-    for (Index i = 0; i < m.nc; ++i) {
-        std::sort(rows[i].begin(), rows[i].end());
-        for (std::size_t k = 0; k < rows[i].size(); ++k) {
-            const Index f = rows[i][k].second;
-            A.val[A.ptr[i] + k] = m.own[f] == i ? upper[f] : lower[f];
-        }
-        A.diag[i] = diag[i];
-    }
+    A.n = m.nc;
+    A.ptr = m.csrPtr;
+    A.col = m.csrCol;
+    A.diag = diag;
+    A.val.resize(m.csrCol.size());
+    const Index ne = static_cast<Index>(m.csrCol.size());
+    parallelFor(ne, [&](Index e) {
+        const Index f = m.csrFace[e];
+        A.val[e] = m.csrOwn[e] ? upper[f] : lower[f];
+    });
 }
 
 void spmv(const Csr& A, const double* x, double* y) {
@@ -165,7 +150,11 @@ void galerkin(const Csr& A, const std::vector<Index>& agg, Index nc, Csr& Ac) {
     Ac.ptr.assign(nc + 1, 0);
     Ac.col.clear(); Ac.val.clear();
     Ac.diag.assign(nc, 0.0);
-    for (Index I = 0; I < nc; ++I) {
+    // Coarse rows are independent: each is formed on its own thread, then the
+    // rows are concatenated in order.
+    std::vector<std::vector<Index>> rowCols(nc);
+    std::vector<std::vector<double>> rowVals(nc);
+    parallelFor(nc, [&](Index I) {
         std::vector<Index> cols;
         for (Index i : members[I])
             for (Index k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
@@ -183,7 +172,12 @@ void galerkin(const Csr& A, const std::vector<Index>& agg, Index nc, Csr& Ac) {
             }
         }
         Ac.diag[I] = d;
-        for (std::size_t e = 0; e < cols.size(); ++e) { Ac.col.push_back(cols[e]); Ac.val.push_back(vals[e]); }
+        rowCols[I] = std::move(cols);
+        rowVals[I] = std::move(vals);
+    });
+    for (Index I = 0; I < nc; ++I) {
+        Ac.col.insert(Ac.col.end(), rowCols[I].begin(), rowCols[I].end());
+        Ac.val.insert(Ac.val.end(), rowVals[I].begin(), rowVals[I].end());
         Ac.ptr[I + 1] = static_cast<Index>(Ac.col.size());
     }
 }
@@ -194,16 +188,24 @@ void Amg::setup(const Csr& A, const SolverConfig& c) {
     cfg = c;
     if (!aggregated) {
         levels.clear();
-        levels.push_back(AmgLevel{A, {}, 0});
+        levels.push_back(AmgLevel{A, {}, 0, {}, {}});
         while (static_cast<int>(levels.size()) < cfg.amgMaxLevels && levels.back().A.n > cfg.amgCoarseSize) {
             AmgLevel& L = levels.back();
             Index nc = 0;
             aggregate(L.A, cfg.amgStrength, L.agg, nc);
             if (nc == 0 || nc >= L.A.n) { L.agg.clear(); break; }
             L.nc = nc;
+            L.memPtr.assign(nc + 1, 0);
+            for (Index i = 0; i < L.A.n; ++i) ++L.memPtr[L.agg[i] + 1];
+            for (Index I = 0; I < nc; ++I) L.memPtr[I + 1] += L.memPtr[I];
+            L.memIdx.resize(L.A.n);
+            {
+                std::vector<Index> fill(L.memPtr.begin(), L.memPtr.end() - 1);
+                for (Index i = 0; i < L.A.n; ++i) L.memIdx[fill[L.agg[i]]++] = i;
+            }
             Csr Ac;
             galerkin(L.A, L.agg, nc, Ac);
-            levels.push_back(AmgLevel{std::move(Ac), {}, 0});
+            levels.push_back(AmgLevel{std::move(Ac), {}, 0, {}, {}});
         }
         aggregated = true;
     } else {
@@ -284,8 +286,14 @@ void Amg::vcycle(int l) {
 #pragma omp parallel for schedule(static)
     for (Index i = 0; i < n; ++i) r[i] = b[i] - r[i];
     std::vector<double>& bc = wb[l + 1];
-    std::fill(bc.begin(), bc.end(), 0.0);
-    for (Index i = 0; i < n; ++i) bc[levels[l].agg[i]] += r[i];
+    // Restriction as a gather over each aggregate's members in ascending row
+    // order: the sums of a serial scatter over the rows, term for term.
+    const AmgLevel& lv = levels[l];
+    parallelFor(lv.nc, [&](Index I) {
+        double s = 0.0;
+        for (Index k = lv.memPtr[I]; k < lv.memPtr[I + 1]; ++k) s += r[lv.memIdx[k]];
+        bc[I] = s;
+    });
     vcycle(l + 1);
     const std::vector<double>& xc = wx[l + 1];
 #pragma omp parallel for schedule(static)
@@ -342,7 +350,7 @@ void LinearSystem::setup(bool reuse) {
             amg->setup(A, cfg);
         } else if (cfg.pc == 1) {
             if (!amg) amg = new Amg();
-            amg->levels.assign(1, AmgLevel{A, {}, 0});
+            amg->levels.assign(1, AmgLevel{A, {}, 0, {}, {}});
         }
         if (cfg.pc != 0) ++pcSetups;
         pcMatrixUpdate = matrixUpdates;

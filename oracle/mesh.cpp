@@ -231,6 +231,25 @@ Mesh buildMesh(const std::vector<V3>& vertices, const std::vector<Index>& fpo,
             if (norm(m.skew[f]) > 1e-12 * std::sqrt(d2)) m.orthogonal = false;
         }
     }
+    // The sorted off-diagonal pattern, built once (linsolve.cpp fills values into it).
+    {
+        std::vector<std::vector<std::pair<Index, Index>>> rows(nc);
+        for (Index f = 0; f < m.nf; ++f) {
+            if (m.nei[f] == kNone) continue;
+            rows[m.own[f]].push_back({m.nei[f], f});
+            rows[m.nei[f]].push_back({m.own[f], f});
+        }
+        m.csrPtr.assign(nc + 1, 0);
+        for (Index i = 0; i < nc; ++i) {
+            std::sort(rows[i].begin(), rows[i].end());
+            m.csrPtr[i + 1] = m.csrPtr[i] + static_cast<Index>(rows[i].size());
+            for (auto& e : rows[i]) {
+                m.csrCol.push_back(e.first);
+                m.csrFace.push_back(e.second);
+                m.csrOwn.push_back(m.own[e.second] == i ? 1 : 0);
+            }
+        }
+    }
     return m;
 }
 
