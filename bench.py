@@ -6,8 +6,14 @@ interior vertices jittered by +-20% of the spacing and ~10% of the hexes cut
 into prism pairs (about 2.2M polyhedral cells), laminar steady SIMPLE with
 least-squares gradients, corrected diffusion, linearUpwind convection;
 momentum by BiCGSTAB+Jacobi, pressure correction by CG with the aggregation
-AMG, FP64.  One step is one SIMPLE outer iteration with every field, matrix
-and preconditioner resident in HBM.
+AMG, both at the reference cavity case's linear tolerances (relative 1e-8,
+absolute 1e-14; cases/cavity/numerics/solution.bs), FP64.  One step is one
+SIMPLE outer iteration with every field, matrix and preconditioner resident
+in HBM; every timed iteration's linear solves must converge.  The same run
+also reports the workload at relative tolerance 1e-6, the mixed-precision
+path with its field error against the FP64 run, a short serving sample
+(16 requests of 1e4-1e5 cells: mixed served, FP64 served, FP64 one at a
+time) and the FP64 oracle on the host cores (median of 3 repeats).
 
 --workload cylinder: transient laminar flow past a circular cylinder (D = 1,
 channel 30 D x 20 D, Re = 100) on a structured O-grid + H-grid hex mesh of
@@ -40,6 +46,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -110,6 +117,12 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=None,
                     help="timed oracle iterations for the CPU baseline (3; 1 for the channel)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="cpu baseline: timed repeats (median reported)")
+    ap.add_argument("--rtol", type=float, default=1e-8,
+                    help="cavity: linear relative tolerance of both solves (the reference cavity case: 1e-8)")
+    ap.add_argument("--no-legs", action="store_true", help="cavity: skip the other-tolerance and mixed legs")
+    ap.add_argument("--serve-sample", type=int, default=16,
+                    help="cavity: requests of the short serving sample (0 to skip)")
     ap.add_argument("--rebuild-interval", type=int, default=8)
     ap.add_argument("--amg-strength", type=float, default=0.65,
                     help="cavity: strength threshold of the pressure AMG aggregation")
@@ -201,29 +214,52 @@ def main():
     nv, nf_in, _, _, nc_in = mesh.sizes()
     log(f"mesh generated: {nc_in} cells, {nf_in} faces ({time.time() - t0:.1f}s)")
     prec = 1 if args.precision == "mixed" else 0
-    cfg = cases.c2_config(
-        mom=solver_config(solver=1, pc=1, rtol=1e-6, atol=1e-14, precision=prec,
-                          rebuild_interval=args.rebuild_interval, warm_start=int(args.warm_start)),
-        pre=solver_config(solver=0, pc=2, rtol=1e-6, atol=1e-14, precision=prec,
-                          rebuild_interval=args.rebuild_interval, warm_start=int(args.warm_start),
-                          amg_strength=args.amg_strength,
-                          amg_correction=args.amg_correction, amg_omega=args.amg_omega))
+
+    def config(precision, rtol):
+        # cases/cavity/numerics/solution.bs:6-16: momentum bcgs and pressure cg
+        # at relativeTolerance 1e-8, absoluteTolerance 1e-14.
+        common = dict(atol=1e-14, precision=precision, rebuild_interval=args.rebuild_interval,
+                      warm_start=int(args.warm_start))
+        return cases.c2_config(
+            mom=solver_config(solver=1, pc=1, rtol=rtol, **common),
+            pre=solver_config(solver=0, pc=2, rtol=rtol, amg_strength=args.amg_strength,
+                              amg_correction=args.amg_correction, amg_omega=args.amg_omega, **common))
+
+    cfg = config(prec, args.rtol)
     ub, pb = cases.cavity_bcs(two_d=False)
     ctx = Context(mesh, renumber=True)
     log(f"context ready: N={ctx.nc} F={ctx.nf} nnz_off={ctx.nnz_off} orthogonal={ctx.orthogonal} "
         f"({time.time() - t0:.1f}s)")
-    sim = Simple(ctx, cfg, ub, pb)
-    sim.start()
-    reps = []
-    for k in range(args.warmup):
-        reps += sim.iterate(1)
-        r = reps[-1]
-        log(f"warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters}")
-    before = sim.timing()
-    t1 = time.time()
-    timed = sim.iterate(args.steps)
-    t2 = time.time()
-    after = sim.timing()
+
+    def run_leg(c, name):
+        """warmup + steps outer iterations from rest; every timed iteration's
+        linear solves must have converged."""
+        sim = Simple(ctx, c, ub, pb)
+        sim.start()
+        for k in range(args.warmup):
+            r = sim.iterate(1)[-1]
+            log(f"{name} warmup {k}: rU={r.rU:.3e} mom_its={list(r.mom_iters)} p_its={r.p_iters}")
+        before = sim.timing()
+        t1 = time.time()
+        timed = sim.iterate(args.steps)
+        t2 = time.time()
+        after = sim.timing()
+        bad = [k for k, r in enumerate(timed) if r.numerical_failure or r.mom_status != 0 or r.p_status != 0]
+        if bad:
+            r = timed[bad[0]]
+            raise RuntimeError(f"{name}: timed iteration {bad[0]} has unconverged linear solves "
+                               f"(mom_status {r.mom_status}, p_status {r.p_status}, failure {r.numerical_failure})")
+        sec = t2 - t1
+        last = timed[-1]
+        log(f"{name}: {args.steps} its in {sec:.3f}s -> {args.steps / sec:.3f} it/s; last rU={last.rU:.3e} "
+            f"mom_its={list(last.mom_iters)} p_its={last.p_iters} mass={last.mass:.2e}")
+        timing = {k: (after[k] - before[k]) / args.steps for k in after if k != "iterations"}
+        return sim, sec, timed, timing
+
+    sim, sec, timed, timing = run_leg(cfg, f"rtol {args.rtol:g}")
+    fields_main = sim.fields()
+    its_per_s = args.steps / sec
+    last = timed[-1]
     # The timed region replays each AMG V-cycle from a captured graph, where
     # events cannot bracket a single kernel; the sweep is timed with HIP
     # events around every launch in the iterations that follow, which launch
@@ -231,15 +267,8 @@ def main():
     sim.probe_start(capacity=512 * args.probe_steps)
     sim.iterate(args.probe_steps)
     launches, sweep_s, sweep_bytes = sim.probe_stop()
-    sec = t2 - t1
-    its_per_s = args.steps / sec
-    last = timed[-1]
-    log(f"timed {args.steps} its in {sec:.3f}s -> {its_per_s:.3f} it/s; last rU={last.rU:.3e} "
-        f"mom_its={list(last.mom_iters)} p_its={last.p_iters} mass={last.mass:.2e}")
-    timing = {k: (after[k] - before[k]) / args.steps for k in after if k != "iterations"}
     used_gb = device_memory_used_gb()
-    # Dominant kernel: the fine-level smoother sweep of the pressure AMG,
-    # timed with HIP events around every launch inside the timed region.
+    # Dominant kernel: the fine-level smoother sweep of the pressure AMG.
     sweep_avg = sweep_s / max(launches, 1)
     achieved = sweep_bytes / sweep_avg / 1e9 if launches else 0.0
     log(f"fine-level sweep: {launches} launches, {sweep_avg * 1e6:.1f} us each, {achieved:.0f} GB/s")
@@ -252,9 +281,37 @@ def main():
     spmv_bytes = 12 * nnz_off + 24 * N
     log(f"spmv: {spmv_s * 1e6:.1f} us per launch from HBM, {spmv_bytes / spmv_s / 1e9:.0f} GB/s; "
         f"back to back {spmv_b2b_s * 1e6:.1f} us, {spmv_bytes / spmv_b2b_s / 1e9:.0f} GB/s")
+    perf_total, perf_systems = sim.performance()
+    del sim
+    legs = {}
+    if not args.no_legs:
+        # The same workload at the looser linear tolerance of the earlier
+        # measurements, and in mixed precision (FP32 AMG and FP32 BiCGSTAB
+        # inside FP64 refinement) with each field's error against the FP64
+        # run after the same iterations from rest.
+        other = 1e-6 if args.rtol != 1e-6 else 1e-8
+        s2, sec2, t2_, _ = run_leg(config(prec, other), f"rtol {other:g}")
+        legs[f"rtol_{other:g}"] = {"value": args.steps / sec2, "unit": "outer_iterations/s",
+                                   "p_its_last": t2_[-1].p_iters, "mom_its_last": list(t2_[-1].mom_iters)}
+        del s2
+        if prec == 0:
+            s3, sec3, t3_, _ = run_leg(config(1, args.rtol), "mixed")
+            Um, pm, phim = s3.fields()
+            Uf, pf, phif = fields_main
+            rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
+            legs["mixed"] = {"value": args.steps / sec3, "unit": "outer_iterations/s",
+                             "speedup_over_fp64": (args.steps / sec3) / its_per_s,
+                             "p_its_last": t3_[-1].p_iters, "mom_its_last": list(t3_[-1].mom_iters),
+                             "field_rel_l2_vs_fp64": {"U": rel(Um, Uf), "p": rel(pm, pf), "phi": rel(phim, phif)},
+                             "compared": f"after {args.warmup + args.steps} outer iterations from rest"}
+            log(f"mixed: {legs['mixed']['value']:.2f} it/s, field rel L2 vs FP64 {legs['mixed']['field_rel_l2_vs_fp64']}")
+            del s3
     traffic = measured_traffic(ctx.nc)
+    serve = None
+    if args.serve_sample > 0:
+        serve = serve_sample(args, log)
     cpu = None
-    cpu_iters = 3 if args.cpu_iters is None else args.cpu_iters
+    cpu_iters = 1 if args.cpu_iters is None else args.cpu_iters
     if not args.no_cpu and cpu_iters > 0:
         import oracle_lib as orc
         arrays = mesh.arrays()
@@ -264,14 +321,21 @@ def main():
         so.start()
         so.iterate(1)  # the first outer iteration is outside the timing window (runtime.cpp:117-130)
         t4 = time.time()
-        so.iterate(cpu_iters)
-        t5 = time.time()
+        rates = []
+        for _ in range(args.cpu_repeats):
+            ta = time.time()
+            so.iterate(cpu_iters)
+            rates.append(cpu_iters / (time.time() - ta))
         threads = int(orc.oracle().orc_threads())
-        cpu = {"value": cpu_iters / (t5 - t4), "unit": "outer_iterations/s", "cores": threads, "kind": "port",
-               "sample": f"{cpu_iters} timed FP64 oracle SIMPLE outer iteration(s) on the same "
-                         f"{nc_in}-cell mesh after one untimed iteration (setup {t4 - t3:.1f}s excluded); "
-                         f"OpenMP over {threads} threads, the assembly face loops serial"}
-        log(f"cpu baseline: {cpu['value']:.4f} it/s")
+        rates_sorted = sorted(rates)
+        cpu = {"value": float(np.median(rates)), "unit": "outer_iterations/s", "cores": threads, "kind": "port",
+               "min": rates_sorted[0], "max": rates_sorted[-1], "repeats": len(rates),
+               "host": host_cpu(),
+               "sample": f"{len(rates)} repeats of {cpu_iters} FP64 oracle SIMPLE outer iteration(s) on the same "
+                         f"{nc_in}-cell mesh at the same linear tolerances, after one untimed iteration (setup "
+                         f"{t4 - t3:.1f}s excluded); median reported; OpenMP over {threads} threads, every face "
+                         "and row loop parallel"}
+        log(f"cpu baseline: median {cpu['value']:.4f} it/s (min {cpu['min']:.4f}, max {cpu['max']:.4f})")
     line = {
         "metric": "SIMPLE outer iterations/s at 2M polyhedral cells (SpMV HBM GB/s in roofline)",
         "value": its_per_s,
@@ -287,11 +351,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3-D lid-driven cavity Re=400, {n}^3 hex lattice, +-20% jitter, ~10% prism splits",
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
+                   "linear_rtol": args.rtol, "linear_atol": 1e-14,
                    "pressure": f"CG + aggregation AMG (strength {args.amg_strength}, correction "
                                f"{args.amg_correction}, Jacobi weight {args.amg_omega})",
                    "momentum": "BiCGSTAB + Jacobi", "warm_start": args.warm_start,
+                   "timed_linear_solves": "all converged",
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
-                   "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb},
+                   "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb,
+                   "performance": {k: perf_total[k] for k in ("linear_solves", "krylov_iterations",
+                                                               "preconditioner_setups", "preconditioner_reuses",
+                                                               "true_residual_checks", "matrix_value_updates")},
+                   "legs": legs, "serve_sample": serve},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "kJacobiSweep (pressure AMG fine-level smoother sweep, sliced rows)",
@@ -307,6 +377,44 @@ def main():
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def host_cpu() -> dict:
+    """Model, sockets and cores of the host, and the CPUs this process may use."""
+    info = {"usable_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[key] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def serve_sample(args, log):
+    """A short serving stream in the default run: mixed precision served,
+    FP64 served, and the plain FP64 GPU path (one request at a time)."""
+    import serve_cases as sc
+    specs = sc.stream(count=args.serve_sample, rate=8.0, cells_min=1e4, cells_max=1e5)
+    meshes = {}
+    for sp in specs:
+        key = (sp.kind, sp.side)
+        if key not in meshes:
+            meshes[key] = sc.mesh_for(sp)
+    out = {"requests": len(specs), "cells": [int(min(sp.cells for sp in specs)), int(max(sp.cells for sp in specs))],
+           "arrival_rate": 8.0, "outer_tolerance": 1e-5}
+    plain = argparse.Namespace(**vars(args))
+    plain.slots = 1
+    for name, prec, a in (("mixed", 1, args), ("fp64", 0, args), ("fp64_plain_one_at_a_time", 0, plain)):
+        r = run_stream(a, specs, meshes, prec, log)
+        out[name] = {k: r[k] for k in ("solves_per_s", "converged", "requests", "makespan_s", "p50_s", "p99_s")}
+        log(f"serve sample {name}: {r['solves_per_s']:.3f} solves/s, p50 {r['p50_s']:.2f}s, p99 {r['p99_s']:.2f}s")
+    out["mixed_over_fp64_served"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["mixed_over_fp64_plain"] = out["mixed"]["solves_per_s"] / out["fp64_plain_one_at_a_time"]["solves_per_s"]
+    return out
 
 
 def bench_cylinder(args):
