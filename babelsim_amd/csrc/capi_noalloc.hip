@@ -51,7 +51,7 @@ double* scratchAddress(int device) {
 // Row pointers non-decreasing within [0, nnz] and every column in [0, n):
 // what bsim_csr_solve checks on the host before uploading.  Only entries
 // inside a row range that passed its own check are read.
-__global__ void kCheckCsr(int n, int nnz, const int* ptr, const int* col, int* bad) {
+__global__ void __launch_bounds__(kBlock) kCheckCsr(int n, int nnz, const int* ptr, const int* col, int* bad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int a = ptr[i], b = ptr[i + 1];

@@ -157,17 +157,47 @@ __device__ inline void st3(double* p, long long i, D3 v) { p[3 * i] = v.x; p[3 *
 // the partials in block order with a fixed tree and hands them to a functor.
 // The kernel boundary orders the two stages, so no device-scope fence or
 // completion counter is needed and the result does not depend on scheduling.
-template <int K, int NT = kBlock> __device__ inline void blockSum(double (&v)[K], double* smem /* K*NT */) {
-    const int t = threadIdx.x;
-    for (int k = 0; k < K; ++k) smem[k * NT + t] = v[k];
+// Wavefront stage: an xor butterfly over the 64 lanes.  Both lanes of every
+// exchange add the same two operands, so all lanes end with one bitwise-equal
+// total and the order of additions is fixed.
+template <class Op> __device__ inline double waveReduce(double v, Op op) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    return v;
+}
+struct SumOp {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct MaxOp {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+// Block stage: the per-wavefront results go through LDS (one slot per
+// wavefront) and every thread combines them in wavefront order.
+template <int K, int NT, class Op> __device__ inline void blockReduce(double (&v)[K], double* smem, Op op) {
+    static_assert(NT % 64 == 0, "blocks of whole wavefronts");
+    constexpr int NW = NT / 64;
+    for (int k = 0; k < K; ++k) v[k] = waveReduce(v[k], op);
+    if (NW == 1) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) smem[k * NW + w] = v[k];
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int k = 0; k < K; ++k) smem[k * NT + t] += smem[k * NT + t + s];
-        __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+        double a = smem[k * NW];
+        for (int j = 1; j < NW; ++j) a = op(a, smem[k * NW + j]);
+        v[k] = a;
     }
-    for (int k = 0; k < K; ++k) v[k] = smem[k * NT];
     __syncthreads();
+}
+// The K sums of the block, in every thread (smem: at least K * NT / 64 doubles).
+template <int K, int NT = kBlock> __device__ inline void blockSum(double (&v)[K], double* smem) {
+    blockReduce<K, NT>(v, smem, SumOp{});
+}
+// The maximum over the block, in every thread.
+template <int NT = kBlock> __device__ inline double blockMaxValue(double v, double* smem) {
+    double a[1] = {v};
+    blockReduce<1, NT>(a, smem, MaxOp{});
+    return a[0];
 }
 
 // Partials are stored under the logical block index, so the final sum runs
@@ -184,11 +214,18 @@ template <int K> __device__ inline void blockPartial(double (&v)[K], double* par
 // ... in order, then a fixed tree.  F: device functor called by thread 0
 // with the K totals.
 template <int K, class F> __global__ void __launch_bounds__(kTail) kReduceTail(int nb, const double* partials, F f) {
-    __shared__ double smem[K * kTail];
+    __shared__ double smem[K * kTail / 64];
     double acc[K];
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kTail)
-        for (int k = 0; k < K; ++k) acc[k] += partials[k * kMaxReduceBlocks + b];
+    // One sum at a time with a short unroll keeps 1024 threads within 128
+    // VGPRs each for every K.
+    for (int k = 0; k < K; ++k) {
+        const double* p = partials + static_cast<long long>(k) * kMaxReduceBlocks;
+        double a = 0.0;
+#pragma unroll 4
+        for (int b = threadIdx.x; b < nb; b += kTail) a += p[b];
+        acc[k] = a;
+    }
     blockSum<K, kTail>(acc, smem);
     if (threadIdx.x == 0) f(acc);
 }

@@ -95,7 +95,7 @@ __device__ D3 areaOf(const D3* p, int n, const unsigned char* tri) {
     return a;
 }
 
-__global__ void kFacePoints(int nf, const int* fpo, const int* fpi, const double* vx, double* psum, int* pcnt) {
+__global__ void __launch_bounds__(kBlock) kFacePoints(int nf, const int* fpo, const int* fpi, const double* vx, double* psum, int* pcnt) {
     const int f = globalThread();
     if (f >= nf) return;
     D3 s = d3(0, 0, 0);
@@ -104,7 +104,7 @@ __global__ void kFacePoints(int nf, const int* fpo, const int* fpi, const double
     pcnt[f] = fpo[f + 1] - fpo[f];
 }
 
-__global__ void kCellReference(MeshView m, const double* psum, const int* pcnt, double* ref) {
+__global__ void __launch_bounds__(kBlock) kCellReference(MeshView m, const double* psum, const int* pcnt, double* ref) {
     const int c = globalThread();
     if (c >= m.nc) return;
     D3 s = d3(0, 0, 0);
@@ -121,7 +121,7 @@ __global__ void kCellReference(MeshView m, const double* psum, const int* pcnt, 
 }
 
 // Orientation from owner toward exterior, area vector, area-weighted centre.
-__global__ void kFaceGeometry(int nf, const int* fpo, int* fpi, const double* vx, const int* own, const double* psum,
+__global__ void __launch_bounds__(kBlock) kFaceGeometry(int nf, const int* fpo, int* fpi, const double* vx, const int* own, const double* psum,
                               const double* ref, double* Cf, double* Sf, double* Af, double* nrm, int* err) {
     const int f = globalThread();
     if (f >= nf) return;
@@ -178,7 +178,7 @@ __device__ void cellFace(MeshView m, const int* fpo, const int* fpi, const doubl
     atot += m.Af[f];
 }
 
-__global__ void kCellGeometry(MeshView m, const int* fpo, const int* fpi, const double* vx, const double* ref,
+__global__ void __launch_bounds__(kBlock) kCellGeometry(MeshView m, const int* fpo, const int* fpi, const double* vx, const double* ref,
                               double* C, double* V, double* invV, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
@@ -198,7 +198,7 @@ __global__ void kCellGeometry(MeshView m, const int* fpo, const int* fpi, const 
     st3(C, c, mom / vol);
 }
 
-__global__ void kFaceFactors(MeshView m, double* w, double* orth, double* kf, double* skew, int* nonOrth, int* err) {
+__global__ void __launch_bounds__(kBlock) kFaceFactors(MeshView m, double* w, double* orth, double* kf, double* skew, int* nonOrth, int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
     const int o = m.own[f], n = m.nei[f];

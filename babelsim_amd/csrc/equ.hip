@@ -16,16 +16,16 @@ using namespace flow;
 
 enum EquFlag : int { kFlagCapacity = 1, kFlagNonFinite = 2, kFlagDiagonal = 4 };
 
-__global__ void kAccumulate(long long n, double f, const double* b, double* a) {  // a += f*b
+__global__ void __launch_bounds__(kBlock) kAccumulate(long long n, double f, const double* b, double* a) {  // a += f*b
     const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
     if (i < n) a[i] += f * b[i];
 }
-__global__ void kScaleArr(long long n, double f, double* x) {
+__global__ void __launch_bounds__(kBlock) kScaleArr(long long n, double f, double* x) {
     const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
     if (i < n) x[i] = f * x[i];
 }
 // reaction (procedural_equation.cpp:194-204): c*V, or factor*c_i*V with a finite c_i.
-__global__ void kReaction(MeshView m, double* diag, const double* c, double value, double factor, int* flag) {
+__global__ void __launch_bounds__(kBlock) kReaction(MeshView m, double* diag, const double* c, double value, double factor, int* flag) {
     const int i = globalThread();
     if (i >= m.nc) return;
     if (c) {
@@ -36,7 +36,7 @@ __global__ void kReaction(MeshView m, double* diag, const double* c, double valu
     }
 }
 // source (procedural_equation.cpp:205-213): V*value, or factor*V*value_i.
-__global__ void kSource(MeshView m, int nk, double* src, const double* field, double c0, double c1, double c2,
+__global__ void __launch_bounds__(kBlock) kSource(MeshView m, int nk, double* src, const double* field, double c0, double c1, double c2,
                         double factor) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -48,7 +48,7 @@ __global__ void kSource(MeshView m, int nk, double* src, const double* field, do
     }
 }
 // timeTerm (procedural_equation.cpp:214-240).
-__global__ void kTime(MeshView m, int nk, double* diag, double* src, const double* capField, double cap,
+__global__ void __launch_bounds__(kBlock) kTime(MeshView m, int nk, double* diag, double* src, const double* capField, double cap,
                       const double* prev, const double* older, double a0, double a1, double a2, int* flag) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -63,7 +63,7 @@ __global__ void kTime(MeshView m, int nk, double* diag, double* src, const doubl
     }
 }
 // relax (procedural_equation.cpp:317-324).
-__global__ void kRelax(int n, int nk, double* diag, double* src, const double* prev, double alpha) {
+__global__ void __launch_bounds__(kBlock) kRelax(int n, int nk, double* diag, double* src, const double* prev, double alpha) {
     const int i = globalThread();
     if (i >= n) return;
     const double extra = diag[i] * (1 / alpha - 1);
@@ -75,7 +75,7 @@ __global__ void kRelax(int n, int nk, double* diag, double* src, const double* p
 }
 // reference(cell, value) (procedural_equation.cpp:325-334); skipped when the
 // row-sum test found a row that is not a constant null mode.
-__global__ void kRefApply(double* diag, double* src, int cell, double value, const int* notConstant, int* flag) {
+__global__ void __launch_bounds__(kBlock) kRefApply(double* diag, double* src, int cell, double value, const int* notConstant, int* flag) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (notConstant && notConstant[0]) return;
     if (!(diag[cell] > 0)) {
@@ -87,7 +87,7 @@ __global__ void kRefApply(double* diag, double* src, int cell, double value, con
 }
 // A x (and b - A x): the diagonal first, then the off-diagonal entries of the
 // row in column order, per component.
-__global__ void kApplyRows(MeshView m, int nk, const double* diag, const double* off, const double* x, const double* b,
+__global__ void __launch_bounds__(kBlock) kApplyRows(MeshView m, int nk, const double* diag, const double* off, const double* x, const double* b,
                            double* y) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -104,9 +104,9 @@ __global__ void kApplyRows(MeshView m, int nk, const double* diag, const double*
     }
 }
 // Partial sums of |b - A x|^2, |A x|^2, |b|^2 (relativeResidual, procedural_equation.cpp:447-452).
-__global__ void kRelResidual(MeshView m, int nk, const double* diag, const double* off, const double* src,
+__global__ void __launch_bounds__(kBlock) kRelResidual(MeshView m, int nk, const double* diag, const double* off, const double* src,
                              const double* x, double* part) {
-    __shared__ double smem[3 * kBlock];
+    __shared__ double smem[3 * kBlock / 64];
     double a[3] = {0, 0, 0};
     for (int i = globalThread(); i < m.nc; i += gridDim.x * blockDim.x) {
         double ax[3];
@@ -128,7 +128,7 @@ __global__ void kRelResidual(MeshView m, int nk, const double* diag, const doubl
 // One diffusion snapshot's face flux (procedural_equation.cpp:352-370): the
 // orthogonal flux of the solution with the snapshot's boundary conditions and
 // frozen coefficient, plus the frozen correction, times the factor.
-__global__ void kFaceFluxAdd(MeshView m, FieldArg sol, const double* kf, const double* corr, double factor,
+__global__ void __launch_bounds__(kBlock) kFaceFluxAdd(MeshView m, FieldArg sol, const double* kf, const double* corr, double factor,
                              double* out, int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
@@ -145,8 +145,8 @@ struct NormArgs {
     const double* v[4];
     long long n[4];
 };
-__global__ void kNorms(NormArgs a, int count, long long nmax, double* part) {
-    __shared__ double smem[4 * kBlock];
+__global__ void __launch_bounds__(kBlock) kNorms(NormArgs a, int count, long long nmax, double* part) {
+    __shared__ double smem[4 * kBlock / 64];
     double s[4] = {0, 0, 0, 0};
     for (long long i = globalThread(); i < nmax; i += static_cast<long long>(gridDim.x) * blockDim.x)
         for (int k = 0; k < count; ++k)
@@ -155,8 +155,8 @@ __global__ void kNorms(NormArgs a, int count, long long nmax, double* part) {
 }
 // relativeChange / relativeMagnitude partial sums (fvm_execution.cpp:91-146):
 // change: |a-b|^2 and |a|^2 per cell; magnitude: |a|^2 and |b|^2.
-__global__ void kPairSums(int nc, int nk, const double* a, const double* b, int change, double* part) {
-    __shared__ double smem[2 * kBlock];
+__global__ void __launch_bounds__(kBlock) kPairSums(int nc, int nk, const double* a, const double* b, int change, double* part) {
+    __shared__ double smem[2 * kBlock / 64];
     double s[2] = {0, 0};
     for (int i = globalThread(); i < nc; i += gridDim.x * blockDim.x) {
         double d2 = 0, a2 = 0, b2 = 0;
@@ -174,32 +174,21 @@ __global__ void kPairSums(int nc, int nk, const double* a, const double* b, int 
 }
 // Block maximum of one value per thread, stored under the logical block index.
 __device__ inline void blockMax(double v, double* partial, double* smem) {
-    const int t = threadIdx.x;
-    smem[t] = v;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (t < s) smem[t] = fmax(smem[t], smem[t + s]);
-        __syncthreads();
-    }
-    if (t == 0) partial[xcdBlock()] = smem[0];
+    const double m = blockMaxValue(v, smem);
+    if (threadIdx.x == 0) partial[xcdBlock()] = m;
 }
 __global__ void __launch_bounds__(kTail) kMaxTail(int nb, const double* partial, double* out) {
-    __shared__ double smem[kTail];
+    __shared__ double smem[kTail / 64];
     double v = 0.0;
     for (int b = threadIdx.x; b < nb; b += kTail) v = fmax(v, partial[b]);
-    smem[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kTail / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) smem[threadIdx.x] = fmax(smem[threadIdx.x], smem[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = smem[0];
+    v = blockMaxValue<kTail>(v, smem);
+    if (threadIdx.x == 0) out[0] = v;
 }
 // fluxBalance (fvm_execution.cpp:148-175): per cell the outward flux sum;
 // partial sums of |imbalance|, imbalance^2 and |outward| and the block maximum
 // of |imbalance|.
-__global__ void kFluxBalance(MeshView m, const double* phi, double* part, double* maxPart) {
-    __shared__ double smem[3 * kBlock];
+__global__ void __launch_bounds__(kBlock) kFluxBalance(MeshView m, const double* phi, double* part, double* maxPart) {
+    __shared__ double smem[3 * kBlock / 64];
     double s[3] = {0, 0, 0}, mx = 0.0;
     for (int c = globalThread(); c < m.nc; c += gridDim.x * blockDim.x) {
         double imb = 0.0, sc = 0.0;
@@ -225,8 +214,8 @@ __global__ void kFluxBalance(MeshView m, const double* phi, double* part, double
 }
 // courantNumber (fvm_execution.cpp:177-199): co = 0.5 dt sum|phi_f| / V per
 // cell; partial sums of co V and V and the block maximum of co.
-__global__ void kCourant(MeshView m, const double* phi, double dt, double* part, double* maxPart) {
-    __shared__ double smem[2 * kBlock];
+__global__ void __launch_bounds__(kBlock) kCourant(MeshView m, const double* phi, double dt, double* part, double* maxPart) {
+    __shared__ double smem[2 * kBlock / 64];
     double s[2] = {0, 0}, mx = 0.0;
     for (int c = globalThread(); c < m.nc; c += gridDim.x * blockDim.x) {
         double sum = 0.0;
@@ -244,7 +233,7 @@ __global__ void kCourant(MeshView m, const double* phi, double dt, double* part,
     blockPartial<2>(s, part, smem);
     blockMax(mx, maxPart, smem);
 }
-__global__ void kFaceAdd(int n, double factor, const double* inc, double* target) {
+__global__ void __launch_bounds__(kBlock) kFaceAdd(int n, double factor, const double* inc, double* target) {
     const int f = globalThread();
     if (f < n) target[f] += factor * inc[f];
 }

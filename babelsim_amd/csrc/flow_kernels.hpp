@@ -50,7 +50,7 @@ __global__ void kMomentumPressureSource(MeshView m, const double* gradP, double*
 // (procedural_equation.cpp:429-433) on the unrelaxed momentum rows.
 __global__ void kMomentumResidual(MeshView m, const double* diag, const double* off, const double* src, const double* U,
                                   double* part) {
-    __shared__ double smem[3 * kBlock];
+    __shared__ double smem[3 * kBlock / 64];
     double a[3] = {0, 0, 0};
     for (int i = globalThread(); i < m.nc; i += gridDim.x * blockDim.x) {
         double ax[3];

@@ -11,7 +11,7 @@ using namespace dev;
 
 // ---------------------------------------------------------------- gradients
 template <int NC, bool PASS2>
-__global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, int* err) {
+__global__ void __launch_bounds__(kBlock) kGradGG(MeshView m, FieldArg x, const double* g0, double* out, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
     D3 acc[NC];
@@ -41,7 +41,7 @@ __global__ void kGradGG(MeshView m, FieldArg x, const double* g0, double* out, i
 }
 
 
-template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, int* err) {
+template <int NC> __global__ void __launch_bounds__(kBlock) kGradLS(MeshView m, FieldArg x, double* out, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
     double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -104,7 +104,7 @@ template <int NC> __global__ void kGradLS(MeshView m, FieldArg x, double* out, i
 }
 
 // -------------------------------------------------------- face quantities
-template <int NC> __global__ void kInterp(MeshView m, FieldArg x, const double* g, double* out, int* err) {
+template <int NC> __global__ void __launch_bounds__(kBlock) kInterp(MeshView m, FieldArg x, const double* g, double* out, int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
     double v[NC];
@@ -118,13 +118,13 @@ template <int NC> __global__ void kInterp(MeshView m, FieldArg x, const double* 
     for (int k = 0; k < NC; ++k) out[static_cast<long long>(NC) * f + k] = v[k];
 }
 
-__global__ void kSeedCtx(MeshView m, const double* U, double* ctx) {
+__global__ void __launch_bounds__(kBlock) kSeedCtx(MeshView m, const double* U, double* ctx) {
     const int f = globalThread();
     if (f >= m.nf) return;
     ctx[f] = f < m.nfi ? 0.0 : dot(ld3(U, m.own[f]), ld3(m.Sf, f));
 }
 
-__global__ void kFlux(MeshView m, FieldArg U, const double* g, double* out, int* err) {
+__global__ void __launch_bounds__(kBlock) kFlux(MeshView m, FieldArg U, const double* g, double* out, int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
     double v[3];
@@ -138,7 +138,7 @@ __global__ void kFlux(MeshView m, FieldArg U, const double* g, double* out, int*
     out[f] = dot(d3(v[0], v[1], v[2]), ld3(m.Sf, f));
 }
 
-__global__ void kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double* out, int* err) {
+__global__ void __launch_bounds__(kBlock) kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double* out, int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
     double v[1];
@@ -146,7 +146,7 @@ __global__ void kSnGrad(MeshView m, FieldArg x, const double* g, int dm, double*
     out[f] = v[0] / m.Af[f];
 }
 
-__global__ void kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double* out,
+__global__ void __launch_bounds__(kBlock) kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double* out,
                                int* err) {
     const int f = globalThread();
     if (f >= m.nf) return;
@@ -157,7 +157,7 @@ __global__ void kDiffusionFlux(MeshView m, FieldArg x, const double* kf, const d
     out[f] = k * v[0];
 }
 
-__global__ void kDivFace(MeshView m, const double* phi, double* out) {
+__global__ void __launch_bounds__(kBlock) kDivFace(MeshView m, const double* phi, double* out) {
     const int c = globalThread();
     if (c >= m.nc) return;
     const double iv = m.invV[c];
@@ -171,7 +171,7 @@ __global__ void kDivFace(MeshView m, const double* phi, double* out) {
     out[c] = acc;
 }
 
-__global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, double kc, const double* g, int dm,
+__global__ void __launch_bounds__(kBlock) kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, double kc, const double* g, int dm,
                                    double* out, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
@@ -204,7 +204,7 @@ __global__ void kExplicitLaplacian(MeshView m, FieldArg x, const double* kf, dou
 }
 
 template <int NC>
-__global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
+__global__ void __launch_bounds__(kBlock) kExplicitConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
                                     double* out, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
@@ -253,7 +253,7 @@ __global__ void kExplicitConvection(MeshView m, FieldArg x, const double* phi, i
 
 // ---------------------------------------------------------------- assembly
 template <int NC>
-__global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
+__global__ void __launch_bounds__(kBlock) kAssembleConvection(MeshView m, FieldArg x, const double* phi, int conv, int interp, const double* g,
                                     double scale, double* diag, double* off, double* src, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
@@ -324,7 +324,7 @@ __global__ void kAssembleConvection(MeshView m, FieldArg x, const double* phi, i
 }
 
 template <int NC>
-__global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double factor,
+__global__ void __launch_bounds__(kBlock) kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, const double* g, int dm, double factor,
                                    double* diag, double* off, double* src, int* err) {
     const int c = globalThread();
     if (c >= m.nc) return;
@@ -378,7 +378,7 @@ __global__ void kAssembleDiffusion(MeshView m, FieldArg x, const double* kf, con
     for (int k = 0; k < NC; ++k) src[static_cast<long long>(NC) * c + k] += factor * sAcc[k];
 }
 
-__global__ void kGradTraces(MeshView m, FieldArg x, const double* g, double* traces) {
+__global__ void __launch_bounds__(kBlock) kGradTraces(MeshView m, FieldArg x, const double* g, double* traces) {
     const int b = globalThread();
     const int f = m.nfi + b;
     if (f >= m.nf) return;

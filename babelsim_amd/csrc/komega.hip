@@ -27,7 +27,7 @@ __device__ inline double strainSquared(const double* g) {
 
 // mut = rho k / max(omega, omegaMin), muEff = mu + mut, cells and boundary
 // traces (k_omega.cpp:146-149).
-__global__ void kUpdateViscosity(MeshView m, FieldArg k, FieldArg w, double rho, double mu, double omegaMin,
+__global__ void __launch_bounds__(kBlock) kUpdateViscosity(MeshView m, FieldArg k, FieldArg w, double rho, double mu, double omegaMin,
                                  double* mut, double* muEff, double* mutT, double* muEffT, int* err) {
     const int i = globalThread();
     if (i < m.nc) {
@@ -49,7 +49,7 @@ __global__ void kUpdateViscosity(MeshView m, FieldArg k, FieldArg w, double rho,
 // Production mut 2S:S, diffusivities mu + sigma mut (with traces), the omega
 // sink rate beta rho omega and source gamma P omega / max(k, kMin)
 // (k_omega.cpp:70-77).
-__global__ void kTransportCoefficients(MeshView m, const double* gradU, const double* mut, const double* mutT,
+__global__ void __launch_bounds__(kBlock) kTransportCoefficients(MeshView m, const double* gradU, const double* mut, const double* mutT,
                                        const double* k, const double* w, double rho, double mu, double sigmaK,
                                        double sigmaW, double beta, double gamma, double kMin, double* prod,
                                        double* kd, double* wd, double* rate, double* srcW, double* kdT, double* wdT) {
@@ -70,13 +70,13 @@ __global__ void kTransportCoefficients(MeshView m, const double* gradU, const do
 }
 
 // The k sink rate beta* rho omega with the new omega (k_omega.cpp:99).
-__global__ void kRateK(int n, const double* w, double betaStarRho, double* rate) {
+__global__ void __launch_bounds__(kBlock) kRateK(int n, const double* w, double betaStarRho, double* rate) {
     const int i = globalThread();
     if (i < n) rate[i] = betaStarRho * w[i];
 }
 
 // reaction (procedural_equation.cpp:181-187) and source (192-196).
-__global__ void kReactionSource(int n, const double* V, const double* rate, const double* s, double* diag,
+__global__ void __launch_bounds__(kBlock) kReactionSource(int n, const double* V, const double* rate, const double* s, double* diag,
                                 double* b) {
     const int i = globalThread();
     if (i >= n) return;
@@ -86,9 +86,9 @@ __global__ void kReactionSource(int n, const double* V, const double* rate, cons
 
 // relativeResidual sums |b - Ax|^2, |Ax|^2, |b|^2 of a scalar equation
 // (procedural_equation.cpp:429-433).
-__global__ void kScalarResidual(MeshView m, const double* diag, const double* off, const double* b, const double* x,
+__global__ void __launch_bounds__(kBlock) kScalarResidual(MeshView m, const double* diag, const double* off, const double* b, const double* x,
                                 double* part) {
-    __shared__ double smem[3 * kBlock];
+    __shared__ double smem[3 * kBlock / 64];
     double a[3] = {0, 0, 0};
     for (int i = globalThread(); i < m.nc; i += gridDim.x * blockDim.x) {
         double ax = diag[i] * x[i];
@@ -107,7 +107,7 @@ __global__ void kScalarResidual(MeshView m, const double* diag, const double* of
 }
 
 // relax without rescaling (procedural_equation.cpp:298-305).
-__global__ void kRelaxScalar(int n, double* diag, double* b, const double* prev, double alpha) {
+__global__ void __launch_bounds__(kBlock) kRelaxScalar(int n, double* diag, double* b, const double* prev, double alpha) {
     const int i = globalThread();
     if (i >= n) return;
     const double extra = diag[i] * (1 / alpha - 1);
@@ -117,9 +117,9 @@ __global__ void kRelaxScalar(int n, double* diag, double* b, const double* prev,
 
 // omega = max(omega, omegaMin, rho k / (ratioMax mu)) and the count of cells
 // it raised (k_omega.cpp:131-144).
-__global__ void kBoundOmega(int n, double* w, const double* k, double omegaMin, double rho, double ratioMu,
+__global__ void __launch_bounds__(kBlock) kBoundOmega(int n, double* w, const double* k, double omegaMin, double rho, double ratioMu,
                             double* part) {
-    __shared__ double smem[kBlock];
+    __shared__ double smem[kBlock / 64];
     double cnt = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const double wi = w[i];
@@ -131,14 +131,14 @@ __global__ void kBoundOmega(int n, double* w, const double* k, double omegaMin, 
     blockPartial<1>(v, part, smem);
 }
 
-__global__ void kBoundK(int n, double* k, double kMin) {
+__global__ void __launch_bounds__(kBlock) kBoundK(int n, double* k, double kMin) {
     const int i = globalThread();
     if (i < n) k[i] = fmax(k[i], kMin);
 }
 
 // relativeChange sums for k and omega (fvm_execution.cpp:110-128).
-__global__ void kChange(int n, const double* k, const double* kp, const double* w, const double* wp, double* part) {
-    __shared__ double smem[4 * kBlock];
+__global__ void __launch_bounds__(kBlock) kChange(int n, const double* k, const double* kp, const double* w, const double* wp, double* part) {
+    __shared__ double smem[4 * kBlock / 64];
     double a[4] = {0, 0, 0, 0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const double dk = k[i] - kp[i], dw = w[i] - wp[i];
@@ -152,7 +152,7 @@ __global__ void kChange(int n, const double* k, const double* kp, const double* 
 
 // Vector-gradient boundary traces (fvm_execution.cpp:243-265), with the
 // symmetry reflection of include/babelsim/vector.h:107-115.
-__global__ void kVectorGradientTraces(MeshView m, FieldArg U, const double* g, double* gt, int* err) {
+__global__ void __launch_bounds__(kBlock) kVectorGradientTraces(MeshView m, FieldArg U, const double* g, double* gt, int* err) {
     const int f = m.nfi + globalThread();
     if (f >= m.nf) return;
     const int o = m.own[f];
@@ -198,7 +198,7 @@ __device__ inline void stressRows(double mu, const double* G, double* r0, double
     for (int c = 0; c < 3; ++c)
         for (int j = 0; j < 3; ++j) rows[c][j] = mu * (G[3 * j + c] - (2.0 / 3.0) * (c == j ? tr : 0.0));
 }
-__global__ void kStressRows(MeshView m, const double* muEff, const double* muEffT, const double* g,
+__global__ void __launch_bounds__(kBlock) kStressRows(MeshView m, const double* muEff, const double* muEffT, const double* g,
                             const double* gt, double* r0, double* r1, double* r2, double* t0, double* t1,
                             double* t2) {
     const long long i = globalThread();
@@ -209,7 +209,7 @@ __global__ void kStressRows(MeshView m, const double* muEff, const double* muEff
 
 // Component `comp` of div(T): the signed row fluxes of the cell over its
 // volume, one face at a time (operators.cpp:1580-1586).
-__global__ void kDivRow(MeshView m, const double* q, int comp, double* divT) {
+__global__ void __launch_bounds__(kBlock) kDivRow(MeshView m, const double* q, int comp, double* divT) {
     const int c = globalThread();
     if (c >= m.nc) return;
     const double v = m.V[c];
@@ -225,7 +225,7 @@ __global__ void kDivRow(MeshView m, const double* q, int comp, double* divT) {
 }
 
 // equ::source(momentum, div T) (procedural_equation.cpp:192-196).
-__global__ void kAddVectorSource(int n, const double* V, const double* s, double* src) {
+__global__ void __launch_bounds__(kBlock) kAddVectorSource(int n, const double* V, const double* s, double* src) {
     const int i = globalThread();
     if (i >= n) return;
     const double v = 1.0 * V[i];

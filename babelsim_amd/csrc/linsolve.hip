@@ -80,29 +80,29 @@ template <class T> __device__ inline T rowDot(const CsrView<T>& A, const T* x, i
 // Krylov kernels run on matrices whose slot q holds row q.
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
 
-__global__ void kFill(int n, double* x, double v) {
+__global__ void __launch_bounds__(kBlock) kFill(int n, double* x, double v) {
     const int i = globalThread();
     if (i < n) x[i] = v;
 }
 // Reads a[0, n) and stores nothing unless the sum is non-zero (a is zero
 // filled by its caller): a read-only pass that leaves no dirty lines.
-__global__ void kTouch(long long n, const double* a, double* sink) {
+__global__ void __launch_bounds__(kBlock) kTouch(long long n, const double* a, double* sink) {
     double acc = 0.0;
     for (long long i = globalThread(); i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) acc += a[i];
     if (acc != 0.0) sink[0] = acc;
 }
-__global__ void kCopy(int n, const double* a, double* b) {
+__global__ void __launch_bounds__(kBlock) kCopy(int n, const double* a, double* b) {
     const int i = globalThread();
     if (i < n) b[i] = a[i];
 }
-__global__ void kAxpy(int n, double a, const double* x, double* y) {  // y += a x
+__global__ void __launch_bounds__(kBlock) kAxpy(int n, double a, const double* x, double* y) {  // y += a x
     const int i = globalThread();
     if (i < n) y[i] += a * x[i];
 }
 
 // Sum of squares of b (optionally copied to r); partial sums per block.
-__global__ void kCopyNorm(int n, const double* b, double* r, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kCopyNorm(int n, const double* b, double* r, double* part) {
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
         const double v = b[i];
@@ -114,8 +114,8 @@ __global__ void kCopyNorm(int n, const double* b, double* r, double* part) {
 }
 
 // r = b - A x and the partial sums of its squares.
-__global__ void kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kResidualNorm(CsrView<double> A, const double* b, const double* x, double* r, double* part) {
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double v = b[i] - rowDot64(A, x, i);
@@ -126,8 +126,8 @@ __global__ void kResidualNorm(CsrView<double> A, const double* b, const double* 
     blockPartial<1>(v, part, smem);
 }
 
-__global__ void kDot(int n, const double* a, const double* b, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kDot(int n, const double* a, const double* b, double* part) {
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
     double v[1] = {acc};
@@ -163,8 +163,8 @@ struct CgBeta {  // beta = z . r
 };
 
 // z = r * (1/d) and the partial sums of z . r
-__global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z, const int* fl, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kJacobiDot(int n, const double* r, const double* jinv, double* z, const int* fl, double* part) {
+    __shared__ double smem[kBlock / 64];
     if (fl[0]) return;
     double acc = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
@@ -175,8 +175,8 @@ __global__ void kJacobiDot(int n, const double* r, const double* jinv, double* z
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
-__global__ void kPlainDot(int n, const double* z, const double* r, const int* fl, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kPlainDot(int n, const double* z, const double* r, const int* fl, double* part) {
+    __shared__ double smem[kBlock / 64];
     if (fl[0]) return;
     double acc = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += z[i] * r[i];
@@ -184,7 +184,7 @@ __global__ void kPlainDot(int n, const double* z, const double* r, const int* fl
     blockPartial<1>(v, part, smem);
 }
 
-__global__ void kCgDirection(int n, const double* z, double* p, const double* sc, const int* fl, int first) {
+__global__ void __launch_bounds__(kBlock) kCgDirection(int n, const double* z, double* p, const double* sc, const int* fl, int first) {
     if (fl[0]) return;
     const int i = globalThread();
     if (i >= n) return;
@@ -196,8 +196,8 @@ __global__ void kCgDirection(int n, const double* z, double* p, const double* sc
 }
 
 // w = A p and the partial sums of p . w
-__global__ void kCgSpmvDot(CsrView<double> A, const double* p, double* w, const int* fl, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kCgSpmvDot(CsrView<double> A, const double* p, double* w, const int* fl, double* part) {
+    __shared__ double smem[kBlock / 64];
     if (fl[0]) return;
     double acc = 0.0;
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
@@ -224,9 +224,9 @@ struct CgAlpha {  // dpi = p . w, a = beta / dpi
 };
 
 // x += a p, r += -a w and the partial sums of r . r
-__global__ void kCgUpdate(int n, double* x, const double* p, double* r, const double* w, const double* sc, const int* fl,
+__global__ void __launch_bounds__(kBlock) kCgUpdate(int n, double* x, const double* p, double* r, const double* w, const double* sc, const int* fl,
                           double* part) {
-    __shared__ double smem[kBlock];
+    __shared__ double smem[kBlock / 64];
     if (fl[0]) return;
     const double a = sc[S_A];
     double acc = 0.0;
@@ -249,8 +249,8 @@ struct CgNorm {  // dp = ||r||
 };
 
 // ---- BiCGSTAB (bcgs.c:14-140, right preconditioning) ----
-__global__ void kBcgsRho(int n, const double* r, const double* rp, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kBcgsRho(int n, const double* r, const double* rp, double* part) {
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) acc += r[i] * rp[i];
     double v[1] = {acc};
@@ -264,15 +264,15 @@ struct BcgsRho {  // rho = r . rp, beta = (rho / rhoold) (alpha / omegaold)
         sc[S_BETA] = (rho / sc[S_RHOOLD]) * (sc[S_ALPHA] / sc[S_OMEGAOLD]);
     }
 };
-__global__ void kBcgsP(int n, double* p, const double* r, const double* v, const double* sc) {
+__global__ void __launch_bounds__(kBlock) kBcgsP(int n, double* p, const double* r, const double* v, const double* sc) {
     const int i = globalThread();
     if (i >= n) return;
     const double beta = sc[S_BETA];
     p[i] = r[i] + (-sc[S_OMEGAOLD] * beta) * v[i] + beta * p[i];
 }
 // v = A t and the partial sums of v . rp
-__global__ void kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* part) {
-    __shared__ double smem[kBlock];
+__global__ void __launch_bounds__(kBlock) kBcgsSpmvD1(CsrView<double> A, const double* t, double* v, const double* rp, double* part) {
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
         const double y = rowDot64(A, t, i);
@@ -293,16 +293,16 @@ struct BcgsAlpha {  // d1 = v . rp, alpha = rho / d1
         else sc[S_ALPHA] = sc[S_RHO] / d1;
     }
 };
-__global__ void kBcgsS(int n, double* s, const double* v, const double* r, const double* sc, const int* fl) {
+__global__ void __launch_bounds__(kBlock) kBcgsS(int n, double* s, const double* v, const double* r, const double* sc, const int* fl) {
     if (fl[0]) return;
     const int i = globalThread();
     if (i >= n) return;
     s[i] = -sc[S_ALPHA] * v[i] + r[i];
 }
 // t = A rt and the partial sums of s . t and t . t
-__global__ void kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, const double* s, const int* fl,
+__global__ void __launch_bounds__(kBlock) kBcgsSpmvD2(CsrView<double> A, const double* rt, double* t, const double* s, const int* fl,
                             double* part) {
-    __shared__ double smem[2 * kBlock];
+    __shared__ double smem[2 * kBlock / 64];
     if (fl[0]) return;
     double a1 = 0.0, a2 = 0.0;
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
@@ -326,9 +326,9 @@ struct BcgsOmega {  // omega = (s . t) / (t . t)
     }
 };
 // x = alpha p + omega s + x, r = -omega t + s and the partial sums of r . r
-__global__ void kBcgsUpdate(int n, double* x, const double* p, const double* s, double* r, const double* t,
+__global__ void __launch_bounds__(kBlock) kBcgsUpdate(int n, double* x, const double* p, const double* s, double* r, const double* t,
                             const double* sc, const int* fl, double* part) {
-    __shared__ double smem[kBlock];
+    __shared__ double smem[kBlock / 64];
     if (fl[0]) return;
     const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
     double acc = 0.0;
@@ -403,8 +403,8 @@ __device__ inline void rowDot3(const CsrView<T>& A, const CVec3p<T>& x, const bo
     }
 }
 
-template <class T> __global__ void k3CopyNorm(int n, CVec3p<T> b, Vec3p<T> r, const int* act, double* part) {
-    __shared__ double smem[3 * kBlock];
+template <class T> __global__ void __launch_bounds__(kBlock) k3CopyNorm(int n, CVec3p<T> b, Vec3p<T> r, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
@@ -416,8 +416,8 @@ template <class T> __global__ void k3CopyNorm(int n, CVec3p<T> b, Vec3p<T> r, co
     blockPartial<3>(acc, part, smem);
 }
 template <class T>
-__global__ void k3ResidualNorm(CsrView<T> A, CVec3p<T> b, CVec3p<T> x, Vec3p<T> r, const int* act, double* part) {
-    __shared__ double smem[3 * kBlock];
+__global__ void __launch_bounds__(kBlock) k3ResidualNorm(CsrView<T> A, CVec3p<T> b, CVec3p<T> x, Vec3p<T> r, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
@@ -432,41 +432,41 @@ __global__ void k3ResidualNorm(CsrView<T> A, CVec3p<T> b, CVec3p<T> x, Vec3p<T> 
     }
     blockPartial<3>(acc, part, smem);
 }
-template <class T> __global__ void k3Dot(int n, CVec3p<T> a, CVec3p<T> b, const int* act, double* part) {
-    __shared__ double smem[3 * kBlock];
+template <class T> __global__ void __launch_bounds__(kBlock) k3Dot(int n, CVec3p<T> a, CVec3p<T> b, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c)
             if (act[c]) acc[c] += static_cast<double>(a.c[c][i]) * static_cast<double>(b.c[c][i]);
     blockPartial<3>(acc, part, smem);
 }
-template <class T> __global__ void k3Copy(int n, CVec3p<T> a, Vec3p<T> b, const int* act) {
+template <class T> __global__ void __launch_bounds__(kBlock) k3Copy(int n, CVec3p<T> a, Vec3p<T> b, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c]) b.c[c][i] = a.c[c][i];
 }
-template <class T> __global__ void k3Fill(int n, Vec3p<T> a, double v, const int* act) {
+template <class T> __global__ void __launch_bounds__(kBlock) k3Fill(int n, Vec3p<T> a, double v, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c]) a.c[c][i] = static_cast<T>(v);
 }
-template <class T> __global__ void k3Axpy(int n, double a, CVec3p<T> x, Vec3p<T> y, const int* act) {  // y += a x
+template <class T> __global__ void __launch_bounds__(kBlock) k3Axpy(int n, double a, CVec3p<T> x, Vec3p<T> y, const int* act) {  // y += a x
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c]) y.c[c][i] = static_cast<T>(static_cast<double>(y.c[c][i]) + a * static_cast<double>(x.c[c][i]));
 }
 // z = r * (1/d), or a copy without a preconditioner
-template <class T> __global__ void k3Jacobi(int n, CVec3p<T> r, const T* jinv, Vec3p<T> z, const int* act) {
+template <class T> __global__ void __launch_bounds__(kBlock) k3Jacobi(int n, CVec3p<T> r, const T* jinv, Vec3p<T> z, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c])
             z.c[c][i] = jinv ? static_cast<T>(static_cast<double>(r.c[c][i]) * static_cast<double>(jinv[i])) : r.c[c][i];
 }
-template <class T> __global__ void k3BcgsP(int n, Vec3p<T> p, CVec3p<T> r, CVec3p<T> v, const double* sc3, const int* act) {
+template <class T> __global__ void __launch_bounds__(kBlock) k3BcgsP(int n, Vec3p<T> p, CVec3p<T> r, CVec3p<T> v, const double* sc3, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c) {
@@ -478,8 +478,8 @@ template <class T> __global__ void k3BcgsP(int n, Vec3p<T> p, CVec3p<T> r, CVec3
     }
 }
 template <class T>
-__global__ void k3SpmvD1(CsrView<T> A, CVec3p<T> t, Vec3p<T> v, CVec3p<T> rp, const int* act, double* part) {
-    __shared__ double smem[3 * kBlock];
+__global__ void __launch_bounds__(kBlock) k3SpmvD1(CsrView<T> A, CVec3p<T> t, Vec3p<T> v, CVec3p<T> rp, const int* act, double* part) {
+    __shared__ double smem[3 * kBlock / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     const bool m[3] = {act[0] != 0, act[1] != 0, act[2] != 0};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
@@ -495,7 +495,7 @@ __global__ void k3SpmvD1(CsrView<T> A, CVec3p<T> t, Vec3p<T> v, CVec3p<T> rp, co
     blockPartial<3>(acc, part, smem);
 }
 template <class T>
-__global__ void k3BcgsS(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const double* sc3, const int* fl3, const int* act) {
+__global__ void __launch_bounds__(kBlock) k3BcgsS(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const double* sc3, const int* fl3, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
@@ -505,9 +505,9 @@ __global__ void k3BcgsS(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const doubl
 }
 // Partials: s . t of the three components at k = 0..2, t . t at k = 3..5.
 template <class T>
-__global__ void k3SpmvD2(CsrView<T> A, CVec3p<T> rt, Vec3p<T> t, CVec3p<T> s, const int* fl3, const int* act,
+__global__ void __launch_bounds__(kBlock) k3SpmvD2(CsrView<T> A, CVec3p<T> rt, Vec3p<T> t, CVec3p<T> s, const int* fl3, const int* act,
                          double* part) {
-    __shared__ double smem[6 * kBlock];
+    __shared__ double smem[6 * kBlock / 64];
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const bool m[3] = {act[0] && !fl3[0], act[1] && !fl3[1], act[2] && !fl3[2]};
     for (int i = globalThread(); i < A.n; i += gridDim.x * blockDim.x) {
@@ -525,9 +525,9 @@ __global__ void k3SpmvD2(CsrView<T> A, CVec3p<T> rt, Vec3p<T> t, CVec3p<T> s, co
     blockPartial<6>(acc, part, smem);
 }
 template <class T>
-__global__ void k3BcgsUpdate(int n, Vec3p<T> x, CVec3p<T> p, CVec3p<T> s, Vec3p<T> r, CVec3p<T> t, const double* sc3,
+__global__ void __launch_bounds__(kBlock) k3BcgsUpdate(int n, Vec3p<T> x, CVec3p<T> p, CVec3p<T> s, Vec3p<T> r, CVec3p<T> t, const double* sc3,
                              const int* fl3, const int* act, double* part) {
-    __shared__ double smem[3 * kBlock];
+    __shared__ double smem[3 * kBlock / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
@@ -589,22 +589,22 @@ struct Tail3Norm {
     }
 };
 
-__global__ void kSpmvPlain(CsrView<double> A, const double* x, double* y) {
+__global__ void __launch_bounds__(kBlock) kSpmvPlain(CsrView<double> A, const double* x, double* y) {
     const int i = globalThread();
     if (i < A.n) y[i] = rowDot64(A, x, i);
 }
-__global__ void kJacobiApply(int n, const double* r, const double* jinv, double* z) {
+__global__ void __launch_bounds__(kBlock) kJacobiApply(int n, const double* r, const double* jinv, double* z) {
     const int i = globalThread();
     if (i < n) z[i] = r[i] * jinv[i];
 }
-__global__ void kInvDiag(int n, const double* d, double* inv) {
+__global__ void __launch_bounds__(kBlock) kInvDiag(int n, const double* d, double* inv) {
     const int i = globalThread();
     if (i < n) inv[i] = 1.0 / d[i];
 }
 // Counts entries where (a0, a1) and (b0, b1) differ, as partial sums per block.
-__global__ void kCompare(int n0, const double* a0, const double* b0, int n1, const double* a1, const double* b1,
+__global__ void __launch_bounds__(kBlock) kCompare(int n0, const double* a0, const double* b0, int n1, const double* a1, const double* b1,
                          double* part) {
-    __shared__ double smem[kBlock];
+    __shared__ double smem[kBlock / 64];
     double cnt = 0.0;
     const int n = n0 + n1;
     for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
@@ -617,11 +617,11 @@ __global__ void kCompare(int n0, const double* a0, const double* b0, int n1, con
 
 // ---- AMG ----
 // The AMG kernels below compute in the storage type T of their level.
-template <class T> __global__ void kJacobiFirst(int n, const T* d, const T* b, T* x, double omega) {
+template <class T> __global__ void __launch_bounds__(kBlock) kJacobiFirst(int n, const T* d, const T* b, T* x, double omega) {
     const int i = globalThread();
     if (i < n) x[i] = static_cast<T>(omega) * b[i] / d[i];
 }
-template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
+template <class T> __global__ void __launch_bounds__(kBlock) kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
@@ -630,9 +630,9 @@ template <class T> __global__ void kJacobiSweep(CsrView<T> A, const T* b, const 
 }
 // The last fine-level sweep of a preconditioner application inside CG, with
 // the partial sums of z . r (z the result, r = b) that kPlainDot would form.
-__global__ void kJacobiSweepDot(CsrView<double> A, const double* b, const double* x, double* xo, double omega,
+__global__ void __launch_bounds__(kBlock) kJacobiSweepDot(CsrView<double> A, const double* b, const double* x, double* xo, double omega,
                                 double* part) {
-    __shared__ double smem[kBlock];
+    __shared__ double smem[kBlock / 64];
     double acc = 0.0;
     for (int q = globalThread(); q < A.n; q += gridDim.x * blockDim.x) {
         const int i = A.perm ? A.perm[q] : q;
@@ -644,20 +644,20 @@ __global__ void kJacobiSweepDot(CsrView<double> A, const double* b, const double
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
-template <class T> __global__ void kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
+template <class T> __global__ void __launch_bounds__(kBlock) kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
     r[i] = b[i] - rowDot(A, x, q, i);
 }
-template <class T> __global__ void kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
+template <class T> __global__ void __launch_bounds__(kBlock) kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
     const int I = globalThread();
     if (I >= nc) return;
     T s = 0;
     for (int k = mp[I]; k < mp[I + 1]; ++k) s += r[mi[k]];
     bc[I] = s;
 }
-template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
+template <class T> __global__ void __launch_bounds__(kBlock) kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
     const int i = globalThread();
     if (i < n) x[i] = x[i] + static_cast<T>(alpha) * xc[agg[i]];
 }
@@ -670,7 +670,7 @@ template <class T> __global__ void kProlong(int n, const int* agg, const T* xc, 
 // kRestrict, then the coarse level's first damped-Jacobi step (kJacobiFirst)
 // into xc.
 template <class T>
-__global__ void kRestrictFirst(int nc, const int* mp, const int* mi, const T* r, T* bc, const T* dc, T* xc,
+__global__ void __launch_bounds__(kBlock) kRestrictFirst(int nc, const int* mp, const int* mi, const T* r, T* bc, const T* dc, T* xc,
                                double omega) {
     const int I = globalThread();
     if (I >= nc) return;
@@ -682,7 +682,7 @@ __global__ void kRestrictFirst(int nc, const int* mp, const int* mi, const T* r,
 // kProlong followed by one kJacobiSweep: the prolongated vector is formed
 // where the sweep reads it and is not stored.
 template <class T>
-__global__ void kProlongSweep(CsrView<T> A, const int* agg, const T* xc, double alpha, const T* b, const T* x, T* xo,
+__global__ void __launch_bounds__(kBlock) kProlongSweep(CsrView<T> A, const int* agg, const T* xc, double alpha, const T* b, const T* x, T* xo,
                               double omega) {
     const int q = globalThread();
     if (q >= A.n) return;
@@ -726,8 +726,11 @@ template <class T> struct TailLevels {
 template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(TailLevels<T> L) {
     const int tid = threadIdx.x;
     const int later = L.pre - 1 + L.post;
-    T* cur[kMaxTailLevels];
-    T* nxt[kMaxTailLevels];
+    // Each level starts in x when the (pre - 1) + post swaps are even, so
+    // that its result lands in x; after the pre-smoothing swaps the current
+    // iterate is in x exactly when curIsX holds.
+    const bool startX = later % 2 == 0;
+    const bool curIsX = (L.pre > 1 ? L.pre - 1 : 0) % 2 == 0 ? startX : !startX;
     const int last = L.count - 1;
     for (int k = 0; k < last; ++k) {  // descend: pre-smoothing, residual, restriction
         const CsrView<T>& A = L.A[k];
@@ -757,8 +760,6 @@ template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(Ta
             L.b[k + 1][I] = s;
         }
         __syncthreads();
-        cur[k] = c;
-        nxt[k] = o;
     }
     {  // the last level: x = inv b (the kDenseApply sum)
         const int n = L.A[last].n;
@@ -784,8 +785,8 @@ template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(Ta
     for (int k = last - 1; k >= 0; --k) {  // ascend: prolongation, post-smoothing
         const CsrView<T>& A = L.A[k];
         const T* b = L.b[k];
-        T* c = cur[k];
-        T* o = nxt[k];
+        T* c = curIsX ? L.x[k] : L.t[k];
+        T* o = curIsX ? L.t[k] : L.x[k];
         const T* xc = L.x[k + 1];
         for (int i = tid; i < A.n; i += kTailThreads) c[i] = c[i] + static_cast<T>(L.alpha) * xc[L.agg[k][i]];
         __syncthreads();
@@ -800,7 +801,7 @@ template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(Ta
         }
     }
 }
-__global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
+__global__ void __launch_bounds__(kBlock) kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, const double* fd, const double* fv,
                           double* cd, double* cv) {
     const int e = globalThread();
     if (e >= ne) return;
@@ -812,18 +813,18 @@ __global__ void kGalerkin(int n, int nc, int ne, const int* gp, const int* gs, c
     if (e < nc) cd[e] = s;
     else cv[e - nc] = s;
 }
-__global__ void kToFloat(int n, const double* a, float* b) {
+__global__ void __launch_bounds__(kBlock) kToFloat(int n, const double* a, float* b) {
     const int i = globalThread();
     if (i < n) b[i] = static_cast<float>(a[i]);
 }
-__global__ void kToDouble(int n, const float* a, double* b) {
+__global__ void __launch_bounds__(kBlock) kToDouble(int n, const float* a, double* b) {
     const int i = globalThread();
     if (i < n) b[i] = static_cast<double>(a[i]);
 }
 // Dense fill + LU with partial pivoting of the coarsest matrix (one block).
 // The coarsest level can be stored as CSR or sliced (a one-level hierarchy is
 // the fine matrix itself); rows are filled in entry order either way.
-__global__ void kDenseLU(CsrView<double> A, double* a, int* piv, int* flag) {
+__global__ void __launch_bounds__(1024) kDenseLU(CsrView<double> A, double* a, int* piv, int* flag) {
     const int n = A.n;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) a[e] = 0.0;
     __syncthreads();
@@ -874,7 +875,7 @@ __global__ void kDenseLU(CsrView<double> A, double* a, int* piv, int* flag) {
 // the LU solve of the unit vector e_j (thread j works in column j of inv).
 // The inverse is stored column-major (thread j owns column j), so that the
 // dense apply below reads it with consecutive lanes on consecutive rows.
-__global__ void kDenseInverse(int n, const double* a, const int* piv, double* inv) {
+__global__ void __launch_bounds__(kBlock) kDenseInverse(int n, const double* a, const int* piv, double* inv) {
     const int j = globalThread();
     if (j >= n) return;
     auto w = [&](int i) -> double& { return inv[static_cast<long long>(j) * n + i]; };
@@ -888,7 +889,7 @@ __global__ void kDenseInverse(int n, const double* a, const int* piv, double* in
     }
 }
 // Coarsest solve as a dense product x = inv * b (FP64 accumulation).
-template <class T> __global__ void kDenseApply(int n, const double* inv, const T* b, T* xo) {
+template <class T> __global__ void __launch_bounds__(kBlock) kDenseApply(int n, const double* inv, const T* b, T* xo) {
     const int i = globalThread();
     if (i >= n) return;
     double s = 0.0;
@@ -1737,13 +1738,13 @@ void DeviceSystem::syncScalars3() {
 namespace {
 template <class T> Vec3p<T> vec3(DBuf<T> (&a)[3]) { return Vec3p<T>{{a[0].p, a[1].p, a[2].p}}; }
 template <class T> CVec3p<T> cvec3(DBuf<T> (&a)[3]) { return CVec3p<T>{{a[0].p, a[1].p, a[2].p}}; }
-__global__ void k3Narrow(int n, CVec3p<double> a, Vec3p<float> b, const int* act) {  // b = float(a)
+__global__ void __launch_bounds__(kBlock) k3Narrow(int n, CVec3p<double> a, Vec3p<float> b, const int* act) {  // b = float(a)
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)
         if (act[c]) b.c[c][i] = static_cast<float>(a.c[c][i]);
 }
-__global__ void k3AddWide(int n, CVec3p<float> d, Vec3p<double> x, const int* act) {  // x += double(d)
+__global__ void __launch_bounds__(kBlock) k3AddWide(int n, CVec3p<float> d, Vec3p<double> x, const int* act) {  // x += double(d)
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c)

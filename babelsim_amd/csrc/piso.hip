@@ -29,7 +29,7 @@ constexpr int kDiagK = 6;
 
 // equ::ddt(momentum, rho, U_old) (procedural_equation.cpp:213-222): w = rho V,
 // diag += w a0, source += w a1 U_prev (+ w a2 U_older).
-__global__ void kTimeTerm(MeshView m, double rho, double a0, double a1, double a2, const double* prev,
+__global__ void __launch_bounds__(kBlock) kTimeTerm(MeshView m, double rho, double a0, double a1, double a2, const double* prev,
                           const double* older, double* diag, double* src) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -45,7 +45,7 @@ __global__ void kTimeTerm(MeshView m, double rho, double a0, double a1, double a
 // rho * temporalFluxCorrection (piso/main.cpp:19-53): the old-time flux
 // mismatch weighted by the time scheme and by the flux-normalised coupling,
 // on interior faces only.
-__global__ void kTemporalFlux(int nf, int nfi, const double* phiP, const double* fluxP, const double* phiO,
+__global__ void __launch_bounds__(kBlock) kTemporalFlux(int nf, int nfi, const double* phiP, const double* fluxP, const double* phiO,
                               const double* fluxO, double wp, double wo, double rho, double* tf) {
     const int f = globalThread();
     if (f >= nf) return;
@@ -61,7 +61,7 @@ __global__ void kTemporalFlux(int nf, int nfi, const double* phiP, const double*
 // relax without rescaling (main.cpp:186; procedural_equation.cpp:298-305),
 // rAU = V/aP, 1/aP (main.cpp:187-189) and the non-pressure right-hand side
 // b + V grad(p) (main.cpp:193).
-__global__ void kPisoRelax(MeshView m, double* diag, double* src, const double* Uiter, double alpha,
+__global__ void __launch_bounds__(kBlock) kPisoRelax(MeshView m, double* diag, double* src, const double* Uiter, double alpha,
                            const double* gradP, double* rAU, double* invAP, double* npr) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -94,7 +94,7 @@ __device__ inline void applyRow(const MeshView& m, const double* diag, const dou
 }
 
 // HbyA = (1/aP) ((b0 - A U) + aP U) (main.cpp:206-207).
-__global__ void kHbyA(MeshView m, const double* diag, const double* off, const double* U, const double* npr,
+__global__ void __launch_bounds__(kBlock) kHbyA(MeshView m, const double* diag, const double* off, const double* U, const double* npr,
                       const double* invAP, double* H) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -108,13 +108,13 @@ __global__ void kHbyA(MeshView m, const double* diag, const double* off, const d
 }
 
 // phiH += rAUf * ddtPhiCorr on interior faces (main.cpp:210).
-__global__ void kAddTemporal(int nfi, const double* rAUf, const double* tf, double* phiH) {
+__global__ void __launch_bounds__(kBlock) kAddTemporal(int nfi, const double* rAUf, const double* tf, double* phiH) {
     const int f = globalThread();
     if (f < nfi) phiH[f] += rAUf[f] * tf[f];
 }
 
 // U = HbyA - rAU grad(p) (main.cpp:230-231).
-__global__ void kPisoCorrectU(int nc, const double* H, const double* rAU, const double* g, double* U) {
+__global__ void __launch_bounds__(kBlock) kPisoCorrectU(int nc, const double* H, const double* rAU, const double* g, double* U) {
     const int i = globalThread();
     if (i >= nc) return;
     for (int k = 0; k < 3; ++k) {
@@ -128,9 +128,9 @@ __global__ void kPisoCorrectU(int nc, const double* H, const double* rAU, const 
 // Squared L2 norms of b0, A U, V grad(p) and the defect b0 - A U - V grad(p)
 // (main.cpp:234-239), and of p - p_before and p (relativeMagnitude,
 // fvm_execution.cpp:130-147).
-__global__ void kCoupling(MeshView m, const double* diag, const double* off, const double* U, const double* npr,
+__global__ void __launch_bounds__(kBlock) kCoupling(MeshView m, const double* diag, const double* off, const double* U, const double* npr,
                           const double* gradP, const double* p, const double* pOld, double* part) {
-    __shared__ double smem[6 * kBlock];
+    __shared__ double smem[6 * kBlock / 64];
     double a[6] = {0, 0, 0, 0, 0, 0};
     for (int i = globalThread(); i < m.nc; i += gridDim.x * blockDim.x) {
         double au[3];
@@ -155,9 +155,9 @@ __global__ void kCoupling(MeshView m, const double* diag, const double* off, con
 // relativeChange(U, U_previous), fluxBalance(phi) and courantNumber(phi, dt)
 // (fvm_execution.cpp:91-205): six sums per block and the block maximum of
 // the cell Courant number.
-__global__ void kStepDiagnostics(MeshView m, const double* U, const double* Uiter, const double* phi, double dt,
+__global__ void __launch_bounds__(kBlock) kStepDiagnostics(MeshView m, const double* U, const double* Uiter, const double* phi, double dt,
                                  double* part) {
-    __shared__ double smem[kDiagK * kBlock];
+    __shared__ double smem[kDiagK * kBlock / 64];
     double a[kDiagK] = {0, 0, 0, 0, 0, 0};
     double mx = 0.0;
     for (int c = globalThread(); c < m.nc; c += gridDim.x * blockDim.x) {
@@ -189,19 +189,14 @@ __global__ void kStepDiagnostics(MeshView m, const double* U, const double* Uite
         a[5] += v;
     }
     blockPartial<kDiagK>(a, part, smem);
-    smem[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) smem[threadIdx.x] = fmax(smem[threadIdx.x], smem[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[kDiagK * kMaxReduceBlocks + xcdBlock()] = smem[0];
+    mx = blockMaxValue(mx, smem);
+    if (threadIdx.x == 0) part[kDiagK * kMaxReduceBlocks + xcdBlock()] = mx;
 }
 
 // Second stage of kStepDiagnostics: the six sums in block order and the
 // maximum, stored to out[0..6].
 __global__ void __launch_bounds__(kTail) kDiagnosticsTail(int nb, const double* part, double* out) {
-    __shared__ double smem[(kDiagK + 1) * kTail];
+    __shared__ double smem[(kDiagK + 1) * kTail / 64];
     double acc[kDiagK];
     for (int k = 0; k < kDiagK; ++k) acc[k] = 0.0;
     double mx = 0.0;
@@ -210,16 +205,10 @@ __global__ void __launch_bounds__(kTail) kDiagnosticsTail(int nb, const double* 
         mx = fmax(mx, part[kDiagK * kMaxReduceBlocks + b]);
     }
     blockSum<kDiagK, kTail>(acc, smem);
-    double* sm = smem + kDiagK * kTail;
-    sm[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = kTail / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
-    }
+    mx = blockMaxValue<kTail>(mx, smem);
     if (threadIdx.x == 0) {
         for (int k = 0; k < kDiagK; ++k) out[k] = acc[k];
-        out[kDiagK] = sm[0];
+        out[kDiagK] = mx;
     }
 }
 

@@ -19,7 +19,7 @@ double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::
 constexpr int kRedK = 6;
 
 // relax (procedural_equation.cpp:298-305), scale (240-247) and rAU = V/aP (main.cpp:62-66).
-__global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, const double* Uprev, double alpha,
+__global__ void __launch_bounds__(kBlock) kRelaxScale(MeshView m, double* diag, double* off, double* src, const double* Uprev, double alpha,
                             double* rAU) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -37,7 +37,7 @@ __global__ void kRelaxScale(MeshView m, double* diag, double* off, double* src, 
 }
 
 // X = rAU * gradP with traces rAU_trace * gradP_trace.
-__global__ void kScaleVector(MeshView m, const double* s, const double* g, double* X, const double* str,
+__global__ void __launch_bounds__(kBlock) kScaleVector(MeshView m, const double* s, const double* g, double* X, const double* str,
                              const double* gtr, double* Xtr) {
     const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
     if (i < m.nc)
@@ -49,7 +49,7 @@ __global__ void kScaleVector(MeshView m, const double* s, const double* g, doubl
 
 // Rhie-Chow on interior faces (main.cpp:71-79): phiH += (rAU gradP)_f . Sf,
 // phiH += -1 * ((rAU_f * snGrad p) * |Sf|).
-__global__ void kRhieChow(MeshView m, double* phiH, const double* Xf, const double* rAUf, const double* sn) {
+__global__ void __launch_bounds__(kBlock) kRhieChow(MeshView m, double* phiH, const double* Xf, const double* rAUf, const double* sn) {
     const int f = globalThread();
     if (f >= m.nfi) return;
     double v = phiH[f] + dot(ld3(Xf, f), ld3(m.Sf, f));
@@ -58,7 +58,7 @@ __global__ void kRhieChow(MeshView m, double* phiH, const double* Xf, const doub
 }
 
 // p += alphaP p'; U -= rAU grad(p')  (main.cpp:95-97)
-__global__ void kCorrectUP(MeshView m, double* p, const double* pp, double alphaP, double* U, const double* rAU,
+__global__ void __launch_bounds__(kBlock) kCorrectUP(MeshView m, double* p, const double* pp, double alphaP, double* U, const double* rAU,
                            const double* gpp) {
     const int i = globalThread();
     if (i >= m.nc) return;
@@ -70,9 +70,9 @@ __global__ void kCorrectUP(MeshView m, double* p, const double* pp, double alpha
 }
 
 // relativeChange, relativeMagnitude and fluxBalance sums (fvm_execution.cpp:91-177).
-__global__ void kConvergence(MeshView m, const double* U, const double* Up, const double* pp, const double* p,
+__global__ void __launch_bounds__(kBlock) kConvergence(MeshView m, const double* U, const double* Up, const double* pp, const double* p,
                              const double* phi, double* part) {
-    __shared__ double smem[kRedK * kBlock];
+    __shared__ double smem[kRedK * kBlock / 64];
     double a[kRedK] = {0, 0, 0, 0, 0, 0};
     for (int c = globalThread(); c < m.nc; c += gridDim.x * blockDim.x) {
         const D3 u = ld3(U, c), up = ld3(Up, c);

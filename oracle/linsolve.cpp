@@ -51,10 +51,24 @@ void spmv(const Csr& A, const double* x, double* y) {
 namespace {
 
 // Inner products are summed in a fixed blocked order: 256 strided partial sums
-// per block over at most 32768 blocks and a pairwise tree inside each block,
-// then 1024 strided sums of the block sums and a pairwise tree over them.  Both sides of the parity check use this order
-// (it is the deterministic reduction of the device library), so FP64 Krylov
-// histories can be compared bit for bit.
+// per block over at most 32768 blocks, combined inside each block as four
+// wavefronts of 64, then 1024 strided sums of the block sums combined as
+// sixteen wavefronts of 64.  A wavefront's 64 values are added by a pairwise
+// tree (lane t takes lane t + s for s = 32, 16, ..., 1: the xor butterfly's
+// result) and the wavefront totals in order.  Both sides of the parity check
+// use this order (it is the deterministic reduction of the device library),
+// so FP64 Krylov histories can be compared bit for bit.
+double wavefrontSum(double* acc, long count) {
+    double total = 0.0;
+    for (long w = 0; w < count / 64; ++w) {
+        double* a = acc + 64 * w;
+        for (long s = 32; s > 0; s >>= 1)
+            for (long t = 0; t < s; ++t) a[t] += a[t + s];
+        total = w == 0 ? a[0] : total + a[0];
+    }
+    return total;
+}
+
 double dotv(const std::vector<double>& a, const std::vector<double>& b) {
     constexpr long B = 256, Gmax = 32768;
     const long n = static_cast<long>(a.size());
@@ -69,9 +83,7 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b) {
             for (long i = g * B + t; i < n; i += G * B) s += a[i] * b[i];
             acc[t] = s;
         }
-        for (long s = B / 2; s > 0; s >>= 1)
-            for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
-        partial[g] = acc[0];
+        partial[g] = wavefrontSum(acc, B);
     }
     constexpr long T = 1024;  // threads of the device's reduction tail
     double acc[T];
@@ -80,9 +92,7 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b) {
         for (long g = t; g < G; g += T) s += partial[g];
         acc[t] = s;
     }
-    for (long s = T / 2; s > 0; s >>= 1)
-        for (long t = 0; t < s; ++t) acc[t] += acc[t + s];
-    return acc[0];
+    return wavefrontSum(acc, T);
 }
 double nrm(const std::vector<double>& a) { return std::sqrt(dotv(a, a)); }
 
