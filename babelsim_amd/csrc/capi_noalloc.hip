@@ -86,7 +86,7 @@ extern "C" int bsim_csr_solve_noalloc(int device, void* stream, int n, const int
             throw std::invalid_argument("matrix has " + std::to_string(n) + " rows; the allocation-free solve holds at most " +
                                         std::to_string(kNoAllocMaxRows));
         const SolverCfg c = toSolverCfg(cfg);
-        if (c.pc == 2 || c.precision != 0)
+        if (c.pc >= 2 || c.precision != 0)
             throw std::invalid_argument("the allocation-free solve runs FP64 CG or BiCGSTAB with no preconditioner "
                                         "or Jacobi only");
         if (DeviceSystem::workspaceDoubles(n) > kWorkspaceDoubles)

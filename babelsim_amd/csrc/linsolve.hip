@@ -593,6 +593,126 @@ __global__ void __launch_bounds__(kBlock) kSpmvPlain(CsrView<double> A, const do
     const int i = globalThread();
     if (i < A.n) y[i] = rowDot64(A, x, i);
 }
+// ---- block Jacobi: ILU(0) inside blocks of 64 consecutive rows ----
+// PETSc's bjacobi with one block per 64 rows and ILU(0) in each block
+// (petsc_backend.cpp:46 names it; PCBJACOBI with -pc_bjacobi_blocks
+// ceil(n/64), sub-PC ILU): one wavefront per block, lane r on row 64*b + r.
+// Only entries whose column lies in the row's own block take part; the
+// factors keep the matrix's pattern (no fill).  Rows past n in the last block
+// are identity rows.
+constexpr int kIluBlock = 64;
+__device__ inline unsigned long long readlane64(unsigned long long v, int lane) {
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(v), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(v >> 32), lane);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+__device__ inline double readlaneD(double v, int lane) {
+    return __longlong_as_double(static_cast<long long>(readlane64(static_cast<unsigned long long>(__double_as_longlong(v)), lane)));
+}
+// Calls f(e) for the stored off-diagonal entries of row i (lane `lane` of
+// block `blk`): the block's slice in the sliced layout, the CSR row otherwise.
+template <class F> __device__ inline void forRowEntries(const CsrView<double>& A, int blk, int lane, int i, F f) {
+    if (i >= A.n) return;
+    if (A.sliceOff) {
+        const int* so = A.sliceOff + static_cast<long long>(blk) * (A.K + 1);
+        for (int k = 0; k < A.K; ++k) {
+            const int width = so[k + 1] - so[k];
+            if (width == 0) break;
+            if (lane < width && A.col[so[k] + lane] >= 0) f(so[k] + lane);
+        }
+    } else {
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) f(e);
+    }
+}
+// Factorisation (IKJ order): for k = 0..62, every row r > k with (r, k) in the
+// pattern takes l = a_rk / a_kk and a_rj -= l * a_kj for each j > k present in
+// rows r and k.  fd: the U diagonal; fv: the factor at each in-block entry
+// (L strictly below, U strictly above; a repeated column keeps it in its first
+// entry and 0 in the others).
+__global__ void __launch_bounds__(kIluBlock) kBlockIluFactor(CsrView<double> A, double* fd, double* fv) {
+    __shared__ double T[kIluBlock][kIluBlock + 1];  // padded rows: lanes reading one column hit distinct banks
+    const int blk = blockIdx.x, r = threadIdx.x, base = blk * kIluBlock, i = base + r;
+    for (int j = 0; j < kIluBlock; ++j) T[r][j] = 0.0;
+    unsigned long long mask = 1ull << r;
+    T[r][r] = i < A.n ? A.diag[i] : 1.0;
+    forRowEntries(A, blk, r, i, [&](int e) {
+        const int c = A.col[e] - base;
+        if (c >= 0 && c < kIluBlock) { T[r][c] += A.val[e]; mask |= 1ull << c; }
+    });
+    __syncthreads();
+    for (int k = 0; k < kIluBlock - 1; ++k) {
+        const unsigned long long mk = readlane64(mask, k);
+        if (r > k && ((mask >> k) & 1ull)) {
+            const double l = T[r][k] / T[k][k];
+            T[r][k] = l;
+            unsigned long long upd = mask & mk & ~((2ull << k) - 1ull);
+            while (upd) {
+                const int j = __builtin_ctzll(upd);
+                upd &= upd - 1ull;
+                T[r][j] -= l * T[k][j];
+            }
+        }
+        __syncthreads();
+    }
+    if (i >= A.n) return;
+    fd[i] = T[r][r];
+    unsigned long long written = 1ull << r;
+    forRowEntries(A, blk, r, i, [&](int e) {
+        const int c = A.col[e] - base;
+        if (c >= 0 && c < kIluBlock) {
+            fv[e] = ((written >> c) & 1ull) ? 0.0 : T[r][c];
+            written |= 1ull << c;
+        }
+    });
+}
+// z = (LU)^-1 r block by block.  Lane r holds its factor row in registers
+// (64 doubles, statically indexed); forward and backward substitution
+// broadcast each finished unknown with readlane.  Zero factors are skipped,
+// so each row's sum runs over its pattern in column order.
+__global__ void __launch_bounds__(kIluBlock) kBlockIluApply(CsrView<double> A, const double* fd, const double* fv,
+                                                           const double* rr, double* zz) {
+    const int blk = blockIdx.x, r = threadIdx.x, base = blk * kIluBlock, i = base + r;
+    double T[kIluBlock];
+#pragma unroll
+    for (int j = 0; j < kIluBlock; ++j) T[j] = 0.0;
+    unsigned long long cols = 0;  // in-block columns of this row's entries
+    forRowEntries(A, blk, r, i, [&](int e) {
+        const int c = A.col[e] - base;
+        if (c >= 0 && c < kIluBlock) {
+            const double v = fv[e];
+            cols |= 1ull << c;
+#pragma unroll
+            for (int j = 0; j < kIluBlock; ++j) T[j] = c == j ? T[j] + v : T[j];
+        }
+    });
+    // Columns some row of the block refers to (an xor-butterfly OR): a
+    // substitution step on any other column changes no row and is skipped.
+    unsigned lo = static_cast<unsigned>(cols), hi = static_cast<unsigned>(cols >> 32);
+    for (int off = 32; off > 0; off >>= 1) {
+        lo |= static_cast<unsigned>(__shfl_xor(static_cast<int>(lo), off, 64));
+        hi |= static_cast<unsigned>(__shfl_xor(static_cast<int>(hi), off, 64));
+    }
+    const unsigned long long used = (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(hi)) << 32) |
+                                    __builtin_amdgcn_readfirstlane(lo);
+    const double d = i < A.n ? fd[i] : 1.0;
+    double y = i < A.n ? rr[i] : 0.0;
+#pragma unroll
+    for (int s = 0; s < kIluBlock - 1; ++s) {
+        if (!((used >> s) & 1ull)) continue;
+        const double ys = readlaneD(y, s);
+        const double t = r > s ? T[s] : 0.0;
+        if (t != 0.0) y = y - t * ys;
+    }
+#pragma unroll
+    for (int s = kIluBlock - 1; s >= 0; --s) {
+        if (r == s) y = y / d;
+        if (!((used >> s) & 1ull)) continue;
+        const double xs = readlaneD(y, s);
+        const double t = r < s ? T[s] : 0.0;
+        if (t != 0.0) y = y - t * xs;
+    }
+    if (i < A.n) zz[i] = y;
+}
 __global__ void __launch_bounds__(kBlock) kJacobiApply(int n, const double* r, const double* jinv, double* z) {
     const int i = globalThread();
     if (i < n) z[i] = r[i] * jinv[i];
@@ -1334,7 +1454,7 @@ bool DeviceSystem::update() {
 }
 
 void DeviceSystem::prepare(const SolverCfg& c) {
-    if (borrowed && (c.pc == 2 || c.precision != 0 || c.solver < 0 || c.solver > 1))
+    if (borrowed && (c.pc >= 2 || c.precision != 0 || c.solver < 0 || c.solver > 1))
         throw std::invalid_argument("a system in a fixed workspace runs FP64 CG or BiCGSTAB with no "
                                     "preconditioner or Jacobi only");
     if (!configured || !(c == cfg)) {
@@ -1356,6 +1476,11 @@ void DeviceSystem::setup(bool reuse) {
         if (cfg.pc == 1) {
             jinv.ensure(n);
             kInvDiag<<<blocksFor(n), kBlock, 0, stream>>>(n, diag(), jinv.p);
+            BSIM_LAUNCH_CHECK();
+        } else if (cfg.pc == 3) {
+            iluDiag.ensure(n);
+            iluVal.ensure(std::max(nnz, 1));
+            kBlockIluFactor<<<(n + kIluBlock - 1) / kIluBlock, kIluBlock, 0, stream>>>(view(), iluDiag.p, iluVal.p);
             BSIM_LAUNCH_CHECK();
         } else if (cfg.pc == 2) {
             if (!amg) amg = std::make_unique<AmgDevice>(stream);
@@ -1380,6 +1505,8 @@ void DeviceSystem::applyPC(const double* rr, double* zz) {
     ++counters.pcApplications;
     if (cfg.pc == 0) kCopy<<<blocksFor(n), kBlock, 0, stream>>>(n, rr, zz);
     else if (cfg.pc == 1) kJacobiApply<<<blocksFor(n), kBlock, 0, stream>>>(n, rr, jinv.p, zz);
+    else if (cfg.pc == 3)
+        kBlockIluApply<<<(n + kIluBlock - 1) / kIluBlock, kIluBlock, 0, stream>>>(view(), iluDiag.p, iluVal.p, rr, zz);
     else amg->apply(rr, zz);
     BSIM_LAUNCH_CHECK();
 }
@@ -1890,7 +2017,7 @@ void DeviceSystem::solve3(const double* const b[3], double* const x[3], SolveRes
         solve3Mixed(b, x, res, hists, 3);
         return;
     }
-    const bool lockstep = cfg.solver == 1 && cfg.pc != 2 && cfg.precision == 0;
+    const bool lockstep = cfg.solver == 1 && cfg.pc <= 1 && cfg.precision == 0;
     if (!lockstep) {
         for (int c = 0; c < 3; ++c) {
             res[c] = solve(b[c], x[c], c);

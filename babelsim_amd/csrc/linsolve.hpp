@@ -14,7 +14,7 @@
 namespace bsim {
 
 struct SolverCfg {
-    int solver = 1, pc = 2;
+    int solver = 1, pc = 2;  // pc: 0 none, 1 Jacobi, 2 aggregation AMG, 3 block Jacobi (ILU(0) per 64 rows)
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000, warmStart = 0, rebuildInterval = 8;
     double reuseGrowth = 2.0;
@@ -191,6 +191,7 @@ private:
     std::uint64_t pcMatrixUpdate = 0, pcRhsSolves = 0;
     int freshIterations[3] = {0, 0, 0};
     DBuf<double> jinv;          // Jacobi: 1/diag of the matrix at PC setup
+    DBuf<double> iluDiag, iluVal;  // block Jacobi: ILU(0) factors of the 64-row blocks at PC setup
     std::unique_ptr<AmgDevice> amg;
 public:
     KernelProbe probe;  // times the fine-level AMG smoother sweeps

@@ -129,6 +129,12 @@ def parse():
     ap.add_argument("--amg-correction", type=float, default=2.0,
                     help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
     ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
+    ap.add_argument("--mom-pc", choices=["jacobi", "bjacobi"], default="jacobi",
+                    help="cavity: momentum BiCGSTAB preconditioner (bjacobi: ILU(0) in 64-row blocks, the "
+                         "reference cavity case's pcType)")
+    ap.add_argument("--amg-pre", type=int, default=2, help="cavity: AMG pre-smoothing sweeps per level")
+    ap.add_argument("--amg-post", type=int, default=2, help="cavity: AMG post-smoothing sweeps per level")
+    ap.add_argument("--amg-coarse-size", type=int, default=48, help="cavity: AMG coarsest-level row target")
     ap.add_argument("--warm-start", action="store_true",
                     help="all workloads: start the linear solves from the current unknowns "
                          "(the equation.<name>.warmStart case setting; off, its default, in the cavity and "
@@ -215,15 +221,18 @@ def main():
     log(f"mesh generated: {nc_in} cells, {nf_in} faces ({time.time() - t0:.1f}s)")
     prec = 1 if args.precision == "mixed" else 0
 
-    def config(precision, rtol):
+    def config(precision, rtol, mom_pc=None):
         # cases/cavity/numerics/solution.bs:6-16: momentum bcgs and pressure cg
         # at relativeTolerance 1e-8, absoluteTolerance 1e-14.
         common = dict(atol=1e-14, precision=precision, rebuild_interval=args.rebuild_interval,
                       warm_start=int(args.warm_start))
+        mpc = 3 if (mom_pc or args.mom_pc) == "bjacobi" and precision == 0 else 1
         return cases.c2_config(
-            mom=solver_config(solver=1, pc=1, rtol=rtol, **common),
+            mom=solver_config(solver=1, pc=mpc, rtol=rtol, **common),
             pre=solver_config(solver=0, pc=2, rtol=rtol, amg_strength=args.amg_strength,
-                              amg_correction=args.amg_correction, amg_omega=args.amg_omega, **common))
+                              amg_correction=args.amg_correction, amg_omega=args.amg_omega,
+                              amg_pre_sweeps=args.amg_pre, amg_post_sweeps=args.amg_post,
+                              amg_coarse_size=args.amg_coarse_size, **common))
 
     cfg = config(prec, args.rtol)
     ub, pb = cases.cavity_bcs(two_d=False)
@@ -306,6 +315,12 @@ def main():
                              "compared": f"after {args.warmup + args.steps} outer iterations from rest"}
             log(f"mixed: {legs['mixed']['value']:.2f} it/s, field rel L2 vs FP64 {legs['mixed']['field_rel_l2_vs_fp64']}")
             del s3
+            # The reference cavity case's momentum preconditioner (pcType bjacobi).
+            other_pc = "bjacobi" if args.mom_pc == "jacobi" else "jacobi"
+            s4, sec4, t4_, _ = run_leg(config(0, args.rtol, other_pc), f"momentum {other_pc}")
+            legs[f"momentum_{other_pc}"] = {"value": args.steps / sec4, "unit": "outer_iterations/s",
+                                            "p_its_last": t4_[-1].p_iters, "mom_its_last": list(t4_[-1].mom_iters)}
+            del s4
     traffic = measured_traffic(ctx.nc)
     serve = None
     if args.serve_sample > 0:
@@ -353,8 +368,10 @@ def main():
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
                    "linear_rtol": args.rtol, "linear_atol": 1e-14,
                    "pressure": f"CG + aggregation AMG (strength {args.amg_strength}, correction "
-                               f"{args.amg_correction}, Jacobi weight {args.amg_omega})",
-                   "momentum": "BiCGSTAB + Jacobi", "warm_start": args.warm_start,
+                               f"{args.amg_correction}, Jacobi weight {args.amg_omega}, sweeps "
+                               f"{args.amg_pre}/{args.amg_post}, coarsest {args.amg_coarse_size} rows)",
+                   "momentum": "BiCGSTAB + " + ("block Jacobi (ILU(0) per 64 rows)" if args.mom_pc == "bjacobi"
+                                                else "Jacobi"), "warm_start": args.warm_start,
                    "timed_linear_solves": "all converged",
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
                    "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb,

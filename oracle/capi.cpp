@@ -256,6 +256,24 @@ int orc_csr_solve(int n, const int* ptr, const int* col, const double* diag, con
     });
 }
 
+int orc_block_ilu(int n, const int* ptr, const int* col, const double* diag, const double* val, double* fd,
+                  double* fv) {
+    return guard([&] {
+        const Csr A = makeCsr(n, ptr, col, diag, val);
+        std::vector<double> d, v;
+        blockIluFactor(A, d, v);
+        std::memcpy(fd, d.data(), sizeof(double) * n);
+        if (!v.empty()) std::memcpy(fv, v.data(), sizeof(double) * v.size());
+    });
+}
+int orc_block_ilu_apply(int n, const int* ptr, const int* col, const double* diag, const double* fd,
+                        const double* fv, const double* r, double* z) {
+    return guard([&] {
+        const Csr A = makeCsr(n, ptr, col, diag, fv);
+        blockIluApply(A, std::vector<double>(fd, fd + n), A.val, r, z);
+    });
+}
+
 struct OrcSystem {
     const Mesh* mesh;
     LinearSystem sys;

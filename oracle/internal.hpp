@@ -190,7 +190,7 @@ struct Csr {  // diagonal stored separately; off-diagonal entries one per interi
 
 struct SolverConfig {
     int solver = 1;          // 0 CG, 1 BiCGSTAB
-    int pc = 2;              // 0 none, 1 Jacobi, 2 AMG
+    int pc = 2;              // 0 none, 1 Jacobi, 2 AMG, 3 block Jacobi (ILU(0) per 64 rows)
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000;
     int warmStart = 0;
@@ -254,11 +254,17 @@ struct Amg {
     SolverConfig cfg;
     std::vector<std::vector<double>> wx, wb, wr;
     bool aggregated = false;  // aggregates are built at the first setup and kept
+    std::vector<double> iluDiag, iluVal;  // block Jacobi factors (pc 3), at the matrix's entries
     void setup(const Csr& A, const SolverConfig& c);
     void apply(const double* r, double* z);
     void vcycle(int l);
 };
 void aggregate(const Csr& A, double theta, std::vector<Index>& agg, Index& nc);
+// Block Jacobi: ILU(0) inside blocks of 64 consecutive rows (PCBJACOBI with
+// ceil(n/64) blocks and ILU(0) sub-preconditioners).
+void blockIluFactor(const Csr& A, std::vector<double>& fd, std::vector<double>& fv);
+void blockIluApply(const Csr& A, const std::vector<double>& fd, const std::vector<double>& fv, const double* r,
+                   double* z);
 void galerkin(const Csr& A, const std::vector<Index>& agg, Index nc, Csr& Ac);
 
 }  // namespace orc

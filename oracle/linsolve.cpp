@@ -351,6 +351,93 @@ void LinearSystem::prepare(const SolverConfig& c) {
     setup(reuse);
 }
 
+// Block Jacobi with ILU(0) in each block of 64 consecutive rows: PETSc's
+// bjacobi (petsc_backend.cpp:46) with ceil(n/64) blocks.  The factorisation
+// runs k = 0..62 over the rows r > k with (r, k) in the pattern: l = a_rk / a_kk,
+// then a_rj -= l * a_kj for each j > k present in rows r and k (no fill).  A
+// repeated column keeps the factor in its first entry and 0 in the others.
+// The substitutions take the columns in order and skip zero factors.  This is
+// the order of the device library's kernels, so the factors and every
+// preconditioned vector agree bit for bit.
+namespace {
+constexpr Index kIlu = 64;
+}
+
+void blockIluFactor(const Csr& A, std::vector<double>& fd, std::vector<double>& fv) {
+    const Index n = A.n, nb = (n + kIlu - 1) / kIlu;
+    fd.assign(n, 0.0);
+    fv.assign(A.val.size(), 0.0);
+#pragma omp parallel for schedule(static)
+    for (Index b = 0; b < nb; ++b) {
+        const Index base = b * kIlu;
+        std::vector<double> T(kIlu * kIlu, 0.0);
+        std::uint64_t mask[kIlu];
+        for (Index r = 0; r < kIlu; ++r) {
+            const Index i = base + r;
+            mask[r] = 1ull << r;
+            T[r * kIlu + r] = i < n ? A.diag[i] : 1.0;
+            if (i >= n) continue;
+            for (Index e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+                const Index c = A.col[e] - base;
+                if (c >= 0 && c < kIlu) { T[r * kIlu + c] += A.val[e]; mask[r] |= 1ull << c; }
+            }
+        }
+        for (Index k = 0; k < kIlu - 1; ++k)
+            for (Index r = k + 1; r < kIlu; ++r) {
+                if (!((mask[r] >> k) & 1ull)) continue;
+                const double l = T[r * kIlu + k] / T[k * kIlu + k];
+                T[r * kIlu + k] = l;
+                for (Index j = k + 1; j < kIlu; ++j)
+                    if (((mask[r] & mask[k]) >> j) & 1ull) T[r * kIlu + j] -= l * T[k * kIlu + j];
+            }
+        for (Index r = 0; r < kIlu && base + r < n; ++r) {
+            const Index i = base + r;
+            fd[i] = T[r * kIlu + r];
+            std::uint64_t written = 1ull << r;
+            for (Index e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+                const Index c = A.col[e] - base;
+                if (c < 0 || c >= kIlu) continue;
+                fv[e] = ((written >> c) & 1ull) ? 0.0 : T[r * kIlu + c];
+                written |= 1ull << c;
+            }
+        }
+    }
+}
+
+void blockIluApply(const Csr& A, const std::vector<double>& fd, const std::vector<double>& fv, const double* r,
+                   double* z) {
+    const Index n = A.n, nb = (n + kIlu - 1) / kIlu;
+#pragma omp parallel for schedule(static)
+    for (Index b = 0; b < nb; ++b) {
+        const Index base = b * kIlu;
+        std::vector<double> T(kIlu * kIlu, 0.0);
+        double y[kIlu], d[kIlu];
+        for (Index q = 0; q < kIlu; ++q) {
+            const Index i = base + q;
+            y[q] = i < n ? r[i] : 0.0;
+            d[q] = i < n ? fd[i] : 1.0;
+            if (i >= n) continue;
+            for (Index e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+                const Index c = A.col[e] - base;
+                if (c >= 0 && c < kIlu) T[q * kIlu + c] += fv[e];
+            }
+        }
+        for (Index s = 0; s < kIlu - 1; ++s)
+            for (Index q = s + 1; q < kIlu; ++q) {
+                const double t = T[q * kIlu + s];
+                if (t != 0.0) y[q] = y[q] - t * y[s];
+            }
+        for (Index s = kIlu - 1; s >= 0; --s) {
+            y[s] = y[s] / d[s];
+            for (Index q = 0; q < s; ++q) {
+                const double t = T[q * kIlu + s];
+                if (t != 0.0) y[q] = y[q] - t * y[s];
+            }
+        }
+        for (Index q = 0; q < kIlu && base + q < n; ++q) z[base + q] = y[q];
+    }
+}
+
 // petsc_linear_system.cpp:303-324; the PC keeps the matrix it was built from.
 static std::vector<double> jacobiInv;
 void LinearSystem::setup(bool reuse) {
@@ -361,6 +448,10 @@ void LinearSystem::setup(bool reuse) {
         } else if (cfg.pc == 1) {
             if (!amg) amg = new Amg();
             amg->levels.assign(1, AmgLevel{A, {}, 0, {}, {}});
+        } else if (cfg.pc == 3) {
+            if (!amg) amg = new Amg();
+            amg->levels.assign(1, AmgLevel{A, {}, 0, {}, {}});
+            blockIluFactor(A, amg->iluDiag, amg->iluVal);
         }
         if (cfg.pc != 0) ++pcSetups;
         pcMatrixUpdate = matrixUpdates;
@@ -388,6 +479,10 @@ struct Ksp {
             const Csr& P = pc->levels[0].A;
 #pragma omp parallel for schedule(static)
             for (Index i = 0; i < n; ++i) z[i] = r[i] * (1.0 / P.diag[i]);
+            return;
+        }
+        if (cfg.pc == 3) {
+            blockIluApply(pc->levels[0].A, pc->iluDiag, pc->iluVal, r.data(), z.data());
             return;
         }
         pc->apply(r.data(), z.data());

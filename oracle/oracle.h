@@ -101,6 +101,12 @@ int orc_system_solve_ldu(void* sys, const double* diag, const double* upper, con
  * Krylov iterations, true-residual checks; last status, initial, final and
  * relative residual. */
 int orc_system_counters(void* sys, uint64_t* out8, int* last_status, double* last3);
+/* Block Jacobi (ILU(0) in 64-row blocks) of a CSR matrix: the factors (U
+ * diagonal fd[n], factor at each entry fv[nnz]) and z = (LU)^-1 r. */
+int orc_block_ilu(int n, const int* ptr, const int* col, const double* diag, const double* val, double* fd,
+                  double* fv);
+int orc_block_ilu_apply(int n, const int* ptr, const int* col, const double* diag, const double* fd,
+                        const double* fv, const double* r, double* z);
 int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,
                       const orc_solver_config* cfg, int* nlevels, int* sizes, int* agg0, double* coarse_diag1);
 

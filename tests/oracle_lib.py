@@ -62,6 +62,8 @@ def oracle():
         L.orc_csr_solve.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.POINTER(SolverConfig),
                                     C.POINTER(SolveResult), dp, C.c_int, ip]
         L.orc_amg_hierarchy.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp]
+        L.orc_block_ilu.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp]
+        L.orc_block_ilu_apply.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, dp]
         L.orc_system_new.restype = vp
         L.orc_system_new.argtypes = [vp]
         L.orc_system_free.restype = None
@@ -502,6 +504,34 @@ def csr_solve(ptr_, col, diag, val, b, x0, cfg: SolverConfig, hist_cap=200000):
     if rc:
         raise ValueError(L.orc_last_error().decode())
     return x, res, hist[:ln.value].copy()
+
+
+def block_ilu(ptr_, col, diag, val):
+    """Block Jacobi factors (ILU(0) in 64-row blocks): U diagonal, factor per entry."""
+    L = oracle()
+    n = len(diag)
+    fd, fv = np.zeros(n), np.zeros(max(len(val), 1))
+    args = (np.ascontiguousarray(ptr_, np.int32), np.ascontiguousarray(col, np.int32),
+            np.ascontiguousarray(diag, np.float64), np.ascontiguousarray(val, np.float64))
+    rc = L.orc_block_ilu(n, ptr(args[0], ip), ptr(args[1], ip), ptr(args[2]), ptr(args[3]), ptr(fd), ptr(fv))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    return fd, fv[:len(val)]
+
+
+def block_ilu_apply(ptr_, col, diag, fd, fv, r):
+    """z = (LU)^-1 r with the block Jacobi factors."""
+    L = oracle()
+    n = len(diag)
+    z = np.zeros(n)
+    args = (np.ascontiguousarray(ptr_, np.int32), np.ascontiguousarray(col, np.int32),
+            np.ascontiguousarray(diag, np.float64), np.ascontiguousarray(fd, np.float64),
+            np.ascontiguousarray(fv if len(fv) else np.zeros(1), np.float64), np.ascontiguousarray(r, np.float64))
+    rc = L.orc_block_ilu_apply(n, ptr(args[0], ip), ptr(args[1], ip), ptr(args[2]), ptr(args[3]), ptr(args[4]),
+                               ptr(args[5]), ptr(z))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    return z
 
 
 def amg_hierarchy(ptr_, col, diag, val, cfg: SolverConfig):

@@ -143,7 +143,7 @@ def test_amg_hierarchy_is_bit_exact():
     assert len(s1) >= 3
 
 
-@pytest.mark.parametrize("solver,pc", [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2)])
+@pytest.mark.parametrize("solver,pc", [(0, 0), (0, 1), (0, 2), (0, 3), (1, 0), (1, 1), (1, 2), (1, 3)])
 def test_csr_solver_histories(solver, pc):
     arrays, om, diag, up, lo, b = laplace_system()
     P, Cc, D, Vv = orc.csr_from_ldu(arrays, diag, up, lo)
@@ -169,6 +169,29 @@ def test_one_level_amg_on_the_mesh_layout():
     assert res[0].status == 0 and res[0].iterations <= 2
     xo, ro, _ = orc.csr_solve(*orc.csr_from_ldu(arrays, diag, up, lo), b, np.zeros(len(b)), cfg)
     assert rel_l2(x, xo) < 1e-9
+
+
+def test_block_jacobi_factors_on_the_mesh_layout():
+    """The device's block Jacobi on a mesh system (sliced rows, caller's cell
+    order, so the 64-row blocks are the oracle's): each preconditioned solve
+    equals the oracle's CSR solve with the same preconditioner to 1e-12, a
+    mixed-precision request is refused, and with renumbered cells (other
+    blocks) both still converge to the same solution."""
+    arrays, om, diag, up, lo, b = laplace_system(anchor=True)
+    csr = orc.csr_from_ldu(arrays, diag, up, lo)
+    for solver in (0, 1):
+        cfg = solver_config(solver=solver, pc=3, rtol=1e-10, atol=1e-30)
+        xo, ro, ho = orc.csr_solve(*csr, b, np.zeros(len(b)), cfg)
+        sys = System(Context(arrays, renumber=False))
+        x, res = sys.solve(diag, up, lo, b, np.zeros_like(b), cfg)
+        assert res[0].status == 0 and res[0].iterations == ro.iterations
+        assert rel_l2(x, xo) < 1e-12
+        sysr = System(Context(arrays, renumber=True))
+        xr, rr = sysr.solve(diag, up, lo, b, np.zeros_like(b), cfg)
+        assert rr[0].status == 0 and rel_l2(xr, xo) < 1e-8
+    with pytest.raises(ValueError):
+        System(Context(arrays, renumber=False)).solve(diag, up, lo, b, np.zeros_like(b),
+                                                      solver_config(solver=1, pc=3, precision=1))
 
 
 def test_manufactured_solution():
@@ -216,13 +239,19 @@ def run_pair(mesh_data, cfg, two_d, iters):
     return so, sg, ro, rg
 
 
-@pytest.mark.parametrize("kind", ["c1", "c2", "c1_retry", "c1_warm"])
+@pytest.mark.parametrize("kind", ["c1", "c2", "c1_retry", "c1_warm", "c1_bjacobi", "c2_bjacobi"])
 def test_simple_parity(kind):
     """Device SIMPLE against the oracle.  c1_retry caps the momentum solves at 3
     BiCGSTAB iterations so that lagged-preconditioner failures trigger the
     rebuild-and-retry path of the three-component solve; c1_warm warm-starts
-    every solve."""
-    if kind == "c1":
+    every solve; the bjacobi kinds solve momentum with BiCGSTAB and block
+    Jacobi, the reference cavity case's pair (cases/cavity/numerics/solution.bs:7-8)."""
+    bj = solver_config(solver=1, pc=3, rtol=1e-8, atol=1e-14)
+    if kind == "c1_bjacobi":
+        mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(mom=bj), True, 12
+    elif kind == "c2_bjacobi":
+        mesh, cfg, two_d, iters = cases.cavity3d_mesh(7, split=0.15, seed=3), cases.c2_config(mom=bj), False, 8
+    elif kind == "c1":
         mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(), True, 12
     elif kind == "c1_retry":
         mom = solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14, max_iter=3)

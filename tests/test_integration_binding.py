@@ -34,10 +34,10 @@ def mpi_include():
 def test_binding_names_every_solver_and_preconditioner_type():
     src = binding_source()
     for name in ("ConjugateGradient", "BiCGSTAB", "PreconditionerType::None", "PreconditionerType::Jacobi",
-                 "PreconditionerType::Hypre", "PreconditionerType::GAMG"):
+                 "PreconditionerType::Hypre", "PreconditionerType::GAMG", "PreconditionerType::BlockJacobi"):
         assert name in src, name
     # Types the library does not implement are refused, not substituted.
-    assert src.count("throw std::invalid_argument") >= 3
+    assert src.count("throw std::invalid_argument") >= 3  # Krylov type, PC type, AMG controls
     assert "bsim_system_performance" in src
 
 

@@ -86,6 +86,7 @@ def test_noalloc_rejects_what_it_cannot_hold():
     assert call(MAX_ROWS + 1, ok) == 2
     assert b"at most 524288" in lib().bsim_last_error()
     assert call(len(D), solver_config(solver=0, pc=2)) == 2
+    assert call(len(D), solver_config(solver=1, pc=3)) == 2  # block Jacobi keeps factors: not in the fixed scratch
     assert call(len(D), solver_config(solver=1, pc=1, precision=1)) == 2
     # A column outside [0, n), found by the device check (every read in bounds).
     bad = Cc.copy()
