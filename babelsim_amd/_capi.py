@@ -363,6 +363,8 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_server_release": [vp, C.c_int],
     "bsim_server_stats": [vp, dp],
     "bsim_server_estimate": [C.c_int, C.c_int, C.c_int, dp, dp],
+    "bsim_device_pool_stats": [C.c_int, dp],
+    "bsim_device_pool_trim": [C.c_int],
 }
 _VOID = {"bsim_ctx_destroy": [vp], "bsim_system_destroy": [vp], "bsim_simple_destroy": [vp],
          "bsim_piso_destroy": [vp], "bsim_server_destroy": [vp], "bsim_field_destroy": [vp],

@@ -20,6 +20,10 @@ Context::Context(const MeshDesc& caller, bool renumber, int dev) : device(dev) {
 }
 
 Context::~Context() {
+    // Work still queued on the stream finishes first: a later stream may get
+    // the same handle, and the block cache treats same-handle reuse as
+    // stream-ordered.
+    if (stream) (void)hipStreamSynchronize(stream);
     mesh.reset();
     err.release();
     if (stream) (void)hipStreamDestroy(stream);

@@ -2,6 +2,8 @@
 // algebra and deterministic two-stage reductions for 64-wide wavefronts.
 #pragma once
 
+#include "pool.hpp"
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,9 +37,9 @@ inline int blocksFor(long long n, int block = kBlock) {
 // released; the serving runtime reads its change across one request.
 inline thread_local long long t_deviceBytes = 0;
 
-// Device allocation of T[n]: owning (alloc) or a view of memory that someone
-// else owns (borrow).  A borrowed buffer never allocates or frees: growing it
-// past its size throws instead.
+// Device allocation of T[n]: owning (alloc, from the block cache of pool.hpp)
+// or a view of memory that someone else owns (borrow).  A borrowed buffer
+// never allocates or frees: growing it past its size throws instead.
 template <class T> struct DBuf {
     T* p = nullptr;
     std::size_t n = 0;
@@ -60,7 +62,7 @@ template <class T> struct DBuf {
         if (!owned) throw std::logic_error("a borrowed device buffer cannot be reallocated");
         release();
         if (count) {
-            BSIM_HIP(hipMalloc(&p, sizeof(T) * count));
+            p = static_cast<T*>(pool::deviceAlloc(sizeof(T) * count));
             t_deviceBytes += static_cast<long long>(sizeof(T) * count);
         }
         n = count;
@@ -74,7 +76,7 @@ template <class T> struct DBuf {
     void ensure(std::size_t count) { if (count > n) alloc(count); }
     void release() {
         if (p && owned) {
-            (void)hipFree(p);
+            pool::deviceFree(p);
             t_deviceBytes -= static_cast<long long>(sizeof(T) * n);
         }
         p = nullptr;

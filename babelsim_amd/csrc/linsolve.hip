@@ -1337,7 +1337,10 @@ bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         // The cycle is launches only (no copies, fills or host waits), all
-        // with arguments fixed by the hierarchy and the key.
+        // with arguments fixed by the hierarchy and the key.  No other
+        // thread returns memory to the runtime (a device-wide wait) while
+        // the capture is open.
+        std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
         BSIM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
         try {
@@ -1359,6 +1362,7 @@ bool AmgDevice::apply(const double* r, double* z, double* zrPartials) {
 }
 
 void AmgDevice::dropGraphs() {
+    std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
     for (auto& kv : graphs) {
         (void)hipGraphExecDestroy(kv.second.exec);
         (void)hipGraphDestroy(kv.second.graph);
@@ -1376,13 +1380,13 @@ DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern)
     sc.alloc(S_COUNT);
     flags.alloc(1);
     partials.alloc(6 * kMaxReduceBlocks);
-    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostScalars), sizeof(double) * (S_COUNT + 2), hipHostMallocDefault));
+    hostScalars = static_cast<double*>(pool::hostAlloc(sizeof(double) * (S_COUNT + 2)));
     for (DBuf<double>* b : {&r, &z, &p, &w, &rp, &v, &t, &s, &tmp, &guess, &saved}) b->alloc(n);
     sc3.alloc(3 * S_COUNT);
     fl3.alloc(3);
     act3.alloc(3);
-    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc3), sizeof(double) * (3 * S_COUNT + 2), hipHostMallocDefault));
-    BSIM_HIP(hipHostMalloc(reinterpret_cast<void**>(&hact3), sizeof(int) * 4 * kActSlots, hipHostMallocDefault));
+    hsc3 = static_cast<double*>(pool::hostAlloc(sizeof(double) * (3 * S_COUNT + 2)));
+    hact3 = static_cast<int*>(pool::hostAlloc(sizeof(int) * 4 * kActSlots));
     counters.patternSeconds = since(t0);
 }
 
@@ -1426,9 +1430,9 @@ DeviceSystem::DeviceSystem(hipStream_t st, Pattern pattern, const SystemWorkspac
 
 DeviceSystem::~DeviceSystem() {
     if (borrowed) return;
-    if (hostScalars) (void)hipHostFree(hostScalars);
-    if (hsc3) (void)hipHostFree(hsc3);
-    if (hact3) (void)hipHostFree(hact3);
+    if (hostScalars) pool::hostFree(hostScalars);
+    if (hsc3) pool::hostFree(hsc3);
+    if (hact3) pool::hostFree(hact3);
 }
 
 bool DeviceSystem::update() {

@@ -228,6 +228,11 @@ void Server::worker(int slot) {
 
 void Server::run(ServeRequest& r, std::unique_ptr<Context>& ctx) {
     BSIM_HIP(hipSetDevice(device));
+    // The solver's buffers return to the block cache stream-ordered on the
+    // context's stream: the next request on this mesh takes the same blocks
+    // back, so its solver, fields and work vectors are built in resident
+    // memory without runtime allocations.
+    pool::StreamScope scope(ctx->stream);
     const long long base = t_deviceBytes;
     long long peak = 0;
     r.tStart = now();
@@ -434,5 +439,18 @@ int bsim_server_estimate(int kind, int ncells, int nfaces, double* context_bytes
 }
 
 void bsim_server_destroy(bsim_server* s) { delete s; }
+
+int bsim_device_pool_stats(int device, double* o) {
+    return guarded([&] {
+        if (!o) throw std::invalid_argument("null argument");
+        const pool::Stats t = pool::deviceStats(device);
+        const double v[5] = {t.cachedBytes, t.liveBytes, t.hits, t.misses, t.released};
+        std::memcpy(o, v, sizeof(v));
+    });
+}
+
+int bsim_device_pool_trim(int device) {
+    return guarded([&] { pool::trim(device); });
+}
 
 }  // extern "C"

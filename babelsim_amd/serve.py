@@ -137,3 +137,18 @@ class Server:
         out = np.zeros(len(STATS_KEYS))
         check(lib().bsim_server_stats(self.h, out.ctypes.data_as(dp)))
         return dict(zip(STATS_KEYS, out.tolist()))
+
+
+POOL_KEYS = ("cached_bytes", "live_bytes", "hits", "misses", "released")
+
+
+def pool_stats(device: int = 0) -> dict:
+    """The device block cache: cached and live bytes, cache hits, runtime
+    allocations and blocks returned to the runtime."""
+    out = np.zeros(len(POOL_KEYS))
+    check(lib().bsim_device_pool_stats(int(device), out.ctypes.data_as(dp)))
+    return dict(zip(POOL_KEYS, out.tolist()))
+
+
+def pool_trim(device: int = 0) -> None:
+    check(lib().bsim_device_pool_trim(int(device)))

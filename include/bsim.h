@@ -524,6 +524,15 @@ int bsim_server_estimate(int kind, int ncells, int nfaces, double* context_bytes
 /* Cancels queued requests, waits for running ones, frees every context. */
 void bsim_server_destroy(bsim_server* s);
 
+/* The library's device block cache (no counterpart in the reference).
+ * Buffers that a served request frees stay cached per size, stream-ordered
+ * on the request's context stream, for the next request on the same mesh;
+ * at most a quarter of the device's memory is held.  Stats: cached bytes,
+ * live bytes, cache hits, runtime allocations, blocks returned to the
+ * runtime.  Trim returns every cached block to the runtime. */
+int bsim_device_pool_stats(int device, double* out5);
+int bsim_device_pool_trim(int device);
+
 #ifdef __cplusplus
 }
 #endif

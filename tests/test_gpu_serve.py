@@ -135,6 +135,30 @@ def test_resident_contexts_are_reused_with_the_same_results(solo):
         assert srv.stats()["context_reuses"] >= 1
 
 
+def test_requests_on_one_mesh_reuse_resident_memory(solo):
+    """One slot, the same steady request three times: after the first, the
+    solver, fields and work vectors are rebuilt from the blocks the previous
+    request returned to the device cache (no runtime allocation), and the
+    results stay equal to the solo run."""
+    from babelsim_amd.serve import pool_stats, pool_trim
+    w = next(w for w in workload() if "steps" not in w)
+    k = [i for i, x in enumerate(workload()) if "steps" not in x][0]
+    pool_trim(0)
+    with Server(slots=1, memory_budget=8e9) as srv:
+        cid = srv.add_case(w["mesh"])
+        ids, misses = [], []
+        for _ in range(3):
+            rid = srv.submit_steady(cid, w["cfg"], *w["bcs"], max_iterations=CAP)
+            srv.wait(rid)
+            ids.append((rid, cid))
+            misses.append(pool_stats(0)["misses"])
+        check_equal(srv, ids, [solo[k]] * 3)
+        assert misses[2] == misses[1], misses  # the third request allocated nothing new
+        assert pool_stats(0)["hits"] > 0
+    pool_trim(0)
+    assert pool_stats(0)["cached_bytes"] == 0
+
+
 def test_tight_budget_runs_one_at_a_time(solo):
     ws = workload()
     footprint = 0.0
