@@ -378,6 +378,192 @@ __global__ void __launch_bounds__(kBlock) kAssembleDiffusion(MeshView m, FieldAr
     for (int k = 0; k < NC; ++k) src[static_cast<long long>(NC) * c + k] += factor * sAcc[k];
 }
 
+// ------------------------------------------------ face-ordered assembly
+// The parts of an interior face's contribution that need the gradients (the
+// linearUpwind or corrected-interpolation convection flux and the
+// non-orthogonal diffusion correction) are formed once per face here, in face
+// order, instead of once from each of the face's two cells; the cell pass
+// (kAssembleTransport) then gathers them.  Each stored value is the product
+// the cell kernels form inline (contraction is off), so the matrices are
+// those of kAssembleConvection then kAssembleDiffusion bit for bit.
+//   qc[NC*f + k] = F * correction   (F = scale * phi[f]);
+//   qd[NC*f + k] = gamma * (ing - orth * (x_n - x_o)).
+template <int NC>
+__global__ void __launch_bounds__(kBlock) kFaceCorrections(MeshView m, FieldArg x, const double* phi, int conv, int interp,
+                                                           const double* g, double scale, const double* kf, int dm,
+                                                           double* qc, double* qd) {
+    const int f = globalThread();
+    if (f >= m.nfi) return;
+    const long long o = m.own[f], n = m.nei[f];
+    if (qc) {
+        const double F = scale * phi[f];
+        double a[NC], b[NC];
+        if (conv == kLinearUpwind) {
+            luCorrection<NC>(g, m, F >= 0.0 ? static_cast<int>(o) : static_cast<int>(n), f, a);
+            for (int k = 0; k < NC; ++k) qc[static_cast<long long>(NC) * f + k] = F * a[k];
+        } else {
+            correctedInternal<NC>(x, g, m, f, a);
+            internalValue<NC>(x, m, f, b);
+            for (int k = 0; k < NC; ++k) {
+                const double cr = a[k] - b[k];
+                qc[static_cast<long long>(NC) * f + k] = F * cr;
+            }
+        }
+    }
+    if (qd) {
+        const double gamma = kf[f];
+        double ing[NC];
+        int dummy = 0;
+        if (gamma > 0.0 && isfinite(gamma)) {
+            integratedNormalGradient<NC>(x, g, m, f, dm, ing, &dummy);
+            for (int k = 0; k < NC; ++k) {
+                const double orthogonal = m.orth[f] * (x.v[NC * n + k] - x.v[NC * o + k]);
+                qd[static_cast<long long>(NC) * f + k] = gamma * (ing[k] - orthogonal);
+            }
+        }
+    }
+}
+
+// kAssembleConvection (when conv >= 0) followed by kAssembleDiffusion (when
+// kf is given) in one cell pass: each cell keeps the two kernels' separate
+// accumulators and adds them to diag, off and src in their order.
+template <int NC>
+__global__ void __launch_bounds__(kBlock) kAssembleTransport(MeshView m, FieldArg x, const double* phi, int conv, int interp,
+                                                             const double* g, double scale, const double* kf, int dm,
+                                                             double factor, const double* qc, const double* qd,
+                                                             double* diag, double* off, double* src, int* err) {
+    const int c = globalThread();
+    if (c >= m.nc) return;
+    const bool doConv = conv >= 0, doDiff = kf != nullptr;
+    const bool haveG = conv == kLinearUpwind || (conv == kCentral && interp == kCorrected);
+    const bool corrected = doDiff && correctedDiffusion(dm);
+    double dC = 0.0, dD = 0.0, sC[NC], sD[NC], a[NC], b[NC], corr[NC];
+    for (int k = 0; k < NC; ++k) { sC[k] = 0.0; sD[k] = 0.0; }
+    for (int k_ = 0, e = 0; m.row(c).valid(k_) && (e = m.row(c).at(k_), true); ++k_) {
+        if (m.mface[e] < 0) continue;
+        const int f = m.mface[e];
+        const bool owner = m.own[f] == c;
+        double oe = off[e];
+        if (doConv) {
+            const double F = scale * phi[f];
+            if (conv == kUpwind || conv == kLinearUpwind) {
+                if (owner) { dC += fmax(F, 0.0); oe += fmin(F, 0.0); }
+                else { dC += fmax(-F, 0.0); oe += fmin(-F, 0.0); }
+            } else {
+                const double w = m.w[f];
+                if (owner) { dC += F * w; oe += F * (1.0 - w); }
+                else { oe -= F * w; dC -= F * (1.0 - w); }
+            }
+            if (haveG)
+                for (int k = 0; k < NC; ++k) {
+                    const double q = qc[static_cast<long long>(NC) * f + k];
+                    sC[k] = owner ? sC[k] - q : sC[k] + q;
+                }
+        }
+        if (doDiff) {
+            const double gamma = kf[f];
+            if (gamma < 0.0 || !isfinite(gamma)) atomicOr(err, kErrNegativeDiffusivity);
+            else if (gamma != 0.0) {
+                const double coef = gamma * m.orth[f];
+                dD += coef;
+                oe += factor * (0.0 - coef);
+                if (corrected)
+                    for (int k = 0; k < NC; ++k) {
+                        const double q = qd[static_cast<long long>(NC) * f + k];
+                        sD[k] = owner ? sD[k] + q : sD[k] - q;
+                    }
+            }
+        }
+        off[e] = oe;
+    }
+    if (doConv)
+        for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+            const int f = m.bface[e];
+            const double F = scale * phi[f];
+            const Cnd<NC> cd = condition<NC>(x, m, f, true, F, err);
+            if (conv == kUpwind || conv == kLinearUpwind) {
+                if (F >= 0.0) {
+                    dC += F;
+                    if (conv == kLinearUpwind) {
+                        luCorrection<NC>(g, m, c, f, corr);
+                        for (int k = 0; k < NC; ++k) sC[k] -= F * corr[k];
+                    }
+                } else {
+                    traceValue<NC>(x, m, f, true, F, a, err);
+                    for (int k = 0; k < NC; ++k) sC[k] -= F * a[k];
+                }
+                continue;
+            }
+            if (cd.type == kFixedValue) {
+                for (int k = 0; k < NC; ++k) sC[k] -= F * cd.v[k];
+            } else if (cd.type == kFixedGradient) {
+                const double dist = dot(ld3(m.Cf, f) - ld3(m.C, c), ld3(m.nrm, f));
+                dC += F;
+                for (int k = 0; k < NC; ++k) sC[k] -= (F * dist) * cd.v[k];
+            } else {
+                dC += F;
+            }
+            if (haveG) {
+                reconstructedBoundary<NC>(x, g, m, f, true, F, a, err);
+                traceValue<NC>(x, m, f, true, F, b, err);
+                for (int k = 0; k < NC; ++k) sC[k] -= F * (a[k] - b[k]);
+            }
+        }
+    if (doDiff)
+        for (int e = m.bptr[c]; e < m.bptr[c + 1]; ++e) {
+            const int f = m.bface[e];
+            const double gamma = kf[f];
+            if (gamma < 0.0 || !isfinite(gamma)) { atomicOr(err, kErrNegativeDiffusivity); continue; }
+            if (gamma == 0.0) continue;
+            const double coef = gamma * m.orth[f];
+            const Cnd<NC> cd = condition<NC>(x, m, f, false, 0.0, err);
+            if (cd.type == kFixedValue || (NC == 3 && cd.type == kSymmetry)) {
+                if (NC == 3) traceValue<NC>(x, m, f, true, -1.0, a, err);
+                else for (int k = 0; k < NC; ++k) a[k] = cd.v[k];
+                dD += coef;
+                for (int k = 0; k < NC; ++k) sD[k] += coef * a[k];
+                if (corrected) {
+                    integratedNormalGradient<NC>(x, g, m, f, dm, b, err);
+                    for (int k = 0; k < NC; ++k) {
+                        const double orthogonal = m.orth[f] * (a[k] - x.v[static_cast<long long>(NC) * c + k]);
+                        sD[k] += gamma * (b[k] - orthogonal);
+                    }
+                }
+            } else if (cd.type == kFixedGradient) {
+                for (int k = 0; k < NC; ++k) sD[k] += NC == 1 ? (gamma * cd.v[k]) * m.Af[f] : (gamma * m.Af[f]) * cd.v[k];
+            }
+        }
+    double d = diag[c];
+    if (doConv) d += dC;
+    if (doDiff) d += factor * dD;
+    diag[c] = d;
+    for (int k = 0; k < NC; ++k) {
+        double v = src[static_cast<long long>(NC) * c + k];
+        if (doConv) v += sC[k];
+        if (doDiff) v += factor * sD[k];
+        src[static_cast<long long>(NC) * c + k] = v;
+    }
+}
+
+// diffusionFlux with the method and orthogonal, then kCorrection, for every
+// face (procedural_equation.cpp:77-87): corr = k ing - k ing_orth, the
+// products and the difference formed as those kernels form them.  Interior
+// faces also store the assembly's correction qd (kFaceCorrections).
+__global__ void __launch_bounds__(kBlock) kDiffusionFaces(MeshView m, FieldArg x, const double* kf, const double* g, int dm,
+                                                          double* corr, double* qd, int* err) {
+    const int f = globalThread();
+    if (f >= m.nf) return;
+    const double k = kf[f];
+    const bool valid = k >= 0.0 && isfinite(k);
+    if (!valid) atomicOr(err, kErrNegativeDiffusivity);
+    double v[1], v0[1];
+    integratedNormalGradient<1>(x, g, m, f, dm, v, err);
+    integratedNormalGradient<1>(x, g, m, f, kOrthogonal, v0, err);
+    const double full = k * v[0], sn = k * v0[0];
+    corr[f] = full + (-1.0) * sn;
+    if (f < m.nfi && valid && k != 0.0) qd[f] = k * (v[0] - v0[0]);
+}
+
 __global__ void __launch_bounds__(kBlock) kGradTraces(MeshView m, FieldArg x, const double* g, double* traces) {
     const int b = globalThread();
     const int f = m.nfi + b;
@@ -485,6 +671,46 @@ void assembleDiffusion(const MeshView& m, int ncomp, FieldArg x, const double* k
         throw std::invalid_argument("unsupported diffusion method");
     if (ncomp == 1) kAssembleDiffusion<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, kf, grad, dmethod, factor, diag, off, src, err);
     else kAssembleDiffusion<3><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, kf, grad, dmethod, factor, diag, off, src, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void assembleTransport(const MeshView& m, int ncomp, FieldArg x, const double* phi, int conv, int interp, double scale,
+                       const double* kf, int dmethod, double factor, const double* grad, double* diag, double* off,
+                       double* src, double* faceWork, int* err, hipStream_t s) {
+    if (phi) {
+        if (conv != kUpwind && conv != kLinearUpwind && conv != kCentral) throw std::invalid_argument("unsupported convection method");
+        if (interp != kLinear && interp != kCorrected) throw std::invalid_argument("unsupported convection interpolation method");
+        if (!std::isfinite(scale)) throw std::invalid_argument("convection flux scale must be finite");
+    }
+    if (kf && dmethod != kOrthogonal && dmethod != kDiffCorrected && dmethod != kLimitedCorrected)
+        throw std::invalid_argument("unsupported diffusion method");
+    const int cv = phi ? conv : -1;
+    const bool needC = phi && (conv == kLinearUpwind || (conv == kCentral && interp == kCorrected));
+    const bool needD = kf && (dmethod == kDiffCorrected || dmethod == kLimitedCorrected);
+    const long long span = static_cast<long long>(ncomp) * m.nfi;
+    double* qc = needC ? faceWork : nullptr;
+    double* qd = needD ? faceWork + (needC ? span : 0) : nullptr;
+    if ((needC || needD) && m.nfi > 0) {
+        if (ncomp == 1) kFaceCorrections<1><<<blocksFor(m.nfi), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, scale, kf, dmethod, qc, qd);
+        else kFaceCorrections<3><<<blocksFor(m.nfi), kBlock, 0, s>>>(m, x, phi, conv, interp, grad, scale, kf, dmethod, qc, qd);
+    }
+    if (ncomp == 1)
+        kAssembleTransport<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, cv, interp, grad, scale, kf, dmethod, factor, qc,
+                                                                 qd, diag, off, src, err);
+    else
+        kAssembleTransport<3><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, phi, cv, interp, grad, scale, kf, dmethod, factor, qc,
+                                                                 qd, diag, off, src, err);
+    BSIM_LAUNCH_CHECK();
+}
+
+void assembleDiffusionCorrected(const MeshView& m, FieldArg x, const double* kf, const double* grad, int dmethod,
+                                double factor, double* diag, double* off, double* src, double* corr, double* faceWork,
+                                int* err, hipStream_t s) {
+    if (dmethod != kDiffCorrected && dmethod != kLimitedCorrected)
+        throw std::invalid_argument("the corrected assembly needs a corrected diffusion method");
+    kDiffusionFaces<<<blocksFor(m.nf), kBlock, 0, s>>>(m, x, kf, grad, dmethod, corr, faceWork, err);
+    kAssembleTransport<1><<<blocksFor(m.nc), kBlock, 0, s>>>(m, x, nullptr, -1, kLinear, grad, 0.0, kf, dmethod, factor,
+                                                             nullptr, faceWork, diag, off, src, err);
     BSIM_LAUNCH_CHECK();
 }
 

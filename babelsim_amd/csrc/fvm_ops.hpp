@@ -62,6 +62,19 @@ void assembleConvection(const MeshView& m, int ncomp, FieldArg x, const double* 
 // `factor` times the contribution computed on its own, like accumulate().
 void assembleDiffusion(const MeshView& m, int ncomp, FieldArg x, const double* kf, const double* grad, int dmethod,
                        double factor, double* diag, double* off, double* src, int* err, hipStream_t s);
+// assembleConvection (phi given) then assembleDiffusion (kf given) with the
+// same results bit for bit, the gradient-dependent face terms formed once per
+// face: faceWork holds 2 * ncomp * nfi doubles.
+void assembleTransport(const MeshView& m, int ncomp, FieldArg x, const double* phi, int conv, int interp, double scale,
+                       const double* kf, int dmethod, double factor, const double* grad, double* diag, double* off,
+                       double* src, double* faceWork, int* err, hipStream_t s);
+// assembleDiffusion with a corrected method on a scalar, plus the face
+// correction of procedural_equation.cpp:77-87 (diffusionFlux with the method
+// minus diffusionFlux orthogonal, every face) into corr, bit for bit as those
+// calls; faceWork holds nfi doubles.
+void assembleDiffusionCorrected(const MeshView& m, FieldArg x, const double* kf, const double* grad, int dmethod,
+                                double factor, double* diag, double* off, double* src, double* corr, double* faceWork,
+                                int* err, hipStream_t s);
 // Computed-gradient boundary traces of a scalar (fvm_execution.cpp:221-235).
 void gradientTraces(const MeshView& m, FieldArg x, const double* grad, double* traces, hipStream_t s);
 

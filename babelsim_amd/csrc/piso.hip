@@ -248,6 +248,7 @@ PisoSolver::PisoSolver(Context& ctx, const PisoCfg& cfg, const int* ubct, const 
     for (DBuf<double>* b : {&phi, &rAUtr, &rAUf, &kfMom, &corr, &full, &orthF, &tf, &fluxTmp, &phih[0], &phih[1],
                             &fluxh[0], &fluxh[1]})
         b->alloc(nf);
+    faceWork.alloc(6 * static_cast<std::size_t>(std::max(M.nfi, 1)));
     red.alloc(kPartOffset + (kDiagK + 1) * static_cast<std::size_t>(kMaxReduceBlocks));
     U.zero(s); p.zero(s); phi.zero(s); corr.zero(s); tf.zero(s);
     const MeshView mv = M.view();
@@ -408,8 +409,8 @@ StepRep PisoSolver::step(double dt) {
         const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
                                c.mDiff != kOrthogonal;
         if (needGradU) ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s);
-        ops::assembleConvection(mv, 3, Uf, phi.p, c.mConv, c.mInterp, gradU.p, c.rho, diag, off, src.p, C.err.p, s);
-        ops::assembleDiffusion(mv, 3, Uf, kfMom.p, gradU.p, c.mDiff, -(-1.0), diag, off, src.p, C.err.p, s);
+        ops::assembleTransport(mv, 3, Uf, phi.p, c.mConv, c.mInterp, c.rho, kfMom.p, c.mDiff, -(-1.0), gradU.p, diag,
+                               off, src.p, faceWork.p, C.err.p, s);
         ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
         kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
         // rU before relaxation (main.cpp:184)
@@ -493,11 +494,9 @@ StepRep PisoSolver::step(double dt) {
                 kFillD<<<nb, kBlock, 0, s>>>(nc, psrc.p, 0.0);
                 if (c.pDiff != kOrthogonal) {
                     ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
-                    ops::assembleDiffusion(mv, 1, fieldP(), rAUf.p, gradP.p, c.pDiff, -(-1.0), pd, po, psrc.p, C.err.p, s);
-                    // diffusionCorrection (procedural_equation.cpp:77-87)
-                    ops::diffusionFlux(mv, fieldP(), rAUf.p, gradP.p, c.pDiff, corr.p, C.err.p, s);
-                    ops::diffusionFlux(mv, fieldP(), rAUf.p, gradP.p, kOrthogonal, orthF.p, C.err.p, s);
-                    kCorrection<<<blocksFor(nf), kBlock, 0, s>>>(nf, corr.p, orthF.p, corr.p);
+                    // The assembly and diffusionCorrection (procedural_equation.cpp:77-87) from one face pass.
+                    ops::assembleDiffusionCorrected(mv, fieldP(), rAUf.p, gradP.p, c.pDiff, -(-1.0), pd, po, psrc.p,
+                                                    corr.p, faceWork.p, C.err.p, s);
                 } else {
                     ops::assembleDiffusion(mv, 1, fieldP(), rAUf.p, nullptr, kOrthogonal, -(-1.0), pd, po, psrc.p, C.err.p, s);
                     kFillD<<<blocksFor(nf), kBlock, 0, s>>>(nf, corr.p, 0.0);

@@ -69,6 +69,7 @@ private:
     DBuf<double> uVal, pVal, zgVal;
     DBuf<double> U, p, phi, Uiter, HbyA, gradU, gradP, work9, rAU, rAUtr, invAP, rAUf, npr, kfMom, mu, imbalance,
         pOld, corr, full, orthF, src, psrc, tf, fluxTmp, red;
+    DBuf<double> faceWork;  // face-ordered assembly terms: 6 per interior face
     // Time levels (history.h:6-27): slot `cur` is the previous level, 1-cur the older.
     DBuf<double> Uh[2], phih[2], fluxh[2];  // fluxh: flux of the saved velocity
     int cur = 0, levels = 0;

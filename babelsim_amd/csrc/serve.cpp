@@ -30,7 +30,7 @@ long long Server::solverEstimate(int kind, int nc, int nf) {
     const long long perCell = kind == kRequestTransient ? 2200 : 2000;
     // The fixed part: two systems' reduction partials (6 x 32768 doubles
     // each), Krylov scalars and the solver's own reduction buffers.
-    return perCell * nc + 160LL * nf + 10 * kMiB;
+    return perCell * nc + 208LL * nf + 10 * kMiB;  // 48 B per face: the face-ordered assembly terms
 }
 
 Server::Server(int dev, int slots, long long budgetBytes) : device(dev), budget(budgetBytes) {

@@ -126,8 +126,9 @@ SimpleSolver::SimpleSolver(Context& ctx, const SimpleCfg& cfg, const int* ubct, 
     for (int c = 0; c < 3; ++c) { bcomp[c].alloc(nc); xcomp[c].alloc(nc); }
     for (DBuf<double>* b : {&U, &Uprev, &gradP, &gradPP, &X, &src}) b->alloc(3 * static_cast<std::size_t>(nc));
     for (DBuf<double>* b : {&gradU, &work9, &gradX}) b->alloc(9 * static_cast<std::size_t>(nc));
-    for (DBuf<double>* b : {&phi, &phiH, &rAUtr, &rAUf, &sn, &kfMom, &kfP, &corr, &full, &ctxSeed}) b->alloc(nf);
+    for (DBuf<double>* b : {&phi, &phiH, &rAUtr, &rAUf, &sn, &kfMom, &kfP, &corr, &ctxSeed}) b->alloc(nf);
     for (DBuf<double>* b : {&gtr, &Xtr, &Xf}) b->alloc(3 * static_cast<std::size_t>(nf));
+    faceWork.alloc(6 * static_cast<std::size_t>(std::max(M.nfi, 1)));
     red.alloc(kRedK * kMaxReduceBlocks + 16);
     U.zero(s); p.zero(s); phi.zero(s); corr.zero(s);
     const MeshView mv = M.view();
@@ -220,7 +221,6 @@ IterRep SimpleSolver::iterate() {
     const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
                            c.mDiff != kOrthogonal || turb;
     if (needGradU) ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s);
-    ops::assembleConvection(mv, 3, Uf, phi.p, c.mConv, c.mInterp, gradU.p, c.rho, diag, off, src.p, C.err.p, s);
     if (turb) {
         // laplacian(muEff) with the current eddy viscosity (main.cpp:53)
         const FieldArg mf{turb->muEff(), zgType.p, zgVal.p, nullptr, turb->muEffTrace()};
@@ -228,7 +228,9 @@ IterRep SimpleSolver::iterate() {
         if (c.mCoefInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 1, mf, c.mCoefGrad, gradPP.p, work9.p, C.err.p, s); gm = gradPP.p; }
         ops::interpolate(mv, 1, mf, c.mCoefInterp, gm, kfMom.p, C.err.p, s);
     }
-    ops::assembleDiffusion(mv, 3, Uf, kfMom.p, gradU.p, c.mDiff, -(-1.0), diag, off, src.p, C.err.p, s);
+    // div(phi, U) then laplacian(mu, U) (main.cpp:51-57) in one face-ordered pass.
+    ops::assembleTransport(mv, 3, Uf, phi.p, c.mConv, c.mInterp, c.rho, kfMom.p, c.mDiff, -(-1.0), gradU.p, diag, off,
+                           src.p, faceWork.p, C.err.p, s);
     ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
     kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
     if (turb) turb->addStressSource(Uf, gradU.p, c.mInterp, c.mGrad, src.p);  // main.cpp:57-59
@@ -319,11 +321,9 @@ IterRep SimpleSolver::iterate() {
         kFillD<<<nb, kBlock, 0, s>>>(nc, psrc.p, 0.0);
         if (c.pDiff != kOrthogonal) {
             ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradPP.p, work9.p, C.err.p, s);
-            ops::assembleDiffusion(mv, 1, fieldPP(), kfP.p, gradPP.p, c.pDiff, -(-1.0), pd, po, psrc.p, C.err.p, s);
-            // diffusionCorrection (procedural_equation.cpp:77-87)
-            ops::diffusionFlux(mv, fieldPP(), kfP.p, gradPP.p, c.pDiff, full.p, C.err.p, s);
-            ops::diffusionFlux(mv, fieldPP(), kfP.p, gradPP.p, kOrthogonal, sn.p, C.err.p, s);
-            kCorrection<<<blocksFor(nf), kBlock, 0, s>>>(nf, full.p, sn.p, corr.p);
+            // The assembly and diffusionCorrection (procedural_equation.cpp:77-87) from one face pass.
+            ops::assembleDiffusionCorrected(mv, fieldPP(), kfP.p, gradPP.p, c.pDiff, -(-1.0), pd, po, psrc.p, corr.p,
+                                            faceWork.p, C.err.p, s);
         } else {
             ops::assembleDiffusion(mv, 1, fieldPP(), kfP.p, nullptr, kOrthogonal, -(-1.0), pd, po, psrc.p, C.err.p, s);
             kFillD<<<blocksFor(nf), kBlock, 0, s>>>(nf, corr.p, 0.0);

@@ -65,7 +65,8 @@ private:
     DBuf<int> uType, pType, ppType, zgType;
     DBuf<double> uVal, pVal, ppVal, zgVal;
     DBuf<double> U, p, phi, phiH, Uprev, gradU, gradP, gradPP, work9, rAU, rAUtr, gtr, X, Xtr, Xf, gradX, rAUf, sn,
-        kfMom, mu, kfP, divPhiH, pPrime, corr, full, src, psrc, ctxSeed, red;
+        kfMom, mu, kfP, divPhiH, pPrime, corr, src, psrc, ctxSeed, red;
+    DBuf<double> faceWork;  // face-ordered assembly terms: 6 per interior face
     DBuf<double> bcomp[3], xcomp[3];  // momentum components, one array each
     std::unique_ptr<DeviceSystem> momSys, pSys;
     std::unique_ptr<KOmegaDevice> turb;
