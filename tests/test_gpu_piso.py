@@ -66,6 +66,11 @@ CASES = {
                                      pc.cavity_piso_config(max_iterations=3, alpha_u=0.8, tol_u=1e-3,
                                                            tol_mom=1e-3, tol_p=1e-2),
                                      cases.cavity_bcs(True), [0.01] * 3),
+    # The cylinder_re100 case's pair: momentum bcgs + bjacobi, pressure cg + hypre
+    # (cases/cylinder_re100/numerics/solution.bs:10-17).
+    "bjacobi_momentum": lambda: (cases.cavity3d_mesh(7, split=0.15, seed=3),
+                                 pc.c2_piso_config(mom=solver_config(solver=1, pc=3, rtol=1e-8, atol=1e-14)),
+                                 cases.cavity_bcs(False), [0.02, 0.01, 0.015]),
     "warm_start": lambda: (cases.cavity2d_mesh(20),
                            pc.cavity_piso_config(mom=solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14,
                                                                    warm_start=1),
