@@ -466,15 +466,21 @@ template <class T> __global__ void __launch_bounds__(kBlock) k3Jacobi(int n, CVe
         if (act[c])
             z.c[c][i] = jinv ? static_cast<T>(static_cast<double>(r.c[c][i]) * static_cast<double>(jinv[i])) : r.c[c][i];
 }
-template <class T> __global__ void __launch_bounds__(kBlock) k3BcgsP(int n, Vec3p<T> p, CVec3p<T> r, CVec3p<T> v, const double* sc3, const int* act) {
+// The BiCGSTAB direction p = r + beta (p - omega v), then the Jacobi apply
+// z = p / d (or a copy), in one pass: z is formed from the stored p.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k3BcgsPJacobi(int n, Vec3p<T> p, CVec3p<T> r, CVec3p<T> v, const double* sc3,
+                                                        const T* jinv, Vec3p<T> z, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
     for (int c = 0; c < 3; ++c) {
         if (!act[c]) continue;
         const double* sc = sc3 + c * S_COUNT;
         const double beta = sc[S_BETA];
-        p.c[c][i] = static_cast<T>(static_cast<double>(r.c[c][i]) + (-sc[S_OMEGAOLD] * beta) * static_cast<double>(v.c[c][i]) +
-                                   beta * static_cast<double>(p.c[c][i]));
+        const T pv = static_cast<T>(static_cast<double>(r.c[c][i]) + (-sc[S_OMEGAOLD] * beta) * static_cast<double>(v.c[c][i]) +
+                                    beta * static_cast<double>(p.c[c][i]));
+        p.c[c][i] = pv;
+        z.c[c][i] = jinv ? static_cast<T>(static_cast<double>(pv) * static_cast<double>(jinv[i])) : pv;
     }
 }
 template <class T>
@@ -494,14 +500,23 @@ __global__ void __launch_bounds__(kBlock) k3SpmvD1(CsrView<T> A, CVec3p<T> t, Ve
     }
     blockPartial<3>(acc, part, smem);
 }
+// s = r - alpha v (components not yet stopped), then the Jacobi apply z = s / d
+// (or a copy) for every active component, from s as it stands after the
+// update, in one pass.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k3BcgsS(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const double* sc3, const int* fl3, const int* act) {
+__global__ void __launch_bounds__(kBlock) k3BcgsSJacobi(int n, Vec3p<T> s, CVec3p<T> v, CVec3p<T> r, const double* sc3,
+                                                        const int* fl3, const T* jinv, Vec3p<T> z, const int* act) {
     const int i = globalThread();
     if (i >= n) return;
-    for (int c = 0; c < 3; ++c)
-        if (act[c] && !fl3[c])
-            s.c[c][i] = static_cast<T>(-sc3[c * S_COUNT + S_ALPHA] * static_cast<double>(v.c[c][i]) +
-                                       static_cast<double>(r.c[c][i]));
+    for (int c = 0; c < 3; ++c) {
+        if (!act[c]) continue;
+        T sv = s.c[c][i];
+        if (!fl3[c]) {
+            sv = static_cast<T>(-sc3[c * S_COUNT + S_ALPHA] * static_cast<double>(v.c[c][i]) + static_cast<double>(r.c[c][i]));
+            s.c[c][i] = sv;
+        }
+        z.c[c][i] = jinv ? static_cast<T>(static_cast<double>(sv) * static_cast<double>(jinv[i])) : sv;
+    }
 }
 // Partials: s . t of the three components at k = 0..2, t . t at k = 3..5.
 template <class T>
@@ -1952,12 +1967,10 @@ void DeviceSystem::bcgs3(Work3<T>& w, const CsrView<T>& A, const T* jv, const T*
     while (active[0] || active[1] || active[2]) {
         k3Dot<T><<<rb, kBlock, 0, stream>>>(n, cvec3(w.r), cvec3(w.rp), act, part);
         kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Rho{sc3.p, act});
-        k3BcgsP<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.p), cvec3(w.r), cvec3(w.v), sc3.p, act);
-        k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.p), jv, vec3(w.tmp), act);
+        k3BcgsPJacobi<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.p), cvec3(w.r), cvec3(w.v), sc3.p, jv, vec3(w.tmp), act);
         k3SpmvD1<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.tmp), vec3(w.v), cvec3(w.rp), act, part);
         kReduceTail<3><<<1, kTail, 0, stream>>>(rb, part, Tail3Alpha{sc3.p, fl3.p, act});
-        k3BcgsS<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.s), cvec3(w.v), cvec3(w.r), sc3.p, fl3.p, act);
-        k3Jacobi<T><<<nb, kBlock, 0, stream>>>(n, cvec3(w.s), jv, vec3(w.r), act);
+        k3BcgsSJacobi<T><<<nb, kBlock, 0, stream>>>(n, vec3(w.s), cvec3(w.v), cvec3(w.r), sc3.p, fl3.p, jv, vec3(w.r), act);
         k3SpmvD2<T><<<rb, kBlock, 0, stream>>>(A, cvec3(w.r), vec3(w.t), cvec3(w.s), fl3.p, act, part);
         kReduceTail<6><<<1, kTail, 0, stream>>>(rb, part, Tail3Omega{sc3.p, fl3.p, act});
         k3BcgsUpdate<T><<<rb, kBlock, 0, stream>>>(n, X, cvec3(w.p), cvec3(w.s), vec3(w.r), cvec3(w.t), sc3.p, fl3.p,
