@@ -487,7 +487,10 @@ struct StandaloneCsr {
         if (device < 0 || device >= count) throw std::invalid_argument("device index is out of range");
         if (n <= 0) throw std::invalid_argument("matrix is empty");
         BSIM_HIP(hipSetDevice(device));
-        BSIM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        {
+            std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
+            BSIM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        }
         std::vector<int> hp(p, p + n + 1), hc(c, c + p[n]);
         for (int k = 0; k < p[n]; ++k)
             if (hc[k] < 0 || hc[k] >= n) throw std::invalid_argument("column index out of range");
@@ -511,7 +514,10 @@ struct StandaloneCsr {
         sys.reset();
         ptr.release();
         col.release();
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) {
+            std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
+            (void)hipStreamDestroy(stream);
+        }
     }
 };
 }  // namespace

@@ -10,7 +10,10 @@ Context::Context(const MeshDesc& caller, bool renumber, int dev) : device(dev) {
     BSIM_HIP(hipGetDeviceCount(&count));
     if (dev < 0 || dev >= count) throw std::invalid_argument("device index is out of range");
     BSIM_HIP(hipSetDevice(dev));
-    BSIM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
+        BSIM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    }
     mesh = std::make_unique<DeviceMesh>(caller, renumber, stream);
     err.alloc(1);
     err.zero(stream);
@@ -26,7 +29,10 @@ Context::~Context() {
     if (stream) (void)hipStreamSynchronize(stream);
     mesh.reset();
     err.release();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+        std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
+        (void)hipStreamDestroy(stream);
+    }
 }
 
 void Context::checkErrors() {

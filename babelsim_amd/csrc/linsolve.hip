@@ -1055,6 +1055,7 @@ bool SolverCfg::operator==(const SolverCfg& o) const {
 
 // ------------------------------------------------------------ probe
 KernelProbe::~KernelProbe() {
+    std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
 }
 void KernelProbe::start(int capacity) {
@@ -1063,6 +1064,7 @@ void KernelProbe::start(int capacity) {
         // Timing-only events: no system-scope fence (cache writeback and
         // invalidate) when they complete, so they do not perturb the kernels.
         hipEvent_t e;
+        std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
         BSIM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         ev.push_back(e);
     }
@@ -1551,11 +1553,13 @@ struct EventSet {
         ev.reserve(count);
         for (int k = 0; k < count; ++k) {
             hipEvent_t e;
+            std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
             BSIM_HIP(hipEventCreate(&e));
             ev.push_back(e);
         }
     }
     ~EventSet() {
+        std::lock_guard<std::recursive_mutex> rl(pool::runtimeLock());
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
     EventSet(const EventSet&) = delete;

@@ -102,7 +102,10 @@ void trimCache(int idx) {
     }
     for (Block& b : blocks) {
         (void)hipEventSynchronize(b.done);
-        (void)hipEventDestroy(b.done);
+        {
+            std::lock_guard<std::recursive_mutex> rl(runtimeLock());
+            (void)hipEventDestroy(b.done);
+        }
         returnToRuntime(idx, b.p);
     }
 }
@@ -130,8 +133,9 @@ void release(void* p) noexcept {
             if (!g_events[ed].empty()) {
                 ev = g_events[ed].back();
                 g_events[ed].pop_back();
-            } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-                ev = nullptr;
+            } else {
+                std::lock_guard<std::recursive_mutex> rl(runtimeLock());
+                if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
             }
             if (ev && hipEventRecord(ev, t_stream) == hipSuccess) {
                 c.free[bytes].push_back({p, t_stream, ev, ed});

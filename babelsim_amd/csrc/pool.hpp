@@ -15,9 +15,9 @@
 //
 // hipFree and hipHostFree synchronise the device, which must not reach a
 // stream another thread is capturing into a graph.  runtimeLock() is held
-// around every call that takes memory from or returns it to the runtime and
-// around each graph capture, instantiation and destruction, so those never
-// overlap across threads.
+// around every call that creates or destroys a runtime object (memory,
+// streams, events, graphs) and around each graph capture and instantiation,
+// so those never overlap across threads.
 #pragma once
 
 #include <hip/hip_runtime.h>
