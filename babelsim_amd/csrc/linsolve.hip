@@ -33,36 +33,56 @@ __device__ inline double sgn(double v) { return (v > 0) - (v < 0); }
 // All lanes of a wavefront hold slots of one slice (see RowRange), so the
 // slice offsets are read with scalar loads.
 template <class T> struct PlainX {
+    static constexpr bool kPlain = true;
     const T* x;
     __device__ T operator()(int j) const { return x[j]; }
 };
+// 1.0 in device memory: the x operand of a padded slot in the pipelined row
+// product.
+template <class T> struct UnitValue {
+    static __device__ const T one;
+};
+template <class T> __device__ const T UnitValue<T>::one = T(1);
 // x + alpha xc[agg] (the kProlong update) evaluated where it is read.
 template <class T> struct ProlongedX {
+    static constexpr bool kPlain = false;
     const T* x;
     const int* agg;
     const T* xc;
     T alpha;
     __device__ T operator()(int j) const { return x[j] + alpha * xc[agg[j]]; }
 };
-template <class T, class XF> __device__ inline T rowDotG(const CsrView<T>& A, const XF& xf, int q, int i) {
+// One row of A times x: the diagonal first, then the stored entries in entry
+// order.  Sliced rows are fetched kC levels at a time so that their column,
+// value and x loads overlap.  With kAhead the column and value loads of the
+// next chunk are issued before the current chunk's x loads are consumed, so a
+// row of m chunks waits for about m + 2 memory latencies instead of 3 m; the
+// coarse AMG levels, whose Galerkin rows run to dozens of entries over few
+// rows, are bound by that chain.  Both forms add the same products in the
+// same order.
+template <class T, bool kAhead, class XF> __device__ inline T rowDotAt(const CsrView<T>& A, const XF& xf, int q, int i) {
     T y = A.diag[i] * xf(i);
-    if (A.sliceOff) {
-        // Entries are fetched kC levels at a time so that their column, value
-        // and x loads overlap; the sum still runs in entry order.
-        const int* so = A.sliceOff + static_cast<long long>(__builtin_amdgcn_readfirstlane(q >> 6)) * (A.K + 1);
-        const int lane = q & 63;
-        constexpr int kC = 8;
+    if (!A.sliceOff) {
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * xf(A.col[k]);
+        return y;
+    }
+    const int* so = A.sliceOff + static_cast<long long>(__builtin_amdgcn_readfirstlane(q >> 6)) * (A.K + 1);
+    const int lane = q & 63;
+    constexpr int kC = 8;
+    auto fetch = [&](int k0, int (&c)[kC], T (&v)[kC]) {
+#pragma unroll
+        for (int j = 0; j < kC; ++j) {
+            const int e = (k0 + j < A.K && lane < so[k0 + j + 1] - so[k0 + j]) ? so[k0 + j] + lane : -1;
+            c[j] = e >= 0 ? A.col[e] : -1;
+            v[j] = e >= 0 ? A.val[e] : T(0);
+        }
+    };
+    if constexpr (!kAhead) {
         for (int k0 = 0; k0 < A.K; k0 += kC) {
             if (so[k0 + 1] == so[k0]) break;  // no row of the slice has more entries
-            int e[kC], c[kC];
+            int c[kC];
             T v[kC], xv[kC];
-#pragma unroll
-            for (int j = 0; j < kC; ++j) e[j] = (k0 + j < A.K && lane < so[k0 + j + 1] - so[k0 + j]) ? so[k0 + j] + lane : -1;
-#pragma unroll
-            for (int j = 0; j < kC; ++j) {
-                c[j] = e[j] >= 0 ? A.col[e[j]] : -1;
-                v[j] = e[j] >= 0 ? A.val[e[j]] : T(0);
-            }
+            fetch(k0, c, v);
 #pragma unroll
             for (int j = 0; j < kC; ++j) xv[j] = c[j] >= 0 ? xf(c[j]) : T(0);
 #pragma unroll
@@ -70,12 +90,87 @@ template <class T, class XF> __device__ inline T rowDotG(const CsrView<T>& A, co
                 if (c[j] >= 0) y += v[j] * xv[j];
         }
     } else {
-        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) y += A.val[k] * xf(A.col[k]);
+        // Branch-free: a padded slot reads entry 0 and the constant 1.0 in
+        // place of x and adds (-0.0) * 1.0, and y + (-0.0) is y for every y,
+        // so each load is issued unconditionally, none is sunk into a
+        // branch, and the waits count only the loads a product needs.  A
+        // fetch keeps the raw loads and a mask of the slots in use; two
+        // register sets alternate, so nothing in flight is copied.
+        static_assert(XF::kPlain, "the pipelined row product reads x directly");
+        if (A.K == 0 || so[1] == so[0]) return y;
+        auto fetchAll = [&](int k0, int (&c)[kC], T (&v)[kC]) {
+            unsigned m = 0;
+#pragma unroll
+            for (int j = 0; j < kC; ++j) {
+                const int kk = min(k0 + j, A.K - 1);
+                const bool in = (k0 + j < A.K) & (lane < so[kk + 1] - so[kk]);
+                const int e = in ? so[kk] + lane : 0;
+                c[j] = A.col[e];
+                v[j] = A.val[e];
+                m |= static_cast<unsigned>(in) << j;
+            }
+            return m;
+        };
+        auto hasChunk = [&](int k0) {
+            const int kk = min(k0, A.K - 1);
+            return (k0 < A.K) & (so[kk + 1] != so[kk]);
+        };
+        auto loadX = [&](unsigned m, const int (&c)[kC], T (&xv)[kC]) {
+#pragma unroll
+            for (int j = 0; j < kC; ++j) xv[j] = *(((m >> j) & 1u) && c[j] >= 0 ? xf.x + c[j] : &UnitValue<T>::one);
+        };
+        // A slot inside a level's width may hold no entry (column -1) when
+        // the slice's rows are not sorted by length.
+        auto consume = [&](unsigned m, const int (&c)[kC], const T (&v)[kC], const T (&xv)[kC]) {
+#pragma unroll
+            for (int j = 0; j < kC; ++j) y = y + (((m >> j) & 1u) && c[j] >= 0 ? v[j] : T(-0.0)) * xv[j];
+        };
+        int ca[kC], cb[kC];
+        T va[kC], vb[kC], xv[kC];
+        unsigned ma = fetchAll(0, ca, va), mb;
+        for (int k0 = 0;; k0 += 2 * kC) {
+            const bool moreB = hasChunk(k0 + kC);
+            loadX(ma, ca, xv);
+            __builtin_amdgcn_sched_barrier(0);  // this chunk's x loads issue first
+            mb = fetchAll(k0 + kC, cb, vb);  // an empty mask past the row's end
+            __builtin_amdgcn_sched_barrier(0);
+            consume(ma, ca, va, xv);
+            if (!moreB) break;
+            const bool moreA = hasChunk(k0 + 2 * kC);
+            loadX(mb, cb, xv);
+            __builtin_amdgcn_sched_barrier(0);
+            ma = fetchAll(k0 + 2 * kC, ca, va);
+            __builtin_amdgcn_sched_barrier(0);
+            consume(mb, cb, vb, xv);
+            if (!moreA) break;
+        }
     }
     return y;
 }
-template <class T> __device__ inline T rowDot(const CsrView<T>& A, const T* x, int q, int i) {
-    return rowDotG<T>(A, PlainX<T>{x}, q, i);
+template <class T, class XF> __device__ inline T rowDotG(const CsrView<T>& A, const XF& xf, int q, int i) {
+    return rowDotAt<T, false>(A, xf, q, i);
+}
+template <class T, bool kAhead = false> __device__ inline T rowDot(const CsrView<T>& A, const T* x, int q, int i) {
+    return rowDotAt<T, kAhead>(A, PlainX<T>{x}, q, i);
+}
+// Sum of r over the members mi[mp[I]] .. mi[mp[I + 1] - 1] in member order;
+// the member indices are fetched kC at a time so that the loads of r overlap.
+template <class T> __device__ inline T memberSum(const int* mp, const int* mi, const T* r, int I) {
+    const int m0 = mp[I], m1 = mp[I + 1];
+    constexpr int kC = 8;
+    T s = 0;
+    for (int k0 = m0; k0 < m1; k0 += kC) {
+        int idx[kC];
+        T rv[kC];
+#pragma unroll
+        for (int j = 0; j < kC; ++j) idx[j] = k0 + j < m1 ? mi[k0 + j] : -1;
+#pragma unroll
+        for (int j = 0; j < kC; ++j) rv[j] = idx[j] >= 0 ? r[idx[j]] : T(0);
+#pragma unroll
+        for (int j = 0; j < kC; ++j)
+            if (idx[j] >= 0) s += rv[j];
+    }
+    return s;
 }
 // Krylov kernels run on matrices whose slot q holds row q.
 __device__ inline double rowDot64(const CsrView<double>& A, const double* x, int i) { return rowDot(A, x, i, i); }
@@ -756,11 +851,13 @@ template <class T> __global__ void __launch_bounds__(kBlock) kJacobiFirst(int n,
     const int i = globalThread();
     if (i < n) x[i] = static_cast<T>(omega) * b[i] / d[i];
 }
-template <class T> __global__ void __launch_bounds__(kBlock) kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
+// kAhead selects the pipelined row product (coarse levels, see rowDotAt).
+template <class T, bool kAhead = false>
+__global__ void __launch_bounds__(kBlock) kJacobiSweep(CsrView<T> A, const T* b, const T* x, T* xo, double omega) {
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
-    const T ax = rowDot(A, x, q, i);
+    const T ax = rowDot<T, kAhead>(A, x, q, i);
     xo[i] = x[i] + static_cast<T>(omega) * (b[i] - ax) / A.diag[i];
 }
 // The last fine-level sweep of a preconditioner application inside CG, with
@@ -779,18 +876,17 @@ __global__ void __launch_bounds__(kBlock) kJacobiSweepDot(CsrView<double> A, con
     double v[1] = {acc};
     blockPartial<1>(v, part, smem);
 }
-template <class T> __global__ void __launch_bounds__(kBlock) kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
+template <class T, bool kAhead = false>
+__global__ void __launch_bounds__(kBlock) kResidual(CsrView<T> A, const T* b, const T* x, T* r) {
     const int q = globalThread();
     if (q >= A.n) return;
     const int i = A.perm ? A.perm[q] : q;
-    r[i] = b[i] - rowDot(A, x, q, i);
+    r[i] = b[i] - rowDot<T, kAhead>(A, x, q, i);
 }
 template <class T> __global__ void __launch_bounds__(kBlock) kRestrict(int nc, const int* mp, const int* mi, const T* r, T* bc) {
     const int I = globalThread();
     if (I >= nc) return;
-    T s = 0;
-    for (int k = mp[I]; k < mp[I + 1]; ++k) s += r[mi[k]];
-    bc[I] = s;
+    bc[I] = memberSum(mp, mi, r, I);
 }
 template <class T> __global__ void __launch_bounds__(kBlock) kProlong(int n, const int* agg, const T* xc, T* x, double alpha) {
     const int i = globalThread();
@@ -809,8 +905,7 @@ __global__ void __launch_bounds__(kBlock) kRestrictFirst(int nc, const int* mp, 
                                double omega) {
     const int I = globalThread();
     if (I >= nc) return;
-    T s = 0;
-    for (int k = mp[I]; k < mp[I + 1]; ++k) s += r[mi[k]];
+    const T s = memberSum(mp, mi, r, I);
     bc[I] = s;
     xc[I] = static_cast<T>(omega) * s / dc[I];
 }
@@ -844,6 +939,13 @@ constexpr int kTailEntries = 8 * kTailThreads;
 // Prolongation and the first post-sweep run as one kernel on coarse levels
 // with thin rows (the fused sweep gathers agg and xc for every entry).
 constexpr int kFuseMaxRowLength = 16;
+// Launched coarse levels take the pipelined row product when their rows span
+// more than one chunk and the level is small enough that memory latency, not
+// bandwidth, sets a sweep's time; on larger levels the plain form's lower
+// register count (more waves per SIMD) wins.  Inside kCycleTail the plain
+// form measured faster on every level.
+constexpr int kAheadMinRowLength = 9;
+constexpr int kAheadMaxRows = 65536;
 template <class T> struct TailLevels {
     int count;
     CsrView<T> A[kMaxTailLevels];
@@ -890,9 +992,7 @@ template <class T> __global__ void __launch_bounds__(kTailThreads) kCycleTail(Ta
         __syncthreads();
         const int nc = L.A[k + 1].n;
         for (int I = tid; I < nc; I += kTailThreads) {
-            T s = 0;
-            for (int m = L.memPtr[k][I]; m < L.memPtr[k][I + 1]; ++m) s += L.r[k][L.memIdx[k][m]];
-            L.b[k + 1][I] = s;
+            L.b[k + 1][I] = memberSum(L.memPtr[k], L.memIdx[k], L.r[k], I);
         }
         __syncthreads();
     }
@@ -1277,9 +1377,11 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut, bool firstD
     T* cur = later % 2 == 0 ? xOut : spare;
     T* nxt = later % 2 == 0 ? spare : xOut;
     KernelProbe* pr = l == 0 ? probe : nullptr;
+    const bool ahead = l > 0 && lv.sliceOff && lv.K >= kAheadMinRowLength && lv.n <= kAheadMaxRows;
     auto sweep = [&]() {  // cur -> nxt (the probed kernel)
         if (pr) pr->begin(s);
-        kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, cur, nxt, w);
+        if (ahead) kJacobiSweep<T, true><<<nb, kBlock, 0, s>>>(A, b, cur, nxt, w);
+        else kJacobiSweep<T><<<nb, kBlock, 0, s>>>(A, b, cur, nxt, w);
         if (pr) pr->end(s);
         std::swap(cur, nxt);
     };
@@ -1292,7 +1394,8 @@ template <class T> void AmgDevice::cycle(int l, const T* b, T* xOut, bool firstD
     const bool childTail = l + 1 >= tailStart;
     T* childCur = childTail ? nullptr : (later % 2 == 0 ? cx : ct);
     const bool fuse = l >= 1 && lv.sliceOff && lv.K <= kFuseMaxRowLength;
-    kResidual<T><<<nb, kBlock, 0, s>>>(A, b, cur, r);
+    if (ahead) kResidual<T, true><<<nb, kBlock, 0, s>>>(A, b, cur, r);
+    else kResidual<T><<<nb, kBlock, 0, s>>>(A, b, cur, r);
     if (childCur)
         kRestrictFirst<T><<<blocksFor(cl.n), kBlock, 0, s>>>(cl.n, lv.memPtr.p, lv.memIdx.p, r, cb, cdg, childCur, w);
     else
