@@ -50,11 +50,23 @@ int currentDevice() {
     return d;
 }
 
+// A runtime call whose failure the cache tolerates (it then takes another
+// path) leaves no error behind for the caller's next launch check: HIP keeps
+// the last failure per thread until hipGetLastError reads it.
+hipError_t tolerated(hipError_t e) {
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
+}
+
+// Event queries, records and waits, like allocation and release, run under
+// runtimeLock(): the runtime refuses them while another thread captures a
+// stream into a graph.
 std::size_t deviceCap(int d) {
     Cache& c = g_cache[d];
     if (c.cap == 0) {
         std::size_t freeB = 0, total = 0;
-        c.cap = hipMemGetInfo(&freeB, &total) == hipSuccess ? total / 4 : std::size_t(8) << 30;
+        std::lock_guard<std::recursive_mutex> rl(runtimeLock());
+        c.cap = tolerated(hipMemGetInfo(&freeB, &total)) == hipSuccess ? total / 4 : std::size_t(8) << 30;
     }
     return c.cap;
 }
@@ -70,7 +82,10 @@ bool takeCached(int idx, std::size_t bytes, void*& out) {
     std::vector<Block>& v = it->second;
     for (std::size_t k = v.size(); k-- > 0;) {
         const bool ordered = idx != kHost && t_scoped && v[k].stream == t_stream;
-        if (!ordered && hipEventQuery(v[k].done) != hipSuccess) continue;
+        if (!ordered) {
+            std::lock_guard<std::recursive_mutex> rl(runtimeLock());
+            if (tolerated(hipEventQuery(v[k].done)) != hipSuccess) continue;
+        }
         out = v[k].p;
         g_events[v[k].eventDevice].push_back(v[k].done);
         v.erase(v.begin() + static_cast<std::ptrdiff_t>(k));
@@ -85,8 +100,8 @@ bool takeCached(int idx, std::size_t bytes, void*& out) {
 
 void returnToRuntime(int idx, void* p) {
     std::lock_guard<std::recursive_mutex> rl(runtimeLock());
-    if (idx == kHost) (void)hipHostFree(p);
-    else (void)hipFree(p);
+    if (idx == kHost) (void)tolerated(hipHostFree(p));
+    else (void)tolerated(hipFree(p));
 }
 
 void trimCache(int idx) {
@@ -101,10 +116,10 @@ void trimCache(int idx) {
         c.cached = 0;
     }
     for (Block& b : blocks) {
-        (void)hipEventSynchronize(b.done);
         {
             std::lock_guard<std::recursive_mutex> rl(runtimeLock());
-            (void)hipEventDestroy(b.done);
+            (void)tolerated(hipEventSynchronize(b.done));
+            (void)tolerated(hipEventDestroy(b.done));
         }
         returnToRuntime(idx, b.p);
     }
@@ -135,9 +150,14 @@ void release(void* p) noexcept {
                 g_events[ed].pop_back();
             } else {
                 std::lock_guard<std::recursive_mutex> rl(runtimeLock());
-                if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+                if (tolerated(hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) ev = nullptr;
             }
-            if (ev && hipEventRecord(ev, t_stream) == hipSuccess) {
+            bool recorded = false;
+            if (ev) {
+                std::lock_guard<std::recursive_mutex> rl(runtimeLock());
+                recorded = tolerated(hipEventRecord(ev, t_stream)) == hipSuccess;
+            }
+            if (recorded) {
                 c.free[bytes].push_back({p, t_stream, ev, ed});
                 c.cached += bytes;
                 cache = true;
