@@ -379,7 +379,9 @@ int bsim_system_counters(bsim_system* sys, uint64_t* o) {
 
 namespace {
 // Names of petsc_backend.cpp:30-60 (kspName, pcName, solveStatusName).
-const char* kspName(int solver) { return solver == 0 ? "cg" : solver == 1 ? "bcgs" : "unknown"; }
+const char* kspName(int solver) {
+    return solver == 0 ? "cg" : solver == 1 ? "bcgs" : solver == 2 ? "gmres" : solver == 3 ? "fgmres" : "unknown";
+}
 const char* pcName(int pc) {
     return pc == 0 ? "none" : pc == 1 ? "jacobi" : pc == 2 ? "amg" : pc == 3 ? "bjacobi" : "unknown";
 }

@@ -240,6 +240,53 @@ struct StoreValue {  // sc[idx] = sum
     __device__ void operator()(const double (&v)[1]) const { sc[idx] = v[0]; }
 };
 
+// ---- GMRES / FGMRES (gmres.c, fgmres/fgmres.c, borthog2.c) ----
+// The Krylov basis is stored as consecutive vectors of `ld` doubles.
+constexpr int kGmDots = 8;  // inner products formed per pass over the new vector
+// Partial sums of w . V[j0 + k] for k < cnt (the other slots hold zeros), each
+// in the order of kDot.
+__global__ void __launch_bounds__(kBlock) kMultiDot(int n, const double* w, const double* V, long long ld, int j0, int cnt,
+                                                    double* part) {
+    __shared__ double smem[kGmDots * kBlock / 64];
+    double acc[kGmDots];
+    for (int k = 0; k < kGmDots; ++k) acc[k] = 0.0;
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
+        const double wi = w[i];
+        for (int k = 0; k < cnt; ++k) acc[k] += wi * V[(j0 + k) * ld + i];
+    }
+    blockPartial<kGmDots>(acc, part, smem);
+}
+// Classical Gram-Schmidt update v += sum_j (-h_j) V_j (VecMAXPY with
+// lhh = -<v, V_j>, j in order) and the partial sums of the updated v . v.
+__global__ void __launch_bounds__(kBlock) kGmMaxpyNorm(int n, double* v, const double* V, long long ld, int cnt,
+                                                       const double* h, double* part) {
+    __shared__ double smem[kBlock / 64];
+    double acc = 0.0;
+    for (int i = globalThread(); i < n; i += gridDim.x * blockDim.x) {
+        double s = v[i];
+        for (int j = 0; j < cnt; ++j) s += -h[j] * V[j * ld + i];
+        v[i] = s;
+        acc += s * s;
+    }
+    double a[1] = {acc};
+    blockPartial<1>(a, part, smem);
+}
+__global__ void __launch_bounds__(kBlock) kScale(int n, double* x, double a) {  // VecScale
+    const int i = globalThread();
+    if (i < n) x[i] *= a;
+}
+struct GmCoef {
+    double c[kGmRestart];
+};
+// out = sum_j c_j V_j, j in order from a zero start (VecMAXPBY with beta 0).
+__global__ void __launch_bounds__(kBlock) kGmCombine(int n, double* out, const double* V, long long ld, int cnt, GmCoef c) {
+    const int i = globalThread();
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < cnt; ++j) s += c.c[j] * V[j * ld + i];
+    out[i] = s;
+}
+
 // ---- CG (cg.c KSPSolve_CG, unpreconditioned norm) ----
 // Stage-1 kernels return at once when a breakdown flag is set; their tails
 // check the same flag so that no stale partials are consumed.
@@ -1553,6 +1600,7 @@ DeviceSystem::~DeviceSystem() {
     if (hostScalars) pool::hostFree(hostScalars);
     if (hsc3) pool::hostFree(hsc3);
     if (hact3) pool::hostFree(hact3);
+    if (gmHost) pool::hostFree(gmHost);
 }
 
 bool DeviceSystem::update() {
@@ -1867,6 +1915,170 @@ void DeviceSystem::bcgs(const double* b, double* x, bool guessZero, int maxIt, d
     BSIM_LAUNCH_CHECK();
 }
 
+// KSPSolve_GMRES (gmres.c:88-235) and KSPSolve_FGMRES (fgmres/fgmres.c:56-244)
+// as the reference configures them: the unpreconditioned norm
+// (petsc_linear_system.cpp:181) puts GMRES on the right side, so both methods
+// precondition from the right and test the residual norm of the Arnoldi
+// recursion.  Restart kGmRestart, classical Gram-Schmidt without refinement
+// (borthog2.c), happy-breakdown tolerance 1e-30 and, for GMRES, the breakdown
+// test at a restart with tolerance 0.1.  The Hessenberg matrix, its plane
+// rotations (KSPGMRESUpdateHessenberg) and the back substitution
+// (KSPGMRESBuildSoln) are host scalars: each iteration reads back its new
+// column of inner products and the norm of the new basis vector.
+void DeviceSystem::gmres(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason,
+                         bool flexible) {
+    if (borrowed) throw std::logic_error("GMRES allocates its Krylov basis; not available in a fixed workspace");
+    constexpr int m = kGmRestart;
+    constexpr double hapTol = 1e-30, breakdownTol = 0.1;
+    const long long ld = (static_cast<long long>(n) + 31) / 32 * 32;
+    gmV.ensure(static_cast<std::size_t>((m + 1) * ld));
+    if (flexible) gmZ.ensure(static_cast<std::size_t>(m * ld));
+    gmPart.ensure(static_cast<std::size_t>(kGmDots) * kMaxReduceBlocks);
+    constexpr int ttSlot = m + kGmDots;  // past every chunk of inner products
+    gmH.ensure(ttSlot + 1);
+    if (!gmHost) gmHost = static_cast<double*>(pool::hostAlloc(sizeof(double) * (ttSlot + 1)));
+    double* const V = gmV.p;
+    auto Vj = [&](int j) { return V + j * ld; };
+    auto Zj = [&](int j) { return gmZ.p + j * ld; };
+    double* part = partials.p;
+    const int rb = reduceBlocks(n), nb = blocksFor(n);
+    // HH(i, j) at hh[i * (m + 1) + j]; rotations cc, ss; right-hand side grs.
+    std::vector<double> hh(static_cast<std::size_t>(m + 1) * (m + 1), 0.0), cc(m + 1, 0.0), ss(m + 1, 0.0),
+        grs(m + 1, 0.0), nrs(m + 1, 0.0);
+    auto H = [&](int i, int j) -> double& { return hh[static_cast<std::size_t>(i) * (m + 1) + j]; };
+    its = 0;
+    reason = 0;
+    auto test = [&](double v) {  // petsc_linear_system.cpp:164-175
+        if (!std::isfinite(v)) { reason = -9; return true; }
+        if (v <= target) { reason = 2; return true; }
+        return false;
+    };
+    double rnorm = -1.0, rnorm0 = 0.0;  // ksp->rnorm (-1: first cycle) and gmres->rnorm0
+    bool zero = guessZero;
+    while (true) {
+        // Initial residual (KSPInitialResidual / KSPFGMRESResidual) into V_0.
+        if (zero) kCopyNorm<<<rb, kBlock, 0, stream>>>(n, b, Vj(0), part);
+        else { kResidualNorm<<<rb, kBlock, 0, stream>>>(view(), b, x, Vj(0), part); ++counters.spmvs; }
+        kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, StoreSqrt{sc.p, S_DP});
+        BSIM_LAUNCH_CHECK();
+        double res = readScalar(S_DP);
+        zero = false;
+        if (!std::isfinite(res)) { reason = -9; return; }  // KSPCheckNorm
+        if (!flexible) {
+            if (res != 0.0) kScale<<<nb, kBlock, 0, stream>>>(n, Vj(0), 1.0 / res);  // VecNormalize
+            if (rnorm > 0.0 && std::fabs(res - rnorm) > breakdownTol * rnorm0) { reason = -5; return; }
+        }
+        grs[0] = rnorm0 = res;
+        rnorm = res;
+        hist.push_back(res);
+        if (!flexible && res == 0.0) { reason = 3; return; }
+        test(res);
+        int it = 0;
+        bool hapend = false;
+        if (!reason && flexible) kScale<<<nb, kBlock, 0, stream>>>(n, Vj(0), 1.0 / res);
+        bool built = true;  // false when a norm check returns from the cycle
+        while (!reason && it < m && its < maxIt) {
+            double* w = Vj(it + 1);
+            if (flexible) {
+                applyPC(Vj(it), Zj(it));
+                kSpmvPlain<<<nb, kBlock, 0, stream>>>(view(), Zj(it), w);
+            } else {
+                applyPC(Vj(it), tmp.p);
+                kSpmvPlain<<<nb, kBlock, 0, stream>>>(view(), tmp.p, w);
+            }
+            ++counters.spmvs;
+            for (int j0 = 0; j0 <= it; j0 += kGmDots) {
+                const int cnt = std::min(kGmDots, it + 1 - j0);
+                kMultiDot<<<rb, kBlock, 0, stream>>>(n, w, V, ld, j0, cnt, gmPart.p);
+                kReduceTail<kGmDots><<<1, kTail, 0, stream>>>(rb, gmPart.p, StoreSums<kGmDots>{gmH.p + j0});
+            }
+            kGmMaxpyNorm<<<rb, kBlock, 0, stream>>>(n, w, V, ld, it + 1, gmH.p, part);
+            kReduceTail<1><<<1, kTail, 0, stream>>>(rb, part, StoreSqrt{gmH.p, ttSlot});
+            BSIM_LAUNCH_CHECK();
+            BSIM_HIP(hipMemcpyAsync(gmHost, gmH.p, sizeof(double) * (ttSlot + 1), hipMemcpyDeviceToHost, stream));
+            BSIM_HIP(hipStreamSynchronize(stream));
+            bool dotFailed = false;  // KSPCheckDot: the orthogonalisation stops
+            for (int j = 0; j <= it; ++j)
+                if (!std::isfinite(gmHost[j])) dotFailed = true;
+            if (dotFailed) { reason = -9; break; }
+            for (int j = 0; j <= it; ++j) H(j, it) = 0.0 - (-gmHost[j]);
+            const double tt = gmHost[ttSlot];
+            if (!std::isfinite(tt)) { reason = -9; built = false; break; }  // KSPCheckNorm returns from the cycle
+            H(it + 1, it) = tt;
+            const double hapbnd = std::min(std::fabs(tt / grs[it]), hapTol);
+            if (flexible) {
+                if (tt > hapbnd) kScale<<<nb, kBlock, 0, stream>>>(n, w, 1.0 / tt);
+                else hapend = true;
+            } else {
+                if (tt != 0.0) kScale<<<nb, kBlock, 0, stream>>>(n, w, 1.0 / tt);
+                if (tt < hapbnd) hapend = true;
+            }
+            // KSPGMRESUpdateHessenberg: earlier rotations, then a new one.
+            bool nullDir = false;
+            for (int j = 1; j <= it; ++j) {
+                const double t0 = H(j - 1, it);
+                H(j - 1, it) = cc[j - 1] * t0 + ss[j - 1] * H(j, it);
+                H(j, it) = cc[j - 1] * H(j, it) - ss[j - 1] * t0;
+            }
+            if (!hapend) {
+                const double a = H(it, it), c = H(it + 1, it);
+                const double r = std::sqrt(a * a + c * c);
+                if (r == 0.0) {
+                    nullDir = true;
+                } else {
+                    cc[it] = a / r;
+                    ss[it] = c / r;
+                    grs[it + 1] = -(ss[it] * grs[it]);
+                    grs[it] = cc[it] * grs[it];
+                    H(it, it) = cc[it] * a + ss[it] * c;
+                    res = std::fabs(grs[it + 1]);
+                }
+            } else {
+                res = 0.0;
+            }
+            if (nullDir) {
+                reason = -2;  // KSP_DIVERGED_NULL
+                if (flexible) break;
+            }
+            ++it;
+            ++its;
+            rnorm = res;
+            if (reason) break;
+            hist.push_back(res);
+            test(res);
+            if (hapend && !reason) { reason = -5; break; }
+        }
+        if (built && it > 0) {
+            // KSPGMRESBuildSoln / KSPFGMRESBuildSoln on columns 0 .. it-1.
+            const int k1 = it - 1;
+            bool solved = true;
+            if (H(k1, k1) != 0.0) nrs[k1] = grs[k1] / H(k1, k1);
+            else if (flexible) nrs[k1] = 0.0;
+            else { reason = -5; solved = false; }
+            for (int k = k1 - 1; solved && k >= 0; --k) {
+                double t0 = grs[k];
+                for (int j = k + 1; j <= k1; ++j) t0 = t0 - H(k, j) * nrs[j];
+                if (!flexible && H(k, k) == 0.0) { reason = -5; solved = false; break; }
+                nrs[k] = t0 / H(k, k);
+            }
+            if (solved) {
+                GmCoef c{};
+                for (int j = 0; j <= k1; ++j) c.c[j] = nrs[j];
+                double* temp = flexible ? tmp.p : this->w.p;
+                kGmCombine<<<nb, kBlock, 0, stream>>>(n, temp, flexible ? gmZ.p : V, ld, it, c);
+                if (!flexible) {
+                    applyPC(temp, tmp.p);  // KSPUnwindPreconditioner
+                    temp = tmp.p;
+                }
+                kAxpy<<<nb, kBlock, 0, stream>>>(n, 1.0, temp, x);
+                BSIM_LAUNCH_CHECK();
+            }
+        }
+        if (!built) return;
+        if (reason == 0 && its >= maxIt) reason = -3;
+        if (reason) return;
+    }
+}
 
 SolveRes DeviceSystem::solve(const double* b, double* x, int rhsIndex) {
     if (ready && configured && !setupPending && cfg.solver == 1 && cfg.precision == 1 && cfg.pc != 2) {
@@ -1939,7 +2151,8 @@ SolveRes DeviceSystem::solveFrom(const double* b, double* x, int rhsIndex, const
             const auto tKsp = Clock::now();
             if (given) { its = first->its; reason = first->reason; }
             else if (cfg.solver == 0) cg(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
-            else bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            else if (cfg.solver == 1) bcgs(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason);
+            else gmres(b, x, guessZero, cfg.maxIter - attemptIts, internalTarget, its, reason, cfg.solver == 3);
             if (!given) counters.solveSeconds += since(tKsp);
             attemptIts += its;
             total += its;

@@ -13,8 +13,13 @@
 
 namespace bsim {
 
+// Restart length of GMRES and FGMRES (PETSc's default for KSPGMRES/KSPFGMRES).
+constexpr int kGmRestart = 30;
+
 struct SolverCfg {
-    int solver = 1, pc = 2;  // pc: 0 none, 1 Jacobi, 2 aggregation AMG, 3 block Jacobi (ILU(0) per 64 rows)
+    // solver: 0 CG, 1 BiCGSTAB, 2 GMRES, 3 FGMRES (LinearSolverType order);
+    // pc: 0 none, 1 Jacobi, 2 aggregation AMG, 3 block Jacobi (ILU(0) per 64 rows)
+    int solver = 1, pc = 2;
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000, warmStart = 0, rebuildInterval = 8;
     double reuseGrowth = 2.0;
@@ -152,6 +157,8 @@ private:
     double trueResidual(const double* b, const double* x, bool check = true);
     void cg(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
     void bcgs(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason);
+    void gmres(const double* b, double* x, bool guessZero, int maxIt, double target, int& its, int& reason,
+               bool flexible);
     double readScalar(int idx, int* flags = nullptr);
     // The prelude and first Krylov call of one component, already executed.
     struct FirstRun {
@@ -197,6 +204,8 @@ public:
     KernelProbe probe;  // times the fine-level AMG smoother sweeps
 private:
     DBuf<double> r, z, p, w, rp, v, t, s, tmp, guess, saved;
+    DBuf<double> gmV, gmZ, gmPart, gmH;  // GMRES basis, FGMRES preconditioned basis, partials, inner products
+    double* gmHost = nullptr;            // pinned mirror of gmH
     DBuf<float> diag32, val32;  // FP32 copy of the effective matrix (mixed precision)
     DBuf<double> sc;            // device scalars
     DBuf<int> flags;

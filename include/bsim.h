@@ -84,7 +84,9 @@ enum { BSIM_LINEAR = 0, BSIM_CORRECTED = 1 };                 /* methods.h:10-13
 enum { BSIM_GREEN_GAUSS = 0, BSIM_LEAST_SQUARES = 1 };        /* methods.h:15-18 */
 enum { BSIM_UPWIND = 0, BSIM_LINEAR_UPWIND = 1, BSIM_CENTRAL = 2 };  /* methods.h:20-24 */
 enum { BSIM_ORTHOGONAL = 0, BSIM_DIFF_CORRECTED = 1, BSIM_LIMITED_CORRECTED = 2 };  /* methods.h:26-30 */
-enum { BSIM_CG = 0, BSIM_BICGSTAB = 1 };                      /* solver_control.h:14-19 */
+/* solver_control.h:13-18.  GMRES and FGMRES restart every 30 iterations
+ * (PETSc's default) and precondition from the right. */
+enum { BSIM_CG = 0, BSIM_BICGSTAB = 1, BSIM_GMRES = 2, BSIM_FGMRES = 3 };
 /* solver_control.h:21-29: hypre/gamg -> AMG; bjacobi -> ILU(0) in blocks of
  * 64 consecutive rows (PCBJACOBI with ceil(n/64) blocks, FP64 only). */
 enum { BSIM_PC_NONE = 0, BSIM_PC_JACOBI = 1, BSIM_PC_AMG = 2, BSIM_PC_BJACOBI = 3 };

@@ -189,7 +189,7 @@ struct Csr {  // diagonal stored separately; off-diagonal entries one per interi
 };
 
 struct SolverConfig {
-    int solver = 1;          // 0 CG, 1 BiCGSTAB
+    int solver = 1;          // 0 CG, 1 BiCGSTAB, 2 GMRES, 3 FGMRES (restart 30, right preconditioning)
     int pc = 2;              // 0 none, 1 Jacobi, 2 AMG, 3 block Jacobi (ILU(0) per 64 rows)
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000;

@@ -601,6 +601,142 @@ struct Ksp {
         if (i >= maxIt) reason = -3;
         finish();
     }
+    // KSPSolve_GMRES (gmres.c:88-235, KSPGMRESUpdateHessenberg, KSPGMRESBuildSoln)
+    // and KSPSolve_FGMRES (fgmres/fgmres.c:56-300) as the reference sets them
+    // up: KSP_NORM_UNPRECONDITIONED (petsc_linear_system.cpp:181) makes PETSc
+    // precondition GMRES from the right, so both test the Arnoldi estimate of
+    // ||b - A x||.  Restart 30, classical Gram-Schmidt without refinement
+    // (borthog2.c), happy-breakdown tolerance 1e-30, and for GMRES the restart
+    // breakdown test with tolerance 0.1.  Vector arithmetic in a fixed order:
+    // the Gram-Schmidt update v += (-h_j) V_j and the combinations sum_j c_j V_j
+    // take j in order from the left.
+    void gmres(const std::vector<double>& b, std::vector<double>& x, bool guessZero, int maxIt, bool flexible) {
+        const Index n = A.n;
+        constexpr int m = 30;
+        constexpr double hapTol = 1e-30, breakdownTol = 0.1;
+        std::vector<std::vector<double>> V(m + 1, std::vector<double>(n)), Z(flexible ? m : 0, std::vector<double>(n));
+        std::vector<double> tmp(n), temp(n), h(m + 1);
+        std::vector<double> hh(static_cast<std::size_t>(m + 1) * (m + 1), 0.0), cc(m + 1, 0.0), ss(m + 1, 0.0),
+            grs(m + 1, 0.0), nrs(m + 1, 0.0);
+        auto H = [&](int i, int j) -> double& { return hh[static_cast<std::size_t>(i) * (m + 1) + j]; };
+        auto scale = [&](std::vector<double>& v, double a) {
+#pragma omp parallel for schedule(static)
+            for (Index i = 0; i < n; ++i) v[i] *= a;
+        };
+        its = 0;
+        reason = 0;
+        double rnorm = -1.0, rnorm0 = 0.0;
+        bool zero = guessZero;
+        while (true) {
+            if (zero) V[0] = b;
+            else { spmv(A, x.data(), V[0].data()); for (Index i = 0; i < n; ++i) V[0][i] = b[i] - V[0][i]; }
+            double res = nrm(V[0]);
+            zero = false;
+            if (!std::isfinite(res)) { reason = -9; return; }
+            if (!flexible) {
+                if (res != 0.0) scale(V[0], 1.0 / res);
+                if (rnorm > 0.0 && std::fabs(res - rnorm) > breakdownTol * rnorm0) { reason = -5; return; }
+            }
+            grs[0] = rnorm0 = res;
+            rnorm = res;
+            if (hist) hist->push_back(res);
+            if (!flexible && res == 0.0) { reason = 3; return; }
+            test(res);
+            int it = 0;
+            bool hapend = false, built = true;
+            if (!reason && flexible) scale(V[0], 1.0 / res);
+            while (!reason && it < m && its < maxIt) {
+                std::vector<double>& w = V[it + 1];
+                if (flexible) { applyPC(V[it], Z[it]); spmv(A, Z[it].data(), w.data()); }
+                else { applyPC(V[it], tmp); spmv(A, tmp.data(), w.data()); }
+                bool dotFailed = false;
+                for (int j = 0; j <= it; ++j) {
+                    h[j] = dotv(w, V[j]);
+                    if (!std::isfinite(h[j])) dotFailed = true;
+                }
+#pragma omp parallel for schedule(static)
+                for (Index i = 0; i < n; ++i) {
+                    double sum = w[i];
+                    for (int j = 0; j <= it; ++j) sum += -h[j] * V[j][i];
+                    w[i] = sum;
+                }
+                const double tt = nrm(w);
+                if (dotFailed) { reason = -9; break; }
+                for (int j = 0; j <= it; ++j) H(j, it) = 0.0 - (-h[j]);
+                if (!std::isfinite(tt)) { reason = -9; built = false; break; }
+                H(it + 1, it) = tt;
+                const double hapbnd = std::min(std::fabs(tt / grs[it]), hapTol);
+                if (flexible) {
+                    if (tt > hapbnd) scale(w, 1.0 / tt);
+                    else hapend = true;
+                } else {
+                    if (tt != 0.0) scale(w, 1.0 / tt);
+                    if (tt < hapbnd) hapend = true;
+                }
+                bool nullDir = false;
+                for (int j = 1; j <= it; ++j) {
+                    const double t0 = H(j - 1, it);
+                    H(j - 1, it) = cc[j - 1] * t0 + ss[j - 1] * H(j, it);
+                    H(j, it) = cc[j - 1] * H(j, it) - ss[j - 1] * t0;
+                }
+                if (!hapend) {
+                    const double a = H(it, it), c = H(it + 1, it);
+                    const double r = std::sqrt(a * a + c * c);
+                    if (r == 0.0) {
+                        nullDir = true;
+                    } else {
+                        cc[it] = a / r;
+                        ss[it] = c / r;
+                        grs[it + 1] = -(ss[it] * grs[it]);
+                        grs[it] = cc[it] * grs[it];
+                        H(it, it) = cc[it] * a + ss[it] * c;
+                        res = std::fabs(grs[it + 1]);
+                    }
+                } else {
+                    res = 0.0;
+                }
+                if (nullDir) {
+                    reason = -2;
+                    if (flexible) break;
+                }
+                ++it;
+                ++its;
+                rnorm = res;
+                if (reason) break;
+                if (hist) hist->push_back(res);
+                test(res);
+                if (hapend && !reason) { reason = -5; break; }
+            }
+            if (built && it > 0) {
+                const int k1 = it - 1;
+                bool solved = true;
+                if (H(k1, k1) != 0.0) nrs[k1] = grs[k1] / H(k1, k1);
+                else if (flexible) nrs[k1] = 0.0;
+                else { reason = -5; solved = false; }
+                for (int k = k1 - 1; solved && k >= 0; --k) {
+                    double t0 = grs[k];
+                    for (int j = k + 1; j <= k1; ++j) t0 = t0 - H(k, j) * nrs[j];
+                    if (!flexible && H(k, k) == 0.0) { reason = -5; solved = false; break; }
+                    nrs[k] = t0 / H(k, k);
+                }
+                if (solved) {
+                    const std::vector<std::vector<double>>& B = flexible ? Z : V;
+#pragma omp parallel for schedule(static)
+                    for (Index i = 0; i < n; ++i) {
+                        double sum = 0.0;
+                        for (int j = 0; j < it; ++j) sum += nrs[j] * B[j][i];
+                        temp[i] = sum;
+                    }
+                    if (!flexible) { applyPC(temp, tmp); temp.swap(tmp); }  // KSPUnwindPreconditioner
+#pragma omp parallel for schedule(static)
+                    for (Index i = 0; i < n; ++i) x[i] += 1.0 * temp[i];
+                }
+            }
+            if (!built) return;
+            if (reason == 0 && its >= maxIt) reason = -3;
+            if (reason) return;
+        }
+    }
 };
 
 }  // namespace
@@ -646,7 +782,8 @@ SolveResult LinearSystem::solve(const std::vector<double>& b, std::vector<double
         for (int restart = 0; restart < 3; ++restart) {
             Ksp k{A, cfg, amg, internalTarget, &hist};
             if (cfg.solver == 0) k.cg(b, x, guessZero, cfg.maxIter - attemptIts);
-            else k.bcgs(b, x, guessZero, cfg.maxIter - attemptIts);
+            else if (cfg.solver == 1) k.bcgs(b, x, guessZero, cfg.maxIter - attemptIts);
+            else k.gmres(b, x, guessZero, cfg.maxIter - attemptIts, cfg.solver == 3);
             attemptIts += k.its; total += k.its; reason = k.reason;
             fin = trueResidual();
             ++trueResidualChecks;

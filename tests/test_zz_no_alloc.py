@@ -88,6 +88,7 @@ def test_noalloc_rejects_what_it_cannot_hold():
     assert call(len(D), solver_config(solver=0, pc=2)) == 2
     assert call(len(D), solver_config(solver=1, pc=3)) == 2  # block Jacobi keeps factors: not in the fixed scratch
     assert call(len(D), solver_config(solver=1, pc=1, precision=1)) == 2
+    assert call(len(D), solver_config(solver=2, pc=1)) == 2  # the GMRES basis does not fit the fixed scratch
     # A column outside [0, n), found by the device check (every read in bounds).
     bad = Cc.copy()
     bad[5] = len(D) + 7
