@@ -114,7 +114,15 @@ class PisoConfig(C.Structure):
                 ("max_correctors", C.c_int), ("non_orth_corr", C.c_int),
                 ("alpha_u", C.c_double), ("coupling_tol", C.c_double), ("tol_mass", C.c_double),
                 ("tol_u", C.c_double), ("tol_mom", C.c_double), ("tol_p", C.c_double),
-                ("mom", SolverConfig), ("pre", SolverConfig)]
+                ("mom", SolverConfig), ("pre", SolverConfig),
+                ("initialize_potential_flow", C.c_int), ("initial_non_orth_corr", C.c_int)]
+
+
+class PisoInitial(C.Structure):
+    """The initial potential-flow projection's record (main.cpp:147-149) and outcome."""
+
+    _fields_ = [("initial_mass", C.c_double), ("initial_u_max", C.c_double),
+                ("initial_pressure_solves", C.c_int), ("p_iters", C.c_int), ("done", C.c_int), ("status", C.c_int)]
 
 
 class StepReport(C.Structure):
@@ -186,7 +194,8 @@ def piso_config(**kw) -> PisoConfig:
     base = dict(rho=1.0, mu=0.01, m_interp=0, m_grad=0, m_conv=1, m_diff=0, m_coef_interp=0, m_coef_grad=0,
                 p_interp=0, p_grad=0, p_diff=0, p_coef_interp=0, p_coef_grad=0, time_method=BDF2,
                 max_iterations=1, n_correctors=2, max_correctors=0, non_orth_corr=1, alpha_u=1.0,
-                coupling_tol=np.finfo(np.float64).max, tol_mass=1e-8, tol_u=1e-7, tol_mom=1e-6, tol_p=1e-6)
+                coupling_tol=np.finfo(np.float64).max, tol_mass=1e-8, tol_u=1e-7, tol_mom=1e-6, tol_p=1e-6,
+                initialize_potential_flow=0, initial_non_orth_corr=-1)
     mom = kw.pop("mom", None) or solver_config(solver=1, pc=1, rtol=1e-8, atol=1e-14)
     pre = kw.pop("pre", None) or solver_config(solver=0, pc=2, rtol=1e-8, atol=1e-14)
     base.update(kw)
@@ -346,6 +355,7 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_piso_create": [vp, C.POINTER(PisoConfig), ip, dp, ip, dp, C.POINTER(vp)],
     "bsim_piso_set_fields": [vp, dp, dp],
     "bsim_piso_start": [vp],
+    "bsim_piso_initial_report": [vp, C.POINTER(PisoInitial)],
     "bsim_piso_step": [vp, C.c_double, C.POINTER(StepReport)],
     "bsim_piso_get_fields": [vp, dp, dp, dp],
     "bsim_piso_temporal_flux": [vp, dp],

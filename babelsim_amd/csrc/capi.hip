@@ -97,6 +97,8 @@ PisoCfg toPisoCfg(const bsim_piso_config& cfg) {
     c.tolMass = cfg.tol_mass; c.tolU = cfg.tol_u; c.tolMom = cfg.tol_mom; c.tolP = cfg.tol_p;
     c.mom = toSolverCfg(&cfg.mom);
     c.pre = toSolverCfg(&cfg.pre);
+    c.potentialInit = cfg.initialize_potential_flow;
+    c.initialNonOrthCorr = cfg.initial_non_orth_corr;
     return c;
 }
 
@@ -714,6 +716,13 @@ int bsim_piso_set_fields(bsim_piso* s, const double* U, const double* p) {
     return guarded([&] { s->s->setFields(U, p); });
 }
 int bsim_piso_start(bsim_piso* s) { return guarded([&] { s->s->start(); }); }
+int bsim_piso_initial_report(bsim_piso* s, bsim_piso_initial* o) {
+    return guarded([&] {
+        if (!s || !o) throw std::invalid_argument("null argument");
+        const InitRep& r = s->s->initial;
+        *o = bsim_piso_initial{r.mass, r.uMax, r.solves, r.pIters, r.done, r.status};
+    });
+}
 int bsim_piso_step(bsim_piso* s, double dt, bsim_step_report* o) {
     return guarded([&] {
         if (!o) throw std::invalid_argument("null report");

@@ -212,6 +212,35 @@ __global__ void __launch_bounds__(kTail) kDiagnosticsTail(int nb, const double* 
     }
 }
 
+// U -= g (Field operator-=, component by component).
+__global__ void __launch_bounds__(kBlock) kSubtractVec(int nc, const double* g, double* U) {
+    const int i = globalThread();
+    if (i >= nc) return;
+    for (int k = 0; k < 3; ++k) {
+        const long long j = 3 * static_cast<long long>(i) + k;
+        U[j] = U[j] - g[j];
+    }
+}
+
+// Per-block maxima of |U| (math::max(map(U, norm))), then their maximum.
+__global__ void __launch_bounds__(kBlock) kSpeedMax(int nc, const double* U, double* part) {
+    __shared__ double smem[kBlock / 64];
+    double mx = 0.0;
+    for (int c = globalThread(); c < nc; c += gridDim.x * blockDim.x) {
+        const D3 u = ld3(U, c);
+        mx = fmax(mx, sqrt(dot(u, u)));
+    }
+    mx = blockMaxValue(mx, smem);
+    if (threadIdx.x == 0) part[xcdBlock()] = mx;
+}
+__global__ void __launch_bounds__(kTail) kMaxTail(int nb, const double* part, double* out) {
+    __shared__ double smem[kTail / 64];
+    double mx = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kTail) mx = fmax(mx, part[b]);
+    mx = blockMaxValue<kTail>(mx, smem);
+    if (threadIdx.x == 0) out[0] = mx;
+}
+
 }  // namespace
 
 PisoSolver::PisoSolver(Context& ctx, const PisoCfg& cfg, const int* ubct, const double* ubcv, const int* pbct,
@@ -231,15 +260,21 @@ PisoSolver::PisoSolver(Context& ctx, const PisoCfg& cfg, const int* ubct, const 
         throw std::invalid_argument("maxCorrectors must lie in [nCorrectors, 100]");
     if (!(c.couplingTol > 0)) throw std::invalid_argument("couplingTolerance must be positive");
     if (!(c.rho > 0) || !(c.mu > 0)) throw std::invalid_argument("density and viscosity must be positive");
-    std::vector<int> zg(np, kZeroGradient);
-    std::vector<double> zgv(3 * np, 0.0);
+    if (c.potentialInit != 0 && c.potentialInit != 1) throw std::invalid_argument("initializePotentialFlow must be 0 or 1");
+    if (c.initialNonOrthCorr < 0) c.initialNonOrthCorr = c.nonOrthCorr;
+    if (c.initialNonOrthCorr > 20) throw std::invalid_argument("PISO iteration counts out of range");
+    std::vector<int> zg(np, kZeroGradient), ppt(np);
+    std::vector<double> zgv(3 * np, 0.0), ppv(np, 0.0);
     for (int k = 0; k < np; ++k) {
         if (ubct[k] < 0 || ubct[k] > 4 || pbct[k] < 0 || pbct[k] > 4) throw std::invalid_argument("invalid boundary type");
         inletOutlet = inletOutlet || ubct[k] == kInletOutlet;
+        // homogeneousLike(p) (field.h:382-409): the potential's boundary conditions
+        ppt[k] = pbct[k] == kFixedValue ? kFixedValue : pbct[k] == kSymmetry ? kSymmetry : kZeroGradient;
     }
     uType.upload(ubct, np, s); uVal.upload(ubcv, 3 * np, s);
     pType.upload(pbct, np, s); pVal.upload(pbcv, np, s);
     zgType.upload(zg, s); zgVal.upload(zgv, s);
+    ppType.upload(ppt, s); ppVal.upload(ppv, s);
     for (DBuf<double>* b : {&p, &rAU, &invAP, &imbalance, &pOld, &psrc, &mu}) b->alloc(nc);
     for (int k = 0; k < 3; ++k) { bcomp[k].alloc(nc); xcomp[k].alloc(nc); }
     for (DBuf<double>* b : {&U, &Uiter, &HbyA, &gradP, &npr, &src, &Uh[0], &Uh[1]})
@@ -300,6 +335,8 @@ void PisoSolver::start() {
     const double* g = nullptr;
     if (c.pInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, fieldU(fluxTmp.p), c.pGrad, gradU.p, work9.p, C.err.p, s); g = gradU.p; }
     ops::flux(mv, fieldU(fluxTmp.p), g, phi.p, C.err.p, s);
+    initial = InitRep{};
+    if (c.potentialInit) initializePotentialFlow();
     const long long n3 = 3LL * M.nc;
     for (int k = 0; k < 2; ++k) {
         kCopyD<<<blocksFor(n3), kBlock, 0, s>>>(n3, U.p, Uh[k].p);
@@ -313,6 +350,78 @@ void PisoSolver::start() {
     dtCur = dtPrev = 0;
     started = true;
     stepped = false;
+}
+
+// initializePotentialFlow (main.cpp:127-150): a conservative initial face flux
+// before the first time history.  The potential equation is the total-pressure
+// equation of the correctors with diffusivity 1 on every face and the
+// divergence of the initial flux as its imbalance; it has its own persistent
+// linear system (another numerical identity, like potentialEquation), which
+// lives only for the projection.  Then U -= grad(potential) and phi +=
+// faceFlux(potential equation, potential); U's flux context is phi as in
+// every step.  A solve that does not converge ends the run as a numerical
+// failure (main.cpp:139), which step() then reports.
+void PisoSolver::initializePotentialFlow() {
+    const DeviceMesh& M = *C.mesh;
+    const MeshView mv = M.view();
+    hipStream_t s = C.stream;
+    const int nc = M.nc, nf = M.nf, nb = blocksFor(nc);
+    const int solves = c.pDiff == kOrthogonal ? 1 : c.initialNonOrthCorr + 1;
+    pPrime.ensure(nc);
+    kFillD<<<nb, kBlock, 0, s>>>(nc, pPrime.p, 0.0);
+    kFillD<<<blocksFor(nf), kBlock, 0, s>>>(nf, full.p, 1.0);  // the constant diffusivity, every face
+    const double* ones = full.p;
+    ops::divergenceFace(mv, phi.p, imbalance.p, s);
+    DeviceSystem pot(s, meshPattern(M));
+    initial.solves = solves;
+    for (int k = 0; k < solves; ++k) {
+        double* pd = pot.candDiag();
+        double* po = pot.candVal();
+        kFillD<<<nb, kBlock, 0, s>>>(nc, pd, 0.0);
+        kFillD<<<blocksFor(std::max(M.nnzSliced, 1)), kBlock, 0, s>>>(M.nnzSliced, po, 0.0);
+        kFillD<<<nb, kBlock, 0, s>>>(nc, psrc.p, 0.0);
+        if (c.pDiff != kOrthogonal) {
+            ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
+            ops::assembleDiffusionCorrected(mv, fieldPP(), ones, gradP.p, c.pDiff, -(-1.0), pd, po, psrc.p, corr.p,
+                                            faceWork.p, C.err.p, s);
+        } else {
+            ops::assembleDiffusion(mv, 1, fieldPP(), ones, nullptr, kOrthogonal, -(-1.0), pd, po, psrc.p, C.err.p, s);
+            kFillD<<<blocksFor(nf), kBlock, 0, s>>>(nf, corr.p, 0.0);
+        }
+        kScalarSource<<<nb, kBlock, 0, s>>>(mv, imbalance.p, psrc.p);
+        int* notConst = reinterpret_cast<int*>(red.p + kSlotFlag);
+        kFillD<<<1, 64, 0, s>>>(1, red.p + kSlotFlag, 0.0);
+        kReferenceTest<<<nb, kBlock, 0, s>>>(mv, pd, po, notConst);
+        kReferenceApply<<<1, 64, 0, s>>>(pd, psrc.p, refCell, 0.0, notConst, C.err.p);
+        BSIM_LAUNCH_CHECK();
+        pot.update();
+        pot.prepare(c.pre);
+        const SolveRes r = pot.solve(psrc.p, pPrime.p, 0);
+        C.checkErrors();
+        initial.pIters += r.iterations;
+        if (r.status != kConverged) {
+            initial.status = kStepNumericalFailure;
+            initial.done = 1;
+            return;
+        }
+    }
+    ops::gradient(mv, M.orthogonal, 1, fieldPP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
+    kSubtractVec<<<nb, kBlock, 0, s>>>(nc, gradP.p, U.p);
+    kPressureFluxPhi<<<blocksFor(nf), kBlock, 0, s>>>(mv, fieldPP(), ones, corr.p, phi.p, phi.p, C.err.p);
+    // initialMass (fluxBalance(phi).relative) and initialUmax.
+    double* part = red.p + kPartOffset;
+    kStepDiagnostics<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, U.p, phi.p, 0.0, part);
+    kDiagnosticsTail<<<1, kTail, 0, s>>>(reduceBlocks(nc), part, red.p + kSlotDiag);
+    kSpeedMax<<<reduceBlocks(nc), kBlock, 0, s>>>(nc, U.p, part);
+    kMaxTail<<<1, kTail, 0, s>>>(reduceBlocks(nc), part, red.p + kSlotDiag + kDiagK + 1);
+    BSIM_LAUNCH_CHECK();
+    double h[kDiagK + 2];
+    BSIM_HIP(hipMemcpyAsync(h, red.p + kSlotDiag, sizeof(h), hipMemcpyDeviceToHost, s));
+    ++run.reductions;
+    C.checkErrors();
+    initial.mass = h[2] / std::max(h[3], 1e-30);
+    initial.uMax = h[kDiagK + 1];
+    initial.done = 1;
 }
 
 // temporalFluxCorrection (main.cpp:19-53).  flux(U_prev) also becomes the
@@ -358,6 +467,9 @@ StepRep PisoSolver::step(double dt) {
     } record{run};
     if (!(dt > 0) || !std::isfinite(dt)) throw std::invalid_argument("history requires positive finite dt");
     if (!started) start();
+    StepRep rep;
+    rep.dt = dt;
+    if (initial.status != kStepCompleted) return rep;  // runPiso ended in the initial projection
     const DeviceMesh& M = *C.mesh;
     const MeshView mv = M.view();
     hipStream_t s = C.stream;
@@ -365,8 +477,6 @@ StepRep PisoSolver::step(double dt) {
     const int nb = blocksFor(nc);
     const long long n3 = 3LL * nc;
     double* part = red.p + kPartOffset;
-    StepRep rep;
-    rep.dt = dt;
     auto t0 = Clock::now();
     // History saves (main.cpp:163-164; history.h:8-13): the previous level
     // becomes the older one and U, phi the previous one.

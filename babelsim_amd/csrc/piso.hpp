@@ -24,6 +24,16 @@ struct PisoCfg {
     double alphaU = 1.0, couplingTol = std::numeric_limits<double>::max();
     double tolMass = 1e-8, tolU = 1e-7, tolMom = 1e-6, tolP = 1e-6;
     SolverCfg mom, pre;
+    // initializePotentialFlow and initialNonOrthogonalCorrections
+    // (main.cpp:104-105, 127-150); a negative count takes nonOrthCorr.
+    int potentialInit = 0, initialNonOrthCorr = -1;
+};
+
+// What runPiso records after the initial projection (main.cpp:147-149), with
+// its Krylov iterations and whether every solve converged.
+struct InitRep {
+    int done = 0, solves = 0, pIters = 0, status = kStepCompleted;
+    double mass = 0, uMax = 0;
 };
 
 struct StepRep {
@@ -49,6 +59,7 @@ public:
     void getFields(double* Ucaller, double* pcaller, double* phicaller);
     void temporalFlux(double* caller);
     void start();
+    InitRep initial;  // the potential-flow projection run by start(), when configured
     StepRep step(double dt);
     std::vector<double> momHist[3], pHist;
     PisoTiming timing;
@@ -61,12 +72,14 @@ private:
     FieldArg fieldP() const { return {p.p, pType.p, pVal.p, nullptr, nullptr}; }
     FieldArg fieldRAU() const { return {rAU.p, zgType.p, zgVal.p, nullptr, rAUtr.p}; }
     void computeTemporalFlux();
+    void initializePotentialFlow();
+    FieldArg fieldPP() const { return {pPrime.p, ppType.p, ppVal.p, nullptr, nullptr}; }
     Context& C;
     PisoCfg c;
     int refCell = 0;
     bool inletOutlet = false;
-    DBuf<int> uType, pType, zgType;
-    DBuf<double> uVal, pVal, zgVal;
+    DBuf<int> uType, pType, zgType, ppType;
+    DBuf<double> uVal, pVal, zgVal, ppVal, pPrime;
     DBuf<double> U, p, phi, Uiter, HbyA, gradU, gradP, work9, rAU, rAUtr, invAP, rAUf, npr, kfMom, mu, imbalance,
         pOld, corr, full, orthF, src, psrc, tf, fluxTmp, red;
     DBuf<double> faceWork;  // face-ordered assembly terms: 6 per interior face

@@ -11,8 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (EquationCounters, IterReport, MeshData, Performance, PisoConfig, SimpleConfig, SolveResult,
-                    SolverConfig, StepReport, check, dp, ip, lib, ptr)
+from ._capi import (EquationCounters, IterReport, MeshData, Performance, PisoConfig, PisoInitial, SimpleConfig,
+                    SolveResult, SolverConfig, StepReport, check, dp, ip, lib, ptr)
 
 
 def _f64(a):
@@ -348,6 +348,12 @@ class Piso:
 
     def start(self):
         check(lib().bsim_piso_start(self.h))
+
+    def initial(self) -> PisoInitial:
+        """The initial potential-flow projection's record (after start)."""
+        rec = PisoInitial()
+        check(lib().bsim_piso_initial_report(self.h, C.byref(rec)))
+        return rec
 
     def step(self, dt) -> StepReport:
         rep = StepReport()

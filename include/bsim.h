@@ -421,7 +421,20 @@ typedef struct {
     int max_iterations, n_correctors, max_correctors, non_orth_corr;
     double alpha_u, coupling_tol, tol_mass, tol_u, tol_mom, tol_p;
     bsim_solver_config mom, pre;
+    /* initializePotentialFlow (0 or 1) and initialNonOrthogonalCorrections
+     * (negative: non_orth_corr), main.cpp:104-105: the start projects the
+     * initial velocity onto a conservative face flux (main.cpp:127-150). */
+    int initialize_potential_flow, initial_non_orth_corr;
 } bsim_piso_config;
+
+/* What the initial projection recorded (main.cpp:147-149): initialMass,
+ * initialUmax, initialPressureSolves, plus its Krylov iterations, whether it
+ * ran, and its outcome (BSIM_STEP_COMPLETED, or BSIM_STEP_NUMERICAL_FAILURE
+ * when a solve did not converge; every later step then reports that). */
+typedef struct {
+    double initial_mass, initial_u_max;
+    int initial_pressure_solves, p_iters, done, status;
+} bsim_piso_initial;
 
 /* The per-step record runPiso reports (main.cpp:282-289): CoMax, CoMean,
  * mass, dU, rU, dP, rCouplingFirst, rCoupling, linU, linP, the outer passes
@@ -438,6 +451,8 @@ int bsim_piso_create(bsim_ctx* ctx, const bsim_piso_config* cfg, const int* u_bc
 int bsim_piso_set_fields(bsim_piso* s, const double* U, const double* p);
 /* phi = flux(U) and the time histories from the current fields (main.cpp:123, 157-158). */
 int bsim_piso_start(bsim_piso* s);
+/* The initial projection's record (zeros with done = 0 when it is not configured). */
+int bsim_piso_initial_report(bsim_piso* s, bsim_piso_initial* out);
 int bsim_piso_step(bsim_piso* s, double dt, bsim_step_report* report);
 int bsim_piso_get_fields(bsim_piso* s, double* U, double* p, double* phi);
 /* rho * temporalFluxCorrection of the last step (main.cpp:19-53, 166-167), one value per face. */

@@ -564,6 +564,8 @@ void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, cons
         c.alphaU = cfg->alpha_u; c.couplingTol = cfg->coupling_tol;
         c.tolMass = cfg->tol_mass; c.tolU = cfg->tol_u; c.tolMom = cfg->tol_mom; c.tolP = cfg->tol_p;
         c.mom = toConfig(&cfg->mom); c.pre = toConfig(&cfg->pre);
+        c.potentialInit = cfg->initialize_potential_flow;
+        c.initialNonOrthCorr = cfg->initial_non_orth_corr;
         std::unique_ptr<Piso> owned(new Piso(m, c));
         std::vector<BC<V3>> ub(m.np);
         std::vector<BC<double>> pb(m.np);
@@ -587,6 +589,12 @@ int orc_piso_set_fields(void* sp, const double* U, const double* p) {
     });
 }
 int orc_piso_start(void* sp) { return guard([&] { static_cast<Piso*>(sp)->start(); }); }
+int orc_piso_initial_report(void* sp, orc_piso_initial* o) {
+    return guard([&] {
+        const InitReport& r = static_cast<Piso*>(sp)->initial;
+        *o = orc_piso_initial{r.mass, r.uMax, r.solves, r.pIters, r.done, r.status};
+    });
+}
 int orc_piso_step(void* sp, double dt, orc_step_report* o) {
     return guard([&] {
         const StepReport r = static_cast<Piso*>(sp)->step(dt);

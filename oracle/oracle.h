@@ -147,7 +147,13 @@ typedef struct {
     int max_iterations, n_correctors, max_correctors, non_orth_corr;
     double alpha_u, coupling_tol, tol_mass, tol_u, tol_mom, tol_p;
     orc_solver_config mom, pre;
+    int initialize_potential_flow, initial_non_orth_corr;
 } orc_piso_config;
+
+typedef struct {
+    double initial_mass, initial_u_max;
+    int initial_pressure_solves, p_iters, done, status;
+} orc_piso_initial;
 
 typedef struct {
     double dt, co_max, co_mean, mass, dU, rU, dP, r_coupling_first, r_coupling, lin_u, lin_p;
@@ -159,6 +165,7 @@ void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, cons
 void orc_piso_free(void* s);
 int orc_piso_set_fields(void* s, const double* U, const double* p);
 int orc_piso_start(void* s);
+int orc_piso_initial_report(void* s, orc_piso_initial* out);
 int orc_piso_step(void* s, double dt, orc_step_report* report);
 int orc_piso_get_fields(void* s, double* U, double* p, double* phi);
 /* rho * temporal flux correction of the last step (nf values). */

@@ -46,6 +46,9 @@ Piso::Piso(const Mesh& mesh, const PisoConfig& cfg) : m(mesh), c(cfg) {
     if (c.maxCorrectors < c.nCorrectors || c.maxCorrectors > 100)
         throw std::invalid_argument("maxCorrectors must lie in [nCorrectors, 100]");
     if (!(c.couplingTol > 0)) throw std::invalid_argument("couplingTolerance must be positive");
+    if (c.potentialInit != 0 && c.potentialInit != 1) throw std::invalid_argument("initializePotentialFlow must be 0 or 1");
+    if (c.initialNonOrthCorr < 0) c.initialNonOrthCorr = c.nonOrthCorr;
+    if (c.initialNonOrthCorr > 20) throw std::invalid_argument("PISO iteration counts out of range");
     U = VField(m); p = SField(m); muEff = SField(m);
     muEff.v.assign(m.nc, c.mu);  // model.cpp:58-63
 }
@@ -58,6 +61,8 @@ void Piso::start() {
     // main.cpp:123: phi = flux(U, pressure options); histories start from the
     // current fields (main.cpp:157-158, history.h:7).
     flux(U, phi, c.pInterp, c.pGrad);
+    initial = InitReport{};
+    if (c.potentialInit) initializePotentialFlow();
     Uold.init(U);
     phiOld.init(phi);
     time = 0;
@@ -104,10 +109,91 @@ void solveOn(const Mesh& m, LinearSystem& sys, const Ldu<double>& q, const Solve
 
 }  // namespace
 
+// main.cpp:127-150: the potential pPrime (homogeneous boundary conditions of
+// p) solves laplacian(1, -1) with the source -div(phi), as many times as the
+// initial non-orthogonal corrections ask (potentialEquation: its own linear
+// system); then U -= grad(pPrime) and phi += faceFlux(potentialEquation,
+// pPrime).  A solve that does not converge ends the run.
+void Piso::initializePotentialFlow() {
+    SField pp(m);
+    pp.bc.resize(m.np);
+    for (Index k = 0; k < m.np; ++k) {
+        const int t = p.bc[k].type;
+        pp.bc[k] = t == FixedValue ? BC<double>{FixedValue, 0.0}
+                 : t == Symmetry ? BC<double>{Symmetry, 0.0} : BC<double>{ZeroGradient, 0.0};
+    }
+    const std::vector<double> one(m.nf, 1.0);
+    std::vector<double> imbalance;
+    divergenceFace(m, phi, imbalance);
+    const int solves = c.pDiff == Orthogonal ? 1 : c.initialNonOrthCorr + 1;
+    initial.solves = solves;
+    LinearSystem potSys;
+    std::vector<double> correction;
+    SField snapshot(m);
+    for (int k = 0; k < solves; ++k) {
+        Ldu<double> peq; peq.init(m);
+        Ldu<double> contrib; contrib.init(m);
+        if (c.pDiff != Orthogonal) {
+            std::vector<V3> g;
+            gradient(pp, g, c.pGrad);
+            addDiffusion<double>(contrib, one, pp, c.pGrad, c.pDiff, &g);
+            std::vector<double> full, orthF;  // procedural_equation.cpp:77-87
+            diffusionFlux(one, pp, g, c.pDiff, full);
+            diffusionFlux(one, pp, g, Orthogonal, orthF);
+            correction.assign(m.nf, 0.0);
+            for (Index f = 0; f < m.nf; ++f) correction[f] = full[f] + (-1.0) * orthF[f];
+        } else {
+            addDiffusion<double>(contrib, one, pp, c.pGrad, c.pDiff, nullptr);
+            correction.assign(m.nf, 0.0);
+        }
+        snapshot = pp;
+        accumulate(peq, contrib, -(-1.0));
+        for (Index i = 0; i < m.nc; ++i) peq.source[i] += (1.0 * m.V[i]) * (-1.0 * imbalance[i]);
+        referenceIfUnanchored(m, peq, 0.0);
+        SolveResult r;
+        solveOn(m, potSys, peq, c.pre, pp.v, r, nullptr);
+        initial.pIters += r.iterations;
+        if (r.status != Converged) {
+            initial.status = StepNumericalFailure;
+            initial.done = 1;
+            return;
+        }
+    }
+    std::vector<V3> g;
+    gradient(pp, g, c.pGrad);
+    for (Index i = 0; i < m.nc; ++i) U.v[i] = U.v[i] - g[i];
+    {
+        SField value = snapshot;
+        value.v = pp.v;
+        std::vector<V3> zero(m.nc, V3{});
+        std::vector<double> inc;
+        diffusionFlux(one, value, zero, Orthogonal, inc);
+        for (Index f = 0; f < m.nf; ++f) inc[f] += 1.0 * correction[f];
+        for (Index f = 0; f < m.nf; ++f) phi[f] = phi[f] + (0.0 + -1.0 * inc[f]);
+    }
+    U.fluxCtx = phi;
+    double l1 = 0, sc = 0, umax = 0;
+    for (Index cidx = 0; cidx < m.nc; ++cidx) {
+        double imb = 0;
+        for (Index k = m.cfo[cidx]; k < m.cfo[cidx + 1]; ++k) {
+            const Index f = m.cfi[k];
+            const double out = m.own[f] == cidx ? phi[f] : -phi[f];
+            imb += out;
+            sc += std::abs(out);
+        }
+        l1 += std::abs(imb);
+        umax = std::max(umax, std::sqrt(dot(U.v[cidx], U.v[cidx])));
+    }
+    initial.mass = l1 / std::max(sc, 1e-30);
+    initial.uMax = umax;
+    initial.done = 1;
+}
+
 StepReport Piso::step(double dt) {
     StepReport rep{};
     rep.dt = dt;
     if (!(dt > 0) || !std::isfinite(dt)) throw std::invalid_argument("history requires positive finite dt");
+    if (initial.status != StepCompleted) return rep;  // runPiso ended in the initial projection
     time += dt;
     // main.cpp:162-167
     Uold.save(U, dt);

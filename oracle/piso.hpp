@@ -23,6 +23,13 @@ struct PisoConfig {
     double couplingTol = std::numeric_limits<double>::max();
     double tolMass = 1e-8, tolU = 1e-7, tolMom = 1e-6, tolP = 1e-6;
     SolverConfig mom, pre;
+    int potentialInit = 0, initialNonOrthCorr = -1;  // main.cpp:104-105 (negative: nonOrthCorr)
+};
+
+// What runPiso records after the initial projection (main.cpp:147-149).
+struct InitReport {
+    int done = 0, solves = 0, pIters = 0, status = StepCompleted;
+    double mass = 0, uMax = 0;
 };
 
 struct StepReport {
@@ -71,6 +78,8 @@ struct Piso {
     Piso(const Mesh& mesh, const PisoConfig& cfg);
     void setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<double>>& pbc);
     void start();
+    void initializePotentialFlow();
+    InitReport initial;
     StepReport step(double dt);
     std::vector<double> temporalFluxCorrection();
 };

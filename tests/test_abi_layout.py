@@ -21,6 +21,7 @@ STRUCTS = {
     "bsim_komega_config": _capi.KOmegaConfig,
     "bsim_piso_config": _capi.PisoConfig,
     "bsim_step_report": _capi.StepReport,
+    "bsim_piso_initial": _capi.PisoInitial,
     "bsim_request": _capi.Request,
     "bsim_request_result": _capi.RequestResult,
     "bsim_equation_counters": _capi.EquationCounters,

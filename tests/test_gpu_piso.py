@@ -198,3 +198,50 @@ def test_piso_mixed_precision_momentum():
         out.append(s.fields())
     for a, b in zip(out[1], out[0]):
         assert rel_l2(a, b) < 1e-6
+
+
+# initializePotentialFlow (piso/main.cpp:127-150) on device and oracle.
+POTENTIAL = {
+    "channel_from_rest": lambda: (pc.channel_mesh(24, 8), pc.cavity_piso_config(initialize_potential_flow=1),
+                                  pc.channel_bcs(), None, [0.05] * 4),
+    "polyhedral_corrected": lambda: (cases.cavity3d_mesh(6, split=0.2, seed=11),
+                                     pc.c2_piso_config(non_orth_corr=2, initialize_potential_flow=1,
+                                                       initial_non_orth_corr=1),
+                                     cases.cavity_bcs(False),
+                                     lambda C: np.stack([np.sin(3 * C[:, 1]), np.cos(2 * C[:, 0]), 0.3 * C[:, 2]], 1),
+                                     [0.02] * 3),
+}
+
+
+@pytest.mark.parametrize("kind", list(POTENTIAL))
+def test_potential_flow_parity(kind):
+    """The initial projection's record and fields, then the steps after it."""
+    mesh, cfg, bcs, U0, dts = POTENTIAL[kind]()
+    so, sg = pair(mesh, cfg, bcs, U0)
+    io, ig = so.initial(), sg.initial()
+    print(f"{kind}: {io.initial_pressure_solves} potential solves, {io.p_iters} CG iterations", flush=True)
+    assert (ig.done, ig.status, ig.initial_pressure_solves, ig.p_iters) == \
+        (io.done, io.status, io.initial_pressure_solves, io.p_iters) == (1, STEP_COMPLETED, io.initial_pressure_solves,
+                                                                          io.p_iters)
+    assert io.p_iters > 0
+    assert ig.initial_u_max == pytest.approx(io.initial_u_max, rel=1e-10)
+    assert abs(ig.initial_mass - io.initial_mass) <= 1e-3 * io.initial_mass + 1e-13
+    for a, b in zip(sg.fields(), so.fields()):
+        assert rel_l2(a, b) < 1e-10
+    for dt in dts:
+        ro, rg = so.step(dt), sg.step(dt)
+        compare_step(rg, ro)
+        assert ro.status == STEP_COMPLETED
+    for a, b in zip(sg.fields(), so.fields()):
+        assert rel_l2(a, b) < 1e-9
+
+
+def test_potential_flow_failure_ends_the_run():
+    """A potential solve that does not converge: the record says numerical
+    failure and every step reports it, as runPiso returns numericalFailure."""
+    cfg = pc.c2_piso_config(initialize_potential_flow=1,
+                            pre=solver_config(solver=0, pc=0, rtol=1e-14, atol=1e-30, max_iter=1))
+    so, sg = pair(cases.cavity3d_mesh(5, split=0.1, seed=2), cfg, cases.cavity_bcs(False),
+                  lambda C: np.tile([0.3, -0.2, 0.1], (len(C), 1)))
+    assert sg.initial().status == so.initial().status == 2
+    assert sg.step(0.01).status == so.step(0.01).status == 2
