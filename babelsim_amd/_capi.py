@@ -132,7 +132,16 @@ class StepReport(C.Structure):
                 ("dU", C.c_double), ("rU", C.c_double), ("dP", C.c_double), ("r_coupling_first", C.c_double),
                 ("r_coupling", C.c_double), ("lin_u", C.c_double), ("lin_p", C.c_double),
                 ("passes", C.c_int), ("correctors", C.c_int), ("mom_iters", C.c_int * 3), ("p_iters", C.c_int),
-                ("accepted", C.c_int), ("converged", C.c_int), ("linear_ok", C.c_int), ("status", C.c_int)]
+                ("accepted", C.c_int), ("converged", C.c_int), ("linear_ok", C.c_int), ("status", C.c_int),
+                ("d_turb", C.c_double), ("r_turb", C.c_double), ("k_iters", C.c_int), ("w_iters", C.c_int),
+                ("turb_linear", C.c_int), ("turb_converged", C.c_int), ("turb_corrections", C.c_int)]
+
+
+class PisoTurbulence(C.Structure):
+    """turbulenceCoupling (0 iterated, 1 segregated), turbulenceCorrectors (<= 0: the reference's
+    default) and turbulenceMaxIterations (src/physics/piso/main.cpp:98-120)."""
+
+    _fields_ = [("coupling", C.c_int), ("correctors", C.c_int), ("max_iterations", C.c_int)]
 
 
 _U64, _F64 = C.c_uint64, C.c_double
@@ -354,6 +363,9 @@ _DEVICE_SIGNATURES: dict = {
     "bsim_simple_probe_stop": [vp, ip, dp, dp],
     "bsim_piso_create": [vp, C.POINTER(PisoConfig), ip, dp, ip, dp, C.POINTER(vp)],
     "bsim_piso_set_fields": [vp, dp, dp],
+    "bsim_piso_enable_komega": [vp, C.POINTER(KOmegaConfig), ip, dp, ip, dp, C.POINTER(PisoTurbulence)],
+    "bsim_piso_set_turbulence": [vp, dp, dp],
+    "bsim_piso_get_turbulence": [vp, dp, dp, dp],
     "bsim_piso_start": [vp],
     "bsim_piso_initial_report": [vp, C.POINTER(PisoInitial)],
     "bsim_piso_step": [vp, C.c_double, C.POINTER(StepReport)],

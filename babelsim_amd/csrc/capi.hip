@@ -83,6 +83,18 @@ SimpleCfg toSimpleCfg(const bsim_simple_config& cfg) {
     return c;
 }
 
+KOmegaCfg toKOmegaCfg(const bsim_komega_config& k) {
+    KOmegaCfg c;
+    c.betaStar = k.beta_star; c.beta = k.beta; c.gamma = k.gamma; c.sigmaK = k.sigma_k;
+    c.sigmaOmega = k.sigma_omega; c.kMin = k.k_min; c.omegaMin = k.omega_min;
+    c.viscosityRatioMax = k.viscosity_ratio_max; c.relaxation = k.relaxation; c.tolerance = k.tolerance;
+    c.kOpts = {k.k_interp, k.k_grad, k.k_conv, k.k_diff, k.k_coef_interp, k.k_coef_grad};
+    c.wOpts = {k.w_interp, k.w_grad, k.w_conv, k.w_diff, k.w_coef_interp, k.w_coef_grad};
+    c.kSolver = toSolverCfg(&k.k_solver);
+    c.wSolver = toSolverCfg(&k.w_solver);
+    return c;
+}
+
 PisoCfg toPisoCfg(const bsim_piso_config& cfg) {
     PisoCfg c;
     c.rho = cfg.rho; c.mu = cfg.mu;
@@ -596,15 +608,7 @@ int bsim_simple_enable_komega(bsim_simple* s, const bsim_komega_config* k, const
                               const int* wbct, const double* wbcv) {
     return guarded([&] {
         if (!k || !kbct || !kbcv || !wbct || !wbcv) throw std::invalid_argument("null argument");
-        KOmegaCfg c;
-        c.betaStar = k->beta_star; c.beta = k->beta; c.gamma = k->gamma; c.sigmaK = k->sigma_k;
-        c.sigmaOmega = k->sigma_omega; c.kMin = k->k_min; c.omegaMin = k->omega_min;
-        c.viscosityRatioMax = k->viscosity_ratio_max; c.relaxation = k->relaxation; c.tolerance = k->tolerance;
-        c.kOpts = {k->k_interp, k->k_grad, k->k_conv, k->k_diff, k->k_coef_interp, k->k_coef_grad};
-        c.wOpts = {k->w_interp, k->w_grad, k->w_conv, k->w_diff, k->w_coef_interp, k->w_coef_grad};
-        c.kSolver = toSolverCfg(&k->k_solver);
-        c.wSolver = toSolverCfg(&k->w_solver);
-        s->s->enableKOmega(c, kbct, kbcv, wbct, wbcv);
+        s->s->enableKOmega(toKOmegaCfg(*k), kbct, kbcv, wbct, wbcv);
     });
 }
 int bsim_simple_set_turbulence(bsim_simple* s, const double* k, const double* omega) {
@@ -733,6 +737,28 @@ int bsim_piso_step(bsim_piso* s, double dt, bsim_step_report* o) {
         for (int c = 0; c < 3; ++c) o->mom_iters[c] = r.momIters[c];
         o->p_iters = r.pIters; o->accepted = r.accepted; o->converged = r.converged; o->linear_ok = r.linearOk;
         o->status = r.status;
+        o->d_turb = r.dTurb; o->r_turb = r.rTurb; o->k_iters = r.kIters; o->w_iters = r.wIters;
+        o->turb_linear = r.turbLinear; o->turb_converged = r.turbConverged; o->turb_corrections = r.turbCorrections;
+    });
+}
+int bsim_piso_enable_komega(bsim_piso* s, const bsim_komega_config* k, const int* kbct, const double* kbcv,
+                            const int* wbct, const double* wbcv, const bsim_piso_turbulence* t) {
+    return guarded([&] {
+        if (!s || !k || !kbct || !kbcv || !wbct || !wbcv || !t) throw std::invalid_argument("null argument");
+        s->s->enableKOmega(toKOmegaCfg(*k), kbct, kbcv, wbct, wbcv,
+                           PisoTurbCfg{t->coupling, t->correctors, t->max_iterations});
+    });
+}
+int bsim_piso_set_turbulence(bsim_piso* s, const double* k, const double* omega) {
+    return guarded([&] {
+        if (!s->s->turbulence()) throw std::invalid_argument("laminar PISO has no turbulence fields");
+        s->s->turbulence()->setFields(k, omega);
+    });
+}
+int bsim_piso_get_turbulence(bsim_piso* s, double* k, double* omega, double* mut) {
+    return guarded([&] {
+        if (!s->s->turbulence()) throw std::invalid_argument("laminar PISO has no turbulence fields");
+        s->s->turbulence()->getFields(k, omega, mut);
     });
 }
 int bsim_piso_get_fields(bsim_piso* s, double* U, double* p, double* phi) {
@@ -762,10 +788,15 @@ int bsim_piso_performance(bsim_piso* s, bsim_performance* total, bsim_equation_c
     return guarded([&] {
         if (!s) throw std::invalid_argument("null argument");
         PisoSolver& S = *s->s;
-        const std::vector<std::pair<const DeviceSystem*, const char*>> sys = {
+        std::vector<std::pair<const DeviceSystem*, const char*>> sys = {
             {&S.momentumSystem(), "vector:momentum:U"}, {&S.pressureSystem(), "scalar:pressure:p"}};
-        runPerformance(sys, S.run, S.run.reductions, S.timing.assembly + S.timing.pressureAssembly, total, systems,
-                       cap, count);
+        std::uint64_t reads = S.run.reductions;
+        if (const KOmegaDevice* t = S.turbulence()) {
+            if (t->kSystem()) sys.push_back({t->kSystem(), "scalar:kTransport:k"});
+            if (t->omegaSystem()) sys.push_back({t->omegaSystem(), "scalar:omegaTransport:omega"});
+            reads += t->reads;
+        }
+        runPerformance(sys, S.run, reads, S.timing.assembly + S.timing.pressureAssembly, total, systems, cap, count);
     });
 }
 int bsim_piso_probe_start(bsim_piso* s, int capacity) {

@@ -76,6 +76,18 @@ __global__ void __launch_bounds__(kBlock) kRateK(int n, const double* w, double 
 }
 
 // reaction (procedural_equation.cpp:181-187) and source (192-196).
+// equ::ddt for a scalar history (procedural_equation.cpp:213-222): w = rho V,
+// diag += w a0, src += w a1 previous (+ w a2 older).
+__global__ void __launch_bounds__(kBlock) kScalarTimeTerm(int n, const double* V, double rho, double a0, double a1,
+                                                          double a2, const double* prev, const double* older,
+                                                          double* diag, double* src) {
+    const int i = globalThread();
+    if (i >= n) return;
+    const double w = rho * V[i];
+    diag[i] += w * a0;
+    src[i] += w * a1 * prev[i];
+    if (a2 != 0) src[i] += w * a2 * older[i];
+}
 __global__ void __launch_bounds__(kBlock) kReactionSource(int n, const double* V, const double* rate, const double* s, double* diag,
                                 double* b) {
     const int i = globalThread();
@@ -337,9 +349,36 @@ void KOmegaDevice::addStressSource(FieldArg U, const double* gradU, int mInterp,
     BSIM_LAUNCH_CHECK();
 }
 
+void KOmegaDevice::setTransient(int method) {
+    if (method != 1 && method != 2) throw std::invalid_argument("transient k-omega needs Euler or BDF2");
+    const int nc = C.mesh->nc;
+    for (int q = 0; q < 2; ++q) { kh[q].ensure(nc); wh[q].ensure(nc); }
+    timeMethod = method;
+    hcur = 0;
+    hlevels = 0;
+    hdt = hdtPrev = 0;
+}
+
+void KOmegaDevice::saveOld(double dt) {
+    if (!timeMethod) throw std::logic_error("saveOld needs the transient form");
+    const int nc = C.mesh->nc;
+    hipStream_t s = C.stream;
+    hcur = 1 - hcur;
+    kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, k.p, kh[hcur].p);
+    kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, w.p, wh[hcur].p);
+    if (hlevels == 0) {  // the older slot of a one-level history holds the same values (history.h:7)
+        kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, k.p, kh[1 - hcur].p);
+        kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, w.p, wh[1 - hcur].p);
+    }
+    BSIM_LAUNCH_CHECK();
+    hdtPrev = hdt;
+    hdt = dt;
+    ++hlevels;
+}
+
 void KOmegaDevice::assemble(DeviceSystem& sys, FieldArg x, double* ctx, const double* phi, const TransportOpts& o,
                             const double* diffusivity, const double* diffusivityTrace, const double* rt,
-                            const double* source) {
+                            const double* source, const TimeTerm* tt) {
     const DeviceMesh& M = *C.mesh;
     const MeshView mv = M.view();
     hipStream_t s = C.stream;
@@ -349,6 +388,9 @@ void KOmegaDevice::assemble(DeviceSystem& sys, FieldArg x, double* ctx, const do
     kFillD<<<blocksFor(nc), kBlock, 0, s>>>(nc, diag, 0.0);
     kFillD<<<blocksFor(std::max(M.nnzSliced, 1)), kBlock, 0, s>>>(M.nnzSliced, off, 0.0);
     kFillD<<<blocksFor(nc), kBlock, 0, s>>>(nc, bsrc.p, 0.0);
+    // equ::ddt(equation, rho, history) first (k_omega.cpp:82, 96)
+    if (tt) kScalarTimeTerm<<<blocksFor(nc), kBlock, 0, s>>>(nc, M.V.p, rho, tt->a0, tt->a1, tt->a2, tt->prev, tt->older,
+                                                             diag, bsrc.p);
     kCopyD<<<blocksFor(nf), kBlock, 0, s>>>(nf, phi, ctx);  // equ::div binds phi (procedural_equation.cpp:121-125)
     const bool needGrad = o.conv == kLinearUpwind || (o.conv == kCentral && o.interp == kCorrected) ||
                           o.diff != kOrthogonal;
@@ -383,6 +425,25 @@ TransportRep KOmegaDevice::solveTransport(FieldArg U, const double* phi, int mGr
     hipStream_t s = C.stream;
     const int nc = M.nc, nf = M.nf;
     TransportRep rep;
+    TimeTerm tk, tw;
+    const TimeTerm* ptk = nullptr;
+    const TimeTerm* ptw = nullptr;
+    if (timeMethod) {  // procedural_equation.cpp:198-239
+        if (hlevels == 0) throw std::logic_error("saveOld must precede a transient assembly");
+        const int method = timeMethod == 2 && hlevels < 2 ? 1 : timeMethod;
+        double a0 = 1 / hdt, a1 = 1 / hdt, a2 = 0;
+        if (method == 2) {
+            const double olddt = hdtPrev == 0 ? hdt : hdtPrev;
+            const double r = hdt / olddt;
+            a0 = (1 + 2 * r) / ((1 + r) * hdt);
+            a1 = (1 + r) / hdt;
+            a2 = -r * r / ((1 + r) * hdt);
+        }
+        tk = TimeTerm{a0, a1, a2, kh[hcur].p, kh[1 - hcur].p};
+        tw = TimeTerm{a0, a1, a2, wh[hcur].p, wh[1 - hcur].p};
+        ptk = &tk;
+        ptw = &tw;
+    }
     kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, k.p, kPrev.p);
     kCopyD<<<blocksFor(nc), kBlock, 0, s>>>(nc, w.p, wPrev.p);
     // k_omega.cpp:62-77: coefficients frozen at the current iterate.
@@ -393,7 +454,7 @@ TransportRep KOmegaDevice::solveTransport(FieldArg U, const double* phi, int mGr
         kd.p, wd.p, rate.p, srcW.p, kdT.p, wdT.p);
     BSIM_LAUNCH_CHECK();
     // omega (k_omega.cpp:80-93)
-    assemble(*wSys, fieldW(), ctxW.p, phi, c.wOpts, wd.p, wdT.p, rate.p, srcW.p);
+    assemble(*wSys, fieldW(), ctxW.p, phi, c.wOpts, wd.p, wdT.p, rate.p, srcW.p, ptw);
     rep.rOmega = residual(*wSys, w.p);
     kRelaxScalar<<<blocksFor(nc), kBlock, 0, s>>>(nc, wSys->candDiag(), bsrc.p, wPrev.p, c.relaxation);
     BSIM_LAUNCH_CHECK();
@@ -408,7 +469,7 @@ TransportRep KOmegaDevice::solveTransport(FieldArg U, const double* phi, int mGr
     double bounded = boundOmega();
     // k with the new omega in its sink (k_omega.cpp:95-106)
     kRateK<<<blocksFor(nc), kBlock, 0, s>>>(nc, w.p, c.betaStar * rho, rate.p);
-    assemble(*kSys, fieldK(), ctxK.p, phi, c.kOpts, kd.p, kdT.p, rate.p, prod.p);
+    assemble(*kSys, fieldK(), ctxK.p, phi, c.kOpts, kd.p, kdT.p, rate.p, prod.p, ptk);
     rep.rK = residual(*kSys, k.p);
     kRelaxScalar<<<blocksFor(nc), kBlock, 0, s>>>(nc, kSys->candDiag(), bsrc.p, kPrev.p, c.relaxation);
     BSIM_LAUNCH_CHECK();

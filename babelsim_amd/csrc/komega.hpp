@@ -1,8 +1,9 @@
 // Wilcox (1988) k-omega on the device (src/physics/RANS/k_omega.cpp:25-157,
 // model.cpp:67-71): eddy viscosity with calculated boundary traces, the
 // explicit deviatoric stress remainder of the momentum equation, and the
-// segregated omega-then-k transport solves with bounding.  Steady form, used
-// by the SIMPLE driver.
+// segregated omega-then-k transport solves with bounding.  Steady form for the
+// SIMPLE driver; the PISO driver adds the k and omega histories and their ddt
+// terms (k_omega.cpp:59-62, 82, 96).
 #pragma once
 
 #include "context.hpp"
@@ -42,6 +43,10 @@ public:
     void addStressSource(FieldArg U, const double* gradU, int mInterp, int mGrad, double* src);
     TransportRep solveTransport(FieldArg U, const double* phi, int mGrad);
     double tolerance() const { return c.tolerance; }
+    double relaxation() const { return c.relaxation; }
+    // Transient form: timeMethod 1 Euler or 2 BDF2 (Euler below two saved levels).
+    void setTransient(int timeMethod);
+    void saveOld(double dt);  // history.h:6-27
     std::vector<double> kHist, wHist;
 
 private:
@@ -49,8 +54,17 @@ private:
     FieldArg fieldW() const { return {w.p, wType.p, wVal.p, ctxW.p, nullptr}; }
     void updateViscosity();
     double boundOmega();
+    struct TimeTerm {
+        double a0 = 0, a1 = 0, a2 = 0;
+        const double *prev = nullptr, *older = nullptr;
+    };
     void assemble(DeviceSystem& sys, FieldArg x, double* ctx, const double* phi, const TransportOpts& o,
-                  const double* diffusivity, const double* diffusivityTrace, const double* rate, const double* source);
+                  const double* diffusivity, const double* diffusivityTrace, const double* rate, const double* source,
+                  const TimeTerm* tt);
+    int timeMethod = 0;
+    DBuf<double> kh[2], wh[2];  // slot hcur: previous level, 1 - hcur: older
+    int hcur = 0, hlevels = 0;
+    double hdt = 0, hdtPrev = 0;
     double residual(DeviceSystem& sys, const double* x);
     Context& C;
     KOmegaCfg c;

@@ -324,6 +324,27 @@ void PisoSolver::temporalFlux(double* out) {
     C.facesOut(tf.toHost(C.stream), out, 1);
 }
 
+void PisoSolver::enableKOmega(const KOmegaCfg& kw, const int* kbct, const double* kbcv, const int* wbct,
+                              const double* wbcv, const PisoTurbCfg& t) {
+    if (started) throw std::logic_error("the turbulence model is enabled before the run starts");
+    // main.cpp:99-120
+    if (t.coupling != 0 && t.coupling != 1) throw std::invalid_argument("turbulenceCoupling must be iterated or segregated");
+    const bool segregated = t.coupling == 1;
+    if (segregated && c.maxIterations != 1)
+        throw std::invalid_argument("segregated turbulence requires one PISO momentum pass");
+    if (segregated && kw.relaxation != 1.0)
+        throw std::invalid_argument("single-pass transient turbulence requires turbulenceRelaxation 1");
+    if (t.maxIterations < 1) throw std::invalid_argument("turbulenceMaxIterations must be positive");
+    PisoTurbCfg v = t;
+    if (v.correctors <= 0) v.correctors = c.maxIterations == 1 && !segregated ? t.maxIterations : 1;
+    if (v.correctors > v.maxIterations) throw std::invalid_argument("turbulenceCorrectors out of range");
+    if (segregated && v.correctors != 1) throw std::invalid_argument("segregated turbulence requires one transport correction");
+    auto model = std::make_unique<KOmegaDevice>(C, kw, c.rho, c.mu, kbct, kbcv, wbct, wbcv);
+    model->setTransient(c.timeMethod);
+    turb = std::move(model);
+    tc = v;
+}
+
 void PisoSolver::start() {
     // main.cpp:123: phi = flux(U, pressure options); the first flux seeds
     // inletOutlet directions from owner velocities (operators.cpp:977-987).
@@ -331,6 +352,7 @@ void PisoSolver::start() {
     const DeviceMesh& M = *C.mesh;
     const MeshView mv = M.view();
     hipStream_t s = C.stream;
+    if (turb) turb->start();  // the model exists before phi (main.cpp:71, 98)
     ops::seedFluxContext(mv, U.p, fluxTmp.p, s);
     const double* g = nullptr;
     if (c.pInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 3, fieldU(fluxTmp.p), c.pGrad, gradU.p, work9.p, C.err.p, s); g = gradU.p; }
@@ -486,6 +508,7 @@ StepRep PisoSolver::step(double dt) {
     dtPrev = dtCur;
     dtCur = dt;
     ++levels;
+    if (turb) turb->saveOld(dt);  // main.cpp:165
     computeTemporalFlux();
     stepped = true;
     // ddt weights (procedural_equation.cpp:204-212; Euler below two levels, 231-237)
@@ -517,12 +540,20 @@ StepRep PisoSolver::step(double dt) {
         kTimeTerm<<<nb, kBlock, 0, s>>>(mv, c.rho, a0, a1, a2, Uh[cur].p, Uh[old].p, diag, src.p);
         const FieldArg Uf = fieldU(phi.p);
         const bool needGradU = c.mConv == kLinearUpwind || (c.mConv == kCentral && c.mInterp == kCorrected) ||
-                               c.mDiff != kOrthogonal;
+                               c.mDiff != kOrthogonal || turb;
         if (needGradU) ops::gradient(mv, M.orthogonal, 3, Uf, c.mGrad, gradU.p, work9.p, C.err.p, s);
+        if (turb) {
+            // laplacian(muEff) with the current eddy viscosity (main.cpp:176)
+            const FieldArg mf{turb->muEff(), zgType.p, zgVal.p, nullptr, turb->muEffTrace()};
+            const double* gm = nullptr;
+            if (c.mCoefInterp == kCorrected) { ops::gradient(mv, M.orthogonal, 1, mf, c.mCoefGrad, gradP.p, work9.p, C.err.p, s); gm = gradP.p; }
+            ops::interpolate(mv, 1, mf, c.mCoefInterp, gm, kfMom.p, C.err.p, s);
+        }
         ops::assembleTransport(mv, 3, Uf, phi.p, c.mConv, c.mInterp, c.rho, kfMom.p, c.mDiff, -(-1.0), gradU.p, diag,
                                off, src.p, faceWork.p, C.err.p, s);
         ops::gradient(mv, M.orthogonal, 1, fieldP(), c.pGrad, gradP.p, work9.p, C.err.p, s);
         kMomentumPressureSource<<<nb, kBlock, 0, s>>>(mv, gradP.p, src.p);
+        if (turb) turb->addStressSource(Uf, gradU.p, c.mInterp, c.mGrad, src.p);  // main.cpp:179-182
         // rU before relaxation (main.cpp:184)
         kMomentumResidual<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, diag, off, src.p, U.p, part);
         kReduceTail<3><<<1, kTail, 0, s>>>(reduceBlocks(nc), part, StoreSums<3>{red.p + kSlotRU});
@@ -651,6 +682,30 @@ StepRep PisoSolver::step(double dt) {
             timing.corrector += since(t0);
             if (done) break;
         }
+        // Turbulence transport at this time level (main.cpp:251-266).
+        bool turbulenceConverged = true, turbulenceLinear = true;
+        if (turb) {
+            rep.kIters = rep.wIters = 0;
+            for (int correction = 0; correction < tc.correctors; ++correction) {
+                t0 = Clock::now();
+                const TransportRep t = turb->solveTransport(fieldU(phi.p), phi.p, c.mGrad);
+                timing.turbulence += since(t0);
+                rep.turbCorrections = correction + 1;
+                rep.kIters += t.itK;
+                rep.wIters += t.itOmega;
+                rep.dTurb = std::max(t.dOmega, t.dK);
+                rep.rTurb = std::max(t.rOmega, t.rK);
+                if (!t.healthy || !std::isfinite(rep.dTurb) || !std::isfinite(rep.rTurb)) {
+                    rep.status = kStepNumericalFailure;
+                    return rep;
+                }
+                turbulenceLinear = t.statusK == kConverged && t.statusOmega == kConverged;
+                turbulenceConverged = turbulenceLinear && rep.dTurb <= turb->tolerance() && rep.rTurb <= turb->tolerance();
+                if (!turbulenceLinear || turbulenceConverged) break;
+            }
+            rep.turbLinear = turbulenceLinear;
+            rep.turbConverged = turbulenceConverged;
+        }
         t0 = Clock::now();
         // Step diagnostics (main.cpp:268-270; fvm_execution.cpp:91-205).
         kStepDiagnostics<<<reduceBlocks(nc), kBlock, 0, s>>>(mv, U.p, Uiter.p, phi.p, dt, part);
@@ -684,11 +739,11 @@ StepRep PisoSolver::step(double dt) {
             rep.status = kStepNumericalFailure;
             return rep;
         }
-        const bool linearConverged = momStatus == kConverged && pressureConverged;
+        const bool linearConverged = momStatus == kConverged && pressureConverged && turbulenceLinear;
         rep.linearOk = linearConverged;
         const bool accepted = linearConverged && rep.mass <= c.tolMass && coupling <= c.couplingTol;
-        const bool settled = rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP;
-        converged = accepted && (c.maxIterations == 1 ? true : settled);
+        const bool settled = rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP && turbulenceConverged;
+        converged = accepted && (c.maxIterations == 1 ? (tc.coupling == 1 || turbulenceConverged) : settled);
         rep.accepted = accepted;
         rep.converged = converged;
         if (!accepted) { rep.status = kStepNotConverged; return rep; }

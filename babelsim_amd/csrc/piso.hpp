@@ -6,6 +6,7 @@
 #pragma once
 
 #include "context.hpp"
+#include "komega.hpp"
 #include "perf.hpp"
 
 #include <limits>
@@ -29,6 +30,14 @@ struct PisoCfg {
     int potentialInit = 0, initialNonOrthCorr = -1;
 };
 
+// The turbulence controls of runPiso (main.cpp:98-120): coupling 0 iterated,
+// 1 segregated; correctors <= 0 takes the reference's default
+// (turbulenceMaxIterations for iterated coupling with one momentum pass,
+// else 1).
+struct PisoTurbCfg {
+    int coupling = 0, correctors = 0, maxIterations = 1000;
+};
+
 // What runPiso records after the initial projection (main.cpp:147-149), with
 // its Krylov iterations and whether every solve converged.
 struct InitRep {
@@ -42,12 +51,16 @@ struct StepRep {
     int passes = 0, correctors = 0;
     int momIters[3] = {0, 0, 0};
     int pIters = 0, accepted = 0, converged = 0, linearOk = 0, status = kStepNumericalFailure;
+    // Turbulence transport at the time level (main.cpp:251-266): the last
+    // correction's dTurb and rTurb, the Krylov iterations of all corrections.
+    double dTurb = 0, rTurb = 0;
+    int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1, turbCorrections = 0;
 };
 
 // Per-phase wall time (seconds) accumulated over steps.
 struct PisoTiming {
     double history = 0, assembly = 0, momentumSolve = 0, corrector = 0, pressureAssembly = 0, pressureSolve = 0,
-           diagnostics = 0;
+           diagnostics = 0, turbulence = 0;
     int steps = 0;
 };
 
@@ -55,6 +68,10 @@ class PisoSolver {
 public:
     PisoSolver(Context& ctx, const PisoCfg& cfg, const int* ubct, const double* ubcv, const int* pbct,
                const double* pbcv);
+    // Wilcox k-omega in transient form (main.cpp:98-121); before start().
+    void enableKOmega(const KOmegaCfg& kw, const int* kbct, const double* kbcv, const int* wbct, const double* wbcv,
+                      const PisoTurbCfg& t);
+    KOmegaDevice* turbulence() { return turb.get(); }
     void setFields(const double* Ucaller, const double* pcaller);
     void getFields(double* Ucaller, double* pcaller, double* phicaller);
     void temporalFlux(double* caller);
@@ -89,6 +106,8 @@ private:
     double dtCur = 0, dtPrev = 0;
     DBuf<double> bcomp[3], xcomp[3];
     std::unique_ptr<DeviceSystem> momSys, pSys;
+    std::unique_ptr<KOmegaDevice> turb;
+    PisoTurbCfg tc;
     bool started = false, stepped = false;
 };
 

@@ -11,8 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (EquationCounters, IterReport, MeshData, Performance, PisoConfig, PisoInitial, SimpleConfig,
-                    SolveResult, SolverConfig, StepReport, check, dp, ip, lib, ptr)
+from ._capi import (EquationCounters, IterReport, KOmegaConfig, MeshData, Performance, PisoConfig, PisoInitial,
+                    PisoTurbulence, SimpleConfig, SolveResult, SolverConfig, StepReport, check, dp, ip, lib, ptr)
 
 
 def _f64(a):
@@ -345,6 +345,23 @@ class Piso:
     def set_fields(self, U=None, p=None):
         check(lib().bsim_piso_set_fields(self.h, ptr(None if U is None else _f64(U)),
                                          ptr(None if p is None else _f64(p))))
+
+    def enable_komega(self, kw: KOmegaConfig, k_bc, w_bc, coupling=0, correctors=0, max_iterations=1000):
+        """Transient Wilcox k-omega (piso/main.cpp:98-121), before start()."""
+        kt, kv = k_bc
+        wt, wv = w_bc
+        t = PisoTurbulence(coupling=coupling, correctors=correctors, max_iterations=max_iterations)
+        check(lib().bsim_piso_enable_komega(self.h, C.byref(kw), ptr(_i32(kt), ip), ptr(_f64(kv)),
+                                            ptr(_i32(wt), ip), ptr(_f64(wv)), C.byref(t)))
+
+    def set_turbulence(self, k=None, omega=None):
+        check(lib().bsim_piso_set_turbulence(self.h, ptr(None if k is None else _f64(k)),
+                                             ptr(None if omega is None else _f64(omega))))
+
+    def turbulence(self):
+        k, w, mut = (np.zeros(self.ctx.nc) for _ in range(3))
+        check(lib().bsim_piso_get_turbulence(self.h, ptr(k), ptr(w), ptr(mut)))
+        return k, w, mut
 
     def start(self):
         check(lib().bsim_piso_start(self.h))

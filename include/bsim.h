@@ -443,12 +443,31 @@ typedef struct {
 typedef struct {
     double dt, co_max, co_mean, mass, dU, rU, dP, r_coupling_first, r_coupling, lin_u, lin_p;
     int passes, correctors, mom_iters[3], p_iters, accepted, converged, linear_ok, status;
+    /* With k-omega (main.cpp:251-266, 290-293): dTurb and rTurb of the last
+     * transport correction, the k and omega Krylov iterations over all of
+     * them, the linear and settled flags and the corrections run. */
+    double d_turb, r_turb;
+    int k_iters, w_iters, turb_linear, turb_converged, turb_corrections;
 } bsim_step_report;
+
+/* turbulenceCoupling (0 iterated, 1 segregated), turbulenceCorrectors (<= 0:
+ * the reference's default) and turbulenceMaxIterations (main.cpp:98-120). */
+typedef struct {
+    int coupling, correctors, max_iterations;
+} bsim_piso_turbulence;
 
 typedef struct bsim_piso bsim_piso;
 int bsim_piso_create(bsim_ctx* ctx, const bsim_piso_config* cfg, const int* u_bc_type, const double* u_bc_value,
                      const int* p_bc_type, const double* p_bc_value, bsim_piso** out);
 int bsim_piso_set_fields(bsim_piso* s, const double* U, const double* p);
+/* Wilcox k-omega in transient form, before the start: k and omega histories
+ * with their ddt terms (k_omega.cpp:59-62, 82, 96), the eddy viscosity and
+ * stress remainder in the momentum predictor, and the transport corrections
+ * after the pressure correctors (main.cpp:98-121, 165, 176-182, 251-266). */
+int bsim_piso_enable_komega(bsim_piso* s, const bsim_komega_config* k, const int* k_bc_type, const double* k_bc_value,
+                            const int* w_bc_type, const double* w_bc_value, const bsim_piso_turbulence* t);
+int bsim_piso_set_turbulence(bsim_piso* s, const double* k, const double* omega);
+int bsim_piso_get_turbulence(bsim_piso* s, double* k, double* omega, double* mut);
 /* phi = flux(U) and the time histories from the current fields (main.cpp:123, 157-158). */
 int bsim_piso_start(bsim_piso* s);
 /* The initial projection's record (zeros with done = 0 when it is not configured). */

@@ -579,6 +579,34 @@ void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, cons
     return s;
 }
 void orc_piso_free(void* s) { delete static_cast<Piso*>(s); }
+int orc_piso_enable_komega(void* sp, const orc_komega_config* kw, const int* kbct, const double* kbcv,
+                           const int* wbct, const double* wbcv, const orc_piso_turbulence* t) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (!kw || !t || !kbct || !kbcv || !wbct || !wbcv) throw std::invalid_argument("null argument");
+        s.enableKOmega(komegaConfig(kw), scalarBcs(s.m, kbct, kbcv), scalarBcs(s.m, wbct, wbcv),
+                       PisoTurbulence{t->coupling, t->correctors, t->max_iterations});
+    });
+}
+int orc_piso_set_turbulence(void* sp, const double* k, const double* omega) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (!s.turb) throw std::invalid_argument("laminar PISO has no turbulence fields");
+        for (Index i = 0; i < s.m.nc; ++i) {
+            if (k) s.turb->k.v[i] = k[i];
+            if (omega) s.turb->omega.v[i] = omega[i];
+        }
+    });
+}
+int orc_piso_get_turbulence(void* sp, double* k, double* omega, double* mut) {
+    return guard([&] {
+        Piso& s = *static_cast<Piso*>(sp);
+        if (!s.turb) throw std::invalid_argument("laminar PISO has no turbulence fields");
+        if (k) store(s.turb->k.v, k);
+        if (omega) store(s.turb->omega.v, omega);
+        if (mut) store(s.turb->mut, mut);
+    });
+}
 int orc_piso_set_fields(void* sp, const double* U, const double* p) {
     return guard([&] {
         Piso& s = *static_cast<Piso*>(sp);
@@ -604,6 +632,8 @@ int orc_piso_step(void* sp, double dt, orc_step_report* o) {
         for (int c = 0; c < 3; ++c) o->mom_iters[c] = r.momIters[c];
         o->p_iters = r.pIters; o->accepted = r.accepted; o->converged = r.converged; o->linear_ok = r.linearOk;
         o->status = r.status;
+        o->d_turb = r.dTurb; o->r_turb = r.rTurb; o->k_iters = r.kIters; o->w_iters = r.wIters;
+        o->turb_linear = r.turbLinear; o->turb_converged = r.turbConverged; o->turb_corrections = r.turbCorrections;
     });
 }
 int orc_piso_get_fields(void* sp, double* U, double* p, double* phi) {

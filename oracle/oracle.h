@@ -158,12 +158,24 @@ typedef struct {
 typedef struct {
     double dt, co_max, co_mean, mass, dU, rU, dP, r_coupling_first, r_coupling, lin_u, lin_p;
     int passes, correctors, mom_iters[3], p_iters, accepted, converged, linear_ok, status;
+    double d_turb, r_turb;
+    int k_iters, w_iters, turb_linear, turb_converged, turb_corrections;
 } orc_step_report;
+
+/* turbulenceCoupling (0 iterated, 1 segregated), turbulenceCorrectors (<= 0:
+ * the reference's default) and turbulenceMaxIterations. */
+typedef struct {
+    int coupling, correctors, max_iterations;
+} orc_piso_turbulence;
 
 void* orc_piso_new(void* mesh, const orc_piso_config* cfg, const int* ubct, const double* ubcv, const int* pbct,
                    const double* pbcv);
 void orc_piso_free(void* s);
 int orc_piso_set_fields(void* s, const double* U, const double* p);
+int orc_piso_enable_komega(void* s, const orc_komega_config* kw, const int* kbct, const double* kbcv,
+                           const int* wbct, const double* wbcv, const orc_piso_turbulence* t);
+int orc_piso_set_turbulence(void* s, const double* k, const double* omega);
+int orc_piso_get_turbulence(void* s, double* k, double* omega, double* mut);
 int orc_piso_start(void* s);
 int orc_piso_initial_report(void* s, orc_piso_initial* out);
 int orc_piso_step(void* s, double dt, orc_step_report* report);

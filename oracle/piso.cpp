@@ -57,7 +57,30 @@ void Piso::setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<do
     U.bc = ubc; p.bc = pbc;
 }
 
+void Piso::enableKOmega(const KOmegaConfig& kw, const std::vector<BC<double>>& kbc,
+                        const std::vector<BC<double>>& wbc, const PisoTurbulence& t) {
+    // main.cpp:99-120
+    if (t.coupling != 0 && t.coupling != 1) throw std::invalid_argument("turbulenceCoupling must be iterated or segregated");
+    const bool segregated = t.coupling == 1;
+    if (segregated && c.maxIterations != 1)
+        throw std::invalid_argument("segregated turbulence requires one PISO momentum pass");
+    if (segregated && kw.relaxation != 1.0)
+        throw std::invalid_argument("single-pass transient turbulence requires turbulenceRelaxation 1");
+    if (t.maxIterations < 1) throw std::invalid_argument("turbulenceMaxIterations must be positive");
+    tc = t;
+    if (tc.correctors <= 0) tc.correctors = c.maxIterations == 1 && !segregated ? t.maxIterations : 1;
+    if (tc.correctors > tc.maxIterations) throw std::invalid_argument("turbulenceCorrectors out of range");
+    if (segregated && tc.correctors != 1)
+        throw std::invalid_argument("segregated turbulence requires one transport correction");
+    turb = std::make_unique<KOmega>(m, kw, c.rho, c.mu);
+    turb->setBoundaries(kbc, wbc);
+    turb->timeMethod = c.timeMethod;
+}
+
 void Piso::start() {
+    // The model exists before phi (main.cpp:71, 98): its first bounding and
+    // viscosity see a zero face flux.
+    if (turb) turb->start(std::vector<double>(m.nf, 0.0));
     // main.cpp:123: phi = flux(U, pressure options); histories start from the
     // current fields (main.cpp:157-158, history.h:7).
     flux(U, phi, c.pInterp, c.pGrad);
@@ -198,6 +221,7 @@ StepReport Piso::step(double dt) {
     // main.cpp:162-167
     Uold.save(U, dt);
     phiOld.save(phi, dt);
+    if (turb) turb->saveOld(dt);
     std::vector<double> tf = temporalFluxCorrection();
     for (auto& v : tf) v = c.rho * v;
     lastTemporalFlux = tf;
@@ -220,7 +244,8 @@ StepReport Piso::step(double dt) {
         addConvection(mom, phi, U, c.mConv, c.mInterp, c.mGrad, c.rho);
         {
             std::vector<double> kf;
-            interpolate(muEff, kf, c.mCoefInterp, c.mCoefGrad);
+            if (turb) interpolate(turb->effectiveViscosity(), kf, c.mCoefInterp, c.mCoefGrad);
+            else interpolate(muEff, kf, c.mCoefInterp, c.mCoefGrad);
             Ldu<V3> contrib; contrib.init(m);
             addDiffusion<V3>(contrib, kf, U, c.mGrad, c.mDiff, nullptr);
             accumulate(mom, contrib, -(-1.0));
@@ -228,6 +253,11 @@ StepReport Piso::step(double dt) {
         std::vector<V3> gradP;
         gradient(p, gradP, c.pGrad);
         for (Index i = 0; i < m.nc; ++i) mom.source[i] += (1.0 * m.V[i]) * (-1.0 * gradP[i]);
+        if (turb) {  // main.cpp:179-182: the explicit remainder of the eddy-viscosity stress
+            std::vector<V3> divT;
+            turb->stressDivergence(U, c.mInterp, c.mGrad, divT);
+            for (Index i = 0; i < m.nc; ++i) mom.source[i] += (1.0 * m.V[i]) * divT[i];
+        }
         rep.rU = relativeResidual(m, mom, U.v);  // main.cpp:184
         if (!std::isfinite(rep.rU)) { rep.status = StepNumericalFailure; return rep; }
         for (Index i = 0; i < m.nc; ++i) {  // relax, no rescaling (main.cpp:186)
@@ -357,6 +387,28 @@ StepReport Piso::step(double dt) {
         rep.rCouplingFirst = first;
         rep.rCoupling = coupling;
         rep.linP = pLin;
+        // Turbulence transport at this time level (main.cpp:251-266).
+        bool turbulenceConverged = true, turbulenceLinear = true;
+        if (turb) {
+            rep.kIters = rep.wIters = 0;
+            for (int correction = 0; correction < tc.correctors; ++correction) {
+                const TransportReport t = turb->solveTransport(U, phi, c.mGrad);
+                rep.turbCorrections = correction + 1;
+                rep.kIters += t.itK;
+                rep.wIters += t.itOmega;
+                rep.dTurb = std::max(t.dOmega, t.dK);
+                rep.rTurb = std::max(t.rOmega, t.rK);
+                if (!t.healthy || !std::isfinite(rep.dTurb) || !std::isfinite(rep.rTurb)) {
+                    rep.status = StepNumericalFailure;
+                    return rep;
+                }
+                turbulenceLinear = t.statusK == Converged && t.statusOmega == Converged;
+                turbulenceConverged = turbulenceLinear && rep.dTurb <= turb->c.tolerance && rep.rTurb <= turb->c.tolerance;
+                if (!turbulenceLinear || turbulenceConverged) break;
+            }
+            rep.turbLinear = turbulenceLinear;
+            rep.turbConverged = turbulenceConverged;
+        }
         // Diagnostics (main.cpp:268-292; fvm_execution.cpp:91-205).
         {
             double d2 = 0, c2 = 0;
@@ -387,11 +439,11 @@ StepReport Piso::step(double dt) {
             rep.status = StepNumericalFailure;
             return rep;
         }
-        const bool linearConverged = momStatus == Converged && pressureConverged;
+        const bool linearConverged = momStatus == Converged && pressureConverged && turbulenceLinear;
         rep.linearOk = linearConverged;
         const bool accepted = linearConverged && rep.mass <= c.tolMass && coupling <= c.couplingTol;
-        const bool settled = rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP;
-        converged = accepted && (c.maxIterations == 1 ? true : settled);
+        const bool settled = rep.rU <= c.tolMom && rep.dU <= c.tolU && rep.dP <= c.tolP && turbulenceConverged;
+        converged = accepted && (c.maxIterations == 1 ? (tc.coupling == 1 || turbulenceConverged) : settled);
         rep.accepted = accepted;
         rep.converged = converged;
         if (!accepted) { rep.status = StepNotConverged; return rep; }

@@ -3,7 +3,10 @@
 
 #include <limits>
 
+#include "rans.hpp"
 #include "simple.hpp"
+
+#include <memory>
 
 namespace orc {
 
@@ -26,6 +29,13 @@ struct PisoConfig {
     int potentialInit = 0, initialNonOrthCorr = -1;  // main.cpp:104-105 (negative: nonOrthCorr)
 };
 
+// The turbulence controls of runPiso (main.cpp:98-120): coupling 0 iterated,
+// 1 segregated; correctors <= 0 takes the default (turbulenceMaxIterations
+// for iterated coupling with one momentum pass, else 1).
+struct PisoTurbulence {
+    int coupling = 0, correctors = 0, maxIterations = 1000;
+};
+
 // What runPiso records after the initial projection (main.cpp:147-149).
 struct InitReport {
     int done = 0, solves = 0, pIters = 0, status = StepCompleted;
@@ -38,6 +48,8 @@ struct StepReport {
     int passes = 0, correctors = 0;
     int momIters[3] = {0, 0, 0};
     int pIters = 0, accepted = 0, converged = 0, linearOk = 0, status = StepNumericalFailure;
+    double dTurb = 0, rTurb = 0;
+    int kIters = 0, wIters = 0, turbLinear = 1, turbConverged = 1, turbCorrections = 0;
 };
 
 // Explicit time levels (include/babelsim/history.h:6-27).
@@ -79,6 +91,11 @@ struct Piso {
     void setBoundaries(const std::vector<BC<V3>>& ubc, const std::vector<BC<double>>& pbc);
     void start();
     void initializePotentialFlow();
+    // Wilcox k-omega (main.cpp:98-121); before start().
+    void enableKOmega(const KOmegaConfig& kw, const std::vector<BC<double>>& kbc, const std::vector<BC<double>>& wbc,
+                      const PisoTurbulence& t);
+    std::unique_ptr<KOmega> turb;
+    PisoTurbulence tc;
     InitReport initial;
     StepReport step(double dt);
     std::vector<double> temporalFluxCorrection();

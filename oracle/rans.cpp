@@ -3,7 +3,8 @@
 // solves, bounding) and model.cpp:67-71 (the deviatoric stress remainder of
 // the momentum equation), with the calculated boundary traces of the field
 // algebra (include/babelsim/math.h:185-258) and the vector-gradient traces of
-// fvm_execution.cpp:243-265.  Steady form (the ddt terms vanish).
+// fvm_execution.cpp:243-265.  Steady form for SIMPLE; in PISO the k and omega
+// histories add the ddt terms of k_omega.cpp:59-62, 82, 96.
 // Test infrastructure only.
 #include "rans.hpp"
 
@@ -191,11 +192,26 @@ void solveScalar(const Mesh& m, LinearSystem& sys, const Ldu<double>& q, const S
 
 // One transport equation: convection, diffusion with a calculated
 // diffusivity, implicit reaction and explicit source (k_omega.cpp:82-87, 96-101).
+// equ::ddt(equation, rho, history) (procedural_equation.cpp:198-239): weights
+// a0, a1, a2 of the history's method (Euler below two saved levels).
+struct TimeTerm {
+    double a0 = 0, a1 = 0, a2 = 0;
+    const std::vector<double>* previous = nullptr;
+    const std::vector<double>* older = nullptr;
+};
+
 Ldu<double> transportEquation(const Mesh& m, SField& x, const std::vector<double>& phi, double rho,
                               const TransportOptions& o, const SField& diffusivity, const std::vector<double>& rate,
-                              const std::vector<double>& src) {
+                              const std::vector<double>& src, const TimeTerm* tt) {
     Ldu<double> eq;
     eq.init(m);
+    if (tt)
+        for (Index i = 0; i < m.nc; ++i) {
+            const double w = rho * m.V[i];
+            eq.diag[i] += w * tt->a0;
+            eq.source[i] += w * tt->a1 * (*tt->previous)[i];
+            if (tt->a2 != 0) eq.source[i] += w * tt->a2 * (*tt->older)[i];
+        }
     addConvection(eq, phi, x, o.conv, o.interp, o.grad, rho);
     {
         std::vector<double> kf;
@@ -220,8 +236,37 @@ void relax(Ldu<double>& eq, const std::vector<double>& prev, double alpha) {  //
 
 }  // namespace
 
+void KOmega::saveOld(double dt) {
+    kOlder = kPrev.empty() ? k.v : kPrev;
+    wOlder = wPrev.empty() ? omega.v : wPrev;
+    kPrev = k.v;
+    wPrev = omega.v;
+    hdtPrev = hdt;
+    hdt = dt;
+    ++hlevels;
+}
+
 TransportReport KOmega::solveTransport(VField& U, const std::vector<double>& phi, int mGrad) {
     TransportReport rep;
+    TimeTerm tk, tw;
+    const TimeTerm* ptk = nullptr;
+    const TimeTerm* ptw = nullptr;
+    if (timeMethod != 0) {
+        if (hlevels == 0) throw std::logic_error("saveOld must precede a transient assembly");
+        const int method = timeMethod == 2 && hlevels < 2 ? 1 : timeMethod;
+        double a0 = 1 / hdt, a1 = 1 / hdt, a2 = 0;
+        if (method == 2) {
+            const double olddt = hdtPrev == 0 ? hdt : hdtPrev;
+            const double r = hdt / olddt;
+            a0 = (1 + 2 * r) / ((1 + r) * hdt);
+            a1 = (1 + r) / hdt;
+            a2 = -r * r / ((1 + r) * hdt);
+        }
+        tk = TimeTerm{a0, a1, a2, &kPrev, &kOlder};
+        tw = TimeTerm{a0, a1, a2, &wPrev, &wOlder};
+        ptk = &tk;
+        ptw = &tw;
+    }
     const std::vector<double> prevK = k.v, prevW = omega.v;
     U.fluxCtx = phi;
     updateViscosity(phi);
@@ -245,7 +290,7 @@ TransportReport KOmega::solveTransport(VField& U, const std::vector<double>& phi
         src[i] = ((c.gamma * production[i]) * omega.v[i]) / std::max(k.v[i], c.kMin);
     }
     // omega first (k_omega.cpp:80-93)
-    Ldu<double> weq = transportEquation(m, omega, phi, rho, c.wOpts, wDiff, rate, src);
+    Ldu<double> weq = transportEquation(m, omega, phi, rho, c.wOpts, wDiff, rate, src, ptw);
     rep.rOmega = relativeResidual(m, weq, omega.v);
     relax(weq, prevW, c.relaxation);
     SolveResult r;
@@ -257,7 +302,7 @@ TransportReport KOmega::solveTransport(VField& U, const std::vector<double>& phi
     boundOmega(bounded);
     // then k with the new omega in its sink (k_omega.cpp:95-106)
     for (Index i = 0; i < m.nc; ++i) rate[i] = (c.betaStar * rho) * omega.v[i];
-    Ldu<double> keq = transportEquation(m, k, phi, rho, c.kOpts, kDiff, rate, production);
+    Ldu<double> keq = transportEquation(m, k, phi, rho, c.kOpts, kDiff, rate, production, ptk);
     rep.rK = relativeResidual(m, keq, k.v);
     relax(keq, prevK, c.relaxation);
     solveScalar(m, kSys, keq, c.kSolver, k.v, r, kHist);

@@ -54,6 +54,13 @@ struct KOmega {
     TransportReport solveTransport(VField& U, const std::vector<double>& phi, int mGrad);
     void updateViscosity(const std::vector<double>& phi);
     void boundOmega(double& bounded);
+    // Transient form (PISO): the k and omega histories (history.h:6-27) and
+    // the ddt terms of k_omega.cpp:82, 96.  timeMethod 0 is the steady form.
+    int timeMethod = 0;
+    std::vector<double> kPrev, kOlder, wPrev, wOlder;
+    double hdt = 0, hdtPrev = 0;
+    int hlevels = 0;
+    void saveOld(double dt);  // k_omega.cpp:59-62
 };
 
 }  // namespace orc

@@ -22,6 +22,7 @@ STRUCTS = {
     "bsim_piso_config": _capi.PisoConfig,
     "bsim_step_report": _capi.StepReport,
     "bsim_piso_initial": _capi.PisoInitial,
+    "bsim_piso_turbulence": _capi.PisoTurbulence,
     "bsim_request": _capi.Request,
     "bsim_request_result": _capi.RequestResult,
     "bsim_equation_counters": _capi.EquationCounters,

@@ -11,7 +11,7 @@ import pytest
 import cases
 import oracle_lib as orc
 import piso_cases as pc
-from babelsim_amd import BDF2, EULER, STEP_COMPLETED, STEP_NOT_CONVERGED, solver_config
+from babelsim_amd import BDF2, EULER, STEP_COMPLETED, STEP_NOT_CONVERGED, MeshData, solver_config
 from babelsim_amd.device import Context, Piso, TimeStepper
 
 pytestmark = pytest.mark.gpu
@@ -245,3 +245,89 @@ def test_potential_flow_failure_ends_the_run():
                   lambda C: np.tile([0.3, -0.2, 0.1], (len(C), 1)))
     assert sg.initial().status == so.initial().status == 2
     assert sg.step(0.01).status == so.step(0.01).status == 2
+
+
+# Transient k-omega inside PISO (piso/main.cpp:98-121, 165, 176-182, 251-266).
+def test_transient_komega_decay_on_device():
+    """The closed box at rest of the oracle's known answer: uniform k and
+    omega follow the implicit step of dk/dt = -beta* k omega, domega/dt =
+    -beta omega^2 (positive root for omega, Euler then variable-step BDF2)."""
+    from babelsim_amd import komega_config
+    arrays, _, _ = MeshData.hex((4, 3, 2)).prepare()
+    ctx = Context(arrays.arrays(), renumber=True)
+    ub, pb = cases.cavity_bcs(False)
+    ub = (ub[0], np.zeros_like(ub[1]))
+    zg = (np.full(6, pc.ZG, np.int32), np.zeros(6))
+    kw = komega_config(tolerance=1e-13, k_solver=solver_config(solver=1, pc=1, rtol=1e-14, atol=1e-30),
+                       w_solver=solver_config(solver=1, pc=1, rtol=1e-14, atol=1e-30))
+    dts = [0.05, 0.1, 0.07]
+    s = Piso(ctx, pc.cavity_piso_config(time_method=BDF2), ub, pb)
+    s.enable_komega(kw, zg, zg)
+    s.set_turbulence(np.full(ctx.nc, 0.3), np.full(ctx.nc, 2.0))
+    s.start()
+    kp, wp, ko, wo, dtp = 0.3, 2.0, 0.3, 2.0, 0.0
+    for n, dt in enumerate(dts):
+        r = s.step(dt)
+        assert r.status == STEP_COMPLETED and r.turb_converged == 1
+        if n > 0:
+            q = dt / dtp
+            a0, a1, a2 = (1 + 2 * q) / ((1 + q) * dt), (1 + q) / dt, -q * q / ((1 + q) * dt)
+        else:
+            a0, a1, a2 = 1 / dt, 1 / dt, 0.0
+        rw, rk = a1 * wp + a2 * wo, a1 * kp + a2 * ko
+        wn = (-a0 + np.sqrt(a0 * a0 + 4 * kw.beta * rw)) / (2 * kw.beta)
+        kn = rk / (a0 + kw.beta_star * wn)
+        k, w, _ = s.turbulence()
+        np.testing.assert_allclose(w, wn, rtol=1e-10)
+        np.testing.assert_allclose(k, kn, rtol=1e-10)
+        ko, wo, kp, wp, dtp = kp, wp, kn, wn, dt
+
+
+TRANSIENT_KOMEGA = {
+    # the URANS cylinder case's coupling: iterated, eight transport corrections
+    "iterated_8": dict(coupling=0, correctors=8, relaxation=0.7, max_iterations=1, tol=3e-3, tol_u=1e-4),
+    # iterated until the transport settles (up to turbulenceMaxIterations)
+    "iterated_settle": dict(coupling=0, correctors=0, relaxation=0.7, max_iterations=1, tol=1e-6, tol_u=1e-4),
+    "segregated": dict(coupling=1, correctors=0, relaxation=1.0, max_iterations=1, tol=1e-6, tol_u=1e-4),
+    "outer_passes": dict(coupling=0, correctors=0, relaxation=0.7, max_iterations=4, tol=5e-2, tol_u=2e-2),
+}
+
+
+@pytest.mark.parametrize("kind", list(TRANSIENT_KOMEGA))
+def test_transient_komega_parity(kind):
+    """A walled channel with inlet turbulence (the steady k-omega tests'
+    boundary values): device and oracle steps agree in every count and to
+    1e-9 in the fields."""
+    from babelsim_amd import komega_config
+    o = TRANSIENT_KOMEGA[kind]
+    mesh = MeshData.hex((16, 10, 1), (0, 0, 0), (2.0, 1.0, 0.1))
+    ub = (np.array([pc.FV, pc.ZG, pc.FV, pc.FV, pc.SYM, pc.SYM], np.int32), np.array([[1.0, 0, 0]] + [[0, 0, 0]] * 5))
+    pb = (np.array([pc.ZG, pc.FV, pc.ZG, pc.ZG, pc.SYM, pc.SYM], np.int32), np.zeros(6))
+    kb = (np.array([pc.FV, pc.ZG, pc.FV, pc.FV, pc.ZG, pc.ZG], np.int32), np.array([1e-3, 0, 1e-10, 1e-10, 0, 0]))
+    wb = (np.array([pc.FV, pc.ZG, pc.FV, pc.FV, pc.ZG, pc.ZG], np.int32), np.array([1.0, 0, 200.0, 200.0, 0, 0]))
+    cfg = pc.cavity_piso_config(mu=1e-3, max_iterations=o["max_iterations"], tol_u=o["tol_u"], tol_mom=o["tol_u"],
+                                tol_p=10 * o["tol_u"])
+    kw = komega_config(relaxation=o["relaxation"], tolerance=o["tol"])
+    arrays, _, _ = mesh.prepare()
+    arrays = arrays.arrays()
+    om = orc.oracle_mesh(arrays)
+    so = orc.OraclePiso(om, cfg, ub, pb)
+    sg = Piso(Context(arrays, renumber=False), cfg, ub, pb)
+    for s in (so, sg):
+        s.enable_komega(kw, kb, wb, coupling=o["coupling"], correctors=o["correctors"])
+        s.set_turbulence(np.full(om.nc, 1e-3), np.full(om.nc, 1.0))
+        s.start()
+    for dt in [0.02] * 4:
+        ro, rg = so.step(dt), sg.step(dt)
+        compare_step(rg, ro)
+        assert (rg.k_iters, rg.w_iters, rg.turb_corrections, rg.turb_linear, rg.turb_converged) == \
+            (ro.k_iters, ro.w_iters, ro.turb_corrections, ro.turb_linear, ro.turb_converged)
+        for key in ("d_turb", "r_turb"):
+            assert abs(getattr(rg, key) - getattr(ro, key)) <= 1e-8 * abs(getattr(ro, key)) + 1e-14, key
+        assert ro.status == STEP_COMPLETED
+    print(f"{kind}: last step {ro.passes} passes, {ro.turb_corrections} corrections, k {ro.k_iters} omega "
+          f"{ro.w_iters} iterations", flush=True)
+    for a, b in zip(sg.fields(), so.fields()):
+        assert rel_l2(a, b) < 1e-9
+    for a, b in zip(sg.turbulence(), so.turbulence()):
+        assert rel_l2(a, b) < 1e-9

@@ -309,3 +309,87 @@ def test_potential_flow_solve_count_and_failure():
     s.start()
     assert s.initial().status == 2
     assert s.step(0.01).status == 2
+
+
+# ------------------------------------------------- transient k-omega in PISO
+def decaying_turbulence(method, dts, coupling=0, relaxation=0.7):
+    """A closed box at rest with zeroGradient k and omega: no convection,
+    diffusion or production, so every cell follows dk/dt = -beta* k omega,
+    domega/dt = -beta omega^2 (k_omega.cpp:80-106 with zero strain)."""
+    from babelsim_amd import komega_config, solver_config
+    arrays, _, _ = MeshData.hex((4, 3, 2)).prepare()
+    om = orc.oracle_mesh(arrays.arrays())
+    ub, pb = cases.cavity_bcs(False)
+    ub = (ub[0], np.zeros_like(ub[1]))  # every wall at rest
+    zg = (np.full(6, pc.ZG, np.int32), np.zeros(6))
+    kw = komega_config(relaxation=relaxation, tolerance=1e-13,
+                       k_solver=solver_config(solver=1, pc=1, rtol=1e-14, atol=1e-30),
+                       w_solver=solver_config(solver=1, pc=1, rtol=1e-14, atol=1e-30))
+    s = orc.OraclePiso(om, pc.cavity_piso_config(time_method=method), ub, pb)
+    s.enable_komega(kw, zg, zg, coupling=coupling)
+    s.set_turbulence(np.full(om.nc, 0.3), np.full(om.nc, 2.0))
+    s.start()
+    out = []
+    for dt in dts:
+        r = s.step(dt)
+        assert r.status == STEP_COMPLETED, r.status
+        k, w, _ = s.turbulence()
+        out.append((k, w, r))
+    return kw, out
+
+
+def test_transient_komega_iterated_is_implicit_time_integration():
+    """Iterated coupling (the default: corrections until the transport
+    settles) reaches the fully implicit step of the decay ODEs: the positive
+    root of beta dt w^2 + a0 dt w - rhs = 0 for omega (Euler, then variable-step
+    BDF2), then k = rhs_k / (a0 + beta* w)."""
+    dts = [0.05, 0.1, 0.07, 0.07]
+    for method in (EULER, BDF2):
+        kw, out = decaying_turbulence(method, dts)
+        beta, bstar = kw.beta, kw.beta_star
+        kp, wp, ko, wo, dtp = 0.3, 2.0, 0.3, 2.0, 0.0
+        for n, (dt, (k, w, r)) in enumerate(zip(dts, out)):
+            if method == BDF2 and n > 0:
+                q = dt / dtp
+                a0, a1, a2 = (1 + 2 * q) / ((1 + q) * dt), (1 + q) / dt, -q * q / ((1 + q) * dt)
+            else:
+                a0, a1, a2 = 1 / dt, 1 / dt, 0.0
+            rw, rk = a1 * wp + a2 * wo, a1 * kp + a2 * ko
+            wn = (-a0 + np.sqrt(a0 * a0 + 4 * beta * rw)) / (2 * beta)
+            kn = rk / (a0 + bstar * wn)
+            np.testing.assert_allclose(w, wn, rtol=1e-10)
+            np.testing.assert_allclose(k, kn, rtol=1e-10)
+            assert r.turb_converged == 1 and r.turb_corrections > 1
+            ko, wo, kp, wp, dtp = kp, wp, kn, wn, dt
+
+
+def test_transient_komega_segregated_is_one_picard_step():
+    """Segregated coupling (one transport correction, relaxation 1): omega's
+    sink uses omega of the previous level, k's sink the new omega."""
+    dts = [0.05, 0.1]
+    kw, out = decaying_turbulence(EULER, dts, coupling=1, relaxation=1.0)
+    k0, w0 = 0.3, 2.0
+    for dt, (k, w, r) in zip(dts, out):
+        w1 = w0 / (1 + dt * kw.beta * w0)
+        k1 = k0 / (1 + dt * kw.beta_star * w1)
+        np.testing.assert_allclose(w, w1, rtol=1e-12)
+        np.testing.assert_allclose(k, k1, rtol=1e-12)
+        assert r.turb_corrections == 1
+        k0, w0 = k1, w1
+
+
+def test_transient_komega_controls_follow_run_piso():
+    """main.cpp:99-120: segregated coupling needs one momentum pass and
+    turbulenceRelaxation 1; turbulenceCorrectors stays within the limit."""
+    from babelsim_amd import komega_config
+    arrays, _, _ = MeshData.hex((3, 3, 2)).prepare()
+    om = orc.oracle_mesh(arrays.arrays())
+    ub, pb = cases.cavity_bcs(False)
+    zg = (np.full(6, pc.ZG, np.int32), np.zeros(6))
+    for cfg, kw, kwargs in [(pc.cavity_piso_config(max_iterations=2), komega_config(relaxation=1.0), dict(coupling=1)),
+                            (pc.cavity_piso_config(), komega_config(relaxation=0.7), dict(coupling=1)),
+                            (pc.cavity_piso_config(), komega_config(), dict(correctors=20, max_iterations=10)),
+                            (pc.cavity_piso_config(), komega_config(relaxation=1.0), dict(coupling=1, correctors=2))]:
+        s = orc.OraclePiso(om, cfg, ub, pb)
+        with pytest.raises(ValueError):
+            s.enable_komega(kw, zg, zg, **kwargs)
