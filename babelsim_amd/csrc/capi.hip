@@ -397,7 +397,7 @@ const char* kspName(int solver) {
     return solver == 0 ? "cg" : solver == 1 ? "bcgs" : solver == 2 ? "gmres" : solver == 3 ? "fgmres" : "unknown";
 }
 const char* pcName(int pc) {
-    return pc == 0 ? "none" : pc == 1 ? "jacobi" : pc == 2 ? "amg" : pc == 3 ? "bjacobi" : "unknown";
+    return pc == 0 ? "none" : pc == 1 ? "jacobi" : pc == 2 ? "amg" : pc == 3 ? "bjacobi" : pc == 4 ? "icc" : pc == 5 ? "asm" : "unknown";
 }
 const char* statusName(int st) {
     return st == kConverged ? "converged" : st == kMaxIterations ? "maxIterations" : st == kNumericalFailure ? "numericalFailure" : "";

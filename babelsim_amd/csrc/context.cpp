@@ -97,8 +97,9 @@ SolverCfg toSolverCfg(const void* p) {
     if (!(s.atol > 0.0) || !std::isfinite(s.atol) || !(s.rtol > 0.0) || !std::isfinite(s.rtol) || s.maxIter <= 0)
         throw std::invalid_argument("linear solver tolerances or iteration limit are invalid");
     if (s.solver < 0 || s.solver > 3) throw std::invalid_argument("unsupported Krylov solver type");
-    if (s.pc < 0 || s.pc > 3) throw std::invalid_argument("unsupported preconditioner type");
-    if (s.pc == 3 && s.precision != 0) throw std::invalid_argument("the block Jacobi preconditioner runs in FP64");
+    if (s.pc < 0 || s.pc > 5) throw std::invalid_argument("unsupported preconditioner type");
+    if (s.pc >= 3 && s.precision != 0)
+        throw std::invalid_argument("the block Jacobi, icc and asm preconditioners run in FP64");
     if (s.pc == 2 && (s.amgMaxLevels <= 0 || s.amgCoarseSize <= 0 || s.amgPre <= 0 || s.amgPost < 0))
         throw std::invalid_argument("AMG hierarchy controls must be positive");
     if (s.rebuildInterval < 1 || !std::isfinite(s.reuseGrowth) || s.reuseGrowth < 1.0)

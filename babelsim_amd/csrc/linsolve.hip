@@ -5,6 +5,7 @@
 // last block of the reduction that produces it, and the host reads one
 // residual norm per iteration to run BabelSim's convergence test.
 #include "linsolve.hpp"
+#include "ilu.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -1654,6 +1655,14 @@ void DeviceSystem::setup(bool reuse) {
             iluVal.ensure(std::max(nnz, 1));
             kBlockIluFactor<<<(n + kIluBlock - 1) / kIluBlock, kIluBlock, 0, stream>>>(view(), iluDiag.p, iluVal.p);
             BSIM_LAUNCH_CHECK();
+        } else if (cfg.pc == 4 || cfg.pc == 5) {
+            // The symbolic analysis depends on the pattern and the kind only.
+            const bool chol = cfg.pc == 4;
+            if (!lilu || lilu->cholesky() != chol) {
+                lilu = std::make_unique<LevelIlu>(stream, chol);
+                lilu->analyse(pat, n);
+            }
+            lilu->factor(view());
         } else if (cfg.pc == 2) {
             if (!amg) amg = std::make_unique<AmgDevice>(stream);
             amg->probe = &probe;
@@ -1679,6 +1688,7 @@ void DeviceSystem::applyPC(const double* rr, double* zz) {
     else if (cfg.pc == 1) kJacobiApply<<<blocksFor(n), kBlock, 0, stream>>>(n, rr, jinv.p, zz);
     else if (cfg.pc == 3)
         kBlockIluApply<<<(n + kIluBlock - 1) / kIluBlock, kIluBlock, 0, stream>>>(view(), iluDiag.p, iluVal.p, rr, zz);
+    else if (cfg.pc >= 4) lilu->apply(rr, zz);
     else amg->apply(rr, zz);
     BSIM_LAUNCH_CHECK();
 }
@@ -1823,7 +1833,13 @@ void DeviceSystem::cg(const double* b, double* x, bool guessZero, int maxIt, dou
         dp = readScalar(S_DP, &fl);
         if (fl & F_BETA_ZERO) { reason = 3; break; }
         if (fl & F_INDEF_PC) { reason = -8; break; }
-        if (fl & F_NONFINITE) { reason = -9; break; }
+        if (fl & F_NONFINITE) {
+            // A non-finite z . r ends the solve where it was formed, one
+            // iteration earlier (KSPCheckDot after the preconditioner).
+            reason = -9;
+            if (!std::isfinite(readScalar(S_BETA))) its = i;
+            break;
+        }
         if (fl & F_INDEF_MAT) { reason = -10; break; }
         hist.push_back(dp);
         if (test(dp)) break;

@@ -18,7 +18,8 @@ constexpr int kGmRestart = 30;
 
 struct SolverCfg {
     // solver: 0 CG, 1 BiCGSTAB, 2 GMRES, 3 FGMRES (LinearSolverType order);
-    // pc: 0 none, 1 Jacobi, 2 aggregation AMG, 3 block Jacobi (ILU(0) per 64 rows)
+    // pc: 0 none, 1 Jacobi, 2 aggregation AMG, 3 block Jacobi (ILU(0) per 64 rows),
+    // 4 IC(0) of the whole matrix (icc), 5 ILU(0) of the whole matrix (asm on one rank)
     int solver = 1, pc = 2;
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000, warmStart = 0, rebuildInterval = 8;
@@ -62,6 +63,7 @@ struct Pattern {
 };
 
 class AmgDevice;
+class LevelIlu;
 
 // HIP-event timing of one kernel's launches: each timed launch is bracketed
 // by a pair of events recorded on the launching stream.  The events are
@@ -199,6 +201,7 @@ private:
     int freshIterations[3] = {0, 0, 0};
     DBuf<double> jinv;          // Jacobi: 1/diag of the matrix at PC setup
     DBuf<double> iluDiag, iluVal;  // block Jacobi: ILU(0) factors of the 64-row blocks at PC setup
+    std::unique_ptr<LevelIlu> lilu;  // icc / asm: level-scheduled factors of the whole matrix
     std::unique_ptr<AmgDevice> amg;
 public:
     KernelProbe probe;  // times the fine-level AMG smoother sweeps

@@ -129,9 +129,12 @@ def parse():
     ap.add_argument("--amg-correction", type=float, default=2.0,
                     help="cavity: over-correction factor of the pressure AMG coarse-grid correction")
     ap.add_argument("--amg-omega", type=float, default=0.9, help="cavity: damped-Jacobi weight of the AMG smoother")
-    ap.add_argument("--mom-pc", choices=["jacobi", "bjacobi"], default="jacobi",
+    ap.add_argument("--mom-pc", choices=["jacobi", "bjacobi", "asm"], default="jacobi",
                     help="cavity: momentum BiCGSTAB preconditioner (bjacobi: ILU(0) in 64-row blocks, the "
-                         "reference cavity case's pcType)")
+                         "reference cavity case's pcType; asm: ILU(0) of the whole matrix)")
+    ap.add_argument("--p-pc", choices=["amg", "icc"], default="amg",
+                    help="cavity: pressure CG preconditioner (icc: IC(0) of the whole matrix, the reference "
+                         "SIMPLE tests' pressure preconditioner)")
     ap.add_argument("--amg-pre", type=int, default=2, help="cavity: AMG pre-smoothing sweeps per level")
     ap.add_argument("--amg-post", type=int, default=2, help="cavity: AMG post-smoothing sweeps per level")
     ap.add_argument("--amg-coarse-size", type=int, default=48, help="cavity: AMG coarsest-level row target")
@@ -226,10 +229,11 @@ def main():
         # at relativeTolerance 1e-8, absoluteTolerance 1e-14.
         common = dict(atol=1e-14, precision=precision, rebuild_interval=args.rebuild_interval,
                       warm_start=int(args.warm_start))
-        mpc = 3 if (mom_pc or args.mom_pc) == "bjacobi" and precision == 0 else 1
+        mpc = {"bjacobi": 3, "asm": 5}.get(mom_pc or args.mom_pc, 1) if precision == 0 else 1
+        ppc = 4 if args.p_pc == "icc" and precision == 0 else 2
         return cases.c2_config(
             mom=solver_config(solver=1, pc=mpc, rtol=rtol, **common),
-            pre=solver_config(solver=0, pc=2, rtol=rtol, amg_strength=args.amg_strength,
+            pre=solver_config(solver=0, pc=ppc, rtol=rtol, amg_strength=args.amg_strength,
                               amg_correction=args.amg_correction, amg_omega=args.amg_omega,
                               amg_pre_sweeps=args.amg_pre, amg_post_sweeps=args.amg_post,
                               amg_coarse_size=args.amg_coarse_size, **common))
@@ -367,11 +371,13 @@ def main():
         "config": {"workload": f"3-D lid-driven cavity Re=400, {n}^3 hex lattice, +-20% jitter, ~10% prism splits",
                    "cells": ctx.nc, "faces": ctx.nf, "nnz": nnz, "precision": args.precision,
                    "linear_rtol": args.rtol, "linear_atol": 1e-14,
-                   "pressure": f"CG + aggregation AMG (strength {args.amg_strength}, correction "
-                               f"{args.amg_correction}, Jacobi weight {args.amg_omega}, sweeps "
-                               f"{args.amg_pre}/{args.amg_post}, coarsest {args.amg_coarse_size} rows)",
-                   "momentum": "BiCGSTAB + " + ("block Jacobi (ILU(0) per 64 rows)" if args.mom_pc == "bjacobi"
-                                                else "Jacobi"), "warm_start": args.warm_start,
+                   "pressure": (f"CG + aggregation AMG (strength {args.amg_strength}, correction "
+                                f"{args.amg_correction}, Jacobi weight {args.amg_omega}, sweeps "
+                                f"{args.amg_pre}/{args.amg_post}, coarsest {args.amg_coarse_size} rows)"
+                                if args.p_pc == "amg" else "CG + IC(0) of the whole matrix"),
+                   "momentum": "BiCGSTAB + " + {"bjacobi": "block Jacobi (ILU(0) per 64 rows)",
+                                                "asm": "ILU(0) of the whole matrix"}.get(args.mom_pc, "Jacobi"),
+                   "warm_start": args.warm_start,
                    "timed_linear_solves": "all converged",
                    "last_iteration": {"p_its": last.p_iters, "mom_its": list(last.mom_iters)},
                    "phase_seconds_per_iteration": timing, "device_memory_used_GB": used_gb,

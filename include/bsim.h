@@ -88,8 +88,12 @@ enum { BSIM_ORTHOGONAL = 0, BSIM_DIFF_CORRECTED = 1, BSIM_LIMITED_CORRECTED = 2 
  * (PETSc's default) and precondition from the right. */
 enum { BSIM_CG = 0, BSIM_BICGSTAB = 1, BSIM_GMRES = 2, BSIM_FGMRES = 3 };
 /* solver_control.h:21-29: hypre/gamg -> AMG; bjacobi -> ILU(0) in blocks of
- * 64 consecutive rows (PCBJACOBI with ceil(n/64) blocks, FP64 only). */
-enum { BSIM_PC_NONE = 0, BSIM_PC_JACOBI = 1, BSIM_PC_AMG = 2, BSIM_PC_BJACOBI = 3 };
+ * 64 consecutive rows (PCBJACOBI with ceil(n/64) blocks); icc -> IC(0) of the
+ * whole matrix from its upper triangle (PCICC); asm -> ILU(0) of the whole
+ * matrix (PCASM on one rank: one subdomain, sub-PC ILU(0)).  The last three
+ * are FP64 only; icc and asm run level-scheduled substitutions. */
+enum { BSIM_PC_NONE = 0, BSIM_PC_JACOBI = 1, BSIM_PC_AMG = 2, BSIM_PC_BJACOBI = 3, BSIM_PC_ICC = 4,
+       BSIM_PC_ASM = 5 };
 enum { BSIM_FP64 = 0, BSIM_MIXED = 1 };
 enum { BSIM_CONVERGED = 0, BSIM_MAX_ITERATIONS = 1, BSIM_NUMERICAL_FAILURE = 2 };  /* solver_control.h:30-34 */
 

@@ -266,6 +266,23 @@ int orc_block_ilu(int n, const int* ptr, const int* col, const double* diag, con
         if (!v.empty()) std::memcpy(fv, v.data(), sizeof(double) * v.size());
     });
 }
+int orc_whole_ilu(int n, const int* ptr, const int* col, const double* diag, const double* val, int chol,
+                  int* fptr, int* fcol, double* f, int cap, int* nf, int* failed) {
+    return guard([&] {
+        const Csr A = makeCsr(n, ptr, col, diag, val);
+        WholeIlu w;
+        w.analyse(A, chol != 0);
+        w.factor(A);
+        *nf = static_cast<int>(w.col.size());
+        *failed = w.fail ? 1 : 0;
+        if (*nf > cap) throw std::invalid_argument("factor capacity too small");
+        std::memcpy(fptr, w.ptr.data(), sizeof(int) * (n + 1));
+        if (*nf) {
+            std::memcpy(fcol, w.col.data(), sizeof(int) * *nf);
+            std::memcpy(f, w.f.data(), sizeof(double) * *nf);
+        }
+    });
+}
 int orc_block_ilu_apply(int n, const int* ptr, const int* col, const double* diag, const double* fd,
                         const double* fv, const double* r, double* z) {
     return guard([&] {

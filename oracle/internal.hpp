@@ -190,7 +190,7 @@ struct Csr {  // diagonal stored separately; off-diagonal entries one per interi
 
 struct SolverConfig {
     int solver = 1;          // 0 CG, 1 BiCGSTAB, 2 GMRES, 3 FGMRES (restart 30, right preconditioning)
-    int pc = 2;              // 0 none, 1 Jacobi, 2 AMG, 3 block Jacobi (ILU(0) per 64 rows)
+    int pc = 2;              // 0 none, 1 Jacobi, 2 AMG, 3 block Jacobi (ILU(0) per 64 rows), 4 IC(0), 5 ILU(0)
     double atol = 1e-12, rtol = 1e-8;
     int maxIter = 1000;
     int warmStart = 0;
@@ -246,6 +246,20 @@ struct AmgLevel {
     Index nc = 0;
     std::vector<Index> memPtr, memIdx;  // coarse -> fine members, ascending (the restriction's gather)
 };
+// IC(0) or ILU(0) of the whole matrix with no fill (pc 4 icc, pc 5 asm on
+// one rank; petsc_linear_system.cpp:204-208, 224-226), in the rows' natural
+// order.  Factor pattern row-major with columns ascending, the diagonal at
+// dpos[i]; IC(0) takes the upper triangle and its transpose.
+struct WholeIlu {
+    bool chol = false, analysed = false, fail = false;
+    Index n = 0;
+    std::vector<Index> ptr, col, dpos, srcPtr, src, uptr, udst, usrc;
+    std::vector<double> f;
+    void analyse(const Csr& A, bool cholesky);
+    void factor(const Csr& A);
+    void apply(const double* r, double* z) const;
+};
+
 struct Amg {
     std::vector<AmgLevel> levels;
     std::vector<double> coarseLU;    // dense LU of coarsest, row-major
@@ -255,6 +269,7 @@ struct Amg {
     std::vector<std::vector<double>> wx, wb, wr;
     bool aggregated = false;  // aggregates are built at the first setup and kept
     std::vector<double> iluDiag, iluVal;  // block Jacobi factors (pc 3), at the matrix's entries
+    WholeIlu whole;                       // icc / asm factors (pc 4, 5)
     void setup(const Csr& A, const SolverConfig& c);
     void apply(const double* r, double* z);
     void vcycle(int l);

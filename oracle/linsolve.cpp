@@ -438,6 +438,117 @@ void blockIluApply(const Csr& A, const std::vector<double>& fd, const std::vecto
     }
 }
 
+// IC(0) / ILU(0) of the whole matrix (PETSc's icc, petsc_linear_system.cpp:
+// 204-208, factors the upper triangle; asm on one rank, :224-226, is one
+// subdomain with the sub-PC's default ILU(0)), natural row order, no fill.
+// Factor entry (i, j) sums its sources: the diagonal first, then the stored
+// entries in row-major order; for IC(0) a lower entry (i, k) takes the upper
+// entry (k, i).  Row i is eliminated in IKJ order: for each lower (i, k) in
+// column order, l = a_ik / u_kk, then a_ij -= l u_kj for the columns j > k
+// present in both rows.  A pivot that is zero (ILU), not positive (IC) or not
+// finite fails the factorisation, after which the preconditioner returns +Inf
+// in every entry (MatSolve on a failed factor).
+void WholeIlu::analyse(const Csr& A, bool cholesky) {
+    chol = cholesky;
+    n = A.n;
+    std::vector<std::vector<Index>> rows(n);
+    for (Index i = 0; i < n; ++i) {
+        rows[i].push_back(i);
+        for (Index q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
+            const Index j = A.col[q];
+            if (j == i) continue;
+            if (!chol) rows[i].push_back(j);
+            else if (j > i) { rows[i].push_back(j); rows[j].push_back(i); }
+        }
+    }
+    ptr.assign(n + 1, 0);
+    col.clear();
+    for (Index i = 0; i < n; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        rows[i].erase(std::unique(rows[i].begin(), rows[i].end()), rows[i].end());
+        col.insert(col.end(), rows[i].begin(), rows[i].end());
+        ptr[i + 1] = static_cast<Index>(col.size());
+    }
+    auto at = [&](Index i, Index j) {
+        for (Index e = ptr[i]; e < ptr[i + 1]; ++e)
+            if (col[e] == j) return e;
+        return Index(-1);
+    };
+    dpos.assign(n, 0);
+    for (Index i = 0; i < n; ++i) dpos[i] = at(i, i);
+    const Index nf = ptr[n];
+    std::vector<std::vector<Index>> sources(nf);
+    for (Index i = 0; i < n; ++i) sources[dpos[i]].push_back(-(i + 1));
+    for (Index i = 0; i < n; ++i)
+        for (Index q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
+            const Index j = A.col[q];
+            if (j == i) sources[dpos[i]].push_back(q);
+            else if (!chol) sources[at(i, j)].push_back(q);
+            else if (j > i) { sources[at(i, j)].push_back(q); sources[at(j, i)].push_back(q); }
+        }
+    srcPtr.assign(nf + 1, 0);
+    src.clear();
+    for (Index t = 0; t < nf; ++t) {
+        src.insert(src.end(), sources[t].begin(), sources[t].end());
+        srcPtr[t + 1] = static_cast<Index>(src.size());
+    }
+    uptr.assign(nf + 1, 0);
+    udst.clear();
+    usrc.clear();
+    for (Index i = 0; i < n; ++i)
+        for (Index e = ptr[i]; e < ptr[i + 1]; ++e) {
+            if (e < dpos[i]) {
+                const Index k = col[e];
+                for (Index a = e + 1; a < ptr[i + 1]; ++a) {
+                    const Index b = at(k, col[a]);
+                    if (b >= 0 && b > dpos[k]) { udst.push_back(a); usrc.push_back(b); }
+                }
+            }
+            uptr[e + 1] = static_cast<Index>(udst.size());
+        }
+    f.assign(nf, 0.0);
+    analysed = true;
+}
+
+void WholeIlu::factor(const Csr& A) {
+    for (Index t = 0; t + 1 < static_cast<Index>(srcPtr.size()); ++t) {
+        double s = 0.0;
+        for (Index q = srcPtr[t]; q < srcPtr[t + 1]; ++q) {
+            const Index p = src[q];
+            const double v = p >= 0 ? A.val[p] : A.diag[-p - 1];
+            s = q == srcPtr[t] ? v : s + v;
+        }
+        f[t] = s;
+    }
+    fail = false;
+    for (Index i = 0; i < n; ++i) {
+        for (Index e = ptr[i]; e < dpos[i]; ++e) {
+            const double l = f[e] / f[dpos[col[e]]];
+            f[e] = l;
+            for (Index q = uptr[e]; q < uptr[e + 1]; ++q) f[udst[q]] -= l * f[usrc[q]];
+        }
+        const double p = f[dpos[i]];
+        if (!(std::isfinite(p) && (chol ? p > 0.0 : p != 0.0))) fail = true;
+    }
+}
+
+void WholeIlu::apply(const double* r, double* z) const {
+    if (fail) {
+        for (Index i = 0; i < n; ++i) z[i] = HUGE_VAL;
+        return;
+    }
+    for (Index i = 0; i < n; ++i) {
+        double y = r[i];
+        for (Index e = ptr[i]; e < dpos[i]; ++e) y -= f[e] * z[col[e]];
+        z[i] = y;
+    }
+    for (Index i = n - 1; i >= 0; --i) {
+        double y = z[i];
+        for (Index e = dpos[i] + 1; e < ptr[i + 1]; ++e) y -= f[e] * z[col[e]];
+        z[i] = y / f[dpos[i]];
+    }
+}
+
 // petsc_linear_system.cpp:303-324; the PC keeps the matrix it was built from.
 static std::vector<double> jacobiInv;
 void LinearSystem::setup(bool reuse) {
@@ -452,6 +563,11 @@ void LinearSystem::setup(bool reuse) {
             if (!amg) amg = new Amg();
             amg->levels.assign(1, AmgLevel{A, {}, 0, {}, {}});
             blockIluFactor(A, amg->iluDiag, amg->iluVal);
+        } else if (cfg.pc == 4 || cfg.pc == 5) {
+            if (!amg) amg = new Amg();
+            // The analysis depends on the pattern and the kind only.
+            if (!amg->whole.analysed || amg->whole.chol != (cfg.pc == 4)) amg->whole.analyse(A, cfg.pc == 4);
+            amg->whole.factor(A);
         }
         if (cfg.pc != 0) ++pcSetups;
         pcMatrixUpdate = matrixUpdates;
@@ -483,6 +599,10 @@ struct Ksp {
         }
         if (cfg.pc == 3) {
             blockIluApply(pc->levels[0].A, pc->iluDiag, pc->iluVal, r.data(), z.data());
+            return;
+        }
+        if (cfg.pc >= 4) {
+            pc->whole.apply(r.data(), z.data());
             return;
         }
         pc->apply(r.data(), z.data());

@@ -105,6 +105,11 @@ int orc_system_counters(void* sys, uint64_t* out8, int* last_status, double* las
  * diagonal fd[n], factor at each entry fv[nnz]) and z = (LU)^-1 r. */
 int orc_block_ilu(int n, const int* ptr, const int* col, const double* diag, const double* val, double* fd,
                   double* fv);
+/* IC(0) (chol != 0) or ILU(0) of the whole matrix: the factor pattern
+ * (fptr[n + 1], fcol[nf], columns ascending, the diagonal included) and the
+ * factors f (L strictly below the diagonal, U on and above it). */
+int orc_whole_ilu(int n, const int* ptr, const int* col, const double* diag, const double* val, int chol,
+                  int* fptr, int* fcol, double* f, int cap, int* nf, int* failed);
 int orc_block_ilu_apply(int n, const int* ptr, const int* col, const double* diag, const double* fd,
                         const double* fv, const double* r, double* z);
 int orc_amg_hierarchy(int n, const int* ptr, const int* col, const double* diag, const double* val,

@@ -64,6 +64,7 @@ def oracle():
         L.orc_amg_hierarchy.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(SolverConfig), ip, ip, ip, dp]
         L.orc_block_ilu.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp]
         L.orc_block_ilu_apply.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, dp]
+        L.orc_whole_ilu.argtypes = [C.c_int, ip, ip, dp, dp, C.c_int, ip, ip, dp, C.c_int, ip, ip]
         L.orc_system_new.restype = vp
         L.orc_system_new.argtypes = [vp]
         L.orc_system_free.restype = None
@@ -556,6 +557,23 @@ def block_ilu_apply(ptr_, col, diag, fd, fv, r):
     if rc:
         raise ValueError(L.orc_last_error().decode())
     return z
+
+
+def whole_ilu(ptr_, col, diag, val, chol):
+    """IC(0) (chol) or ILU(0) factors of the whole matrix: (fptr, fcol, f, failed)
+    with L strictly below the diagonal and U on and above it."""
+    L = oracle()
+    n = len(diag)
+    cap = len(col) * 2 + n + 1
+    fptr, fcol, f = np.zeros(n + 1, np.int32), np.zeros(cap, np.int32), np.zeros(cap)
+    nf, failed = C.c_int(), C.c_int()
+    args = (np.ascontiguousarray(ptr_, np.int32), np.ascontiguousarray(col, np.int32),
+            np.ascontiguousarray(diag, np.float64), np.ascontiguousarray(val if len(val) else np.zeros(1), np.float64))
+    rc = L.orc_whole_ilu(n, ptr(args[0], ip), ptr(args[1], ip), ptr(args[2]), ptr(args[3]), 1 if chol else 0,
+                         ptr(fptr, ip), ptr(fcol, ip), ptr(f), cap, C.byref(nf), C.byref(failed))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    return fptr, fcol[:nf.value].copy(), f[:nf.value].copy(), bool(failed.value)
 
 
 def amg_hierarchy(ptr_, col, diag, val, cfg: SolverConfig):
