@@ -82,6 +82,7 @@ Pattern meshPattern(const DeviceMesh& M) {
     P.hptr = M.hmptr;
     P.hcol = M.hmcol;
     P.csrToStore = M.csrToSliced;
+    P.order = M.cellNewToOld;
     return P;
 }
 

@@ -1,11 +1,15 @@
 // Level-scheduled incomplete factorisations with no fill: IC(0) for PETSc's
 // icc (petsc_linear_system.cpp:204-208; PCICC factors the upper triangle) and
 // ILU(0) for asm on one rank (:224-226; one subdomain covering every row, the
-// sub-PC PETSc's default ILU(0)).  Both use the natural order of the system's
-// rows, like PETSc's default factor ordering.
+// sub-PC PETSc's default ILU(0)).
+//
+// The rows are eliminated in the caller's numbering (the system's own rows
+// keep the renumbered storage; each row's entries are sorted by their
+// columns' positions in the caller's order), so the factors are those PETSc
+// forms from the caller's matrix in its natural order.
 //
 // The symbolic analysis runs once per pattern on the host: the factor pattern
-// (row-major, columns ascending), the stored entries each factor entry sums,
+// (row-major, entries in elimination order), the stored entries each factor entry sums,
 // the (i, j) <- (k, j) update pairs of each lower entry, and the levels of the
 // lower and upper dependency graphs.  Rows of one level are independent; each
 // is one thread's sequential work (IKJ elimination, or one row of a
@@ -142,8 +146,19 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
     const std::vector<int>& rc = P.hcol;
     if (static_cast<int>(rp.size()) != n + 1) throw std::invalid_argument("incomplete factorisation: pattern size");
     auto stored = [&](int q) { return P.csrToStore.empty() ? q : P.csrToStore[q]; };
+    // Elimination order: each row's position in the caller's numbering, so
+    // that the factors are those of the caller's matrix in its natural order.
+    std::vector<int> pos(n), at(n, -1);
+    for (int i = 0; i < n; ++i) {
+        pos[i] = P.order.empty() ? i : P.order[i];
+        if (pos[i] < 0 || pos[i] >= n || at[pos[i]] >= 0)
+            throw std::invalid_argument("incomplete factorisation: the row order is not a permutation");
+        at[pos[i]] = i;
+    }
+    auto before = [&](int a, int b) { return pos[a] < pos[b]; };
     // Factor pattern: the diagonal and the off-diagonal columns of each row;
-    // for IC(0) the upper triangle's columns and their transposes.
+    // for IC(0) the upper triangle's columns (later in the order) and their
+    // transposes.
     hptr.assign(n + 1, 0);
     for (int i = 0; i < n; ++i) {
         ++hptr[i + 1];
@@ -152,7 +167,7 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
             if (j < 0 || j >= n) throw std::invalid_argument("incomplete factorisation: column out of range");
             if (j == i) continue;
             if (!chol) ++hptr[i + 1];
-            else if (j > i) { ++hptr[i + 1]; ++hptr[j + 1]; }
+            else if (before(i, j)) { ++hptr[i + 1]; ++hptr[j + 1]; }
         }
     }
     for (int i = 0; i < n; ++i) hptr[i + 1] += hptr[i];
@@ -165,29 +180,29 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
                 const int j = rc[q];
                 if (j == i) continue;
                 if (!chol) cols[fill[i]++] = j;
-                else if (j > i) { cols[fill[i]++] = j; cols[fill[j]++] = i; }
+                else if (before(i, j)) { cols[fill[i]++] = j; cols[fill[j]++] = i; }
             }
         }
     }
-    // Sort and merge repeated columns.
+    // Entries in elimination order; repeated columns merged.
     std::vector<int> cptr(n + 1, 0);
     hcol.clear();
     hcol.reserve(cols.size());
     for (int i = 0; i < n; ++i) {
         auto b = cols.begin() + hptr[i], e = cols.begin() + hptr[i + 1];
-        std::sort(b, e);
+        std::sort(b, e, before);
         e = std::unique(b, e);
         hcol.insert(hcol.end(), b, e);
         cptr[i + 1] = static_cast<int>(hcol.size());
     }
     hptr.swap(cptr);
     const int nf = hptr[n];
-    hdpos.assign(n, 0);
-    for (int i = 0; i < n; ++i)
-        hdpos[i] = static_cast<int>(std::lower_bound(hcol.begin() + hptr[i], hcol.begin() + hptr[i + 1], i) - hcol.begin());
     auto find = [&](int i, int j) {
-        return static_cast<int>(std::lower_bound(hcol.begin() + hptr[i], hcol.begin() + hptr[i + 1], j) - hcol.begin());
+        return static_cast<int>(std::lower_bound(hcol.begin() + hptr[i], hcol.begin() + hptr[i + 1], j, before) -
+                                hcol.begin());
     };
+    hdpos.assign(n, 0);
+    for (int i = 0; i < n; ++i) hdpos[i] = find(i, i);
     // Sources of each factor entry: the diagonal first, then the stored
     // entries in row-major order.
     std::vector<int> scount(nf + 1, 0);
@@ -197,7 +212,7 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
             const int j = rc[q];
             if (j == i) ++scount[hdpos[i] + 1];
             else if (!chol) ++scount[find(i, j) + 1];
-            else if (j > i) { ++scount[find(i, j) + 1]; ++scount[find(j, i) + 1]; }
+            else if (before(i, j)) { ++scount[find(i, j) + 1]; ++scount[find(j, i) + 1]; }
         }
     }
     for (int t = 0; t < nf; ++t) scount[t + 1] += scount[t];
@@ -211,7 +226,7 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
                 const int s = stored(q);
                 if (j == i) hsrc[fill[hdpos[i]]++] = s;
                 else if (!chol) { const int t = find(i, j); hsrc[fill[t]++] = s; }
-                else if (j > i) {
+                else if (before(i, j)) {
                     const int t = find(i, j), tt = find(j, i);
                     hsrc[fill[t]++] = s;
                     hsrc[fill[tt]++] = s;
@@ -219,7 +234,7 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
             }
     }
     // Update pairs of each lower entry (i, k): (i, j) and (k, j) for the
-    // columns j > k in both rows, ascending.
+    // columns j after k in both rows, in elimination order.
     std::vector<int> uptr(nf + 1, 0), udst, usrc;
     for (int i = 0; i < n; ++i) {
         for (int e = hptr[i]; e < hptr[i + 1]; ++e) {
@@ -228,7 +243,7 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
                 int a = e + 1, b = hdpos[k] + 1;
                 while (a < hptr[i + 1] && b < hptr[k + 1]) {
                     if (hcol[a] == hcol[b]) { udst.push_back(a); usrc.push_back(b); ++a; ++b; }
-                    else if (hcol[a] < hcol[b]) ++a;
+                    else if (before(hcol[a], hcol[b])) ++a;
                     else ++b;
                 }
             }
@@ -239,13 +254,15 @@ void LevelIlu::analyse(const Pattern& P, int nn) {
     // entries refer to.
     std::vector<int> lf(n, 0), lb(n, 0);
     int nlf = 0, nlb = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int p = 0; p < n; ++p) {
+        const int i = at[p];
         int l = 0;
         for (int e = hptr[i]; e < hdpos[i]; ++e) l = std::max(l, lf[hcol[e]] + 1);
         lf[i] = l;
         nlf = std::max(nlf, l + 1);
     }
-    for (int i = n - 1; i >= 0; --i) {
+    for (int p = n - 1; p >= 0; --p) {
+        const int i = at[p];
         int l = 0;
         for (int e = hdpos[i] + 1; e < hptr[i + 1]; ++e) l = std::max(l, lb[hcol[e]] + 1);
         lb[i] = l;

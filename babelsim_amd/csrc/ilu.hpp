@@ -10,9 +10,9 @@
 
 namespace bsim {
 
-// Factor pattern and values, row-major with columns ascending and the
-// diagonal at dpos[i].  L (unit lower) strictly below the diagonal, U on and
-// above it.
+// Factor pattern and values, row-major with each row's entries in
+// elimination order and the diagonal at dpos[i]: L (unit lower) before the
+// diagonal, U from it on.
 struct IluView {
     const int* ptr;
     const int* col;
@@ -25,7 +25,8 @@ public:
     // cholesky: IC(0) of the matrix's upper triangle (the lower factor's
     // pattern and values are the upper triangle's transpose), else ILU(0).
     LevelIlu(hipStream_t s, bool cholesky) : stream(s), chol(cholesky) {}
-    // Symbolic analysis of the system's pattern (once per pattern).
+    // Symbolic analysis of the system's pattern (once per pattern), in the
+    // elimination order P.order (empty: the row order).
     void analyse(const Pattern& P, int n);
     // Numeric factorisation of A's current values.  False when a pivot is
     // zero (ILU) or not positive (IC), or not finite; apply() then returns +Inf

@@ -60,6 +60,8 @@ struct Pattern {
     const int* sliceOff = nullptr;  // device slice offsets when sliced
     std::vector<int> hptr, hcol;    // host row-major CSR
     std::vector<int> csrToStore;    // row-major entry -> storage position (empty: identity)
+    std::vector<int> order;         // each row's position in the caller's numbering (empty: identity);
+                                    // incomplete factorisations eliminate in that order
 };
 
 class AmgDevice;
