@@ -239,15 +239,24 @@ def run_pair(mesh_data, cfg, two_d, iters):
     return so, sg, ro, rg
 
 
-@pytest.mark.parametrize("kind", ["c1", "c2", "c1_retry", "c1_warm", "c1_bjacobi", "c2_bjacobi"])
+@pytest.mark.parametrize("kind", ["c1", "c2", "c1_retry", "c1_warm", "c1_bjacobi", "c2_bjacobi", "c1_icc", "c2_asm"])
 def test_simple_parity(kind):
     """Device SIMPLE against the oracle.  c1_retry caps the momentum solves at 3
     BiCGSTAB iterations so that lagged-preconditioner failures trigger the
     rebuild-and-retry path of the three-component solve; c1_warm warm-starts
     every solve; the bjacobi kinds solve momentum with BiCGSTAB and block
-    Jacobi, the reference cavity case's pair (cases/cavity/numerics/solution.bs:7-8)."""
+    Jacobi, the reference cavity case's pair (cases/cavity/numerics/solution.bs:7-8);
+    c1_icc solves pressure with CG and icc, as the reference's SIMPLE tests do
+    (tests/cavity_regression_test.cpp:52-53); c2_asm solves momentum with
+    BiCGSTAB and asm."""
     bj = solver_config(solver=1, pc=3, rtol=1e-8, atol=1e-14)
-    if kind == "c1_bjacobi":
+    if kind == "c1_icc":
+        pre = solver_config(solver=0, pc=4, rtol=1e-8, atol=1e-14)
+        mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(pre=pre), True, 12
+    elif kind == "c2_asm":
+        mom = solver_config(solver=1, pc=5, rtol=1e-8, atol=1e-14)
+        mesh, cfg, two_d, iters = cases.cavity3d_mesh(7, split=0.15, seed=3), cases.c2_config(mom=mom), False, 8
+    elif kind == "c1_bjacobi":
         mesh, cfg, two_d, iters = cases.cavity2d_mesh(24), cases.c1_config(mom=bj), True, 12
     elif kind == "c2_bjacobi":
         mesh, cfg, two_d, iters = cases.cavity3d_mesh(7, split=0.15, seed=3), cases.c2_config(mom=bj), False, 8
