@@ -1,0 +1,84 @@
+"""The reference's SIMPLE regression expectations on the device, with its
+solver pairs: pressure CG + icc, momentum BiCGSTAB + bjacobi (the default
+LinearSolverConfig, solver_control.h:166-167).
+
+- tests/cavity_regression_test.cpp:13-77: 12 x 12 x 1 Re = 100 cavity
+  (upwind, Green-Gauss, orthogonal; relaxation 0.5 / 0.3; continuity 1e-8,
+  velocity 1e-6; linear rtol 1e-10): converged, primary vortex at the centre
+  (u_x < -0.05), circulation up the left wall and down the right.
+- tests/cavity_3d_test.cpp:14-84: 6^3 cavity with the lid on +y (linear rtol
+  1e-9): converged, centre vortex u_x < -0.02, the two cells astride the
+  z = 1/2 plane mirror each other to 2e-5, spanwise velocity above 1e-4,
+  mass conserved to the continuity tolerance.
+
+Cells are located by their centres, so the checks do not depend on the
+numbering."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+from babelsim_amd import simple_config, solver_config
+from babelsim_amd.device import Context, Simple
+
+pytestmark = pytest.mark.gpu
+
+
+def regression_config(rtol, atol):
+    return simple_config(rho=1.0, mu=0.01, m_interp=0, m_grad=0, m_conv=0, m_diff=0, p_interp=0, p_grad=0,
+                         p_diff=0, alpha_u=0.5, alpha_p=0.3, non_orth_corr=1, tol_mass=1e-8, tol_u=1e-6,
+                         tol_mom=1e-6, tol_p=1e-6,
+                         mom=solver_config(solver=1, pc=3, rtol=rtol, atol=atol),
+                         pre=solver_config(solver=0, pc=4, rtol=rtol, atol=atol))
+
+
+def run(mesh, cfg, two_d, cap):
+    ctx = Context(mesh, renumber=True)
+    ub, pb = cases.cavity_bcs(two_d=two_d)
+    sim = Simple(ctx, cfg, ub, pb)
+    sim.start()
+    its, last = 0, None
+    while its < cap:
+        reps = sim.iterate(100)
+        its += len(reps)
+        last = reps[-1]
+        assert all(r.mom_status != 2 and r.p_status != 2 for r in reps)
+        if any(r.converged for r in reps):
+            last = next(r for r in reps if r.converged)
+            break
+    U, p, _ = sim.fields()
+    return ctx.geometry()["C"], U, last, its
+
+
+def cell_at(C, x, y, z):
+    return int(np.argmin(np.sum((C - np.array([x, y, z])) ** 2, axis=1)))
+
+
+def test_cavity_regression_12x12():
+    n = 12
+    C, U, last, its = run(cases.cavity2d_mesh(n), regression_config(1e-10, 1e-16), True, 3000)
+    print(f"12x12 cavity: converged={last.converged} after {its} iterations, mass {last.mass:.2e}", flush=True)
+    assert last.converged
+    h = 1.0 / n
+    centre = cell_at(C, (n // 2 - 0.5) * h, (n // 2 - 0.5) * h, C[:, 2].mean())
+    assert U[centre, 0] < -0.05
+    upper_left = cell_at(C, 0.5 * h, (n - 2.5) * h, C[:, 2].mean())
+    upper_right = cell_at(C, (n - 0.5) * h, (n - 2.5) * h, C[:, 2].mean())
+    assert U[upper_left, 1] > 0.0 and U[upper_right, 1] < 0.0
+
+
+def test_cavity_3d_6cubed():
+    n = 6
+    mesh = cases.cavity3d_mesh(n, jitter=0.0, split=0.0)
+    C, U, last, its = run(mesh, regression_config(1e-9, 1e-15), False, 2000)
+    print(f"6^3 cavity: converged={last.converged} after {its} iterations, mass {last.mass:.2e}", flush=True)
+    assert last.converged
+    h = 1.0 / n
+    lower = U[cell_at(C, (n // 2 - 0.5) * h, (n // 2 - 0.5) * h, (n // 2 - 0.5) * h)]
+    upper = U[cell_at(C, (n // 2 - 0.5) * h, (n // 2 - 0.5) * h, (n // 2 + 0.5) * h)]
+    assert 0.5 * (lower[0] + upper[0]) < -0.02
+    assert abs(lower[0] - upper[0]) <= 2e-5 and abs(lower[1] - upper[1]) <= 2e-5
+    assert abs(lower[2] + upper[2]) <= 2e-5
+    assert np.abs(U[:, 2]).max() > 1e-4
+    assert last.mass <= 1e-8
