@@ -108,3 +108,37 @@ def test_nonorthogonal_cavity_3d():
     assert U[centre, 0] < -0.015
     assert np.abs(U[:, 2]).max() > 1e-4
     assert last.mass <= 2e-8
+
+
+def test_channel_fixture():
+    """tests/simple_solver_test.cpp:58-116 on the reference's own channel mesh
+    file (tests/data/babelsim_channel.mesh): rho 2, mu 0.2, upwind,
+    Green-Gauss, orthogonal, relaxation 0.5 / 0.3, continuity 1e-9, velocity
+    1e-7, linear rtol 1e-10 / atol 1e-13, at most 500 iterations.  Converged,
+    mass conserved, every velocity finite and, with one cell in z, no z
+    velocity above 1e-13."""
+    from pathlib import Path
+    from babelsim_amd import MeshData
+    mesh = MeshData.read(Path(__file__).resolve().parent / "golden" / "meshes" / "babelsim_channel.mesh")
+    names = [nm for nm, _ in mesh.patches()]
+    ut = np.array([cases.FIXED_VALUE, cases.ZERO_GRADIENT, cases.FIXED_VALUE, cases.FIXED_VALUE, cases.SYMMETRY,
+                   cases.SYMMETRY], np.int32)
+    uv = np.zeros((6, 3))
+    uv[names.index("inlet")] = (1.0, 0.0, 0.0)
+    pt = np.array([cases.ZERO_GRADIENT, cases.FIXED_VALUE, cases.ZERO_GRADIENT, cases.ZERO_GRADIENT, cases.SYMMETRY,
+                   cases.SYMMETRY], np.int32)
+    cfg = simple_config(rho=2.0, mu=0.2, m_interp=0, m_grad=0, m_conv=0, m_diff=0, p_interp=0, p_grad=0,
+                        p_diff=0, alpha_u=0.5, alpha_p=0.3, non_orth_corr=1, tol_mass=1e-9, tol_u=1e-7,
+                        tol_mom=1e-6, tol_p=1e-6,
+                        mom=solver_config(solver=1, pc=3, rtol=1e-10, atol=1e-13),
+                        pre=solver_config(solver=0, pc=4, rtol=1e-10, atol=1e-13))
+    sim = Simple(Context(mesh, renumber=True), cfg, (ut, uv), (pt, np.zeros(6)))
+    sim.start()
+    reps = sim.iterate(500)
+    conv = [r for r in reps if r.converged]
+    print(f"channel fixture: converged after {len(reps)} iterations", flush=True)
+    assert conv and all(r.mom_status != 2 and r.p_status != 2 for r in reps)
+    assert conv[0].mass <= 1e-9
+    U, _, _ = sim.fields()
+    assert np.all(np.isfinite(U))
+    assert np.abs(U[:, 2]).max() < 1e-13
