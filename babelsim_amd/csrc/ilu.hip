@@ -64,16 +64,35 @@ __device__ inline void factorRow(const IluView& V, const IluUpdates& U, int i, b
     const bool ok = chol ? (p > 0.0 && __builtin_isfinite(p)) : (p != 0.0 && __builtin_isfinite(p));
     if (!ok) *flag = 1;
 }
+// y -= sum over entries [e0, e1) of f[e] z[col[e]], in entry order.  The
+// entries are taken kIluChunk at a time: their columns and factors load
+// together, then the z values together, then the products are subtracted in
+// order.
+constexpr int kIluChunk = 8;
+__device__ inline double subtractRow(const IluView& V, int e0, int e1, const double* z, double y) {
+    for (int e = e0; e < e1; e += kIluChunk) {
+        int c[kIluChunk];
+        double fv[kIluChunk], zv[kIluChunk];
+#pragma unroll
+        for (int k = 0; k < kIluChunk; ++k) {
+            const bool in = e + k < e1;
+            c[k] = in ? V.col[e + k] : 0;
+            fv[k] = in ? V.f[e + k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kIluChunk; ++k) zv[k] = e + k < e1 ? z[c[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kIluChunk; ++k)
+            if (e + k < e1) y -= fv[k] * zv[k];
+    }
+    return y;
+}
 __device__ inline void forwardRow(const IluView& V, int i, const double* r, double* z) {
-    double y = r[i];
-    for (int e = V.ptr[i]; e < V.dpos[i]; ++e) y -= V.f[e] * z[V.col[e]];
-    z[i] = y;
+    z[i] = subtractRow(V, V.ptr[i], V.dpos[i], z, r[i]);
 }
 __device__ inline void backwardRow(const IluView& V, int i, double* z) {
     const int d = V.dpos[i];
-    double y = z[i];
-    for (int e = d + 1; e < V.ptr[i + 1]; ++e) y -= V.f[e] * z[V.col[e]];
-    z[i] = y / V.f[d];
+    z[i] = subtractRow(V, d + 1, V.ptr[i + 1], z, z[i]) / V.f[d];
 }
 
 __global__ void __launch_bounds__(kBlock) kIluFactorLevel(IluView V, IluUpdates U, const int* rows, int count,
